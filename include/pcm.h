@@ -1,0 +1,216 @@
+/*
+ * pcm.h — C-ABI of the MI355X (gfx950) mechanics-timestep library `libpcm.so`.
+ *
+ * This is the drop-in boundary for paraPhysiCell's mechanics solver (src/solver/).
+ * Each entry point replaces one reference entry point; the replaced symbol is cited
+ * beside it.  All pointers are plain host pointers, all sizes are plain integers, and
+ * every function returns a status code (PCM_OK == 0).  pcm_last_error() gives the
+ * message of the most recent failure on the calling thread.
+ *
+ * Data contract (mirrors src/cell_data.h:14-196):
+ *   - agent fields are laid out exactly like the reference's std::vector members:
+ *     one array per field, `count * width` elements, agent-major (positions of agent i
+ *     at [i*dims, i*dims+dims)); 2D simulations store exactly two coordinates;
+ *   - real-valued fields hold `real_bytes`-wide IEEE values (4 = float, 8 = double),
+ *     index fields are int32 (biofvm::index_t), flags are uint8;
+ *   - per-agent lists (springs, attached cells, neighbors) travel as CSR
+ *     (offsets[count+1], indices[offsets[count]]).
+ *
+ * Residency: the handle owns device-resident state.  Host buffers are borrowed only for
+ * the duration of a call.  A handle is not thread-safe; use one handle per thread.
+ */
+#ifndef PCM_H
+#define PCM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define PCM_OK 0
+#define PCM_ERR_ARGUMENT 1    /* bad argument (null pointer, size out of range, ...)     */
+#define PCM_ERR_DEVICE 2      /* a HIP runtime call failed                                */
+#define PCM_ERR_CAPACITY 3    /* agent capacity, list capacity or budget exceeded         */
+#define PCM_ERR_UNSUPPORTED 4 /* configuration the device path does not implement        */
+#define PCM_ERR_STATE 5       /* the device status word reports an aborted step           */
+
+/* ---------------------------------------------------------------- field table */
+/* element types */
+#define PCM_T_REAL 0 /* real_bytes wide */
+#define PCM_T_I32 1
+#define PCM_T_U8 2
+/* width kinds: elements per agent */
+#define PCM_W_ONE 0  /* 1                      */
+#define PCM_W_DIMS 1 /* mesh dims (1, 2 or 3)   */
+#define PCM_W_DEFS 2 /* cell definitions count  */
+#define PCM_W_SUBS 3 /* substrates count        */
+
+/* X(name, element type, width kind) — order defines the field ids. */
+#define PCM_FIELD_LIST(X)                                                                                              \
+	X(POSITIONS, PCM_T_REAL, PCM_W_DIMS)                          /* agent_data.positions            */            \
+	X(VELOCITIES, PCM_T_REAL, PCM_W_DIMS)                         /* velocities                      */            \
+	X(PREVIOUS_VELOCITIES, PCM_T_REAL, PCM_W_DIMS)                /* previous_velocities             */            \
+	X(RADIUS, PCM_T_REAL, PCM_W_ONE)                              /* geometries.radius               */            \
+	X(NUCLEAR_RADIUS, PCM_T_REAL, PCM_W_ONE)                      /* geometries.nuclear_radius       */            \
+	X(SURFACE_AREA, PCM_T_REAL, PCM_W_ONE)                        /* geometries.surface_area         */            \
+	X(VOLUME, PCM_T_REAL, PCM_W_ONE)                              /* agent_data.volumes (total)      */            \
+	X(FLUID, PCM_T_REAL, PCM_W_ONE)                               /* volumes.fluid                   */            \
+	X(SOLID, PCM_T_REAL, PCM_W_ONE)                               /* volumes.solid                   */            \
+	X(NUCLEAR, PCM_T_REAL, PCM_W_ONE)                             /* volumes.nuclear                 */            \
+	X(NUCLEAR_FLUID, PCM_T_REAL, PCM_W_ONE)                       /* volumes.nuclear_fluid           */            \
+	X(NUCLEAR_SOLID, PCM_T_REAL, PCM_W_ONE)                       /* volumes.nuclear_solid           */            \
+	X(CYTOPLASMIC, PCM_T_REAL, PCM_W_ONE)                         /* volumes.cytoplasmic             */            \
+	X(CYTOPLASMIC_FLUID, PCM_T_REAL, PCM_W_ONE)                   /* volumes.cytoplasmic_fluid       */            \
+	X(CYTOPLASMIC_SOLID, PCM_T_REAL, PCM_W_ONE)                   /* volumes.cytoplasmic_solid       */            \
+	X(FLUID_FRACTION, PCM_T_REAL, PCM_W_ONE)                      /* volumes.fluid_fraction          */            \
+	X(CYTOPLASMIC_TO_NUCLEAR_RATIO, PCM_T_REAL, PCM_W_ONE)        /* volumes.cytoplasmic_to_nuclear_ratio */       \
+	X(TARGET_SOLID_CYTOPLASMIC, PCM_T_REAL, PCM_W_ONE)            /* volumes.target_solid_cytoplasmic */           \
+	X(TARGET_SOLID_NUCLEAR, PCM_T_REAL, PCM_W_ONE)                /* volumes.target_solid_nuclear    */            \
+	X(CELL_CELL_ADHESION_STRENGTH, PCM_T_REAL, PCM_W_ONE)         /* mechanics.*                     */            \
+	X(CELL_CELL_REPULSION_STRENGTH, PCM_T_REAL, PCM_W_ONE)                                                         \
+	X(CELL_BM_REPULSION_STRENGTH, PCM_T_REAL, PCM_W_ONE)                                                           \
+	X(RELATIVE_MAXIMUM_ADHESION_DISTANCE, PCM_T_REAL, PCM_W_ONE)                                                   \
+	X(CELL_ADHESION_AFFINITIES, PCM_T_REAL, PCM_W_DEFS)                                                            \
+	X(MAXIMUM_NUMBER_OF_ATTACHMENTS, PCM_T_I32, PCM_W_ONE)                                                         \
+	X(ATTACHMENT_ELASTIC_CONSTANT, PCM_T_REAL, PCM_W_ONE)                                                          \
+	X(ATTACHMENT_RATE, PCM_T_REAL, PCM_W_ONE)                                                                      \
+	X(DETACHMENT_RATE, PCM_T_REAL, PCM_W_ONE)                                                                      \
+	X(IS_MOTILE, PCM_T_U8, PCM_W_ONE)                             /* motilities.*                    */            \
+	X(PERSISTENCE_TIME, PCM_T_REAL, PCM_W_ONE)                                                                     \
+	X(MIGRATION_SPEED, PCM_T_REAL, PCM_W_ONE)                                                                      \
+	X(MIGRATION_BIAS_DIRECTION, PCM_T_REAL, PCM_W_DIMS)                                                            \
+	X(MIGRATION_BIAS, PCM_T_REAL, PCM_W_ONE)                                                                       \
+	X(MOTILITY_VECTOR, PCM_T_REAL, PCM_W_DIMS)                                                                     \
+	X(RESTRICT_TO_2D, PCM_T_U8, PCM_W_ONE)                                                                         \
+	X(DEAD, PCM_T_U8, PCM_W_ONE)                                  /* deaths.dead                     */            \
+	X(DEAD_PHAGOCYTOSIS_RATE, PCM_T_REAL, PCM_W_ONE)              /* interactions.*                  */            \
+	X(LIVE_PHAGOCYTOSIS_RATES, PCM_T_REAL, PCM_W_DEFS)                                                             \
+	X(DAMAGE_RATE, PCM_T_REAL, PCM_W_ONE)                                                                          \
+	X(ATTACK_RATES, PCM_T_REAL, PCM_W_DEFS)                                                                        \
+	X(IMMUNOGENICITIES, PCM_T_REAL, PCM_W_DEFS)                                                                    \
+	X(FUSION_RATES, PCM_T_REAL, PCM_W_DEFS)                                                                        \
+	X(SIMPLE_PRESSURE, PCM_T_REAL, PCM_W_ONE)                     /* states.*                        */            \
+	X(NUMBER_OF_NUCLEI, PCM_T_I32, PCM_W_ONE)                                                                      \
+	X(DAMAGE, PCM_T_REAL, PCM_W_ONE)                                                                               \
+	X(TOTAL_ATTACK_TIME, PCM_T_REAL, PCM_W_ONE)                                                                    \
+	X(CELL_DEFINITION_INDEX, PCM_T_I32, PCM_W_ONE)                /* cell_definition_indices         */            \
+	X(IS_MOVABLE, PCM_T_U8, PCM_W_ONE)                            /* is_movable                      */            \
+	X(FLAGS, PCM_T_U8, PCM_W_ONE)                                 /* flags (cell_state_flag)         */            \
+	X(INTERNALIZED_SUBSTRATES, PCM_T_REAL, PCM_W_SUBS)            /* agent_data.internalized_substrates */         \
+	X(FRACTION_TRANSFERRED_WHEN_INGESTED, PCM_T_REAL, PCM_W_SUBS) /* agent_data.fraction_transferred_... */        \
+	X(ORIGIN, PCM_T_I32, PCM_W_ONE) /* index of the agent in the last host upload; moves with the agent */
+
+#define PCM_FIELD_ENUM_(name, type, width) PCM_F_##name,
+enum pcm_field
+{
+	PCM_FIELD_LIST(PCM_FIELD_ENUM_) PCM_FIELD_COUNT
+};
+#undef PCM_FIELD_ENUM_
+
+/* cell_state_flag values (src/cell_data.h:155-160) */
+#define PCM_FLAG_NONE 0
+#define PCM_FLAG_TO_REMOVE 1
+#define PCM_FLAG_TO_DIVIDE 2
+
+/* link lists */
+#define PCM_LINKS_SPRINGS 0  /* states.springs        */
+#define PCM_LINKS_ATTACHED 1 /* states.attached_cells */
+
+/* ---------------------------------------------------------------- configuration */
+typedef struct pcm_mesh_desc
+{
+	int32_t dims;        /* 1, 2 or 3 (microenvironment mesh dims)                         */
+	int32_t bbox_min[3]; /* microenvironment bounding box, integer microns                  */
+	int32_t bbox_max[3];
+	int32_t voxel[3]; /* mechanics voxel shape (default {30,30,30}, src/environment.h:10)     */
+} pcm_mesh_desc;
+
+typedef struct pcm_config
+{
+	pcm_mesh_desc mesh;
+	int32_t real_bytes;  /* 4 (FP32 real_t) or 8 (FP64 real_t)                          */
+	int32_t cell_defs;   /* environment::cell_definitions_count                           */
+	int32_t substrates;  /* microenvironment::substrates_count                             */
+	int32_t capacity;    /* maximum agents the handle will hold                          */
+	uint64_t seed;       /* key of the counter-based random stream                       */
+} pcm_config;
+
+/* The environment scalars a mechanics phase reads (src/environment.h:30-41). */
+typedef struct pcm_step_params
+{
+	double mechanics_time_step;        /* environment::mechanics_time_step             */
+	int32_t virtual_wall_at_domain_edges; /* environment::virtual_wall_at_domain_edges */
+	int32_t automated_spring_adhesion; /* environment::automated_spring_adhesion        */
+} pcm_step_params;
+
+typedef struct pcm_sim pcm_sim; /* one simulation's device-resident mechanics state */
+
+/* ---------------------------------------------------------------- handle lifetime */
+/* solver::initialize (src/solver/solver.cpp:5) + containers_solver::initialize
+ * (src/solver/host/containers_solver.cpp:10-13): allocates the device SoA, the mechanics
+ * voxel grid and the list storage. */
+int pcm_create(const pcm_config* config, pcm_sim** out);
+int pcm_destroy(pcm_sim* sim);
+const char* pcm_last_error(void);
+/* Field metadata; needs no GPU. */
+int pcm_field_info(int32_t field, int32_t* type, int32_t* width_kind, const char** name);
+int pcm_version(void);
+
+/* ---------------------------------------------------------------- data movement */
+/* cell_data::agents_count.  Setting the count resets ORIGIN to the identity. */
+int pcm_set_count(pcm_sim* sim, int32_t count);
+int pcm_get_count(pcm_sim* sim, int32_t* count);
+int pcm_set_field(pcm_sim* sim, int32_t field, const void* host);
+int pcm_get_field(pcm_sim* sim, int32_t field, void* host);
+/* springs / attached_cells as CSR (offsets has count+1 entries) */
+int pcm_set_links(pcm_sim* sim, int32_t which, const int32_t* offsets, const int32_t* indices);
+int pcm_get_links_size(pcm_sim* sim, int32_t which, int64_t* total);
+int pcm_get_links(pcm_sim* sim, int32_t which, int32_t* offsets, int32_t* indices);
+/* states.neighbors as built by update_cell_neighbors (empty for non-movable agents) */
+int pcm_get_neighbors_size(pcm_sim* sim, int64_t* total);
+int pcm_get_neighbors(pcm_sim* sim, int32_t* offsets, int32_t* indices);
+/* environment::cells_in_mechanics_voxels: counts[voxel_count], agents concatenated in
+ * voxel order (each bucket in the reference's bucket order). */
+int pcm_get_voxel_count(pcm_sim* sim, int64_t* voxel_count);
+int pcm_get_voxel_buckets(pcm_sim* sim, int32_t* counts, int32_t* agents);
+/* environment::deaths_count increments made on the device since creation */
+int pcm_get_deaths_count(pcm_sim* sim, int64_t* deaths);
+/* mechanics-step counter that keys the random stream (incremented by
+ * pcm_update_cell_forces and by every pcm_step iteration) */
+int pcm_set_step_index(pcm_sim* sim, uint64_t step);
+int pcm_get_step_index(pcm_sim* sim, uint64_t* step);
+
+/* ---------------------------------------------------------------- reference phases */
+/* containers_solver::update_mechanics_mesh (src/solver/host/containers_solver.cpp:15-56) */
+int pcm_update_mechanics_mesh(pcm_sim* sim);
+/* position_solver::update_cell_neighbors (src/solver/host/position_solver.cpp:262-286) */
+int pcm_update_cell_neighbors(pcm_sim* sim);
+/* position_solver::update_cell_forces (src/solver/host/position_solver.cpp:233-260) */
+int pcm_update_cell_forces(pcm_sim* sim, const pcm_step_params* params);
+/* position_solver::update_motility (src/solver/host/position_solver.cpp:288-314) */
+int pcm_update_motility(pcm_sim* sim, const pcm_step_params* params);
+/* position_solver::update_basement_membrane_interactions (position_solver.cpp:316-335) */
+int pcm_update_basement_membrane_interactions(pcm_sim* sim, const pcm_step_params* params);
+/* position_solver::update_spring_attachments (src/solver/host/position_solver.cpp:451-479) */
+int pcm_update_spring_attachments(pcm_sim* sim, const pcm_step_params* params);
+/* position_solver::update_positions (src/solver/host/position_solver.cpp:506-520) */
+int pcm_update_positions(pcm_sim* sim, const pcm_step_params* params);
+/* interactions_solver::update_cell_cell_interactions (interactions_solver.cpp:255-265) */
+int pcm_update_cell_cell_interactions(pcm_sim* sim, const pcm_step_params* params);
+/* containers_solver::update_cell_container_for_mechanics (containers_solver.cpp:140-147) */
+int pcm_update_cell_container_for_mechanics(pcm_sim* sim, const pcm_step_params* params);
+
+/* The mechanics step of simulator::simulate_mechanics (src/simulator.cpp:173-192), run
+ * `steps` times without host round trips: forces, motility, basement membrane, springs,
+ * positions, cell-cell interactions, container compaction, mechanics mesh, neighbors. */
+int pcm_step(pcm_sim* sim, const pcm_step_params* params, int32_t steps);
+/* Wait for queued device work; reports PCM_ERR_STATE if a step aborted. */
+int pcm_synchronize(pcm_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCM_H */

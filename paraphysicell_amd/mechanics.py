@@ -1,0 +1,308 @@
+"""Python host side of the MI355X mechanics library (libpcm.so, include/pcm.h).
+
+``DeviceSim`` owns one simulation's device-resident state.  ``Solver`` mirrors the
+reference's solver aggregate (src/solver/solver.h:9-17): ``solver.position``,
+``solver.containers`` and ``solver.interactions`` expose the reference's entry points by
+name, each one forwarding to the matching C-ABI call.
+
+There is no CPU fallback: constructing a ``DeviceSim`` without the built library, or on
+a machine without a GPU, raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+from .state import AgentState
+
+
+class PcmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"pcm error {code}: {message}")
+        self.code = code
+
+
+_PTR = ctypes.c_void_p
+
+
+def _declare(lib, prefix: str):
+    """Argument types of the C entry points shared by libpcm.so and the oracle."""
+    P = _PTR
+    sig = {
+        "create": [ctypes.POINTER(abi.Config), ctypes.POINTER(P)],
+        "destroy": [P],
+        "set_count": [P, ctypes.c_int32],
+        "get_count": [P, ctypes.POINTER(ctypes.c_int32)],
+        "set_field": [P, ctypes.c_int32, P],
+        "get_field": [P, ctypes.c_int32, P],
+        "set_links": [P, ctypes.c_int32, P, P],
+        "get_links_size": [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)],
+        "get_links": [P, ctypes.c_int32, P, P],
+        "get_neighbors_size": [P, ctypes.POINTER(ctypes.c_int64)],
+        "get_neighbors": [P, P, P],
+        "get_voxel_count": [P, ctypes.POINTER(ctypes.c_int64)],
+        "get_voxel_buckets": [P, P, P],
+        "get_deaths_count": [P, ctypes.POINTER(ctypes.c_int64)],
+        "set_step_index": [P, ctypes.c_uint64],
+        "get_step_index": [P, ctypes.POINTER(ctypes.c_uint64)],
+        "update_mechanics_mesh": None,
+        "update_cell_neighbors": None,
+        "update_cell_forces": [P, ctypes.POINTER(abi.StepParams)],
+        "update_motility": [P, ctypes.POINTER(abi.StepParams)],
+        "update_basement_membrane_interactions": [P, ctypes.POINTER(abi.StepParams)],
+        "update_spring_attachments": [P, ctypes.POINTER(abi.StepParams)],
+        "update_positions": [P, ctypes.POINTER(abi.StepParams)],
+        "update_cell_cell_interactions": [P, ctypes.POINTER(abi.StepParams)],
+        "update_cell_container_for_mechanics": [P, ctypes.POINTER(abi.StepParams)],
+        "step": [P, ctypes.POINTER(abi.StepParams), ctypes.c_int32],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = ctypes.c_int
+        if args is not None:
+            fn.argtypes = args
+    getattr(lib, prefix + "last_error").restype = ctypes.c_char_p
+
+
+class _LibSim:
+    """Common driver over a C library exposing the pcm.h entry points under ``prefix``."""
+
+    prefix = "pcm_"
+    _mesh_only_phases = ("update_mechanics_mesh", "update_cell_neighbors")
+
+    def __init__(self, lib, mesh: abi.MeshSpec, real_bytes: int = 4, defs: int = 1, subs: int = 1,
+                 capacity: int = 1024, seed: int = 0):
+        self._lib = lib
+        self.mesh = mesh
+        self.real_bytes = real_bytes
+        self.defs = defs
+        self.subs = subs
+        self.capacity = capacity
+        self.seed = seed
+        cfg = abi.make_config(mesh, real_bytes, defs, subs, capacity, seed)
+        h = _PTR()
+        self._h = None
+        self._call("create", ctypes.byref(cfg), ctypes.byref(h))
+        self._h = h
+
+    # ------------------------------------------------------------------ plumbing
+    def _fn(self, name):
+        return getattr(self._lib, self.prefix + name)
+
+    def _call(self, name, *args):
+        rc = self._fn(name)(*args)
+        if rc != abi.PCM_OK:
+            msg = self._fn("last_error")()
+            raise PcmError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self._h is not None:
+            self._fn("destroy")(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------ data movement
+    @property
+    def count(self) -> int:
+        n = ctypes.c_int32()
+        self._call("get_count", self._h, ctypes.byref(n))
+        return n.value
+
+    def upload(self, state: AgentState, fields=None) -> None:
+        if state.count > self.capacity:
+            raise PcmError(abi.PCM_ERR_CAPACITY, "state larger than capacity")
+        self._call("set_count", self._h, state.count)
+        for name, _, _ in abi.FIELDS:
+            if fields is not None and name not in fields:
+                continue
+            a = np.ascontiguousarray(state.arrays[name])
+            self._call("set_field", self._h, abi.FIELD_ID[name], a.ctypes.data)
+        for which in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED):
+            off, idx = state.links[which]
+            off = np.ascontiguousarray(off, dtype=np.int32)
+            idx = np.ascontiguousarray(idx, dtype=np.int32)
+            if idx.size == 0:
+                idx = np.zeros(1, np.int32)
+            self._call("set_links", self._h, which, off.ctypes.data, idx.ctypes.data)
+
+    def get_field(self, name: str) -> np.ndarray:
+        ftype, wkind = abi.FIELDS[abi.FIELD_ID[name]][1:]
+        n = self.count
+        w = abi.field_width(wkind, self.mesh.dims, self.defs, self.subs)
+        out = np.zeros(max(n * w, 1), dtype=abi.field_dtype(ftype, self.real_bytes))
+        self._call("get_field", self._h, abi.FIELD_ID[name], out.ctypes.data)
+        return out[: n * w]
+
+    def download(self) -> AgentState:
+        n = self.count
+        st = AgentState(self.mesh, self.real_bytes, self.defs, self.subs, n)
+        for name, _, _ in abi.FIELDS:
+            st.arrays[name] = self.get_field(name)
+        for which in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED):
+            st.links[which] = self.links(which)
+        return st
+
+    def links(self, which: int):
+        total = ctypes.c_int64()
+        self._call("get_links_size", self._h, which, ctypes.byref(total))
+        off = np.zeros(self.count + 1, np.int32)
+        idx = np.zeros(max(total.value, 1), np.int32)
+        self._call("get_links", self._h, which, off.ctypes.data, idx.ctypes.data)
+        return off, idx[: total.value]
+
+    def neighbors(self):
+        total = ctypes.c_int64()
+        self._call("get_neighbors_size", self._h, ctypes.byref(total))
+        off = np.zeros(self.count + 1, np.int32)
+        idx = np.zeros(max(total.value, 1), np.int32)
+        self._call("get_neighbors", self._h, off.ctypes.data, idx.ctypes.data)
+        return off, idx[: total.value]
+
+    def buckets(self):
+        v = ctypes.c_int64()
+        self._call("get_voxel_count", self._h, ctypes.byref(v))
+        counts = np.zeros(v.value, np.int32)
+        agents = np.zeros(max(self.count, 1), np.int32)
+        self._call("get_voxel_buckets", self._h, counts.ctypes.data, agents.ctypes.data)
+        return counts, agents[: int(counts.sum())]
+
+    @property
+    def deaths_count(self) -> int:
+        d = ctypes.c_int64()
+        self._call("get_deaths_count", self._h, ctypes.byref(d))
+        return d.value
+
+    @property
+    def step_index(self) -> int:
+        s = ctypes.c_uint64()
+        self._call("get_step_index", self._h, ctypes.byref(s))
+        return s.value
+
+    @step_index.setter
+    def step_index(self, value: int) -> None:
+        self._call("set_step_index", self._h, int(value))
+
+    # ------------------------------------------------------------------ phases
+    def _phase(self, name, params):
+        if name in self._mesh_only_phases:
+            self._call(name, self._h)
+        else:
+            p = (params or abi.Params()).to_c()
+            self._call(name, self._h, ctypes.byref(p))
+
+    def update_mechanics_mesh(self, params=None):
+        self._phase("update_mechanics_mesh", params)
+
+    def update_cell_neighbors(self, params=None):
+        self._phase("update_cell_neighbors", params)
+
+    def update_cell_forces(self, params=None):
+        self._phase("update_cell_forces", params)
+
+    def update_motility(self, params=None):
+        self._phase("update_motility", params)
+
+    def update_basement_membrane_interactions(self, params=None):
+        self._phase("update_basement_membrane_interactions", params)
+
+    def update_spring_attachments(self, params=None):
+        self._phase("update_spring_attachments", params)
+
+    def update_positions(self, params=None):
+        self._phase("update_positions", params)
+
+    def update_cell_cell_interactions(self, params=None):
+        self._phase("update_cell_cell_interactions", params)
+
+    def update_cell_container_for_mechanics(self, params=None):
+        self._phase("update_cell_container_for_mechanics", params)
+
+    def initialize_mesh(self):
+        """simulator::initialize tail (src/simulator.cpp:47-48): mesh, then neighbors."""
+        self.update_mechanics_mesh()
+        self.update_cell_neighbors()
+
+    def step(self, params=None, steps: int = 1):
+        p = (params or abi.Params()).to_c()
+        self._call("step", self._h, ctypes.byref(p), int(steps))
+
+    def run_phases(self, params=None):
+        """The mechanics step as the sequence of per-phase calls (src/simulator.cpp:173-192)."""
+        for name in ("update_cell_forces", "update_motility", "update_basement_membrane_interactions",
+                     "update_spring_attachments", "update_positions", "update_cell_cell_interactions",
+                     "update_cell_container_for_mechanics", "update_mechanics_mesh", "update_cell_neighbors"):
+            getattr(self, name)(params)
+
+
+_LIB = None
+
+
+def load_library(path: str | None = None):
+    """Load libpcm.so; raises if it has not been built (no fallback exists)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or abi.LIB_PATH
+    if not os.path.exists(p):
+        raise PcmError(abi.PCM_ERR_UNSUPPORTED,
+                       f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(p)
+    _declare(lib, "pcm_")
+    lib.pcm_synchronize.argtypes = [_PTR]
+    lib.pcm_synchronize.restype = ctypes.c_int
+    lib.pcm_field_info.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_char_p)]
+    lib.pcm_field_info.restype = ctypes.c_int
+    lib.pcm_version.restype = ctypes.c_int
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+class DeviceSim(_LibSim):
+    """One simulation's mechanics state resident on the GPU (libpcm.so)."""
+
+    prefix = "pcm_"
+
+    def __init__(self, mesh: abi.MeshSpec, real_bytes: int = 4, defs: int = 1, subs: int = 1,
+                 capacity: int = 1024, seed: int = 0, lib=None):
+        super().__init__(lib or load_library(), mesh, real_bytes, defs, subs, capacity, seed)
+
+    def synchronize(self):
+        self._call("synchronize", self._h)
+
+
+class _Facade:
+    def __init__(self, sim, names):
+        self._sim = sim
+        for n in names:
+            setattr(self, n, getattr(sim, n))
+
+
+class Solver:
+    """Reference-shaped aggregate: solver.position / .containers / .interactions."""
+
+    def __init__(self, sim: _LibSim):
+        self.sim = sim
+        self.position = _Facade(sim, ["update_cell_forces", "update_cell_neighbors", "update_motility",
+                                      "update_basement_membrane_interactions", "update_spring_attachments",
+                                      "update_positions"])
+        self.containers = _Facade(sim, ["update_mechanics_mesh", "update_cell_container_for_mechanics"])
+        self.interactions = _Facade(sim, ["update_cell_cell_interactions"])
+
+    def initialize(self):
+        self.sim.initialize_mesh()
