@@ -1,0 +1,383 @@
+// pcm_device.h — device-side state of one simulation and the per-agent arithmetic of the
+// mechanics step, shared by all kernels of libpcm.so.
+//
+// Layout in HBM (all arrays sized by the handle's agent capacity):
+//   - every agent field of include/pcm.h in the reference's own layout (agent-major,
+//     `width` elements per agent), so uploads and downloads are plain copies;
+//   - neighbor lists: fixed stride K per agent (nbr_idx[i*K + s], nbr_cnt[i]); the list
+//     of every agent is built (non-movable ones too, they feed simple_pressure), the
+//     reference-visible list of a non-movable agent is empty;
+//   - springs / attached cells: fixed stride L / LA per agent with counts;
+//   - mechanics voxel grid: vcnt[V], vstart[V] (only voxels touched by agents are ever
+//     written), bucket[] holding each voxel's agents contiguously in the reference's
+//     bucket order, and a voxel-sorted copy of (x, y, z, rel*r) for the neighbor search.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pcm.h"
+#include "pcm_rng.h"
+
+namespace pcm {
+
+// device status bits (Scalars::status); any set bit makes later kernels of the step return
+enum : int
+{
+	ST_NEIGHBOR_OVERFLOW = 1, // some agent has more than K neighbors: grow K, rebuild
+	ST_LIST_OVERFLOW = 2,     // a link list would exceed its fixed capacity
+	ST_CAPACITY = 4,          // a scratch buffer capacity was exceeded
+};
+
+struct Scalars
+{
+	int n;                 // agents_count
+	int status;            // ST_* bits
+	unsigned long long step; // mechanics step index (keys the random stream)
+	int cursor;            // bucket allocation cursor of the voxel counting sort
+	int n_binned;          // agents binned by the last mesh update (for the sparse clear)
+	int att_events;        // spring attachment events collected this step
+	int cand_count;        // interaction candidate slots collected this step
+	int conflict;          // interaction speculation found a cross-agent dependency
+	int bad_count;         // agents the container update removes this step
+	int max_neighbors;     // largest neighbor count seen by the last neighbor build
+	int moves;             // agents moved by the container update
+	long long deaths;      // environment::deaths_count increments
+	int pad[4];
+};
+
+template <class R>
+struct SimCtx
+{
+	// configuration
+	int dims, T, S, cap;
+	int bmin[3], bmax[3], vox[3], grid[3];
+	long long vcount;
+	int K;       // neighbor slots per agent
+	int L;       // spring slots per agent
+	int LA;      // attached-cell slots per agent
+	int ev_cap;  // attachment event buffer
+	int cand_cap; // interaction candidate buffer
+	int bad_cap;  // removal list buffer
+	unsigned long long seed;
+	// environment scalars of the current phase call
+	R dt;
+	int wall;
+	int springs;
+	Scalars* sc;
+
+	// agent fields (include/pcm.h order); fptr/fbytes give the generic view
+	void* fptr[PCM_FIELD_COUNT];
+	int fbytes[PCM_FIELD_COUNT]; // bytes per agent
+	R *pos, *vel, *pvel, *radius, *nuc_radius, *surface, *volume, *fluid, *solid, *nuclear, *nuc_fluid, *nuc_solid,
+		*cyto, *cyto_fluid, *cyto_solid, *fluid_frac, *c2n, *target_cyto, *target_nuc;
+	R *adhesion, *repulsion, *bm_repulsion, *rel_adh, *affinity;
+	int* max_att;
+	R *elastic, *att_rate, *det_rate;
+	uint8_t* motile;
+	R *persistence, *speed, *bias_dir, *bias, *motility;
+	uint8_t* restrict2d;
+	uint8_t* dead;
+	R *dead_phago, *live_phago, *damage_rate, *attack, *immuno, *fusion;
+	R* pressure;
+	int* nuclei;
+	R *damage, *attack_time;
+	int* type;
+	uint8_t* movable;
+	uint8_t* flags;
+	R *internal, *frac_ingest;
+	int* origin;
+
+	// link lists
+	int *spr_cnt, *spr_idx, *atc_cnt, *atc_idx;
+	// neighbor lists
+	int *nbr_cnt, *nbr_idx;
+	// mechanics voxel grid
+	int *vcnt, *vstart, *vox_of, *vrank, *btmp, *bucket;
+	R* sorted; // 4 reals per bucket slot: x, y, z, rel*radius
+
+	// spring scratch
+	uint32_t *fire, *erase;  // per agent slot masks (L <= 32)
+	int* ev;                 // attachment events: 3 ints (actor, slot, target) each
+	// interaction scratch
+	int* cand;               // candidate (actor, slot) pairs
+	int *n_rem_t, *n_atk_t;  // removal / attack events targeting an agent
+	int* rem_cnt;            // tentative removal events of an actor (<= 4), -1 = none recorded
+	int* rem_ev;             // 4 per actor: slot | kind << 24
+	int* atk_tgt;            // attacked agent of an actor, -1 = none
+	uint8_t* acted;          // actor reached a lock at least once
+	// compaction scratch
+	int *bad, *remap, *seq, *move_src, *move_dst;
+};
+
+// -------------------------------------------------------------------------- helpers
+template <class R>
+__device__ __forceinline__ R real_max(R a, R b)
+{
+	return a < b ? b : a; // std::max(a, b)
+}
+
+template <class R>
+__device__ __forceinline__ R zero_threshold()
+{
+	return (R)1e-16;
+}
+
+template <class R>
+__device__ __forceinline__ R pressure_coefficient()
+{
+	return (R)36.64504274775163;
+}
+
+// position_helper<D>::distance (solver_helper.h:54-58, 127-131, 212-217)
+template <int D, class R>
+__device__ __forceinline__ R pair_distance(const R* a, const R* b)
+{
+	if constexpr (D == 1)
+		return fabs(a[0] - b[0]);
+	else if constexpr (D == 2)
+	{
+		const R s = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]);
+		return sqrt(s);
+	}
+	else
+	{
+		const R s = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+		return sqrt(s);
+	}
+}
+
+// voxel coordinate of a position: (index_t)((p - min) / voxel), clamped to the grid
+template <class R>
+__device__ __forceinline__ int voxel_index(const SimCtx<R>& c, const R* p)
+{
+	int co[3] = { 0, 0, 0 };
+	for (int k = 0; k < c.dims; k++)
+	{
+		const R q = (p[k] - (R)c.bmin[k]) / (R)c.vox[k];
+		int v = (int)q;
+		v = v < 0 ? 0 : (v >= c.grid[k] ? c.grid[k] - 1 : v);
+		co[k] = v;
+	}
+	return co[0] + c.grid[0] * (co[1] + c.grid[1] * co[2]);
+}
+
+template <class R>
+__device__ __forceinline__ void voxel_coords(const SimCtx<R>& c, int v, int co[3])
+{
+	co[0] = v % c.grid[0];
+	co[1] = (v / c.grid[0]) % c.grid[1];
+	co[2] = v / (c.grid[0] * c.grid[1]);
+}
+
+// solve_pair (position_solver.cpp:24-77) for one neighbor: adds the pair force to v and
+// returns the repulsion term that feeds simple_pressure.
+template <int D, class R>
+__device__ __forceinline__ R pair_force(const SimCtx<R>& c, int i, int j, const R* pi, R* v)
+{
+	const R* pj = c.pos + (size_t)j * D;
+	R diff[D];
+	R sq = 0;
+	for (int k = 0; k < D; k++)
+		diff[k] = pi[k] - pj[k];
+	R distance;
+	if constexpr (D == 1)
+		distance = fabs(diff[0]);
+	else if constexpr (D == 2)
+	{
+		sq = diff[0] * diff[0] + diff[1] * diff[1];
+		distance = sqrt(sq);
+	}
+	else
+	{
+		sq = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2];
+		distance = sqrt(sq);
+	}
+	distance = real_max<R>(distance, (R)0.00001);
+
+	R rep = 1 - distance / (c.radius[i] + c.radius[j]);
+	rep = rep < 0 ? (R)0 : rep;
+	rep *= rep;
+	const R pressure_term = rep;
+	rep *= sqrt(c.repulsion[i] * c.repulsion[j]);
+
+	const R reach = c.rel_adh[i] * c.radius[i] + c.rel_adh[j] * c.radius[j];
+	R adh = 1 - distance / reach;
+	adh *= adh;
+	const int ti = c.type[i], tj = c.type[j];
+	adh *= sqrt(c.adhesion[i] * c.adhesion[j] * c.affinity[(size_t)i * c.T + tj] * c.affinity[(size_t)j * c.T + ti]);
+
+	const R force = (rep - adh) / distance;
+	for (int k = 0; k < D; k++)
+		v[k] += force * diff[k];
+	return pressure_term;
+}
+
+// repulsion term of a pair only (the simple_pressure contribution, position_solver.cpp:41-51)
+template <int D, class R>
+__device__ __forceinline__ R pair_pressure_term(const SimCtx<R>& c, int i, int j, const R* pi)
+{
+	const R* pj = c.pos + (size_t)j * D;
+	R distance = pair_distance<D>(pi, pj);
+	distance = real_max<R>(distance, (R)0.00001);
+	R rep = 1 - distance / (c.radius[i] + c.radius[j]);
+	rep = rep < 0 ? (R)0 : rep;
+	return rep * rep;
+}
+
+// update_membrane_velocity (solver_helper.h:14-28)
+template <class R>
+__device__ __forceinline__ void membrane_push(R position, R wall, R sign, R radius, R strength, R& velocity)
+{
+	R distance = fabs(wall - position);
+	distance = real_max<R>(distance, (R)0.00001);
+	R rep = 1 - distance / radius;
+	rep = rep < 0 ? (R)0 : rep;
+	rep *= rep * strength * sign;
+	velocity += rep * distance;
+}
+
+// update_motility_single (position_solver.cpp:115-145) for one agent.  The 3D walk is
+// planar because the reference passes the restrict_to_2d array pointer as the bool.
+template <int D, class R>
+__device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsigned long long step, R* v)
+{
+	if (c.motile[i] == 0)
+		return;
+	R* mv = c.motility + (size_t)i * D;
+	R m[D];
+	for (int k = 0; k < D; k++)
+		m[k] = mv[k];
+	const pcm_u32x4 blk = pcm_block(c.seed, step, PCM_STREAM_MOTILITY, (uint32_t)i, 0u, 0u);
+	const R u0 = pcm_draw<R>::from(blk, 0);
+	if (u0 < c.dt / c.persistence[i])
+	{
+		R walk[D];
+		const R u1 = pcm_draw<R>::from(blk, 1);
+		if constexpr (D == 1)
+			walk[0] = u1 < (R)0.5 ? (R)-1 : (R)1;
+		else
+		{
+			const R two_pi = (R)2 * (R)3.14159265358979323846;
+			const R theta = pcm_scale_uniform<R>(u1, (R)0, two_pi);
+			walk[0] = cos(theta);
+			walk[1] = sin(theta);
+			if constexpr (D == 3)
+				walk[2] = 0;
+		}
+		const R b = c.bias[i];
+		const R* dir = c.bias_dir + (size_t)i * D;
+		for (int k = 0; k < D; k++)
+			m[k] = (1 - b) * walk[k] + b * dir[k];
+		R length;
+		if constexpr (D == 1)
+			length = fabs(m[0]);
+		else if constexpr (D == 2)
+			length = sqrt(m[0] * m[0] + m[1] * m[1]);
+		else
+			length = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+		const R speed = c.speed[i];
+		for (int k = 0; k < D; k++)
+		{
+			m[k] = length > zero_threshold<R>() ? m[k] * speed / length : (R)0;
+			mv[k] = m[k];
+		}
+	}
+	for (int k = 0; k < D; k++)
+		v[k] += m[k];
+}
+
+// position_helper<D>::update_membrane_velocities (solver_helper.h:103-110, 184-193, 288-299)
+template <int D, class R>
+__device__ __forceinline__ void membrane_agent(const SimCtx<R>& c, int i, const R* p, R* v)
+{
+	const R r = c.radius[i];
+	const R s = c.bm_repulsion[i];
+	for (int k = 0; k < D; k++)
+	{
+		const int upper = (D == 1) ? c.bmax[1] : c.bmax[k];
+		membrane_push<R>(p[k], (R)c.bmin[k], (R)1, r, s, v[k]);
+		membrane_push<R>(p[k], (R)upper, (R)-1, r, s, v[k]);
+	}
+}
+
+// spring_contract_function (position_solver.cpp:337-370) for one agent
+template <int D, class R>
+__device__ __forceinline__ void contract_agent(const SimCtx<R>& c, int i, const R* p, R* v)
+{
+	const int cnt = c.spr_cnt[i];
+	const int ti = c.type[i];
+	const R ki = c.elastic[i];
+	const R ai_row = 0;
+	(void)ai_row;
+	for (int s = 0; s < cnt; s++)
+	{
+		const int j = c.spr_idx[(size_t)i * c.L + s];
+		const int tj = c.type[j];
+		const R kij = ki * c.elastic[j] * c.affinity[(size_t)i * c.T + tj] * c.affinity[(size_t)j * c.T + ti];
+		const R adhesion = (R)sqrt((double)kij);
+		const R* pj = c.pos + (size_t)j * D;
+		for (int k = 0; k < D; k++)
+			v[k] += adhesion * (pj[k] - p[k]);
+	}
+}
+
+// update_geometry (interactions_solver.cpp:89-96): double arithmetic as in the reference
+template <class R>
+__device__ __forceinline__ void update_geometry(const SimCtx<R>& c, int i)
+{
+	const double pi_r = (double)(R)3.14159265358979323846;
+	const double sphere = pi_r * 4.0 / 3.0;
+	c.radius[i] = (R)cbrt((double)c.volume[i] / sphere);
+	c.nuc_radius[i] = (R)cbrt((double)c.nuclear[i] / sphere);
+	c.surface[i] = (R)(4.0 * pi_r * (double)c.radius[i] * (double)c.radius[i]);
+}
+
+// ingest (interactions_solver.cpp:8-27, 57-67, 98-112)
+template <class R>
+__device__ __forceinline__ void ingest(const SimCtx<R>& c, int lhs, int rhs)
+{
+	c.cyto_fluid[lhs] += c.fluid[rhs];
+	c.cyto_solid[lhs] += c.solid[rhs];
+	c.fluid[lhs] += c.fluid[rhs];
+	c.solid[lhs] += c.solid[rhs];
+	c.cyto[lhs] += c.fluid[rhs] + c.solid[rhs];
+	c.volume[lhs] += c.fluid[rhs] + c.solid[rhs];
+	c.fluid_frac[lhs] = c.fluid[lhs] / (c.volume[lhs] + zero_threshold<R>());
+	c.c2n[lhs] = c.cyto_solid[lhs] / (c.nuc_solid[lhs] + zero_threshold<R>());
+	for (int k = 0; k < c.S; k++)
+		c.internal[(size_t)lhs * c.S + k] += c.internal[(size_t)rhs * c.S + k] * c.frac_ingest[(size_t)rhs * c.S + k];
+	update_geometry<R>(c, lhs);
+	c.flags[rhs] = PCM_FLAG_TO_REMOVE;
+}
+
+// fuse (interactions_solver.cpp:29-55, 69-87, 114-134)
+template <class R>
+__device__ __forceinline__ void fuse(const SimCtx<R>& c, int lhs, int rhs)
+{
+	const int D = c.dims;
+	const R both = c.volume[lhs] + c.volume[rhs];
+	for (int k = 0; k < D; k++)
+		c.pos[(size_t)lhs * D + k] =
+			(c.pos[(size_t)lhs * D + k] * c.volume[lhs] + c.pos[(size_t)rhs * D + k] * c.volume[rhs]) / both;
+	c.cyto_fluid[lhs] += c.cyto_fluid[rhs];
+	c.nuc_fluid[lhs] += c.nuc_fluid[rhs];
+	c.cyto_solid[lhs] += c.cyto_solid[rhs];
+	c.nuc_solid[lhs] += c.nuc_solid[rhs];
+	c.target_cyto[lhs] += c.target_cyto[rhs];
+	c.target_nuc[lhs] += c.target_nuc[rhs];
+	c.fluid[lhs] = c.nuc_fluid[lhs] + c.cyto_fluid[lhs];
+	c.solid[lhs] = c.nuc_solid[lhs] + c.cyto_solid[lhs];
+	c.nuclear[lhs] = c.nuc_fluid[lhs] + c.nuc_solid[lhs];
+	c.cyto[lhs] = c.cyto_fluid[lhs] + c.cyto_solid[lhs];
+	c.volume[lhs] = c.nuclear[lhs] + c.cyto[lhs];
+	c.fluid_frac[lhs] = c.fluid[lhs] / (c.volume[lhs] + zero_threshold<R>());
+	c.c2n[lhs] = c.cyto_solid[lhs] / (c.nuc_solid[lhs] + zero_threshold<R>());
+	for (int k = 0; k < c.S; k++)
+		c.internal[(size_t)lhs * c.S + k] += c.internal[(size_t)rhs * c.S + k];
+	update_geometry<R>(c, lhs);
+	c.nuclei[lhs] += c.nuclei[rhs];
+	c.flags[rhs] = PCM_FLAG_TO_REMOVE;
+}
+
+} // namespace pcm

@@ -1,0 +1,1256 @@
+// pcm_kernels.h — gfx950 kernels of the mechanics step (included once, by pcm_lib.hip).
+//
+// Every per-agent kernel takes the simulation contexts in device memory and uses
+// blockIdx.y as the simulation index, so one launch can advance a batch of simulations;
+// blocks past a simulation's agent count return at once.  Every kernel returns when the
+// simulation's status word is non-zero (an aborted step).
+#pragma once
+
+#include "pcm_device.h"
+
+namespace pcm {
+
+constexpr int BLOCK = 256;
+constexpr int RESOLVE_THREADS = 1024;
+constexpr int ATT_CHUNK = 2048;     // attachment events resolved per LDS pass
+constexpr int CAND_CHUNK = 4096;    // interaction candidates ordered per LDS pass
+constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
+
+__device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+#define PCM_PROLOGUE(c)                                                                                                \
+	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
+	if (c.sc->status)                                                                                                  \
+		return;
+
+// ------------------------------------------------------------------ step bookkeeping
+template <class R>
+__global__ void k_begin_step(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	if (gid() == 0)
+		c.sc->step++;
+}
+
+// ------------------------------------------------------------------ counting sort into voxels
+// containers_solver::update_mechanics_mesh (containers_solver.cpp:15-56).  The reference's
+// single-thread scatter (fetch_sub from the voxel count) leaves every bucket in descending
+// agent order; the device reproduces that order exactly: histogram -> bucket allocation ->
+// unordered scatter -> rank by "number of bucket members with a larger index".
+
+template <class R>
+__global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i < c.sc->n_binned)
+		c.vcnt[c.vox_of[i]] = 0;
+	if (i == 0)
+		c.sc->cursor = 0;
+}
+
+template <class R, int D>
+__global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	const int n = c.sc->n;
+	if (i == 0)
+		c.sc->n_binned = n;
+	if (i >= n)
+		return;
+	const int v = voxel_index<R>(c, c.pos + (size_t)i * D);
+	c.vox_of[i] = v;
+	c.vrank[i] = atomicAdd(&c.vcnt[v], 1);
+}
+
+template <class R>
+__global__ void k_bin_claim(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || c.vrank[i] != 0)
+		return;
+	const int v = c.vox_of[i];
+	c.vstart[v] = atomicAdd(&c.sc->cursor, c.vcnt[v]);
+}
+
+template <class R>
+__global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	c.btmp[c.vstart[c.vox_of[i]] + c.vrank[i]] = i;
+}
+
+template <class R, int D>
+__global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	const int v = c.vox_of[i];
+	const int s = c.vstart[v], m = c.vcnt[v];
+	int rank = 0;
+	for (int k = s; k < s + m; k++)
+		rank += c.btmp[k] > i;
+	const int slot = s + rank;
+	c.bucket[slot] = i;
+	R* out = c.sorted + (size_t)slot * 4;
+	for (int k = 0; k < 3; k++)
+		out[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
+	out[3] = c.rel_adh[i] * c.radius[i];
+}
+
+// ------------------------------------------------------------------ neighbor lists
+// position_solver::update_cell_neighbors (position_solver.cpp:79-97, 180-195, 262-286)
+// with the 3^d stencil of common_solver::for_each_in_mech_neighborhood
+// (common_solver.h:77-105): z outer, y, x inner, each bucket in bucket order, self
+// skipped.  Lists are built for every agent; a non-movable agent's list only feeds
+// simple_pressure and is reported empty.
+template <class R, int D>
+__global__ void k_neighbors(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	R pi[3] = { 0, 0, 0 };
+	for (int k = 0; k < D; k++)
+		pi[k] = c.pos[(size_t)i * D + k];
+	const R mi = c.rel_adh[i] * c.radius[i];
+	int co[3];
+	voxel_coords<R>(c, c.vox_of[i], co);
+	int* out = c.nbr_idx + (size_t)i * c.K;
+	int cnt = 0;
+	for (int z = -1; z <= 1; z++)
+	{
+		if (co[2] + z >= c.grid[2] || co[2] + z < 0)
+			continue;
+		for (int y = -1; y <= 1; y++)
+		{
+			if (co[1] + y >= c.grid[1] || co[1] + y < 0)
+				continue;
+			for (int x = -1; x <= 1; x++)
+			{
+				if (co[0] + x >= c.grid[0] || co[0] + x < 0)
+					continue;
+				const int nv = (co[0] + x) + c.grid[0] * ((co[1] + y) + c.grid[1] * (co[2] + z));
+				const int m = c.vcnt[nv];
+				if (m == 0)
+					continue;
+				const int s = c.vstart[nv];
+				for (int k = s; k < s + m; k++)
+				{
+					const int j = c.bucket[k];
+					if (j == i)
+						continue;
+					const R* q = c.sorted + (size_t)k * 4;
+					const R dist = pair_distance<D, R>(pi, q);
+					if (dist <= mi + q[3])
+					{
+						if (cnt < c.K)
+							out[cnt] = j;
+						cnt++;
+					}
+				}
+			}
+		}
+	}
+	if (cnt > c.K)
+	{
+		atomicOr(&c.sc->status, ST_NEIGHBOR_OVERFLOW);
+		atomicMax(&c.sc->max_neighbors, cnt);
+		cnt = c.K;
+	}
+	c.nbr_cnt[i] = cnt;
+}
+
+// number of neighbors of i the reference would list (empty for non-movable agents)
+template <class R>
+__device__ __forceinline__ int visible_neighbors(const SimCtx<R>& c, int i)
+{
+	return c.movable[i] ? c.nbr_cnt[i] : 0;
+}
+
+// ------------------------------------------------------------------ velocity phases
+// update_cell_forces (position_solver.cpp:15-22, 99-113, 158-178, 233-260).  Velocity is
+// summed in neighbor-list order; simple_pressure is gathered: agent k receives the
+// repulsion term of every pair (k, j) once for each movable member of the pair, which is
+// what the reference's two-sided `simple_pressure[lhs/rhs] +=` accumulates.
+template <int D, class R>
+__device__ __forceinline__ R forces_agent(const SimCtx<R>& c, int i, const R* pi, R* v)
+{
+	const bool mov = c.movable[i] != 0;
+	const int cnt = c.nbr_cnt[i];
+	const int* nb = c.nbr_idx + (size_t)i * c.K;
+	R p = 0;
+	for (int s = 0; s < cnt; s++)
+	{
+		const int j = nb[s];
+		const R term = mov ? pair_force<D, R>(c, i, j, pi, v) : pair_pressure_term<D, R>(c, i, j, pi);
+		const R t = term * pressure_coefficient<R>();
+		if (mov)
+			p += t;
+		if (c.movable[j])
+			p += t;
+	}
+	return p;
+}
+
+template <class R, int D>
+__global__ void k_forces(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	R pi[D], v[D];
+	for (int k = 0; k < D; k++)
+	{
+		pi[k] = c.pos[(size_t)i * D + k];
+		v[k] = c.vel[(size_t)i * D + k];
+	}
+	c.pressure[i] = forces_agent<D, R>(c, i, pi, v);
+	if (c.movable[i])
+		for (int k = 0; k < D; k++)
+			c.vel[(size_t)i * D + k] = v[k];
+}
+
+template <class R, int D>
+__global__ void k_motility(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || c.motile[i] == 0)
+		return;
+	R v[D];
+	for (int k = 0; k < D; k++)
+		v[k] = c.vel[(size_t)i * D + k];
+	motility_agent<D, R>(c, i, c.sc->step, v);
+	for (int k = 0; k < D; k++)
+		c.vel[(size_t)i * D + k] = v[k];
+}
+
+template <class R, int D>
+__global__ void k_membrane(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || !c.movable[i])
+		return;
+	R p[D], v[D];
+	for (int k = 0; k < D; k++)
+	{
+		p[k] = c.pos[(size_t)i * D + k];
+		v[k] = c.vel[(size_t)i * D + k];
+	}
+	membrane_agent<D, R>(c, i, p, v);
+	for (int k = 0; k < D; k++)
+		c.vel[(size_t)i * D + k] = v[k];
+}
+
+template <class R, int D>
+__global__ void k_contract(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || !c.movable[i])
+		return;
+	R p[D], v[D];
+	for (int k = 0; k < D; k++)
+	{
+		p[k] = c.pos[(size_t)i * D + k];
+		v[k] = c.vel[(size_t)i * D + k];
+	}
+	contract_agent<D, R>(c, i, p, v);
+	for (int k = 0; k < D; k++)
+		c.vel[(size_t)i * D + k] = v[k];
+}
+
+// update_positions_internal (position_solver.cpp:481-504): Adams-Bashforth 2
+template <class R, int D>
+__global__ void k_positions(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || !c.movable[i])
+		return;
+	const R factor = (R)((double)c.dt * 1.5);
+	const R previous_factor = (R)((double)c.dt * -0.5);
+	for (int k = 0; k < D; k++)
+	{
+		const size_t o = (size_t)i * D + k;
+		const R v = c.vel[o];
+		c.pos[o] += v * factor + c.pvel[o] * previous_factor;
+		c.pvel[o] = v;
+		c.vel[o] = 0;
+	}
+}
+
+// Forces, motility, basement membrane and spring contraction of one agent in the
+// reference's summation order (simulator.cpp:173-176), written to the velocity.
+template <class R, int D>
+__global__ void k_velocity_fused(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	const bool mov = c.movable[i] != 0;
+	R p[D], v[D];
+	for (int k = 0; k < D; k++)
+	{
+		p[k] = c.pos[(size_t)i * D + k];
+		v[k] = c.vel[(size_t)i * D + k];
+	}
+	c.pressure[i] = forces_agent<D, R>(c, i, p, v);
+	motility_agent<D, R>(c, i, c.sc->step, v);
+	if (mov && c.wall)
+		membrane_agent<D, R>(c, i, p, v);
+	if (mov && c.springs)
+		contract_agent<D, R>(c, i, p, v);
+	for (int k = 0; k < D; k++)
+		c.vel[(size_t)i * D + k] = v[k];
+}
+
+// ------------------------------------------------------------------ springs
+// update_spring_attachments_internal (position_solver.cpp:374-449).
+//
+// Detachment: one draw per (agent, slot).  The single-thread reference visits agents in
+// ascending order; a fired slot of agent a erases its own entry and the first remaining
+// entry of a in the partner's list.  For a slot s of a holding partner p that gives:
+//   a < p: erased iff s fired, or s is among the first g non-fired entries of p in a's
+//          list, where g counts p's fired entries of a after the first f_a (f_a = fired
+//          entries of p in a's list);
+//   a > p: erased iff s fired, or s is among the first f_p entries of p in a's list
+//          (f_p = fired entries of a in p's list).
+template <class R>
+__global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	const int cnt = c.spr_cnt[i];
+	const R rate = c.det_rate[i] * c.dt;
+	const unsigned long long step = c.sc->step;
+	uint32_t mask = 0;
+	for (int s = 0; s < cnt; s++)
+	{
+		const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_DETACH, (uint32_t)i, (uint32_t)s, 0u);
+		if (u <= rate)
+			mask |= 1u << s;
+	}
+	c.fire[i] = mask;
+}
+
+template <class R>
+__global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	const int n = c.sc->n;
+	if (i >= n)
+		return;
+	const int cnt = c.spr_cnt[i];
+	const int* li = c.spr_idx + (size_t)i * c.L;
+	const uint32_t fi = c.fire[i];
+	uint32_t erased = 0;
+	for (int s = 0; s < cnt; s++)
+	{
+		const int p = li[s];
+		if (p < 0 || p >= n)
+			continue;
+		const bool fired = (fi >> s) & 1u;
+		if (fired)
+		{
+			erased |= 1u << s;
+			continue;
+		}
+		const int* lp = c.spr_idx + (size_t)p * c.L;
+		const int cp = c.spr_cnt[p];
+		const uint32_t fp = c.fire[p];
+		if (i < p)
+		{
+			int f_i = 0, nf_rank = 0;
+			for (int t = 0; t < cnt; t++)
+				if (li[t] == p)
+				{
+					const bool ft = (fi >> t) & 1u;
+					f_i += ft;
+					if (!ft && t < s)
+						nf_rank++;
+				}
+			int g = 0, occ = 0;
+			for (int t = 0; t < cp; t++)
+				if (lp[t] == i)
+				{
+					occ++;
+					if (occ > f_i && ((fp >> t) & 1u))
+						g++;
+				}
+			if (nf_rank < g)
+				erased |= 1u << s;
+		}
+		else
+		{
+			int rank = 0;
+			for (int t = 0; t < s; t++)
+				rank += li[t] == p;
+			int f_p = 0;
+			for (int t = 0; t < cp; t++)
+				f_p += lp[t] == i && ((fp >> t) & 1u);
+			if (rank < f_p)
+				erased |= 1u << s;
+		}
+	}
+	c.erase[i] = erased;
+}
+
+template <class R>
+__global__ void k_spring_compact(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	const uint32_t e = c.erase[i];
+	if (e == 0)
+		return;
+	int* li = c.spr_idx + (size_t)i * c.L;
+	const int cnt = c.spr_cnt[i];
+	int w = 0;
+	for (int s = 0; s < cnt; s++)
+		if (!((e >> s) & 1u))
+			li[w++] = li[s];
+	c.spr_cnt[i] = w;
+}
+
+// Attachment candidates: every (i, slot) with j = neighbor > i whose draws succeed.
+template <class R>
+__global__ void k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	const int cnt = visible_neighbors<R>(c, i);
+	if (cnt == 0)
+		return;
+	const int* nb = c.nbr_idx + (size_t)i * c.K;
+	const int ti = c.type[i];
+	const R rate_i = c.att_rate[i] * c.dt;
+	const unsigned long long step = c.sc->step;
+	for (int s = 0; s < cnt; s++)
+	{
+		const int j = nb[s];
+		if (j < i)
+			continue;
+		const R p_l = rate_i * c.affinity[(size_t)i * c.T + c.type[j]];
+		const R p_r = c.att_rate[j] * c.dt * c.affinity[(size_t)j * c.T + ti];
+		const pcm_u32x4 b = pcm_block(c.seed, step, PCM_STREAM_ATTACH, (uint32_t)i, (uint32_t)s, 0u);
+		if (pcm_draw<R>::from(b, 0) <= p_l || pcm_draw<R>::from(b, 1) <= p_r)
+		{
+			const int e = atomicAdd(&c.sc->att_events, 1);
+			if (e < c.ev_cap)
+			{
+				c.ev[3 * e] = i;
+				c.ev[3 * e + 1] = s;
+				c.ev[3 * e + 2] = j;
+			}
+			else
+				atomicOr(&c.sc->status, ST_CAPACITY);
+		}
+	}
+}
+
+// Block-wide bitonic sort of `count` 64-bit keys (with 32-bit payloads) in shared memory.
+__device__ inline void block_sort_pairs(unsigned long long* key, int* val, int count)
+{
+	int P = 1;
+	while (P < count)
+		P <<= 1;
+	for (int t = count + threadIdx.x; t < P; t += blockDim.x)
+	{
+		key[t] = ~0ull;
+		val[t] = -1;
+	}
+	__syncthreads();
+	for (int k = 2; k <= P; k <<= 1)
+		for (int j = k >> 1; j > 0; j >>= 1)
+		{
+			for (int t = threadIdx.x; t < P; t += blockDim.x)
+			{
+				const int u = t ^ j;
+				if (u > t)
+				{
+					const bool up = (t & k) == 0;
+					const unsigned long long a = key[t], b = key[u];
+					if ((a > b) == up)
+					{
+						key[t] = b;
+						key[u] = a;
+						const int tv = val[t];
+						val[t] = val[u];
+						val[u] = tv;
+					}
+				}
+			}
+			__syncthreads();
+		}
+}
+
+// Block-wide count of records (stride ints apart, actor first) whose actor lies in [lo, hi).
+__device__ inline int block_count_range(const int* rec, int stride, int count, int lo, int hi, int* red)
+{
+	int local = 0;
+	for (int e = threadIdx.x; e < count; e += blockDim.x)
+	{
+		const int a = rec[stride * e];
+		local += a >= lo && a < hi;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+		*red = 0;
+	__syncthreads();
+	atomicAdd(red, local);
+	__syncthreads();
+	const int r = *red;
+	__syncthreads();
+	return r;
+}
+
+// Largest hi in (lo, n] such that at most `chunk` records have their actor in [lo, hi).
+__device__ inline int block_chunk_end(const int* rec, int stride, int count, int lo, int n, int chunk, int* red)
+{
+	if (block_count_range(rec, stride, count, lo, n, red) <= chunk)
+		return n;
+	int a = lo + 1, b = n;
+	while (a < b)
+	{
+		const int mid = a + (b - a + 1) / 2;
+		if (block_count_range(rec, stride, count, lo, mid, red) <= chunk)
+			a = mid;
+		else
+			b = mid - 1;
+	}
+	return a;
+}
+
+// Attachment resolution in the reference's order: event (i, j) succeeds iff, when it is
+// reached in (i ascending, neighbor order), both endpoints hold fewer springs than their
+// maximum.  Events are taken in actor ranges of at most ATT_CHUNK; within a range, each
+// event's endpoint counts are bounded below (successes so far) and above (successes plus
+// undecided) by the earlier events touching the endpoint, and the bounds are iterated until
+// every event is decided — the earliest undecided event is always decidable, so the loop
+// ends.  Successful springs are appended to both lists in event order.
+template <class R>
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs)
+{
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	__shared__ unsigned long long ekey[ATT_CHUNK];
+	__shared__ int etgt[ATT_CHUNK];
+	__shared__ unsigned long long nkey[2 * ATT_CHUNK];
+	__shared__ int npos[2 * ATT_CHUNK];
+	__shared__ uint8_t est[ATT_CHUNK];
+	__shared__ int s_count, s_unknown, s_red;
+
+	if (c.sc->status)
+		return;
+	const int E = min(c.sc->att_events, c.ev_cap);
+	const int n = c.sc->n;
+	volatile int* spr_cnt = c.spr_cnt;
+	int lo = 0;
+	while (E > 0 && lo < n)
+	{
+		// choose the actor range [lo, hi); one actor never has more than K < ATT_CHUNK events
+		const int hi = block_chunk_end(c.ev, 3, E, lo, n, ATT_CHUNK, &s_red);
+		// gather the range's events
+		if (threadIdx.x == 0)
+			s_count = 0;
+		__syncthreads();
+		for (int e = threadIdx.x; e < E; e += blockDim.x)
+		{
+			const int a = c.ev[3 * e];
+			if (a >= lo && a < hi)
+			{
+				const int k = atomicAdd(&s_count, 1);
+				if (k < ATT_CHUNK)
+				{
+					ekey[k] = ((unsigned long long)a << 32) | (unsigned)c.ev[3 * e + 1];
+					etgt[k] = c.ev[3 * e + 2];
+				}
+			}
+		}
+		__syncthreads();
+		const int m = min(s_count, ATT_CHUNK);
+		if (s_count > ATT_CHUNK)
+		{
+			if (threadIdx.x == 0)
+				atomicOr(&c.sc->status, ST_CAPACITY);
+			return;
+		}
+		block_sort_pairs(ekey, etgt, m);
+		// endpoint entries keyed by (agent, event rank)
+		for (int r = threadIdx.x; r < m; r += blockDim.x)
+		{
+			const int actor = (int)(ekey[r] >> 32);
+			nkey[2 * r] = ((unsigned long long)actor << 32) | (unsigned)r;
+			nkey[2 * r + 1] = ((unsigned long long)etgt[r] << 32) | (unsigned)r;
+			npos[2 * r] = 2 * r;
+			npos[2 * r + 1] = 2 * r + 1;
+			est[r] = 0; // 0 undecided, 1 success, 2 failure
+		}
+		__syncthreads();
+		block_sort_pairs(nkey, npos, 2 * m);
+		// npos[q] now holds the (2 * rank + side) tag of sorted entry q
+		// rounds: each decides at least the earliest undecided event, so m + 1 rounds suffice
+		for (int round = 0;; round++)
+		{
+			if (round > m + 1)
+			{
+				if (threadIdx.x == 0)
+					atomicOr(&c.sc->status, ST_CAPACITY);
+				return;
+			}
+			if (threadIdx.x == 0)
+				s_unknown = 0;
+			__syncthreads();
+			for (int q = threadIdx.x; q < 2 * m; q += blockDim.x)
+			{
+				// each entry evaluates its endpoint; decisions are taken by side-0 entries of
+				// the pair after both sides are known: entries of side 0 drive the event
+				const int tag = npos[q];
+				const int r = tag >> 1;
+				if ((tag & 1) != 0 || est[r] != 0)
+					continue;
+				// locate the partner entry (side 1) by scanning the target's segment
+				int cond[2];
+				for (int side = 0; side < 2; side++)
+				{
+					int qq = q;
+					if (side == 1)
+					{
+						const unsigned long long want = ((unsigned long long)etgt[r] << 32) | (unsigned)r;
+						// binary search in the sorted entry keys
+						int a = 0, b = 2 * m - 1;
+						while (a < b)
+						{
+							const int mid = (a + b) >> 1;
+							if (nkey[mid] < want)
+								a = mid + 1;
+							else
+								b = mid;
+						}
+						qq = a;
+					}
+					const int agent = (int)(nkey[qq] >> 32);
+					int lo_c = spr_cnt[agent], hi_c = lo_c;
+					const int maxa = c.max_att[agent];
+					for (int w = qq - 1; w >= 0 && (int)(nkey[w] >> 32) == agent; w--)
+					{
+						const int rs = est[(int)(nkey[w] & 0xffffffffu)];
+						lo_c += rs == 1;
+						hi_c += rs != 2;
+					}
+					cond[side] = lo_c >= maxa ? 2 : (hi_c < maxa ? 1 : 0);
+				}
+				if (cond[0] == 2 || cond[1] == 2)
+					est[r] = 2;
+				else if (cond[0] == 1 && cond[1] == 1)
+					est[r] = 1;
+				else
+					atomicAdd(&s_unknown, 1);
+			}
+			__syncthreads();
+			const int u = s_unknown;
+			__syncthreads();
+			if (u == 0)
+				break;
+		}
+		// append successes per agent segment, in event order
+		for (int q = threadIdx.x; q < 2 * m; q += blockDim.x)
+		{
+			const int agent = (int)(nkey[q] >> 32);
+			if (q > 0 && (int)(nkey[q - 1] >> 32) == agent)
+				continue; // not a segment head
+			int cnt = spr_cnt[agent];
+			for (int w = q; w < 2 * m && (int)(nkey[w] >> 32) == agent; w++)
+			{
+				const int r = (int)(nkey[w] & 0xffffffffu);
+				if (est[r] != 1)
+					continue;
+				const int actor = (int)(ekey[r] >> 32);
+				const int other = actor == agent ? etgt[r] : actor;
+				if (cnt < c.L)
+					c.spr_idx[(size_t)agent * c.L + cnt] = other;
+				else
+					atomicOr(&c.sc->status, ST_LIST_OVERFLOW);
+				cnt++;
+			}
+			spr_cnt[agent] = min(cnt, c.L);
+		}
+		__threadfence();
+		__syncthreads();
+		lo = hi;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+		c.sc->att_events = 0;
+}
+
+// ------------------------------------------------------------------ cell-cell interactions
+// update_cell_cell_interactions_internal (interactions_solver.cpp:143-253).
+//
+// k_interact_tentative: every live actor walks its neighbor list, draws one number per
+// neighbor, records the slots where any branch can fire (candidates) and plays the
+// reference's branch logic against the flags as they stand before the phase.  The
+// outcome equals the reference's order-dependent result unless agents interact through
+// more than one event (an agent removed and also acting, attacked or removed twice);
+// k_interact_check detects that, and then k_interact_serial replays all candidates in
+// the reference's order on one thread instead of k_interact_apply.
+enum : int
+{
+	EV_INGEST = 0,
+	EV_FUSE = 2
+};
+
+template <class R>
+__global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	c.atk_tgt[i] = -1;
+	int n_rem = 0;
+	bool acted = false;
+	if (c.dead[i] != 1)
+	{
+		const int cnt = visible_neighbors<R>(c, i);
+		const int* nb = c.nbr_idx + (size_t)i * c.K;
+		const int ti = c.type[i];
+		const R dt = c.dt;
+		const unsigned long long step = c.sc->step;
+		const bool self_removed = c.flags[i] == PCM_FLAG_TO_REMOVE;
+		bool attacked_once = false, phagocytosed_once = false, fused_once = false;
+		for (int s = 0; s < cnt; s++)
+		{
+			const int j = nb[s];
+			const int tj = c.type[j];
+			const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
+			const bool dead_j = c.dead[j] == 1;
+			const R p_live = c.live_phago[(size_t)i * c.T + tj] * dt;
+			const R p_attack = c.attack[(size_t)i * c.T + tj] * c.immuno[(size_t)j * c.T + ti] * dt;
+			const R p_fuse = c.fusion[(size_t)i * c.T + tj] * dt;
+			const bool candidate =
+				dead_j ? (u < c.dead_phago[i] * dt) : (u < p_live || u < p_attack || u < p_fuse);
+			if (!candidate)
+				continue;
+			const int k = atomicAdd(&c.sc->cand_count, 1);
+			if (k < c.cand_cap)
+			{
+				c.cand[2 * k] = i;
+				c.cand[2 * k + 1] = s;
+			}
+			else
+				atomicOr(&c.sc->status, ST_CAPACITY);
+
+			const bool target_removed = c.flags[j] == PCM_FLAG_TO_REMOVE;
+			int kind = -1; // -1 none, 0 ingest, 1 attack, 2 fuse
+			if (dead_j)
+				kind = 0;
+			else if (!phagocytosed_once && u < p_live)
+				kind = 0;
+			else if (!attacked_once && u < p_attack)
+				kind = 1;
+			else if (!fused_once && u < p_fuse)
+				kind = 2;
+			if (kind < 0)
+				continue;
+			acted = true;
+			if (self_removed)
+				break;
+			if (target_removed)
+				continue;
+			if (kind == 1)
+			{
+				c.atk_tgt[i] = j;
+				atomicAdd(&c.n_atk_t[j], 1);
+				attacked_once = true;
+				continue;
+			}
+			if (!dead_j)
+			{
+				if (kind == 0)
+					phagocytosed_once = true;
+				else
+					fused_once = true;
+			}
+			atomicAdd(&c.n_rem_t[j], 1);
+			if (n_rem < 4)
+				c.rem_ev[4 * i + n_rem] = s | ((kind == 2 ? EV_FUSE : EV_INGEST) << 24);
+			else
+				c.sc->conflict = 1;
+			n_rem++;
+		}
+	}
+	c.rem_cnt[i] = n_rem;
+	c.acted[i] = acted;
+}
+
+template <class R>
+__global__ void k_interact_check(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	const int r = c.n_rem_t[i];
+	if (r >= 1 && (r >= 2 || c.n_atk_t[i] >= 1 || c.acted[i]))
+		c.sc->conflict = 1;
+}
+
+template <class R>
+__global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || c.sc->conflict)
+		return;
+	const int nr = c.rem_cnt[i];
+	for (int k = 0; k < nr; k++)
+	{
+		const int e = c.rem_ev[4 * i + k];
+		const int j = c.nbr_idx[(size_t)i * c.K + (e & 0xFFFFFF)];
+		if ((e >> 24) == EV_FUSE)
+			fuse<R>(c, i, j);
+		else
+			ingest<R>(c, i, j);
+	}
+	const int na = c.n_atk_t[i];
+	if (na > 0)
+	{
+		// attackers, ascending: the reference applies attacks in actor order
+		const int cnt = c.nbr_cnt[i];
+		const int* nb = c.nbr_idx + (size_t)i * c.K;
+		int last = -1;
+		for (int found = 0; found < na; found++)
+		{
+			int best = 0x7fffffff;
+			for (int s = 0; s < cnt; s++)
+			{
+				const int k = nb[s];
+				if (k > last && k < best && c.atk_tgt[k] == i)
+					best = k;
+			}
+			if (best == 0x7fffffff)
+				break;
+			c.damage[i] += c.damage_rate[best] * c.dt;
+			c.attack_time[i] += c.dt;
+			last = best;
+		}
+	}
+	c.n_rem_t[i] = 0;
+	c.n_atk_t[i] = 0;
+}
+
+// Exact replay of the reference loop over the candidates, in (actor, slot) order, when the
+// speculative pass found a dependency.  Candidates are ordered CAND_CHUNK at a time.
+template <class R>
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCtx<R>* __restrict__ ctxs)
+{
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	__shared__ unsigned long long key[CAND_CHUNK];
+	__shared__ int val[CAND_CHUNK];
+	__shared__ int s_count, s_red;
+	if (c.sc->status)
+		return;
+	const int C = min(c.sc->cand_count, c.cand_cap);
+	if (c.sc->conflict && C > 0)
+	{
+		const int n = c.sc->n;
+		const R dt = c.dt;
+		const unsigned long long step = c.sc->step;
+		// per-actor state survives chunk boundaries
+		int cur = -1;
+		bool broken = false, attacked_once = false, phagocytosed_once = false, fused_once = false;
+		int lo = 0;
+		while (lo < n)
+		{
+			const int hi = block_chunk_end(c.cand, 2, C, lo, n, CAND_CHUNK, &s_red);
+			if (threadIdx.x == 0)
+				s_count = 0;
+			__syncthreads();
+			for (int e = threadIdx.x; e < C; e += blockDim.x)
+			{
+				const int ac = c.cand[2 * e];
+				if (ac >= lo && ac < hi)
+				{
+					const int k = atomicAdd(&s_count, 1);
+					if (k < CAND_CHUNK)
+					{
+						key[k] = ((unsigned long long)ac << 32) | (unsigned)c.cand[2 * e + 1];
+						val[k] = e;
+					}
+				}
+			}
+			__syncthreads();
+			const int m = min(s_count, CAND_CHUNK);
+			block_sort_pairs(key, val, m);
+			if (threadIdx.x == 0)
+			{
+				for (int q = 0; q < m; q++)
+				{
+					const int i = (int)(key[q] >> 32);
+					const int s = (int)(key[q] & 0xffffffffu);
+					if (i != cur)
+					{
+						cur = i;
+						broken = attacked_once = phagocytosed_once = fused_once = false;
+					}
+					if (broken)
+						continue;
+					const int j = c.nbr_idx[(size_t)i * c.K + s];
+					const int ti = c.type[i], tj = c.type[j];
+					const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
+					if (c.dead[j] == 1)
+					{
+						if (u < c.dead_phago[i] * dt)
+						{
+							if (c.flags[i] == PCM_FLAG_TO_REMOVE)
+							{
+								broken = true;
+								continue;
+							}
+							if (c.flags[j] == PCM_FLAG_TO_REMOVE)
+								continue;
+							ingest<R>(c, i, j);
+						}
+						continue;
+					}
+					if (!phagocytosed_once && u < c.live_phago[(size_t)i * c.T + tj] * dt)
+					{
+						if (c.flags[i] == PCM_FLAG_TO_REMOVE)
+						{
+							broken = true;
+							continue;
+						}
+						if (c.flags[j] == PCM_FLAG_TO_REMOVE)
+							continue;
+						ingest<R>(c, i, j);
+						phagocytosed_once = true;
+						continue;
+					}
+					const R p_attack = c.attack[(size_t)i * c.T + tj] * c.immuno[(size_t)j * c.T + ti] * dt;
+					if (!attacked_once && u < p_attack)
+					{
+						if (c.flags[i] == PCM_FLAG_TO_REMOVE)
+						{
+							broken = true;
+							continue;
+						}
+						if (c.flags[j] == PCM_FLAG_TO_REMOVE)
+							continue;
+						c.damage[j] += c.damage_rate[i] * dt;
+						c.attack_time[j] += dt;
+						attacked_once = true;
+						continue;
+					}
+					if (!fused_once && u < c.fusion[(size_t)i * c.T + tj] * dt)
+					{
+						if (c.flags[i] == PCM_FLAG_TO_REMOVE)
+						{
+							broken = true;
+							continue;
+						}
+						if (c.flags[j] == PCM_FLAG_TO_REMOVE)
+							continue;
+						fuse<R>(c, i, j);
+						fused_once = true;
+						continue;
+					}
+				}
+			}
+			__threadfence();
+			__syncthreads();
+			lo = hi;
+		}
+		// clear the per-target counters the tentative pass raised
+		for (int e = threadIdx.x; e < C; e += blockDim.x)
+		{
+			const int i = c.cand[2 * e];
+			const int j = c.nbr_idx[(size_t)i * c.K + c.cand[2 * e + 1]];
+			c.n_rem_t[j] = 0;
+			c.n_atk_t[j] = 0;
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		c.sc->conflict = 0;
+		c.sc->cand_count = 0;
+	}
+}
+
+// ------------------------------------------------------------------ container compaction
+// update_cell_container_for_mechanics (containers_solver.cpp:58-147).  In the reference's
+// single-thread order removal i -> (last agent moves into i) leaves: kept agents below
+// N' = N - removed stay; holes below N' (ascending) receive the kept agents of [N', N) in
+// descending order.  Link lists of survivors are edited as the reference's sequence of
+// remove_attached (swap with back) and rename_attached calls would edit them.
+template <class R, int D>
+__global__ void k_compact_flag(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	bool bad = c.flags[i] == PCM_FLAG_TO_REMOVE;
+	for (int k = 0; k < D; k++)
+	{
+		const R p = c.pos[(size_t)i * D + k];
+		if (p < (R)c.bmin[k] || p > (R)c.bmax[k])
+			bad = true;
+	}
+	if (bad)
+	{
+		const int k = atomicAdd(&c.sc->bad_count, 1);
+		c.bad[k] = i;
+	}
+}
+
+__device__ inline void block_sort_ints(int* a, int count)
+{
+	int P = 1;
+	while (P < count)
+		P <<= 1;
+	for (int t = count + threadIdx.x; t < P; t += blockDim.x)
+		a[t] = 0x7fffffff;
+	__syncthreads();
+	for (int k = 2; k <= P; k <<= 1)
+		for (int j = k >> 1; j > 0; j >>= 1)
+		{
+			for (int t = threadIdx.x; t < P; t += blockDim.x)
+			{
+				const int u = t ^ j;
+				if (u > t)
+				{
+					const bool up = (t & k) == 0;
+					const int x = a[t], y = a[u];
+					if ((x > y) == up)
+					{
+						a[t] = y;
+						a[u] = x;
+					}
+				}
+			}
+			__syncthreads();
+		}
+}
+
+template <class R>
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R>* __restrict__ ctxs)
+{
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	__shared__ int sb[BAD_CHUNK];
+	if (c.sc->status)
+		return;
+	const int b = c.sc->bad_count;
+	if (b == 0)
+		return;
+	// ascending list of removed agents: in LDS when it fits, else sorted in place in HBM
+	int* list = sb;
+	if (b <= BAD_CHUNK)
+	{
+		for (int t = threadIdx.x; t < b; t += blockDim.x)
+			sb[t] = c.bad[t];
+		__syncthreads();
+		block_sort_ints(sb, b);
+	}
+	else
+	{
+		// the removal list buffer holds cap + padding entries
+		block_sort_ints(c.bad, b);
+		list = c.bad;
+	}
+	__syncthreads();
+	if (threadIdx.x != 0)
+		return;
+	const int n0 = c.sc->n;
+	const int n1 = n0 - b;
+	long long deaths = 0;
+	for (int t = 0; t < b; t++)
+		deaths += c.flags[list[t]] == PCM_FLAG_TO_REMOVE;
+	int holes = 0;
+	while (holes < b && list[holes] < n1)
+		holes++;
+	// walk the tail [n1, n0) downwards: good agents fill holes, bad ones get their
+	// position in the removal sequence
+	int q = 0;
+	int tb = b - 1; // index of the largest not-yet-visited bad tail agent
+	int top = n0;   // agents >= top have been consumed
+	int moves = 0;
+	for (int h = 0; h < holes; h++)
+	{
+		const int hole = list[h];
+		c.remap[hole] = -1 - q++;
+		// consume tail agents from top-1 down until a good one fills the hole
+		for (;;)
+		{
+			const int t = --top;
+			if (t < n1)
+			{
+				atomicOr(&c.sc->status, ST_CAPACITY); // inconsistent removal list
+				return;
+			}
+			if (tb >= holes && list[tb] == t)
+			{
+				c.remap[t] = -1 - q++;
+				tb--;
+				continue;
+			}
+			c.move_src[moves] = t;
+			c.move_dst[moves] = hole;
+			c.remap[t] = hole;
+			moves++;
+			break;
+		}
+	}
+	// remaining bad agents in [n1, top): n1 first, then top-1 down to n1+1
+	if (top > n1)
+	{
+		c.remap[n1] = -1 - q++;
+		for (int t = top - 1; t > n1; t--)
+			c.remap[t] = -1 - q++;
+	}
+	c.sc->moves = moves;
+	c.sc->n = n1;
+	c.sc->deaths += deaths;
+}
+
+// move every field (and the list rows) of each surviving tail agent into its hole
+template <class R>
+__global__ void k_compact_move(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int k = gid();
+	if (k >= c.sc->moves)
+		return;
+	const int src = c.move_src[k], dst = c.move_dst[k];
+	for (int f = 0; f < PCM_FIELD_COUNT; f++)
+	{
+		const int nb = c.fbytes[f];
+		const uint8_t* s = (const uint8_t*)c.fptr[f] + (size_t)src * nb;
+		uint8_t* d = (uint8_t*)c.fptr[f] + (size_t)dst * nb;
+		for (int b = 0; b < nb; b++)
+			d[b] = s[b];
+	}
+	c.spr_cnt[dst] = c.spr_cnt[src];
+	for (int s = 0; s < c.L; s++)
+		c.spr_idx[(size_t)dst * c.L + s] = c.spr_idx[(size_t)src * c.L + s];
+	c.atc_cnt[dst] = c.atc_cnt[src];
+	for (int s = 0; s < c.LA; s++)
+		c.atc_idx[(size_t)dst * c.LA + s] = c.atc_idx[(size_t)src * c.LA + s];
+	c.nbr_cnt[dst] = c.nbr_cnt[src];
+	for (int s = 0; s < c.K; s++)
+		c.nbr_idx[(size_t)dst * c.K + s] = c.nbr_idx[(size_t)src * c.K + s];
+}
+
+// remove_attached (swap with back) for removed partners in removal order, then
+// rename_attached for moved partners, on one survivor's list
+__device__ inline void edit_links(int* list, int* cnt_ptr, const int* remap)
+{
+	int cnt = *cnt_ptr;
+	if (cnt == 0)
+		return;
+	bool touched = false;
+	for (int s = 0; s < cnt; s++)
+		if (remap[list[s]] != list[s])
+		{
+			touched = true;
+			break;
+		}
+	if (!touched)
+		return;
+	for (;;)
+	{
+		// next removal op touching this list: smallest sequence number among removed entries
+		int best_seq = 0x7fffffff, best_id = -1;
+		for (int s = 0; s < cnt; s++)
+		{
+			const int r = remap[list[s]];
+			if (r < 0 && -1 - r < best_seq)
+			{
+				best_seq = -1 - r;
+				best_id = list[s];
+			}
+		}
+		if (best_id < 0)
+			break;
+		// the removed agent strips every entry of itself, each time the first one
+		for (;;)
+		{
+			int pos = -1;
+			for (int s = 0; s < cnt; s++)
+				if (list[s] == best_id)
+				{
+					pos = s;
+					break;
+				}
+			if (pos < 0)
+				break;
+			list[pos] = list[cnt - 1];
+			cnt--;
+		}
+	}
+	for (int s = 0; s < cnt; s++)
+		list[s] = remap[list[s]];
+	*cnt_ptr = cnt;
+}
+
+template <class R>
+__global__ void k_compact_links(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (c.sc->bad_count == 0 || i >= c.sc->n)
+		return;
+	edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
+	edit_links(c.atc_idx + (size_t)i * c.LA, c.atc_cnt + i, c.remap);
+}
+
+template <class R>
+__global__ void k_compact_restore(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int k = gid();
+	const int b = c.sc->bad_count;
+	if (k < b)
+	{
+		const int x = c.bad[k];
+		c.remap[x] = x;
+	}
+	if (k < c.sc->moves)
+	{
+		const int x = c.move_src[k];
+		c.remap[x] = x;
+	}
+}
+
+template <class R>
+__global__ void k_compact_done(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	if (gid() == 0)
+	{
+		c.sc->bad_count = 0;
+		c.sc->moves = 0;
+	}
+}
+
+} // namespace pcm

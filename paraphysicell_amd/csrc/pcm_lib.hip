@@ -1,0 +1,1019 @@
+// pcm_lib.hip — libpcm.so: the C-ABI of include/pcm.h over device-resident state.
+//
+// One pcm_sim owns one simulation: its agent SoA in HBM, list storage, the mechanics
+// voxel grid, scratch buffers and a HIP stream of default priority.  Phase entry points
+// enqueue kernels and return; pcm_step enqueues whole mechanics steps back to back and
+// synchronizes once at the end (re-running the neighbor build when the fixed per-agent
+// neighbor capacity had to grow).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pcm_kernels.h"
+
+using namespace pcm;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+{
+	g_error = msg;
+	return code;
+}
+
+#define PCM_HIP(expr)                                                                                                  \
+	do                                                                                                                 \
+	{                                                                                                                  \
+		hipError_t e_ = (expr);                                                                                        \
+		if (e_ != hipSuccess)                                                                                          \
+			return fail(PCM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                           \
+	} while (0)
+
+#define PCM_TRY(expr)                                                                                                  \
+	do                                                                                                                 \
+	{                                                                                                                  \
+		int rc_ = (expr);                                                                                              \
+		if (rc_ != PCM_OK)                                                                                             \
+			return rc_;                                                                                                \
+	} while (0)
+
+struct field_meta
+{
+	int type, width_kind;
+	const char* name;
+};
+
+#define PCM_FIELD_META_(name, type, width) { type, width, #name },
+const field_meta k_fields[PCM_FIELD_COUNT] = { PCM_FIELD_LIST(PCM_FIELD_META_) };
+#undef PCM_FIELD_META_
+
+int next_pow2(long long x)
+{
+	int p = 1;
+	while (p < x)
+		p <<= 1;
+	return p;
+}
+
+} // namespace
+
+struct pcm_sim
+{
+	virtual ~pcm_sim() = default;
+	virtual int init() = 0;
+	virtual int set_count(int n) = 0;
+	virtual int get_count(int* n) = 0;
+	virtual int set_field(int f, const void* src) = 0;
+	virtual int get_field(int f, void* dst) = 0;
+	virtual int set_links(int which, const int32_t* off, const int32_t* idx) = 0;
+	virtual int get_links(int which, int32_t* off, int32_t* idx, int64_t* total) = 0;
+	virtual int get_neighbors(int32_t* off, int32_t* idx, int64_t* total) = 0;
+	virtual int get_buckets(int32_t* counts, int32_t* agents) = 0;
+	virtual long long voxel_count() const = 0;
+	virtual int get_deaths(int64_t* d) = 0;
+	virtual int set_step(uint64_t s) = 0;
+	virtual int get_step(uint64_t* s) = 0;
+	virtual int phase(int which, const pcm_step_params* p) = 0;
+	virtual int step(const pcm_step_params* p, int steps) = 0;
+	virtual int synchronize() = 0;
+};
+
+namespace {
+
+enum phase_id
+{
+	PH_MESH,
+	PH_NEIGHBORS,
+	PH_FORCES,
+	PH_MOTILITY,
+	PH_MEMBRANE,
+	PH_SPRINGS,
+	PH_POSITIONS,
+	PH_INTERACTIONS,
+	PH_CONTAINER
+};
+
+template <class R>
+class sim_impl final : public pcm_sim
+{
+public:
+	explicit sim_impl(const pcm_config& cfg) : cfg_(cfg) {}
+
+	~sim_impl() override
+	{
+		if (stream_)
+		{
+			(void)hipStreamSynchronize(stream_);
+			(void)hipStreamDestroy(stream_);
+		}
+		for (void* p : allocations_)
+			(void)hipFree(p);
+	}
+
+	int init() override
+	{
+		auto& h = h_;
+		std::memset(&h, 0, sizeof(h));
+		h.dims = cfg_.mesh.dims;
+		h.T = cfg_.cell_defs;
+		h.S = cfg_.substrates;
+		h.cap = cfg_.capacity;
+		for (int k = 0; k < 3; k++)
+		{
+			h.bmin[k] = cfg_.mesh.bbox_min[k];
+			h.bmax[k] = cfg_.mesh.bbox_max[k];
+			h.vox[k] = cfg_.mesh.voxel[k];
+			if (k < h.dims)
+			{
+				if (h.vox[k] <= 0 || h.bmax[k] < h.bmin[k])
+					return fail(PCM_ERR_ARGUMENT, "bad mesh box or voxel shape");
+				h.grid[k] = std::max(1, (h.bmax[k] - h.bmin[k] + h.vox[k] - 1) / h.vox[k]);
+			}
+			else
+				h.grid[k] = 1;
+		}
+		h.vcount = (long long)h.grid[0] * h.grid[1] * h.grid[2];
+		if (h.vcount > 0x7fffffffLL)
+			return fail(PCM_ERR_CAPACITY, "mechanics voxel grid larger than 2^31 voxels");
+		h.K = h.dims == 3 ? 32 : (h.dims == 2 ? 16 : 8);
+		h.L = 12;
+		h.LA = 1;
+		h.seed = cfg_.seed;
+		h.dt = (R)0.1;
+		h.springs = 1;
+
+		PCM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+		const int cap = std::max(1, h.cap);
+		const int dims = h.dims;
+
+		for (int f = 0; f < PCM_FIELD_COUNT; f++)
+		{
+			const int w = width(k_fields[f].width_kind);
+			const int eb = k_fields[f].type == PCM_T_REAL ? (int)sizeof(R) : (k_fields[f].type == PCM_T_I32 ? 4 : 1);
+			h.fbytes[f] = w * eb;
+			PCM_TRY(alloc(&h.fptr[f], (size_t)cap * std::max(1, h.fbytes[f])));
+		}
+		bind_fields();
+
+		PCM_TRY(alloc_t(&h.sc, 1));
+		PCM_TRY(alloc_t(&h.spr_cnt, cap));
+		PCM_TRY(alloc_t(&h.spr_idx, (size_t)cap * h.L));
+		PCM_TRY(alloc_t(&h.atc_cnt, cap));
+		PCM_TRY(alloc_t(&h.atc_idx, (size_t)cap * h.LA));
+		PCM_TRY(alloc_t(&h.nbr_cnt, cap));
+		PCM_TRY(alloc_neighbor_storage());
+		PCM_TRY(alloc_t(&h.vcnt, (size_t)h.vcount));
+		PCM_TRY(alloc_t(&h.vstart, (size_t)h.vcount));
+		PCM_TRY(alloc_t(&h.vox_of, cap));
+		PCM_TRY(alloc_t(&h.vrank, cap));
+		PCM_TRY(alloc_t(&h.btmp, cap));
+		PCM_TRY(alloc_t(&h.bucket, cap));
+		PCM_TRY(alloc_t(&h.sorted, (size_t)cap * 4));
+		PCM_TRY(alloc_t(&h.fire, cap));
+		PCM_TRY(alloc_t(&h.erase, cap));
+		PCM_TRY(alloc_t(&h.n_rem_t, cap));
+		PCM_TRY(alloc_t(&h.n_atk_t, cap));
+		PCM_TRY(alloc_t(&h.rem_cnt, cap));
+		PCM_TRY(alloc_t(&h.rem_ev, (size_t)cap * 4));
+		PCM_TRY(alloc_t(&h.atk_tgt, cap));
+		PCM_TRY(alloc_t(&h.acted, cap));
+		h.bad_cap = next_pow2(cap);
+		PCM_TRY(alloc_t(&h.bad, (size_t)h.bad_cap));
+		PCM_TRY(alloc_t(&h.remap, cap));
+		PCM_TRY(alloc_t(&h.move_src, cap));
+		PCM_TRY(alloc_t(&h.move_dst, cap));
+		(void)dims;
+
+		std::vector<int> ident(cap);
+		for (int i = 0; i < cap; i++)
+			ident[i] = i;
+		PCM_HIP(hipMemcpy(h.remap, ident.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+		PCM_TRY(alloc_t(&d_, 1));
+		PCM_TRY(push_ctx());
+		PCM_HIP(hipStreamSynchronize(stream_));
+		return PCM_OK;
+	}
+
+	// ---------------------------------------------------------------- data movement
+	int set_count(int n) override
+	{
+		if (n < 0 || n > h_.cap)
+			return fail(PCM_ERR_CAPACITY, "agent count exceeds the handle's capacity");
+		PCM_HIP(hipStreamSynchronize(stream_));
+		Scalars s;
+		PCM_HIP(hipMemcpy(&s, h_.sc, sizeof(s), hipMemcpyDeviceToHost));
+		s.n = n;
+		s.status = 0;
+		s.att_events = 0;
+		s.cand_count = 0;
+		s.conflict = 0;
+		s.bad_count = 0;
+		s.moves = 0;
+		PCM_HIP(hipMemcpy(h_.sc, &s, sizeof(s), hipMemcpyHostToDevice));
+		n_launch_ = n;
+		std::vector<int> ident(std::max(n, 1));
+		for (int i = 0; i < n; i++)
+			ident[i] = i;
+		if (n > 0)
+		{
+			PCM_HIP(hipMemcpy(h_.origin, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+			PCM_HIP(hipMemset(h_.spr_cnt, 0, sizeof(int) * n));
+			PCM_HIP(hipMemset(h_.atc_cnt, 0, sizeof(int) * n));
+			PCM_HIP(hipMemset(h_.nbr_cnt, 0, sizeof(int) * n));
+		}
+		return PCM_OK;
+	}
+
+	int get_count(int* n) override
+	{
+		PCM_TRY(read_scalars());
+		*n = sc_.n;
+		return PCM_OK;
+	}
+
+	int set_field(int f, const void* src) override
+	{
+		if (f < 0 || f >= PCM_FIELD_COUNT || !src)
+			return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer");
+		PCM_HIP(hipStreamSynchronize(stream_));
+		const int n = n_launch_;
+		if (f == PCM_F_MAXIMUM_NUMBER_OF_ATTACHMENTS && n > 0)
+		{
+			const int* m = (const int*)src;
+			int mx = 0;
+			for (int i = 0; i < n; i++)
+				mx = std::max(mx, m[i]);
+			if (mx > 32)
+				return fail(PCM_ERR_UNSUPPORTED, "maximum_number_of_attachments above 32 is not supported");
+			if (mx > h_.L)
+				PCM_TRY(grow_springs(mx));
+		}
+		PCM_HIP(hipMemcpy(h_.fptr[f], src, (size_t)n * h_.fbytes[f], hipMemcpyHostToDevice));
+		return PCM_OK;
+	}
+
+	int get_field(int f, void* dst) override
+	{
+		if (f < 0 || f >= PCM_FIELD_COUNT || !dst)
+			return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer");
+		PCM_TRY(read_scalars());
+		PCM_HIP(hipMemcpy(dst, h_.fptr[f], (size_t)sc_.n * h_.fbytes[f], hipMemcpyDeviceToHost));
+		return PCM_OK;
+	}
+
+	int set_links(int which, const int32_t* off, const int32_t* idx) override
+	{
+		if (which != PCM_LINKS_SPRINGS && which != PCM_LINKS_ATTACHED)
+			return fail(PCM_ERR_ARGUMENT, "bad link list id");
+		if (!off)
+			return fail(PCM_ERR_ARGUMENT, "null offsets");
+		PCM_HIP(hipStreamSynchronize(stream_));
+		const int n = n_launch_;
+		int longest = 0;
+		for (int i = 0; i < n; i++)
+		{
+			if (off[i + 1] < off[i])
+				return fail(PCM_ERR_ARGUMENT, "offsets must be non-decreasing");
+			longest = std::max(longest, off[i + 1] - off[i]);
+		}
+		if (which == PCM_LINKS_SPRINGS)
+		{
+			if (longest > 32)
+				return fail(PCM_ERR_UNSUPPORTED, "more than 32 springs on one agent");
+			if (longest > h_.L)
+				PCM_TRY(grow_springs(longest));
+		}
+		else if (longest > h_.LA)
+			PCM_TRY(grow_attached(longest));
+		for (int i = 0; i < n; i++)
+			for (int s = off[i]; s < off[i + 1]; s++)
+				if (!idx || idx[s] < 0 || idx[s] >= n || idx[s] == i)
+					return fail(PCM_ERR_ARGUMENT, "link list entry is not another agent's index");
+		const int W = which == PCM_LINKS_SPRINGS ? h_.L : h_.LA;
+		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * W, -1);
+		for (int i = 0; i < n; i++)
+		{
+			cnt[i] = off[i + 1] - off[i];
+			for (int s = 0; s < cnt[i]; s++)
+				rows[(size_t)i * W + s] = idx[off[i] + s];
+		}
+		int* dc = which == PCM_LINKS_SPRINGS ? h_.spr_cnt : h_.atc_cnt;
+		int* di = which == PCM_LINKS_SPRINGS ? h_.spr_idx : h_.atc_idx;
+		if (n > 0)
+		{
+			PCM_HIP(hipMemcpy(dc, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+			PCM_HIP(hipMemcpy(di, rows.data(), sizeof(int) * (size_t)n * W, hipMemcpyHostToDevice));
+		}
+		return PCM_OK;
+	}
+
+	int get_links(int which, int32_t* off, int32_t* idx, int64_t* total) override
+	{
+		if (which != PCM_LINKS_SPRINGS && which != PCM_LINKS_ATTACHED)
+			return fail(PCM_ERR_ARGUMENT, "bad link list id");
+		PCM_TRY(read_scalars());
+		const int n = sc_.n;
+		const int W = which == PCM_LINKS_SPRINGS ? h_.L : h_.LA;
+		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * W);
+		if (n > 0)
+		{
+			PCM_HIP(hipMemcpy(cnt.data(), which == PCM_LINKS_SPRINGS ? h_.spr_cnt : h_.atc_cnt, sizeof(int) * n,
+							  hipMemcpyDeviceToHost));
+			PCM_HIP(hipMemcpy(rows.data(), which == PCM_LINKS_SPRINGS ? h_.spr_idx : h_.atc_idx,
+							  sizeof(int) * (size_t)n * W, hipMemcpyDeviceToHost));
+		}
+		int64_t t = 0;
+		for (int i = 0; i < n; i++)
+		{
+			if (off)
+				off[i] = (int32_t)t;
+			for (int s = 0; s < cnt[i]; s++, t++)
+				if (idx)
+					idx[t] = rows[(size_t)i * W + s];
+		}
+		if (off)
+			off[n] = (int32_t)t;
+		*total = t;
+		return PCM_OK;
+	}
+
+	int get_neighbors(int32_t* off, int32_t* idx, int64_t* total) override
+	{
+		PCM_TRY(read_scalars());
+		const int n = sc_.n;
+		const int K = h_.K;
+		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * K);
+		std::vector<uint8_t> mov(std::max(n, 1));
+		if (n > 0)
+		{
+			PCM_HIP(hipMemcpy(cnt.data(), h_.nbr_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+			PCM_HIP(hipMemcpy(rows.data(), h_.nbr_idx, sizeof(int) * (size_t)n * K, hipMemcpyDeviceToHost));
+			PCM_HIP(hipMemcpy(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
+		}
+		int64_t t = 0;
+		for (int i = 0; i < n; i++)
+		{
+			if (off)
+				off[i] = (int32_t)t;
+			if (!mov[i])
+				continue;
+			for (int s = 0; s < cnt[i]; s++, t++)
+				if (idx)
+					idx[t] = rows[(size_t)i * K + s];
+		}
+		if (off)
+			off[n] = (int32_t)t;
+		*total = t;
+		return PCM_OK;
+	}
+
+	long long voxel_count() const override { return h_.vcount; }
+
+	int get_buckets(int32_t* counts, int32_t* agents) override
+	{
+		PCM_TRY(read_scalars());
+		const size_t V = (size_t)h_.vcount;
+		std::vector<int> vc(V), vs(V);
+		PCM_HIP(hipMemcpy(vc.data(), h_.vcnt, sizeof(int) * V, hipMemcpyDeviceToHost));
+		PCM_HIP(hipMemcpy(vs.data(), h_.vstart, sizeof(int) * V, hipMemcpyDeviceToHost));
+		const int nb = std::max(sc_.n_binned, 1);
+		std::vector<int> b(nb);
+		if (sc_.n_binned > 0)
+			PCM_HIP(hipMemcpy(b.data(), h_.bucket, sizeof(int) * sc_.n_binned, hipMemcpyDeviceToHost));
+		int64_t t = 0;
+		for (size_t v = 0; v < V; v++)
+		{
+			counts[v] = vc[v];
+			for (int k = 0; k < vc[v]; k++)
+				agents[t++] = b[vs[v] + k];
+		}
+		return PCM_OK;
+	}
+
+	int get_deaths(int64_t* d) override
+	{
+		PCM_TRY(read_scalars());
+		*d = sc_.deaths;
+		return PCM_OK;
+	}
+
+	int set_step(uint64_t s) override
+	{
+		PCM_TRY(read_scalars());
+		sc_.step = s;
+		PCM_HIP(hipMemcpy(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
+		return PCM_OK;
+	}
+
+	int get_step(uint64_t* s) override
+	{
+		PCM_TRY(read_scalars());
+		*s = sc_.step;
+		return PCM_OK;
+	}
+
+	// ---------------------------------------------------------------- phases
+	int phase(int which, const pcm_step_params* p) override
+	{
+		if (p)
+			PCM_TRY(set_params(*p));
+		switch (h_.dims)
+		{
+			case 1:
+				return phase_d<1>(which);
+			case 2:
+				return phase_d<2>(which);
+			default:
+				return phase_d<3>(which);
+		}
+	}
+
+	int step(const pcm_step_params* p, int steps) override
+	{
+		if (!p || steps < 0)
+			return fail(PCM_ERR_ARGUMENT, "null params or negative step count");
+		PCM_TRY(set_params(*p));
+		PCM_TRY(read_scalars());
+		if (sc_.status)
+			return fail(PCM_ERR_STATE, "device status word is set");
+		const unsigned long long target = sc_.step + (unsigned long long)steps;
+		for (int guard = 0; guard < 64; guard++)
+		{
+			const long long remaining = (long long)(target - sc_.step);
+			for (long long s = 0; s < remaining; s++)
+				PCM_TRY(dispatch_step());
+			PCM_TRY(read_scalars());
+			if (sc_.status == 0)
+				return PCM_OK;
+			PCM_TRY(recover());
+			if (sc_.step >= target)
+				return PCM_OK;
+		}
+		return fail(PCM_ERR_STATE, "mechanics step did not converge after repeated neighbor-capacity growth");
+	}
+
+	int synchronize() override
+	{
+		PCM_TRY(read_scalars());
+		if (sc_.status)
+			return fail(PCM_ERR_STATE, "device status word is set: " + std::to_string(sc_.status));
+		return PCM_OK;
+	}
+
+private:
+	pcm_config cfg_;
+	SimCtx<R> h_ {};
+	SimCtx<R>* d_ = nullptr;
+	Scalars sc_ {};
+	hipStream_t stream_ = nullptr;
+	std::vector<void*> allocations_;
+	int n_launch_ = 0; // agents at the last upload: an upper bound of the device count
+
+	int width(int kind) const
+	{
+		switch (kind)
+		{
+			case PCM_W_DIMS:
+				return h_.dims;
+			case PCM_W_DEFS:
+				return h_.T;
+			case PCM_W_SUBS:
+				return h_.S;
+			default:
+				return 1;
+		}
+	}
+
+	int alloc(void** p, size_t bytes)
+	{
+		PCM_HIP(hipMalloc(p, std::max<size_t>(bytes, 16)));
+		PCM_HIP(hipMemset(*p, 0, std::max<size_t>(bytes, 16)));
+		allocations_.push_back(*p);
+		return PCM_OK;
+	}
+
+	template <class T>
+	int alloc_t(T** p, size_t n)
+	{
+		void* v = nullptr;
+		PCM_TRY(alloc(&v, sizeof(T) * n));
+		*p = (T*)v;
+		return PCM_OK;
+	}
+
+	void release(void* p)
+	{
+		auto it = std::find(allocations_.begin(), allocations_.end(), p);
+		if (it != allocations_.end())
+		{
+			(void)hipFree(p);
+			allocations_.erase(it);
+		}
+	}
+
+	void bind_fields()
+	{
+		auto& h = h_;
+		auto F = [&](int f) { return h.fptr[f]; };
+		h.pos = (R*)F(PCM_F_POSITIONS);
+		h.vel = (R*)F(PCM_F_VELOCITIES);
+		h.pvel = (R*)F(PCM_F_PREVIOUS_VELOCITIES);
+		h.radius = (R*)F(PCM_F_RADIUS);
+		h.nuc_radius = (R*)F(PCM_F_NUCLEAR_RADIUS);
+		h.surface = (R*)F(PCM_F_SURFACE_AREA);
+		h.volume = (R*)F(PCM_F_VOLUME);
+		h.fluid = (R*)F(PCM_F_FLUID);
+		h.solid = (R*)F(PCM_F_SOLID);
+		h.nuclear = (R*)F(PCM_F_NUCLEAR);
+		h.nuc_fluid = (R*)F(PCM_F_NUCLEAR_FLUID);
+		h.nuc_solid = (R*)F(PCM_F_NUCLEAR_SOLID);
+		h.cyto = (R*)F(PCM_F_CYTOPLASMIC);
+		h.cyto_fluid = (R*)F(PCM_F_CYTOPLASMIC_FLUID);
+		h.cyto_solid = (R*)F(PCM_F_CYTOPLASMIC_SOLID);
+		h.fluid_frac = (R*)F(PCM_F_FLUID_FRACTION);
+		h.c2n = (R*)F(PCM_F_CYTOPLASMIC_TO_NUCLEAR_RATIO);
+		h.target_cyto = (R*)F(PCM_F_TARGET_SOLID_CYTOPLASMIC);
+		h.target_nuc = (R*)F(PCM_F_TARGET_SOLID_NUCLEAR);
+		h.adhesion = (R*)F(PCM_F_CELL_CELL_ADHESION_STRENGTH);
+		h.repulsion = (R*)F(PCM_F_CELL_CELL_REPULSION_STRENGTH);
+		h.bm_repulsion = (R*)F(PCM_F_CELL_BM_REPULSION_STRENGTH);
+		h.rel_adh = (R*)F(PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE);
+		h.affinity = (R*)F(PCM_F_CELL_ADHESION_AFFINITIES);
+		h.max_att = (int*)F(PCM_F_MAXIMUM_NUMBER_OF_ATTACHMENTS);
+		h.elastic = (R*)F(PCM_F_ATTACHMENT_ELASTIC_CONSTANT);
+		h.att_rate = (R*)F(PCM_F_ATTACHMENT_RATE);
+		h.det_rate = (R*)F(PCM_F_DETACHMENT_RATE);
+		h.motile = (uint8_t*)F(PCM_F_IS_MOTILE);
+		h.persistence = (R*)F(PCM_F_PERSISTENCE_TIME);
+		h.speed = (R*)F(PCM_F_MIGRATION_SPEED);
+		h.bias_dir = (R*)F(PCM_F_MIGRATION_BIAS_DIRECTION);
+		h.bias = (R*)F(PCM_F_MIGRATION_BIAS);
+		h.motility = (R*)F(PCM_F_MOTILITY_VECTOR);
+		h.restrict2d = (uint8_t*)F(PCM_F_RESTRICT_TO_2D);
+		h.dead = (uint8_t*)F(PCM_F_DEAD);
+		h.dead_phago = (R*)F(PCM_F_DEAD_PHAGOCYTOSIS_RATE);
+		h.live_phago = (R*)F(PCM_F_LIVE_PHAGOCYTOSIS_RATES);
+		h.damage_rate = (R*)F(PCM_F_DAMAGE_RATE);
+		h.attack = (R*)F(PCM_F_ATTACK_RATES);
+		h.immuno = (R*)F(PCM_F_IMMUNOGENICITIES);
+		h.fusion = (R*)F(PCM_F_FUSION_RATES);
+		h.pressure = (R*)F(PCM_F_SIMPLE_PRESSURE);
+		h.nuclei = (int*)F(PCM_F_NUMBER_OF_NUCLEI);
+		h.damage = (R*)F(PCM_F_DAMAGE);
+		h.attack_time = (R*)F(PCM_F_TOTAL_ATTACK_TIME);
+		h.type = (int*)F(PCM_F_CELL_DEFINITION_INDEX);
+		h.movable = (uint8_t*)F(PCM_F_IS_MOVABLE);
+		h.flags = (uint8_t*)F(PCM_F_FLAGS);
+		h.internal = (R*)F(PCM_F_INTERNALIZED_SUBSTRATES);
+		h.frac_ingest = (R*)F(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED);
+		h.origin = (int*)F(PCM_F_ORIGIN);
+	}
+
+	int alloc_neighbor_storage()
+	{
+		const size_t cap = std::max(1, h_.cap);
+		const size_t K = h_.K;
+		PCM_TRY(alloc_t(&h_.nbr_idx, cap * K));
+		h_.ev_cap = (int)std::min<size_t>(0x7ffffff0, cap * K / 2 + 16);
+		PCM_TRY(alloc_t(&h_.ev, (size_t)h_.ev_cap * 3));
+		h_.cand_cap = (int)std::min<size_t>(0x7ffffff0, cap * K + 16);
+		PCM_TRY(alloc_t(&h_.cand, (size_t)h_.cand_cap * 2));
+		return PCM_OK;
+	}
+
+	int grow_neighbors(int needed)
+	{
+		PCM_HIP(hipStreamSynchronize(stream_));
+		release(h_.nbr_idx);
+		release(h_.ev);
+		release(h_.cand);
+		h_.K = next_pow2(std::max(needed, 2 * h_.K));
+		PCM_TRY(alloc_neighbor_storage());
+		return push_ctx();
+	}
+
+	int grow_rows(int** idx, int oldW, int newW)
+	{
+		const size_t cap = std::max(1, h_.cap);
+		std::vector<int> rows(cap * oldW), grown(cap * newW, -1);
+		PCM_HIP(hipMemcpy(rows.data(), *idx, sizeof(int) * cap * oldW, hipMemcpyDeviceToHost));
+		for (size_t i = 0; i < cap; i++)
+			for (int s = 0; s < oldW; s++)
+				grown[i * newW + s] = rows[i * oldW + s];
+		release(*idx);
+		PCM_TRY(alloc_t(idx, cap * newW));
+		PCM_HIP(hipMemcpy(*idx, grown.data(), sizeof(int) * cap * newW, hipMemcpyHostToDevice));
+		return PCM_OK;
+	}
+
+	int grow_springs(int L)
+	{
+		PCM_HIP(hipStreamSynchronize(stream_));
+		PCM_TRY(grow_rows(&h_.spr_idx, h_.L, L));
+		h_.L = L;
+		return push_ctx();
+	}
+
+	int grow_attached(int LA)
+	{
+		PCM_HIP(hipStreamSynchronize(stream_));
+		PCM_TRY(grow_rows(&h_.atc_idx, h_.LA, LA));
+		h_.LA = LA;
+		return push_ctx();
+	}
+
+	int push_ctx()
+	{
+		PCM_HIP(hipMemcpyAsync(d_, &h_, sizeof(h_), hipMemcpyHostToDevice, stream_));
+		// the host copy may change before the async copy runs
+		PCM_HIP(hipStreamSynchronize(stream_));
+		return PCM_OK;
+	}
+
+	int set_params(const pcm_step_params& p)
+	{
+		const R dt = (R)p.mechanics_time_step;
+		const int wall = p.virtual_wall_at_domain_edges != 0;
+		const int springs = p.automated_spring_adhesion != 0;
+		if (dt == h_.dt && wall == h_.wall && springs == h_.springs)
+			return PCM_OK;
+		h_.dt = dt;
+		h_.wall = wall;
+		h_.springs = springs;
+		return push_ctx();
+	}
+
+	int read_scalars()
+	{
+		PCM_HIP(hipStreamSynchronize(stream_));
+		PCM_HIP(hipMemcpy(&sc_, h_.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+		return PCM_OK;
+	}
+
+	// neighbor overflow: grow K, rebuild the lists of the aborted step, clear the status
+	int recover()
+	{
+		if (sc_.status & ~ST_NEIGHBOR_OVERFLOW)
+			return fail(PCM_ERR_STATE, "device status word is set: " + std::to_string(sc_.status));
+		PCM_TRY(grow_neighbors(sc_.max_neighbors));
+		sc_.status = 0;
+		sc_.max_neighbors = 0;
+		PCM_HIP(hipMemcpy(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
+		PCM_TRY(launch_neighbors());
+		PCM_TRY(read_scalars());
+		if (sc_.status)
+			return fail(PCM_ERR_STATE, "neighbor rebuild failed after growth");
+		return PCM_OK;
+	}
+
+	int grid_agents() const { return std::max(1, (n_launch_ + BLOCK - 1) / BLOCK); }
+
+	template <class Kern>
+	int launch(Kern kernel, int blocks, int threads)
+	{
+		hipLaunchKernelGGL(kernel, dim3(blocks, 1), dim3(threads), 0, stream_, (const SimCtx<R>*)d_);
+		PCM_HIP(hipGetLastError());
+		return PCM_OK;
+	}
+
+	template <class Kern>
+	int per_agent(Kern kernel)
+	{
+		return launch(kernel, grid_agents(), BLOCK);
+	}
+
+	int launch_neighbors()
+	{
+		switch (h_.dims)
+		{
+			case 1:
+				return per_agent(k_neighbors<R, 1>);
+			case 2:
+				return per_agent(k_neighbors<R, 2>);
+			default:
+				return per_agent(k_neighbors<R, 3>);
+		}
+	}
+
+	template <int D>
+	int mesh_kernels()
+	{
+		PCM_TRY(per_agent(k_bin_clear<R>));
+		PCM_TRY(per_agent(k_bin_count<R, D>));
+		PCM_TRY(per_agent(k_bin_claim<R>));
+		PCM_TRY(per_agent(k_bin_scatter<R>));
+		return per_agent(k_bin_order<R, D>);
+	}
+
+	int spring_update_kernels()
+	{
+		PCM_TRY(per_agent(k_spring_fire<R>));
+		PCM_TRY(per_agent(k_spring_erase<R>));
+		PCM_TRY(per_agent(k_spring_compact<R>));
+		PCM_TRY(per_agent(k_spring_collect<R>));
+		return launch(k_spring_resolve<R>, 1, RESOLVE_THREADS);
+	}
+
+	int interaction_kernels()
+	{
+		PCM_TRY(per_agent(k_interact_tentative<R>));
+		PCM_TRY(per_agent(k_interact_check<R>));
+		PCM_TRY(per_agent(k_interact_apply<R>));
+		return launch(k_interact_serial<R>, 1, RESOLVE_THREADS);
+	}
+
+	template <int D>
+	int container_kernels()
+	{
+		PCM_TRY(per_agent(k_compact_flag<R, D>));
+		PCM_TRY(launch(k_compact_plan<R>, 1, RESOLVE_THREADS));
+		PCM_TRY(per_agent(k_compact_move<R>));
+		PCM_TRY(per_agent(k_compact_links<R>));
+		PCM_TRY(per_agent(k_compact_restore<R>));
+		return launch(k_compact_done<R>, 1, 64);
+	}
+
+	template <int D>
+	int phase_d(int which)
+	{
+		switch (which)
+		{
+			case PH_MESH:
+				return mesh_kernels<D>();
+			case PH_NEIGHBORS:
+			{
+				PCM_TRY(per_agent(k_neighbors<R, D>));
+				PCM_TRY(read_scalars());
+				if (sc_.status & ST_NEIGHBOR_OVERFLOW)
+					PCM_TRY(recover());
+				return PCM_OK;
+			}
+			case PH_FORCES:
+				PCM_TRY(launch(k_begin_step<R>, 1, 64));
+				return per_agent(k_forces<R, D>);
+			case PH_MOTILITY:
+				return per_agent(k_motility<R, D>);
+			case PH_MEMBRANE:
+				if (!h_.wall)
+					return PCM_OK;
+				return per_agent(k_membrane<R, D>);
+			case PH_SPRINGS:
+				if (!h_.springs)
+					return PCM_OK;
+				PCM_TRY(spring_update_kernels());
+				return per_agent(k_contract<R, D>);
+			case PH_POSITIONS:
+				return per_agent(k_positions<R, D>);
+			case PH_INTERACTIONS:
+				return interaction_kernels();
+			case PH_CONTAINER:
+				return container_kernels<D>();
+		}
+		return fail(PCM_ERR_ARGUMENT, "unknown phase");
+	}
+
+	// one mechanics step (simulator.cpp:173-192): the spring-list update runs first since
+	// forces, motility and the membrane do not read springs; velocity contributions are
+	// then summed in the reference's order by the fused velocity kernel.
+	template <int D>
+	int step_d()
+	{
+		PCM_TRY(launch(k_begin_step<R>, 1, 64));
+		if (h_.springs)
+			PCM_TRY(spring_update_kernels());
+		PCM_TRY(per_agent(k_velocity_fused<R, D>));
+		PCM_TRY(per_agent(k_positions<R, D>));
+		PCM_TRY(interaction_kernels());
+		PCM_TRY(container_kernels<D>());
+		PCM_TRY(mesh_kernels<D>());
+		return per_agent(k_neighbors<R, D>);
+	}
+
+	int dispatch_step()
+	{
+		switch (h_.dims)
+		{
+			case 1:
+				return step_d<1>();
+			case 2:
+				return step_d<2>();
+			default:
+				return step_d<3>();
+		}
+	}
+};
+
+pcm_sim* checked(pcm_sim* s) { return s; }
+
+} // namespace
+
+// ============================================================================ C-ABI
+extern "C" {
+
+int pcm_version(void) { return 1; }
+
+const char* pcm_last_error(void) { return g_error.c_str(); }
+
+int pcm_field_info(int32_t field, int32_t* type, int32_t* width_kind, const char** name)
+{
+	if (field < 0 || field >= PCM_FIELD_COUNT)
+		return fail(PCM_ERR_ARGUMENT, "bad field id");
+	if (type)
+		*type = k_fields[field].type;
+	if (width_kind)
+		*width_kind = k_fields[field].width_kind;
+	if (name)
+		*name = k_fields[field].name;
+	return PCM_OK;
+}
+
+int pcm_create(const pcm_config* config, pcm_sim** out)
+{
+	if (!config || !out)
+		return fail(PCM_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	const auto& c = *config;
+	if (c.mesh.dims < 1 || c.mesh.dims > 3)
+		return fail(PCM_ERR_ARGUMENT, "mesh dims must be 1, 2 or 3");
+	if (c.cell_defs < 1 || c.substrates < 0 || c.capacity < 0)
+		return fail(PCM_ERR_ARGUMENT, "bad cell definition, substrate or capacity count");
+	int devices = 0;
+	if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
+		return fail(PCM_ERR_DEVICE, "no HIP device is available");
+	std::unique_ptr<pcm_sim> s;
+	if (c.real_bytes == 4)
+		s.reset(new sim_impl<float>(c));
+	else if (c.real_bytes == 8)
+		s.reset(new sim_impl<double>(c));
+	else
+		return fail(PCM_ERR_ARGUMENT, "real_bytes must be 4 or 8");
+	const int rc = s->init();
+	if (rc != PCM_OK)
+		return rc;
+	*out = s.release();
+	return PCM_OK;
+}
+
+int pcm_destroy(pcm_sim* sim)
+{
+	delete sim;
+	return PCM_OK;
+}
+
+#define PCM_NEED(sim)                                                                                                  \
+	if (!(sim))                                                                                                        \
+	return fail(PCM_ERR_ARGUMENT, "null handle")
+
+int pcm_set_count(pcm_sim* sim, int32_t count)
+{
+	PCM_NEED(sim);
+	return sim->set_count(count);
+}
+
+int pcm_get_count(pcm_sim* sim, int32_t* count)
+{
+	PCM_NEED(sim);
+	if (!count)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_count(count);
+}
+
+int pcm_set_field(pcm_sim* sim, int32_t field, const void* host)
+{
+	PCM_NEED(sim);
+	return sim->set_field(field, host);
+}
+
+int pcm_get_field(pcm_sim* sim, int32_t field, void* host)
+{
+	PCM_NEED(sim);
+	return sim->get_field(field, host);
+}
+
+int pcm_set_links(pcm_sim* sim, int32_t which, const int32_t* offsets, const int32_t* indices)
+{
+	PCM_NEED(sim);
+	return sim->set_links(which, offsets, indices);
+}
+
+int pcm_get_links_size(pcm_sim* sim, int32_t which, int64_t* total)
+{
+	PCM_NEED(sim);
+	if (!total)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_links(which, nullptr, nullptr, total);
+}
+
+int pcm_get_links(pcm_sim* sim, int32_t which, int32_t* offsets, int32_t* indices)
+{
+	PCM_NEED(sim);
+	int64_t t = 0;
+	if (!offsets || !indices)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_links(which, offsets, indices, &t);
+}
+
+int pcm_get_neighbors_size(pcm_sim* sim, int64_t* total)
+{
+	PCM_NEED(sim);
+	if (!total)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_neighbors(nullptr, nullptr, total);
+}
+
+int pcm_get_neighbors(pcm_sim* sim, int32_t* offsets, int32_t* indices)
+{
+	PCM_NEED(sim);
+	int64_t t = 0;
+	if (!offsets || !indices)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_neighbors(offsets, indices, &t);
+}
+
+int pcm_get_voxel_count(pcm_sim* sim, int64_t* voxel_count)
+{
+	PCM_NEED(sim);
+	if (!voxel_count)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	*voxel_count = sim->voxel_count();
+	return PCM_OK;
+}
+
+int pcm_get_voxel_buckets(pcm_sim* sim, int32_t* counts, int32_t* agents)
+{
+	PCM_NEED(sim);
+	if (!counts || !agents)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_buckets(counts, agents);
+}
+
+int pcm_get_deaths_count(pcm_sim* sim, int64_t* deaths)
+{
+	PCM_NEED(sim);
+	if (!deaths)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_deaths(deaths);
+}
+
+int pcm_set_step_index(pcm_sim* sim, uint64_t step)
+{
+	PCM_NEED(sim);
+	return sim->set_step(step);
+}
+
+int pcm_get_step_index(pcm_sim* sim, uint64_t* step)
+{
+	PCM_NEED(sim);
+	if (!step)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_step(step);
+}
+
+int pcm_update_mechanics_mesh(pcm_sim* sim)
+{
+	PCM_NEED(sim);
+	return sim->phase(PH_MESH, nullptr);
+}
+
+int pcm_update_cell_neighbors(pcm_sim* sim)
+{
+	PCM_NEED(sim);
+	return sim->phase(PH_NEIGHBORS, nullptr);
+}
+
+#define PCM_PHASE_ENTRY(NAME, ID)                                                                                      \
+	int NAME(pcm_sim* sim, const pcm_step_params* params)                                                              \
+	{                                                                                                                  \
+		PCM_NEED(sim);                                                                                                 \
+		if (!params)                                                                                                   \
+			return fail(PCM_ERR_ARGUMENT, "null params");                                                              \
+		return sim->phase(ID, params);                                                                                 \
+	}
+
+PCM_PHASE_ENTRY(pcm_update_cell_forces, PH_FORCES)
+PCM_PHASE_ENTRY(pcm_update_motility, PH_MOTILITY)
+PCM_PHASE_ENTRY(pcm_update_basement_membrane_interactions, PH_MEMBRANE)
+PCM_PHASE_ENTRY(pcm_update_spring_attachments, PH_SPRINGS)
+PCM_PHASE_ENTRY(pcm_update_positions, PH_POSITIONS)
+PCM_PHASE_ENTRY(pcm_update_cell_cell_interactions, PH_INTERACTIONS)
+PCM_PHASE_ENTRY(pcm_update_cell_container_for_mechanics, PH_CONTAINER)
+
+int pcm_step(pcm_sim* sim, const pcm_step_params* params, int32_t steps)
+{
+	PCM_NEED(sim);
+	return sim->step(params, steps);
+}
+
+int pcm_synchronize(pcm_sim* sim)
+{
+	PCM_NEED(sim);
+	return sim->synchronize();
+}
+
+} // extern "C"

@@ -1,0 +1,74 @@
+"""The C-ABI library: it loads, exports every entry point include/pcm.h declares, its field
+table agrees with the Python mirror, and it refuses to run without a GPU (no fallback)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from paraphysicell_amd import abi, mechanics
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "pcm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pcm_[a-z_0-9]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(abi.LIB_PATH):
+        from paraphysicell_amd import build
+
+        build.build()
+    return mechanics.load_library()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = set(re.findall(r"\bT (pcm_[a-z_0-9]+)", out.stdout))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(lib, n)
+
+
+def test_field_table_matches_header(lib):
+    t = ctypes.c_int32()
+    w = ctypes.c_int32()
+    name = ctypes.c_char_p()
+    for fid, (pname, ptype, pwidth) in enumerate(abi.FIELDS):
+        assert lib.pcm_field_info(fid, ctypes.byref(t), ctypes.byref(w), ctypes.byref(name)) == abi.PCM_OK
+        assert (t.value, w.value) == (ptype, pwidth), pname
+        assert name.value.decode().lower() == pname.lower(), (name.value, pname)
+    assert lib.pcm_field_info(len(abi.FIELDS), None, None, None) == abi.PCM_ERR_ARGUMENT
+
+
+def test_version(lib):
+    assert lib.pcm_version() >= 1
+
+
+def test_no_silent_fallback(lib):
+    """Without a usable GPU the library reports an error instead of computing on the CPU."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(mechanics.PcmError) as e:
+        mechanics.DeviceSim(abi.MeshSpec(3, (0, 0, 0), (90, 90, 90)), 4, 1, 1, capacity=8)
+    assert e.value.code == abi.PCM_ERR_DEVICE
+
+
+def test_argument_errors(lib):
+    h = ctypes.c_void_p()
+    assert lib.pcm_create(None, ctypes.byref(h)) == abi.PCM_ERR_ARGUMENT
+    cfg = abi.make_config(abi.MeshSpec(4, (0, 0, 0), (9, 9, 9)), 4, 1, 1, 8, 0)
+    assert lib.pcm_create(ctypes.byref(cfg), ctypes.byref(h)) == abi.PCM_ERR_ARGUMENT
+    cfg = abi.make_config(abi.MeshSpec(3, (0, 0, 0), (9, 9, 9)), 2, 1, 1, 8, 0)
+    assert lib.pcm_create(ctypes.byref(cfg), ctypes.byref(h)) in (abi.PCM_ERR_ARGUMENT, abi.PCM_ERR_DEVICE)
+    assert lib.pcm_step(None, None, 1) == abi.PCM_ERR_ARGUMENT

@@ -1,0 +1,194 @@
+"""Parity of the MI355X library (through the C-ABI) with the oracle and the reference tests.
+
+Bar: voxel buckets and neighbor lists bit-exact; integer/byte fields and link lists
+bit-exact; real fields within 1e-5 (FP32) / 1e-12 (FP64) relative to max(|oracle|, 1).
+"""
+import numpy as np
+import pytest
+
+import parity
+import scenarios as sc
+from paraphysicell_amd import abi
+from paraphysicell_amd.mechanics import DeviceSim
+from test_oracle_reference_cases import float_eq
+
+pytestmark = pytest.mark.gpu
+
+
+def device(mesh, st, seed=7, capacity=None):
+    d = DeviceSim(mesh, st.real_bytes, st.defs, st.subs, capacity=capacity or max(st.count, 1), seed=seed)
+    d.upload(st)
+    return d
+
+
+# ------------------------------------------------------------------ reference test cases
+@pytest.mark.parametrize("real_bytes", [4, 8])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_velocity_simple(dims, real_bytes):
+    mesh, st, p = sc.velocity_simple(dims, real_bytes)
+    d = device(mesh, st)
+    d.update_mechanics_mesh()
+    d.update_cell_neighbors()
+    d.update_cell_forces(p)
+    v = d.get_field("velocities").reshape(3, dims)
+    expected = sc.VELOCITY_SIMPLE_EXPECTED[dims]
+    for k in range(dims):
+        assert float_eq(v[0, k], expected) and float_eq(v[1, k], -expected)
+        assert v[2, k] == 0
+
+
+@pytest.mark.parametrize("real_bytes", [4, 8])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_velocity_complex(dims, real_bytes):
+    mesh, st, p = sc.velocity_complex(dims, real_bytes)
+    d = device(mesh, st)
+    d.update_mechanics_mesh()
+    d.update_cell_neighbors()
+    d.update_cell_forces(p)
+    off, _ = d.neighbors()
+    assert list(np.diff(off)) == [1, 2, 1]
+    v = d.get_field("velocities").reshape(3, dims)
+    expected = sc.brute_force_velocities(st)
+    for c in range(3):
+        for k in range(dims):
+            assert float_eq(v[c, k], expected[c, k])
+
+
+@pytest.mark.parametrize("adhesion", [1, 3])
+@pytest.mark.parametrize("per_voxel", [1, 5, 50])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_neighbors_grid(dims, per_voxel, adhesion):
+    mesh, st, _ = sc.neighbors_grid(dims, per_voxel, adhesion)
+    dev, ora = parity.make_pair(mesh, st)
+    for s in (dev, ora):
+        s.update_mechanics_mesh()
+    counts, _ = dev.buckets()
+    assert np.all(counts == per_voxel)
+    parity.assert_same_buckets(dev, ora, "grid buckets")
+    for s in (dev, ora):
+        s.update_cell_neighbors()
+    parity.assert_same_neighbors(dev, ora, "grid neighbors")
+    off, _ = dev.neighbors()
+    grid = mesh.grid_shape()
+    pos = st.view("positions")
+    for i in range(0, st.count, max(1, st.count // 200)):
+        c = [int((pos[i, a] - mesh.bbox_min[a]) // mesh.voxel[a]) if a < dims else 0 for a in range(3)]
+        assert off[i + 1] - off[i] == sc.expected_grid_neighbor_count(c, grid, per_voxel, adhesion)
+
+
+@pytest.mark.parametrize("kind", ["attack", "dead_phagocytosis", "phagocytosis", "fusion"])
+def test_interactions_reference(kind):
+    mesh, st, p = sc.interactions_case(kind, 4)
+    dev, ora = parity.make_pair(mesh, st)
+    for s in (dev, ora):
+        s.update_mechanics_mesh()
+        s.update_cell_neighbors()
+        s.update_cell_cell_interactions(p)
+        if kind in ("phagocytosis", "fusion"):
+            s.update_cell_cell_interactions(p)
+    flags = list(dev.get_field("flags"))
+    if kind == "attack":
+        dmg = dev.get_field("damage")
+        assert dmg[0] == 0 and dmg[1] == 0 and dmg[2] != 0
+    else:
+        assert flags == [abi.FLAG_NONE, abi.FLAG_TO_REMOVE, abi.FLAG_NONE]
+    parity.assert_same_state(dev, ora, what=kind)
+
+
+def test_container_add_and_remove():
+    mesh = abi.MeshSpec(1, (0, 0, 0), (80, 0, 0))
+    st = sc.zero_cell_state(mesh, 4, 3, 2, 4)
+    st["volume"] = [1, 2, 3, 4]
+    st["velocities"] = [10, 20, 30, 40]
+    st["positions"] = [10, 20, 30, 40]
+    st["flags"][2] = abi.FLAG_TO_REMOVE
+    d = device(mesh, st)
+    d.update_cell_container_for_mechanics(abi.Params())
+    assert list(d.get_field("volume")) == [1, 2, 4]
+    assert list(d.get_field("velocities")) == [10, 20, 40]
+    s2 = d.download()
+    s2["flags"][0] = abi.FLAG_TO_REMOVE
+    d.upload(s2)
+    d.update_cell_container_for_mechanics(abi.Params())
+    assert list(d.get_field("volume")) == [4, 2]
+    assert list(d.get_field("velocities")) == [40, 20]
+    assert d.deaths_count == 2
+
+
+# ------------------------------------------------------------------ per-phase parity
+PHASES = ["update_cell_forces", "update_motility", "update_basement_membrane_interactions",
+          "update_spring_attachments", "update_positions", "update_cell_cell_interactions",
+          "update_cell_container_for_mechanics", "update_mechanics_mesh", "update_cell_neighbors"]
+
+
+@pytest.mark.parametrize("rates", ["busy", "rare"])
+@pytest.mark.parametrize("real_bytes", [4, 8])
+@pytest.mark.parametrize("dims", [2, 3])
+def test_phase_by_phase(dims, real_bytes, rates):
+    mesh, st, p = parity.rich_population(dims, 1500 if dims == 3 else 800, 10 + dims, real_bytes, rates=rates)
+    dev, ora = parity.make_pair(mesh, st, seed=5)
+    for s in (dev, ora):
+        s.initialize_mesh()
+    parity.assert_same_buckets(dev, ora, "initial mesh")
+    parity.assert_same_neighbors(dev, ora, "initial neighbors")
+    for step in range(3):
+        for ph in PHASES:
+            getattr(dev, ph)(p)
+            getattr(ora, ph)(p)
+            what = f"step {step} after {ph}"
+            parity.assert_same_state(dev, ora, what=what)
+            if ph == "update_mechanics_mesh":
+                parity.assert_same_buckets(dev, ora, what)
+            if ph == "update_cell_neighbors":
+                parity.assert_same_neighbors(dev, ora, what)
+        # continue both from the oracle's state so rounding differences never accumulate
+        dev.upload(ora.download())
+        dev.step_index = ora.step_index
+        dev.initialize_mesh()
+
+
+@pytest.mark.parametrize("real_bytes", [4, 8])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_fused_step(dims, real_bytes):
+    """pcm_step (all phases fused into one enqueue) against the oracle's step."""
+    n = {1: 200, 2: 800, 3: 2000}[dims]
+    mesh, st, p = parity.rich_population(dims, n, 20 + dims, real_bytes, rates="busy")
+    dev, ora = parity.make_pair(mesh, st, seed=9)
+    for s in (dev, ora):
+        s.initialize_mesh()
+    for step in range(4):
+        dev.step(p, 1)
+        ora.step(p, 1)
+        parity.assert_same_state(dev, ora, what=f"fused step {step}")
+        parity.assert_same_buckets(dev, ora, f"fused step {step}")
+        parity.assert_same_neighbors(dev, ora, f"fused step {step}")
+        dev.upload(ora.download())
+        dev.step_index = ora.step_index
+        dev.initialize_mesh()
+
+
+def test_fused_equals_phases():
+    """The fused step and the per-phase entry points give bit-identical states."""
+    mesh, st, p = parity.rich_population(3, 1500, 33, 4, rates="busy")
+    a = device(mesh, st, seed=3)
+    b = device(mesh, st, seed=3)
+    for s in (a, b):
+        s.initialize_mesh()
+    for _ in range(3):
+        a.step(p, 1)
+        b.run_phases(p)
+    parity.assert_same_state(a, b, tol=0.0, what="fused vs phases")
+    parity.assert_same_neighbors(a, b, "fused vs phases")
+
+
+def test_deterministic():
+    """Two runs from the same state are bitwise identical (no schedule dependence)."""
+    mesh, st, p = parity.rich_population(3, 2000, 44, 4, rates="busy")
+    runs = []
+    for _ in range(2):
+        d = device(mesh, st, seed=1)
+        d.initialize_mesh()
+        d.step(p, 5)
+        runs.append(d)
+    parity.assert_same_state(runs[0], runs[1], tol=0.0, what="rerun")
+    parity.assert_same_neighbors(runs[0], runs[1], "rerun")
