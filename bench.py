@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Mechanics-step benchmark on one MI355X: agent-steps/s of the 88.7k-agent 3D workload.
+
+Workload (one "step" = one mechanics timestep of the whole population): 88,700 agents in
+a ball of radius 380 um inside a 4000 um box, two cell types 70/30, springs (elastic
+0.05, attach 0.01/min, detach 0.001/min, max 12), type B attacking type A (0.005/min,
+damage rate 1) and eating dead cells (0.01/min, 1% start dead), random motility 1 um/min,
+basement membrane on, dt_mech 0.1 min, FP32 real_t.  Inputs are resident in HBM before
+the timed region starts.
+
+Reports one JSON line: whole-step throughput, the HBM roofline of the dominant kernel
+(algorithmic bytes per launch / its HIP-event-timed average duration), and the oracle
+(CPU restatement of the reference solver, OpenMP) timed on the host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "agent-steps/s at 88.7k-agent 3D; ensemble agent-steps/s + p99 request latency"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public specification
+
+
+def kernel_bytes(kernel: str, d: int, r: int, T: int, k: float, s: float) -> float:
+    """Algorithmic HBM bytes per agent of one kernel launch: each agent field the kernel
+    needs is read once and each output written once; neighbor and spring partner data are
+    counted once per agent (the unique data), not once per gather."""
+    pos = d * r
+    table = {
+        # own pos, rel, radius, voxel id; every agent's voxel-sorted record (x,y,z,rel*r) and
+        # bucket slot once; voxel start/count; writes: list entries and count
+        "NEIGHBORS": pos + 2 * r + 4 + (4 * r + 4) + 8 + 4 * k + 4,
+        # forces: pos, vel (rw), radius, R, A, rel, type, affinity row, movable, list;
+        # motility: vector (rw), bias direction, bias, speed, persistence, motile flag;
+        # membrane strength; springs: count, list, elastic; writes pressure
+        "VELOCITY_FUSED": pos + 2 * pos + 4 * r + 4 + T * r + 1 + 4 + 4 * k
+        + 2 * pos + pos + 3 * r + 1 + r + 4 + 4 * s + r + r,
+        "POSITIONS": 1 + 3 * pos + 3 * pos,
+        "BIN_COUNT": pos + 8 + 8,
+        "BIN_ORDER": 4 + 8 + 4 + pos + 2 * r + 4 + 4 * r,
+        "INTERACT_TENTATIVE": 1 + 4 + 4 * k + 4 + 4 * T * r + r + 4 + 1 + 1 + 4 + 4 + 1,
+        "SPRING_COLLECT": 1 + 4 + 4 * k + 4 + r + T * r,
+    }
+    return float(table.get(kernel, 0.0))
+
+
+def time_cpu_baseline(seconds: float, threads: int, n_agents: int):
+    """Oracle in OpenMP mode (reference RNG) on the host: bounded sample of cfg2 steps."""
+    from oracle.pyoracle import OracleSim
+    from paraphysicell_amd import synth
+
+    mesh, st, params = synth.config2(4, n=n_agents)
+    ora = OracleSim(mesh, 4, st.defs, st.subs, capacity=st.count, seed=2, threads=threads, rng="mt")
+    ora.upload(st)
+    ora.initialize_mesh()
+    ora.time_steps(params, 2, 0)  # warm-up: 2 steps
+    steps, ms, agent_steps = 0, 0.0, 0.0
+    while ms < seconds * 1000.0 and steps < 600:
+        n0 = ora.count
+        chunk = 5
+        ms += ora.time_steps(params, 0, chunk)
+        agent_steps += 0.5 * (n0 + ora.count) * chunk
+        steps += chunk
+    return agent_steps / (ms / 1000.0), steps, ms
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600, help="timed mechanics steps (600 = 60 simulated minutes)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--agents", type=int, default=88700)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS or all host cores")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (every simulation here fits one "
+              "MI355X; multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from paraphysicell_amd import synth
+    from paraphysicell_amd.mechanics import DeviceSim
+
+    mesh, st, params = synth.config2(4, n=args.agents)
+    dev = DeviceSim(mesh, 4, st.defs, st.subs, capacity=st.count, seed=2)
+    dev.upload(st)
+    dev.initialize_mesh()
+
+    # warm-up steps; the kernel profile of the last few picks the dominant kernel
+    dev.step(params, max(args.warmup - 5, 0))
+    from paraphysicell_amd.mechanics import kernel_names
+
+    names = kernel_names()
+    dev.profile_begin(names)
+    dev.step(params, min(5, args.warmup))
+    warm_prof = dev.profile_end()
+    dominant = max(warm_prof, key=lambda k: warm_prof[k][0] / max(warm_prof[k][1], 1) * warm_prof[k][1])
+
+    n_before = dev.count
+    torch.cuda.synchronize()
+    dev.profile_begin([dominant])
+    t0 = time.perf_counter()
+    dev.step(params, args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = dev.profile_end()
+    n_after = dev.count
+    elapsed = t1 - t0
+    agent_steps = 0.5 * (n_before + n_after) * args.steps
+    value = agent_steps / elapsed
+
+    # mean neighbors / springs per agent at the end of the timed region
+    off, _ = dev.neighbors()
+    k_mean = float(off[-1]) / max(n_after, 1)
+    soff, _ = dev.links(0)
+    s_mean = float(soff[-1]) / max(n_after, 1)
+
+    ms_total, launches = prof[dominant]
+    avg_ms = ms_total / launches
+    bytes_per_agent = kernel_bytes(dominant, 3, 4, st.defs, k_mean, s_mean)
+    bytes_per_launch = bytes_per_agent * 0.5 * (n_before + n_after)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_agent else None
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1000.0 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D mechanics step, {args.agents} agents in a 380 um ball, 2 cell types 70/30, springs "
+                        "+ attack + dead phagocytosis, BM on, dt_mech 0.1 min (configs[2])",
+            "agents": args.agents,
+            "agents_at_end": n_after,
+            "dims": 3,
+            "real_t": "float32",
+            "mean_neighbors": round(k_mean, 2),
+            "mean_springs": round(s_mean, 2),
+        },
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None,
+            "bytes_per_agent": round(bytes_per_agent, 1),
+            "avg_launch_us": round(avg_ms * 1000.0, 2),
+            "launches": launches,
+        },
+        "kernel_share_warmup": {k: round(v[0] / max(sum(x[0] for x in warm_prof.values()), 1e-9), 3)
+                                for k, v in sorted(warm_prof.items(), key=lambda kv: -kv[1][0])[:8]},
+    }
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        threads = min(threads, 16)
+        cpu_value, cpu_steps, cpu_ms = time_cpu_baseline(args.cpu_seconds, threads, args.agents)
+        result["cpu_baseline"] = {
+            "value": round(cpu_value, 1),
+            "unit": "agent-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{cpu_steps} mechanics steps of the same workload (after 2 warm-up steps), OpenMP "
+                      f"oracle with the reference's mt19937 draws, {cpu_ms / 1000.0:.1f} s",
+            "gpu_over_cpu": round(value / cpu_value, 1),
+        }
+    print(json.dumps(result), flush=True)
+    del np
+
+
+if __name__ == "__main__":
+    main()

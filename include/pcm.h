@@ -209,6 +209,28 @@ int pcm_step(pcm_sim* sim, const pcm_step_params* params, int32_t steps);
 /* Wait for queued device work; reports PCM_ERR_STATE if a step aborted. */
 int pcm_synchronize(pcm_sim* sim);
 
+/* ---------------------------------------------------------------- kernel timing */
+/* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,
+ * simulator_measurer.h:5-33), at kernel granularity: while a profile is open, every launch
+ * of a kernel whose bit is set in `kernel_mask` is bracketed by HIP events on the handle's
+ * stream; pcm_profile_end waits for the work and returns, per kernel id, the summed event
+ * time in milliseconds and the launch count (arrays of PCM_KERNEL_COUNT entries). */
+#define PCM_KERNEL_LIST(X)                                                                                             \
+	X(BEGIN_STEP) X(BIN_CLEAR) X(BIN_COUNT) X(BIN_CLAIM) X(BIN_SCATTER) X(BIN_ORDER) X(NEIGHBORS) X(FORCES)              \
+	X(MOTILITY) X(MEMBRANE) X(CONTRACT) X(POSITIONS) X(VELOCITY_FUSED) X(SPRING_FIRE) X(SPRING_ERASE)                   \
+	X(SPRING_COMPACT) X(SPRING_COLLECT) X(SPRING_RESOLVE) X(INTERACT_TENTATIVE) X(INTERACT_CHECK)                        \
+	X(INTERACT_APPLY) X(INTERACT_SERIAL) X(COMPACT_FLAG) X(COMPACT_PLAN) X(COMPACT_MOVE) X(COMPACT_LINKS)               \
+	X(COMPACT_RESTORE) X(COMPACT_DONE)
+#define PCM_KERNEL_ENUM_(name) PCM_K_##name,
+enum pcm_kernel
+{
+	PCM_KERNEL_LIST(PCM_KERNEL_ENUM_) PCM_KERNEL_COUNT
+};
+#undef PCM_KERNEL_ENUM_
+int pcm_kernel_name(int32_t kernel, const char** name);
+int pcm_profile_begin(pcm_sim* sim, uint64_t kernel_mask);
+int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

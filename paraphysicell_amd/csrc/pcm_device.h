@@ -43,7 +43,9 @@ struct Scalars
 	int max_neighbors;     // largest neighbor count seen by the last neighbor build
 	int moves;             // agents moved by the container update
 	long long deaths;      // environment::deaths_count increments
-	int pad[4];
+	int n_occ;             // occupied mechanics voxels of the last mesh update
+	int order_valid;       // bucket[0, n) is a permutation of the current agents
+	int pad[2];
 };
 
 template <class R>
@@ -93,7 +95,7 @@ struct SimCtx
 	// neighbor lists
 	int *nbr_cnt, *nbr_idx;
 	// mechanics voxel grid
-	int *vcnt, *vstart, *vox_of, *vrank, *btmp, *bucket;
+	int *vcnt, *vstart, *vox_of, *vrank, *btmp, *bucket, *occ;
 	R* sorted; // 4 reals per bucket slot: x, y, z, rel*radius
 
 	// spring scratch
@@ -108,7 +110,73 @@ struct SimCtx
 	uint8_t* acted;          // actor reached a lock at least once
 	// compaction scratch
 	int *bad, *remap, *seq, *move_src, *move_dst;
+	// compaction copy plan: one entry per 4-byte word (or byte) of an agent's fields
+	int mv_n;
+	uint16_t mv_field[192];
+	uint16_t mv_unit[192];
+	uint8_t mv_bytes[192];
 };
+
+// index of the agent a thread handles: voxel (bucket) order when the last mesh update
+// still describes the agents, so neighboring lanes gather neighboring agents' data
+template <class R>
+__device__ __forceinline__ int agent_at(const SimCtx<R>& c, int g)
+{
+	return c.sc->order_valid ? c.bucket[g] : g;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// exclusive prefix sum over the 64 lanes of a wavefront; `total` receives the wave sum
+__device__ __forceinline__ int wave_exclusive_scan(int x, int& total)
+{
+	int v = x;
+	const int lane = lane_id();
+	for (int o = 1; o < 64; o <<= 1)
+	{
+		const int y = __shfl_up(v, o, 64);
+		if (lane >= o)
+			v += y;
+	}
+	total = __shfl(v, 63, 64);
+	return v - x;
+}
+
+// exclusive prefix sum over a block of up to 1024 threads (all threads must call it)
+__device__ __forceinline__ int block_exclusive_scan(int x, int& total)
+{
+	__shared__ int s_wave[16];
+	int wt;
+	const int pre = wave_exclusive_scan(x, wt);
+	const int w = threadIdx.x >> 6;
+	if (lane_id() == 63)
+		s_wave[w] = wt;
+	__syncthreads();
+	int off = 0, tot = 0;
+	const int nw = (blockDim.x + 63) >> 6;
+	for (int k = 0; k < nw; k++)
+	{
+		if (k < w)
+			off += s_wave[k];
+		tot += s_wave[k];
+	}
+	__syncthreads();
+	total = tot;
+	return off + pre;
+}
+
+// Reserve `count` consecutive slots of a global counter for each lane with one atomic per
+// wavefront; returns the lane's first slot.  All lanes of the wavefront must call it.
+__device__ __forceinline__ int wave_reserve(int* counter, int count)
+{
+	int total;
+	const int pre = wave_exclusive_scan(count, total);
+	int base = 0;
+	if (lane_id() == 0 && total > 0)
+		base = atomicAdd(counter, total);
+	base = __shfl(base, 0, 64);
+	return base + pre;
+}
 
 // -------------------------------------------------------------------------- helpers
 template <class R>

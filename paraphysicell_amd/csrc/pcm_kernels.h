@@ -46,7 +46,11 @@ __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs)
 	if (i < c.sc->n_binned)
 		c.vcnt[c.vox_of[i]] = 0;
 	if (i == 0)
+	{
 		c.sc->cursor = 0;
+		c.sc->n_occ = 0;
+		c.sc->order_valid = 0;
+	}
 }
 
 template <class R, int D>
@@ -64,15 +68,36 @@ __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs)
 	c.vrank[i] = atomicAdd(&c.vcnt[v], 1);
 }
 
+// Each occupied voxel's first agent claims the voxel's bucket range; claims are summed
+// per block so a block issues one atomic for the bucket cursor and one for the list of
+// occupied voxels.
 template <class R>
 __global__ void k_bin_claim(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int i = gid();
-	if (i >= c.sc->n || c.vrank[i] != 0)
-		return;
-	const int v = c.vox_of[i];
-	c.vstart[v] = atomicAdd(&c.sc->cursor, c.vcnt[v]);
+	const int n = c.sc->n;
+	int amount = 0, v = -1;
+	if (i < n && c.vrank[i] == 0)
+	{
+		v = c.vox_of[i];
+		amount = c.vcnt[v];
+	}
+	int total_amount, total_voxels;
+	const int pre_amount = block_exclusive_scan(amount, total_amount);
+	const int pre_voxels = block_exclusive_scan(amount > 0 ? 1 : 0, total_voxels);
+	__shared__ int s_base[2];
+	if (threadIdx.x == 0)
+	{
+		s_base[0] = total_amount ? atomicAdd(&c.sc->cursor, total_amount) : 0;
+		s_base[1] = total_voxels ? atomicAdd(&c.sc->n_occ, total_voxels) : 0;
+	}
+	__syncthreads();
+	if (amount > 0)
+	{
+		c.vstart[v] = s_base[0] + pre_amount;
+		c.occ[s_base[1] + pre_voxels] = v;
+	}
 }
 
 template <class R>
@@ -103,6 +128,8 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 	for (int k = 0; k < 3; k++)
 		out[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
 	out[3] = c.rel_adh[i] * c.radius[i];
+	if (i == 0)
+		c.sc->order_valid = 1;
 }
 
 // ------------------------------------------------------------------ neighbor lists
@@ -111,19 +138,25 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 // (common_solver.h:77-105): z outer, y, x inner, each bucket in bucket order, self
 // skipped.  Lists are built for every agent; a non-movable agent's list only feeds
 // simple_pressure and is reported empty.
-template <class R, int D>
-__global__ void k_neighbors(const SimCtx<R>* __restrict__ ctxs)
+//
+// One wavefront per occupied voxel: the voxel's 3^d stencil buckets are staged once in
+// LDS in stencil order, then every agent of the voxel tests 64 candidates per iteration;
+// a ballot and a prefix popcount write the passing candidates in candidate order.
+constexpr int NBR_TILE = 1024;
+
+// One agent's list straight from HBM.  DIRECT: the buckets may predate the current
+// positions, so the stencil is centred on the agent's current voxel and candidates are
+// tested with their current positions and radii (what the reference does with stale
+// buckets); otherwise the voxel-sorted copy made by the mesh update is used.
+template <class R, int D, bool DIRECT>
+__device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
 {
-	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n)
-		return;
 	R pi[3] = { 0, 0, 0 };
 	for (int k = 0; k < D; k++)
 		pi[k] = c.pos[(size_t)i * D + k];
 	const R mi = c.rel_adh[i] * c.radius[i];
 	int co[3];
-	voxel_coords<R>(c, c.vox_of[i], co);
+	voxel_coords<R>(c, DIRECT ? voxel_index<R>(c, c.pos + (size_t)i * D) : c.vox_of[i], co);
 	int* out = c.nbr_idx + (size_t)i * c.K;
 	int cnt = 0;
 	for (int z = -1; z <= 1; z++)
@@ -148,7 +181,16 @@ __global__ void k_neighbors(const SimCtx<R>* __restrict__ ctxs)
 					const int j = c.bucket[k];
 					if (j == i)
 						continue;
-					const R* q = c.sorted + (size_t)k * 4;
+					R q[4] = { 0, 0, 0, 0 };
+					if (DIRECT)
+					{
+						for (int d = 0; d < D; d++)
+							q[d] = c.pos[(size_t)j * D + d];
+						q[3] = c.rel_adh[j] * c.radius[j];
+					}
+					else
+						for (int d = 0; d < 4; d++)
+							q[d] = c.sorted[(size_t)k * 4 + d];
 					const R dist = pair_distance<D, R>(pi, q);
 					if (dist <= mi + q[3])
 					{
@@ -167,6 +209,120 @@ __global__ void k_neighbors(const SimCtx<R>* __restrict__ ctxs)
 		cnt = c.K;
 	}
 	c.nbr_cnt[i] = cnt;
+}
+
+template <class R, int D>
+__global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	__shared__ R s_rec[NBR_TILE * 4];
+	__shared__ int s_idx[NBR_TILE];
+	__shared__ int s_pre[28];
+	__shared__ int s_vs[27];
+	const int lane = threadIdx.x;
+	const int n_occ = c.sc->n_occ;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	for (int w = blockIdx.x; w < n_occ; w += gridDim.x)
+	{
+		const int v = c.occ[w];
+		int co[3];
+		voxel_coords<R>(c, v, co);
+		int cnt = 0, vs = 0;
+		if (lane < 27)
+		{
+			const int x = co[0] + lane % 3 - 1, y = co[1] + (lane / 3) % 3 - 1, z = co[2] + lane / 9 - 1;
+			if (x >= 0 && x < c.grid[0] && y >= 0 && y < c.grid[1] && z >= 0 && z < c.grid[2])
+			{
+				const int nv = x + c.grid[0] * (y + c.grid[1] * z);
+				cnt = c.vcnt[nv];
+				vs = cnt ? c.vstart[nv] : 0;
+			}
+		}
+		int C;
+		const int pre = wave_exclusive_scan(cnt, C);
+		if (lane < 27)
+		{
+			s_pre[lane] = pre;
+			s_vs[lane] = vs;
+		}
+		if (lane == 0)
+			s_pre[27] = C;
+		__syncthreads();
+		const int m = c.vcnt[v], s0 = c.vstart[v];
+		if (C <= NBR_TILE)
+		{
+			for (int t = lane; t < C; t += 64)
+			{
+				int sv = 0;
+				while (s_pre[sv + 1] <= t)
+					sv++;
+				const int g = s_vs[sv] + t - s_pre[sv];
+				s_idx[t] = c.bucket[g];
+				for (int k = 0; k < 4; k++)
+					s_rec[t * 4 + k] = c.sorted[(size_t)g * 4 + k];
+			}
+			__syncthreads();
+			for (int a = 0; a < m; a++)
+			{
+				const int i = c.bucket[s0 + a];
+				R pi[4];
+				for (int k = 0; k < 4; k++)
+					pi[k] = c.sorted[(size_t)(s0 + a) * 4 + k];
+				int* out = c.nbr_idx + (size_t)i * c.K;
+				int base = 0;
+				for (int t0 = 0; t0 < C; t0 += 64)
+				{
+					const int t = t0 + lane;
+					bool pass = false;
+					int j = -1;
+					if (t < C)
+					{
+						j = s_idx[t];
+						if (j != i)
+						{
+							const R* q = s_rec + t * 4;
+							pass = pair_distance<D, R>(pi, q) <= pi[3] + q[3];
+						}
+					}
+					const unsigned long long mask = __ballot(pass);
+					if (pass)
+					{
+						const int slot = base + __popcll(mask & below);
+						if (slot < c.K)
+							out[slot] = j;
+					}
+					base += __popcll(mask);
+				}
+				if (lane == 0)
+				{
+					if (base > c.K)
+					{
+						atomicOr(&c.sc->status, ST_NEIGHBOR_OVERFLOW);
+						atomicMax(&c.sc->max_neighbors, base);
+						base = c.K;
+					}
+					c.nbr_cnt[i] = base;
+				}
+			}
+		}
+		else
+		{
+			// stencil larger than the LDS tile: one lane per agent, straight from HBM
+			for (int a = lane; a < m; a += 64)
+				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
+		}
+		__syncthreads();
+	}
+}
+
+template <class R, int D>
+__global__ void k_neighbors_direct(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n)
+		return;
+	neighbors_one_agent<R, D, true>(c, i);
 }
 
 // number of neighbors of i the reference would list (empty for non-movable agents)
@@ -205,9 +361,10 @@ template <class R, int D>
 __global__ void k_forces(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n)
+	const int g = gid();
+	if (g >= c.sc->n)
 		return;
+	const int i = agent_at<R>(c, g);
 	R pi[D], v[D];
 	for (int k = 0; k < D; k++)
 	{
@@ -297,9 +454,10 @@ template <class R, int D>
 __global__ void k_velocity_fused(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n)
+	const int g = gid();
+	if (g >= c.sc->n)
 		return;
+	const int i = agent_at<R>(c, g);
 	const bool mov = c.movable[i] != 0;
 	R p[D], v[D];
 	for (int k = 0; k < D; k++)
@@ -431,40 +589,68 @@ __global__ void k_spring_compact(const SimCtx<R>* __restrict__ ctxs)
 }
 
 // Attachment candidates: every (i, slot) with j = neighbor > i whose draws succeed.
+// Events are buffered per lane and appended with one atomic per wavefront.
+constexpr int LANE_BUF = 8;
+
 template <class R>
 __global__ void k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n)
-		return;
-	const int cnt = visible_neighbors<R>(c, i);
-	if (cnt == 0)
-		return;
-	const int* nb = c.nbr_idx + (size_t)i * c.K;
-	const int ti = c.type[i];
-	const R rate_i = c.att_rate[i] * c.dt;
-	const unsigned long long step = c.sc->step;
-	for (int s = 0; s < cnt; s++)
+	const int g = gid();
+	int buf_s[LANE_BUF], buf_j[LANE_BUF];
+	int nb_ev = 0;
+	int i = -1;
+	if (g < c.sc->n)
 	{
-		const int j = nb[s];
-		if (j < i)
-			continue;
-		const R p_l = rate_i * c.affinity[(size_t)i * c.T + c.type[j]];
-		const R p_r = c.att_rate[j] * c.dt * c.affinity[(size_t)j * c.T + ti];
-		const pcm_u32x4 b = pcm_block(c.seed, step, PCM_STREAM_ATTACH, (uint32_t)i, (uint32_t)s, 0u);
-		if (pcm_draw<R>::from(b, 0) <= p_l || pcm_draw<R>::from(b, 1) <= p_r)
+		i = agent_at<R>(c, g);
+		const int cnt = visible_neighbors<R>(c, i);
+		const int* nb = c.nbr_idx + (size_t)i * c.K;
+		const int ti = c.type[i];
+		const R rate_i = c.att_rate[i] * c.dt;
+		const unsigned long long step = c.sc->step;
+		for (int s = 0; s < cnt; s++)
 		{
-			const int e = atomicAdd(&c.sc->att_events, 1);
-			if (e < c.ev_cap)
+			const int j = nb[s];
+			if (j < i)
+				continue;
+			const R p_l = rate_i * c.affinity[(size_t)i * c.T + c.type[j]];
+			const R p_r = c.att_rate[j] * c.dt * c.affinity[(size_t)j * c.T + ti];
+			const pcm_u32x4 b = pcm_block(c.seed, step, PCM_STREAM_ATTACH, (uint32_t)i, (uint32_t)s, 0u);
+			if (pcm_draw<R>::from(b, 0) <= p_l || pcm_draw<R>::from(b, 1) <= p_r)
 			{
-				c.ev[3 * e] = i;
-				c.ev[3 * e + 1] = s;
-				c.ev[3 * e + 2] = j;
+				if (nb_ev < LANE_BUF)
+				{
+					buf_s[nb_ev] = s;
+					buf_j[nb_ev] = j;
+					nb_ev++;
+				}
+				else
+				{
+					const int e = atomicAdd(&c.sc->att_events, 1);
+					if (e < c.ev_cap)
+					{
+						c.ev[3 * e] = i;
+						c.ev[3 * e + 1] = s;
+						c.ev[3 * e + 2] = j;
+					}
+					else
+						atomicOr(&c.sc->status, ST_CAPACITY);
+				}
 			}
-			else
-				atomicOr(&c.sc->status, ST_CAPACITY);
 		}
+	}
+	const int base = wave_reserve(&c.sc->att_events, nb_ev);
+	for (int k = 0; k < nb_ev; k++)
+	{
+		const int e = base + k;
+		if (e < c.ev_cap)
+		{
+			c.ev[3 * e] = i;
+			c.ev[3 * e + 1] = buf_s[k];
+			c.ev[3 * e + 2] = buf_j[k];
+		}
+		else
+			atomicOr(&c.sc->status, ST_CAPACITY);
 	}
 }
 
@@ -557,6 +743,10 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 	__shared__ unsigned long long nkey[2 * ATT_CHUNK];
 	__shared__ int npos[2 * ATT_CHUNK];
 	__shared__ uint8_t est[ATT_CHUNK];
+	__shared__ int s_base[2 * ATT_CHUNK];
+	__shared__ int s_max[2 * ATT_CHUNK];
+	__shared__ int s_q0[ATT_CHUNK];
+	__shared__ int s_q1[ATT_CHUNK];
 	__shared__ int s_count, s_unknown, s_red;
 
 	if (c.sc->status)
@@ -607,7 +797,20 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 		}
 		__syncthreads();
 		block_sort_pairs(nkey, npos, 2 * m);
-		// npos[q] now holds the (2 * rank + side) tag of sorted entry q
+		// npos[q] now holds the (2 * rank + side) tag of sorted entry q.  Stage each
+		// endpoint's spring count and maximum, and each event's two entry positions.
+		for (int q = threadIdx.x; q < 2 * m; q += blockDim.x)
+		{
+			const int tag = npos[q];
+			const int agent = (int)(nkey[q] >> 32);
+			s_base[q] = spr_cnt[agent];
+			s_max[q] = c.max_att[agent];
+			if (tag & 1)
+				s_q1[tag >> 1] = q;
+			else
+				s_q0[tag >> 1] = q;
+		}
+		__syncthreads();
 		// rounds: each decides at least the earliest undecided event, so m + 1 rounds suffice
 		for (int round = 0;; round++)
 		{
@@ -620,44 +823,23 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 			if (threadIdx.x == 0)
 				s_unknown = 0;
 			__syncthreads();
-			for (int q = threadIdx.x; q < 2 * m; q += blockDim.x)
+			for (int r = threadIdx.x; r < m; r += blockDim.x)
 			{
-				// each entry evaluates its endpoint; decisions are taken by side-0 entries of
-				// the pair after both sides are known: entries of side 0 drive the event
-				const int tag = npos[q];
-				const int r = tag >> 1;
-				if ((tag & 1) != 0 || est[r] != 0)
+				if (est[r] != 0)
 					continue;
-				// locate the partner entry (side 1) by scanning the target's segment
 				int cond[2];
 				for (int side = 0; side < 2; side++)
 				{
-					int qq = q;
-					if (side == 1)
-					{
-						const unsigned long long want = ((unsigned long long)etgt[r] << 32) | (unsigned)r;
-						// binary search in the sorted entry keys
-						int a = 0, b = 2 * m - 1;
-						while (a < b)
-						{
-							const int mid = (a + b) >> 1;
-							if (nkey[mid] < want)
-								a = mid + 1;
-							else
-								b = mid;
-						}
-						qq = a;
-					}
-					const int agent = (int)(nkey[qq] >> 32);
-					int lo_c = spr_cnt[agent], hi_c = lo_c;
-					const int maxa = c.max_att[agent];
-					for (int w = qq - 1; w >= 0 && (int)(nkey[w] >> 32) == agent; w--)
+					const int q = side ? s_q1[r] : s_q0[r];
+					const unsigned agent = (unsigned)(nkey[q] >> 32);
+					int lo_c = s_base[q], hi_c = lo_c;
+					for (int w = q - 1; w >= 0 && (unsigned)(nkey[w] >> 32) == agent; w--)
 					{
 						const int rs = est[(int)(nkey[w] & 0xffffffffu)];
 						lo_c += rs == 1;
 						hi_c += rs != 2;
 					}
-					cond[side] = lo_c >= maxa ? 2 : (hi_c < maxa ? 1 : 0);
+					cond[side] = lo_c >= s_max[q] ? 2 : (hi_c < s_max[q] ? 1 : 0);
 				}
 				if (cond[0] == 2 || cond[1] == 2)
 					est[r] = 2;
@@ -678,7 +860,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 			const int agent = (int)(nkey[q] >> 32);
 			if (q > 0 && (int)(nkey[q - 1] >> 32) == agent)
 				continue; // not a segment head
-			int cnt = spr_cnt[agent];
+			int cnt = s_base[q];
 			for (int w = q; w < 2 * m && (int)(nkey[w] >> 32) == agent; w++)
 			{
 				const int r = (int)(nkey[w] & 0xffffffffu);
@@ -723,13 +905,16 @@ template <class R>
 __global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n)
-		return;
-	c.atk_tgt[i] = -1;
+	const int g = gid();
+	const bool live_thread = g < c.sc->n;
+	const int i = live_thread ? agent_at<R>(c, g) : 0;
+	int buf_s[LANE_BUF];
+	int n_cand = 0;
 	int n_rem = 0;
 	bool acted = false;
-	if (c.dead[i] != 1)
+	if (live_thread)
+		c.atk_tgt[i] = -1;
+	if (live_thread && c.dead[i] != 1)
 	{
 		const int cnt = visible_neighbors<R>(c, i);
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
@@ -751,14 +936,19 @@ __global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 				dead_j ? (u < c.dead_phago[i] * dt) : (u < p_live || u < p_attack || u < p_fuse);
 			if (!candidate)
 				continue;
-			const int k = atomicAdd(&c.sc->cand_count, 1);
-			if (k < c.cand_cap)
-			{
-				c.cand[2 * k] = i;
-				c.cand[2 * k + 1] = s;
-			}
+			if (n_cand < LANE_BUF)
+				buf_s[n_cand++] = s;
 			else
-				atomicOr(&c.sc->status, ST_CAPACITY);
+			{
+				const int k = atomicAdd(&c.sc->cand_count, 1);
+				if (k < c.cand_cap)
+				{
+					c.cand[2 * k] = i;
+					c.cand[2 * k + 1] = s;
+				}
+				else
+					atomicOr(&c.sc->status, ST_CAPACITY);
+			}
 
 			const bool target_removed = c.flags[j] == PCM_FLAG_TO_REMOVE;
 			int kind = -1; // -1 none, 0 ingest, 1 attack, 2 fuse
@@ -799,8 +989,22 @@ __global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 			n_rem++;
 		}
 	}
-	c.rem_cnt[i] = n_rem;
-	c.acted[i] = acted;
+	const int base = wave_reserve(&c.sc->cand_count, n_cand);
+	for (int k = 0; k < n_cand; k++)
+	{
+		if (base + k < c.cand_cap)
+		{
+			c.cand[2 * (base + k)] = i;
+			c.cand[2 * (base + k) + 1] = buf_s[k];
+		}
+		else
+			atomicOr(&c.sc->status, ST_CAPACITY);
+	}
+	if (live_thread)
+	{
+		c.rem_cnt[i] = n_rem;
+		c.acted[i] = acted;
+	}
 }
 
 template <class R>
@@ -1131,34 +1335,51 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R
 	c.sc->moves = moves;
 	c.sc->n = n1;
 	c.sc->deaths += deaths;
+	c.sc->order_valid = 0;
 }
 
-// move every field (and the list rows) of each surviving tail agent into its hole
+// move every field (and the list rows) of each surviving tail agent into its hole: one
+// wavefront per move, its lanes spread over the words of the copy plan and the rows
 template <class R>
 __global__ void k_compact_move(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int k = gid();
-	if (k >= c.sc->moves)
-		return;
-	const int src = c.move_src[k], dst = c.move_dst[k];
-	for (int f = 0; f < PCM_FIELD_COUNT; f++)
+	const int lane = lane_id();
+	const int waves = gridDim.x * (blockDim.x >> 6);
+	const int moves = c.sc->moves;
+	for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
 	{
-		const int nb = c.fbytes[f];
-		const uint8_t* s = (const uint8_t*)c.fptr[f] + (size_t)src * nb;
-		uint8_t* d = (uint8_t*)c.fptr[f] + (size_t)dst * nb;
-		for (int b = 0; b < nb; b++)
-			d[b] = s[b];
+		const int src = c.move_src[k], dst = c.move_dst[k];
+		for (int u = lane; u < c.mv_n; u += 64)
+		{
+			const int f = c.mv_field[u];
+			const int nb = c.fbytes[f];
+			if (c.mv_bytes[u] == 4)
+			{
+				const uint32_t* sp = (const uint32_t*)((const uint8_t*)c.fptr[f] + (size_t)src * nb);
+				uint32_t* dp = (uint32_t*)((uint8_t*)c.fptr[f] + (size_t)dst * nb);
+				dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
+			}
+			else
+			{
+				const uint8_t* sp = (const uint8_t*)c.fptr[f] + (size_t)src * nb;
+				uint8_t* dp = (uint8_t*)c.fptr[f] + (size_t)dst * nb;
+				dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
+			}
+		}
+		for (int t = lane; t < c.L; t += 64)
+			c.spr_idx[(size_t)dst * c.L + t] = c.spr_idx[(size_t)src * c.L + t];
+		for (int t = lane; t < c.LA; t += 64)
+			c.atc_idx[(size_t)dst * c.LA + t] = c.atc_idx[(size_t)src * c.LA + t];
+		for (int t = lane; t < c.K; t += 64)
+			c.nbr_idx[(size_t)dst * c.K + t] = c.nbr_idx[(size_t)src * c.K + t];
+		if (lane == 0)
+		{
+			c.spr_cnt[dst] = c.spr_cnt[src];
+			c.atc_cnt[dst] = c.atc_cnt[src];
+			c.nbr_cnt[dst] = c.nbr_cnt[src];
+		}
 	}
-	c.spr_cnt[dst] = c.spr_cnt[src];
-	for (int s = 0; s < c.L; s++)
-		c.spr_idx[(size_t)dst * c.L + s] = c.spr_idx[(size_t)src * c.L + s];
-	c.atc_cnt[dst] = c.atc_cnt[src];
-	for (int s = 0; s < c.LA; s++)
-		c.atc_idx[(size_t)dst * c.LA + s] = c.atc_idx[(size_t)src * c.LA + s];
-	c.nbr_cnt[dst] = c.nbr_cnt[src];
-	for (int s = 0; s < c.K; s++)
-		c.nbr_idx[(size_t)dst * c.K + s] = c.nbr_idx[(size_t)src * c.K + s];
 }
 
 // remove_attached (swap with back) for removed partners in removal order, then

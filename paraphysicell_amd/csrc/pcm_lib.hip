@@ -82,6 +82,8 @@ struct pcm_sim
 	virtual int phase(int which, const pcm_step_params* p) = 0;
 	virtual int step(const pcm_step_params* p, int steps) = 0;
 	virtual int synchronize() = 0;
+	virtual int profile_begin(uint64_t mask) = 0;
+	virtual int profile_end(double* total_ms, int64_t* launches) = 0;
 };
 
 namespace {
@@ -112,6 +114,13 @@ public:
 			(void)hipStreamSynchronize(stream_);
 			(void)hipStreamDestroy(stream_);
 		}
+		for (auto& r : prof_records_)
+		{
+			(void)hipEventDestroy(r.start);
+			(void)hipEventDestroy(r.stop);
+		}
+		for (hipEvent_t e : event_pool_)
+			(void)hipEventDestroy(e);
 		for (void* p : allocations_)
 			(void)hipFree(p);
 	}
@@ -160,6 +169,23 @@ public:
 			PCM_TRY(alloc(&h.fptr[f], (size_t)cap * std::max(1, h.fbytes[f])));
 		}
 		bind_fields();
+		// copy plan of the compaction move: every 4-byte word of the word-sized fields, every
+		// byte of the byte-sized ones
+		h.mv_n = 0;
+		for (int f = 0; f < PCM_FIELD_COUNT; f++)
+		{
+			const bool bytes = k_fields[f].type == PCM_T_U8;
+			const int units = bytes ? h.fbytes[f] : h.fbytes[f] / 4;
+			for (int u = 0; u < units; u++)
+			{
+				if (h.mv_n >= (int)(sizeof(h.mv_field) / sizeof(h.mv_field[0])))
+					return fail(PCM_ERR_UNSUPPORTED, "too many cell definitions or substrates for the copy plan");
+				h.mv_field[h.mv_n] = (uint16_t)f;
+				h.mv_unit[h.mv_n] = (uint16_t)u;
+				h.mv_bytes[h.mv_n] = bytes ? 1 : 4;
+				h.mv_n++;
+			}
+		}
 
 		PCM_TRY(alloc_t(&h.sc, 1));
 		PCM_TRY(alloc_t(&h.spr_cnt, cap));
@@ -174,6 +200,7 @@ public:
 		PCM_TRY(alloc_t(&h.vrank, cap));
 		PCM_TRY(alloc_t(&h.btmp, cap));
 		PCM_TRY(alloc_t(&h.bucket, cap));
+		PCM_TRY(alloc_t(&h.occ, cap));
 		PCM_TRY(alloc_t(&h.sorted, (size_t)cap * 4));
 		PCM_TRY(alloc_t(&h.fire, cap));
 		PCM_TRY(alloc_t(&h.erase, cap));
@@ -215,8 +242,10 @@ public:
 		s.conflict = 0;
 		s.bad_count = 0;
 		s.moves = 0;
+		s.order_valid = 0;
 		PCM_HIP(hipMemcpy(h_.sc, &s, sizeof(s), hipMemcpyHostToDevice));
 		n_launch_ = n;
+		mesh_fresh_ = false;
 		std::vector<int> ident(std::max(n, 1));
 		for (int i = 0; i < n; i++)
 			ident[i] = i;
@@ -255,6 +284,8 @@ public:
 				PCM_TRY(grow_springs(mx));
 		}
 		PCM_HIP(hipMemcpy(h_.fptr[f], src, (size_t)n * h_.fbytes[f], hipMemcpyHostToDevice));
+		if (f == PCM_F_POSITIONS || f == PCM_F_RADIUS || f == PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE)
+			mesh_fresh_ = false;
 		return PCM_OK;
 	}
 
@@ -474,6 +505,7 @@ private:
 	hipStream_t stream_ = nullptr;
 	std::vector<void*> allocations_;
 	int n_launch_ = 0; // agents at the last upload: an upper bound of the device count
+	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
 
 	int width(int kind) const
 	{
@@ -675,17 +707,107 @@ private:
 	int grid_agents() const { return std::max(1, (n_launch_ + BLOCK - 1) / BLOCK); }
 
 	template <class Kern>
-	int launch(Kern kernel, int blocks, int threads)
+	int launch(int tag, Kern kernel, int blocks, int threads)
 	{
+		const bool timed = (prof_mask_ >> tag) & 1ull;
+		hipEvent_t ev[2] = { nullptr, nullptr };
+		if (timed)
+		{
+			PCM_TRY(take_events(ev));
+			PCM_HIP(hipEventRecord(ev[0], stream_));
+		}
 		hipLaunchKernelGGL(kernel, dim3(blocks, 1), dim3(threads), 0, stream_, (const SimCtx<R>*)d_);
 		PCM_HIP(hipGetLastError());
+		if (timed)
+		{
+			PCM_HIP(hipEventRecord(ev[1], stream_));
+			prof_records_.push_back({ tag, ev[0], ev[1] });
+		}
 		return PCM_OK;
 	}
 
 	template <class Kern>
-	int per_agent(Kern kernel)
+	int per_agent(int tag, Kern kernel)
 	{
-		return launch(kernel, grid_agents(), BLOCK);
+		return launch(tag, kernel, grid_agents(), BLOCK);
+	}
+
+	struct prof_record
+	{
+		int tag;
+		hipEvent_t start, stop;
+	};
+	uint64_t prof_mask_ = 0;
+	std::vector<prof_record> prof_records_;
+	std::vector<hipEvent_t> event_pool_;
+
+	int take_events(hipEvent_t* ev)
+	{
+		for (int k = 0; k < 2; k++)
+		{
+			if (!event_pool_.empty())
+			{
+				ev[k] = event_pool_.back();
+				event_pool_.pop_back();
+			}
+			else
+				PCM_HIP(hipEventCreate(&ev[k]));
+		}
+		return PCM_OK;
+	}
+
+public:
+	int profile_begin(uint64_t mask) override
+	{
+		PCM_HIP(hipStreamSynchronize(stream_));
+		for (auto& r : prof_records_)
+		{
+			event_pool_.push_back(r.start);
+			event_pool_.push_back(r.stop);
+		}
+		prof_records_.clear();
+		prof_mask_ = mask;
+		return PCM_OK;
+	}
+
+	int profile_end(double* total_ms, int64_t* launches) override
+	{
+		PCM_HIP(hipStreamSynchronize(stream_));
+		for (int k = 0; k < PCM_KERNEL_COUNT; k++)
+		{
+			if (total_ms)
+				total_ms[k] = 0;
+			if (launches)
+				launches[k] = 0;
+		}
+		for (auto& r : prof_records_)
+		{
+			float ms = 0;
+			PCM_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+			if (total_ms)
+				total_ms[r.tag] += ms;
+			if (launches)
+				launches[r.tag]++;
+			event_pool_.push_back(r.start);
+			event_pool_.push_back(r.stop);
+		}
+		prof_records_.clear();
+		prof_mask_ = 0;
+		return PCM_OK;
+	}
+
+private:
+
+	int neighbor_blocks() const { return std::max(1, std::min(n_launch_, 16384)); }
+
+	template <int D>
+	int neighbors_d()
+	{
+		if (mesh_fresh_)
+			return launch(PCM_K_NEIGHBORS, k_neighbors<R, D>, neighbor_blocks(), 64);
+		// the voxel buckets predate position changes: search each agent's stencil around
+		// its current position, as the reference does with stale buckets
+		return per_agent(PCM_K_NEIGHBORS, k_neighbors_direct<R, D>);
 	}
 
 	int launch_neighbors()
@@ -693,50 +815,51 @@ private:
 		switch (h_.dims)
 		{
 			case 1:
-				return per_agent(k_neighbors<R, 1>);
+				return neighbors_d<1>();
 			case 2:
-				return per_agent(k_neighbors<R, 2>);
+				return neighbors_d<2>();
 			default:
-				return per_agent(k_neighbors<R, 3>);
+				return neighbors_d<3>();
 		}
 	}
 
 	template <int D>
 	int mesh_kernels()
 	{
-		PCM_TRY(per_agent(k_bin_clear<R>));
-		PCM_TRY(per_agent(k_bin_count<R, D>));
-		PCM_TRY(per_agent(k_bin_claim<R>));
-		PCM_TRY(per_agent(k_bin_scatter<R>));
-		return per_agent(k_bin_order<R, D>);
+		mesh_fresh_ = true;
+		PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
+		PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D>));
+		PCM_TRY(per_agent(PCM_K_BIN_CLAIM, k_bin_claim<R>));
+		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R>));
+		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 	}
 
 	int spring_update_kernels()
 	{
-		PCM_TRY(per_agent(k_spring_fire<R>));
-		PCM_TRY(per_agent(k_spring_erase<R>));
-		PCM_TRY(per_agent(k_spring_compact<R>));
-		PCM_TRY(per_agent(k_spring_collect<R>));
-		return launch(k_spring_resolve<R>, 1, RESOLVE_THREADS);
+		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R>));
+		PCM_TRY(per_agent(PCM_K_SPRING_ERASE, k_spring_erase<R>));
+		PCM_TRY(per_agent(PCM_K_SPRING_COMPACT, k_spring_compact<R>));
+		PCM_TRY(per_agent(PCM_K_SPRING_COLLECT, k_spring_collect<R>));
+		return launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS);
 	}
 
 	int interaction_kernels()
 	{
-		PCM_TRY(per_agent(k_interact_tentative<R>));
-		PCM_TRY(per_agent(k_interact_check<R>));
-		PCM_TRY(per_agent(k_interact_apply<R>));
-		return launch(k_interact_serial<R>, 1, RESOLVE_THREADS);
+		PCM_TRY(per_agent(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R>));
+		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
+		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R>));
+		return launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R>, 1, RESOLVE_THREADS);
 	}
 
 	template <int D>
 	int container_kernels()
 	{
-		PCM_TRY(per_agent(k_compact_flag<R, D>));
-		PCM_TRY(launch(k_compact_plan<R>, 1, RESOLVE_THREADS));
-		PCM_TRY(per_agent(k_compact_move<R>));
-		PCM_TRY(per_agent(k_compact_links<R>));
-		PCM_TRY(per_agent(k_compact_restore<R>));
-		return launch(k_compact_done<R>, 1, 64);
+		PCM_TRY(per_agent(PCM_K_COMPACT_FLAG, k_compact_flag<R, D>));
+		PCM_TRY(launch(PCM_K_COMPACT_PLAN, k_compact_plan<R>, 1, RESOLVE_THREADS));
+		PCM_TRY(launch(PCM_K_COMPACT_MOVE, k_compact_move<R>, 64, BLOCK));
+		PCM_TRY(per_agent(PCM_K_COMPACT_LINKS, k_compact_links<R>));
+		PCM_TRY(per_agent(PCM_K_COMPACT_RESTORE, k_compact_restore<R>));
+		return launch(PCM_K_COMPACT_DONE, k_compact_done<R>, 1, 64);
 	}
 
 	template <int D>
@@ -748,31 +871,34 @@ private:
 				return mesh_kernels<D>();
 			case PH_NEIGHBORS:
 			{
-				PCM_TRY(per_agent(k_neighbors<R, D>));
+				PCM_TRY(neighbors_d<D>());
 				PCM_TRY(read_scalars());
 				if (sc_.status & ST_NEIGHBOR_OVERFLOW)
 					PCM_TRY(recover());
 				return PCM_OK;
 			}
 			case PH_FORCES:
-				PCM_TRY(launch(k_begin_step<R>, 1, 64));
-				return per_agent(k_forces<R, D>);
+				PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
+				return per_agent(PCM_K_FORCES, k_forces<R, D>);
 			case PH_MOTILITY:
-				return per_agent(k_motility<R, D>);
+				return per_agent(PCM_K_MOTILITY, k_motility<R, D>);
 			case PH_MEMBRANE:
 				if (!h_.wall)
 					return PCM_OK;
-				return per_agent(k_membrane<R, D>);
+				return per_agent(PCM_K_MEMBRANE, k_membrane<R, D>);
 			case PH_SPRINGS:
 				if (!h_.springs)
 					return PCM_OK;
 				PCM_TRY(spring_update_kernels());
-				return per_agent(k_contract<R, D>);
+				return per_agent(PCM_K_CONTRACT, k_contract<R, D>);
 			case PH_POSITIONS:
-				return per_agent(k_positions<R, D>);
+				mesh_fresh_ = false;
+				return per_agent(PCM_K_POSITIONS, k_positions<R, D>);
 			case PH_INTERACTIONS:
+				mesh_fresh_ = false; // fusion moves agents, ingestion changes radii
 				return interaction_kernels();
 			case PH_CONTAINER:
+				mesh_fresh_ = false;
 				return container_kernels<D>();
 		}
 		return fail(PCM_ERR_ARGUMENT, "unknown phase");
@@ -784,15 +910,15 @@ private:
 	template <int D>
 	int step_d()
 	{
-		PCM_TRY(launch(k_begin_step<R>, 1, 64));
+		PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 		if (h_.springs)
 			PCM_TRY(spring_update_kernels());
-		PCM_TRY(per_agent(k_velocity_fused<R, D>));
-		PCM_TRY(per_agent(k_positions<R, D>));
+		PCM_TRY(per_agent(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D>));
+		PCM_TRY(per_agent(PCM_K_POSITIONS, k_positions<R, D>));
 		PCM_TRY(interaction_kernels());
 		PCM_TRY(container_kernels<D>());
 		PCM_TRY(mesh_kernels<D>());
-		return per_agent(k_neighbors<R, D>);
+		return neighbors_d<D>();
 	}
 
 	int dispatch_step()
@@ -1014,6 +1140,29 @@ int pcm_synchronize(pcm_sim* sim)
 {
 	PCM_NEED(sim);
 	return sim->synchronize();
+}
+
+int pcm_kernel_name(int32_t kernel, const char** name)
+{
+#define PCM_KERNEL_NAME_(n) #n,
+	static const char* names[PCM_KERNEL_COUNT] = { PCM_KERNEL_LIST(PCM_KERNEL_NAME_) };
+#undef PCM_KERNEL_NAME_
+	if (kernel < 0 || kernel >= PCM_KERNEL_COUNT || !name)
+		return fail(PCM_ERR_ARGUMENT, "bad kernel id");
+	*name = names[kernel];
+	return PCM_OK;
+}
+
+int pcm_profile_begin(pcm_sim* sim, uint64_t kernel_mask)
+{
+	PCM_NEED(sim);
+	return sim->profile_begin(kernel_mask);
+}
+
+int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches)
+{
+	PCM_NEED(sim);
+	return sim->profile_end(total_ms, launches);
 }
 
 } // extern "C"

@@ -176,7 +176,7 @@ class _LibSim:
         v = ctypes.c_int64()
         self._call("get_voxel_count", self._h, ctypes.byref(v))
         counts = np.zeros(v.value, np.int32)
-        agents = np.zeros(max(self.count, 1), np.int32)
+        agents = np.zeros(max(self.capacity, self.count, 1), np.int32)  # last binning may predate removals
         self._call("get_voxel_buckets", self._h, counts.ctypes.data, agents.ctypes.data)
         return counts, agents[: int(counts.sum())]
 
@@ -268,6 +268,12 @@ def load_library(path: str | None = None):
                                    ctypes.POINTER(ctypes.c_char_p)]
     lib.pcm_field_info.restype = ctypes.c_int
     lib.pcm_version.restype = ctypes.c_int
+    lib.pcm_kernel_name.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p)]
+    lib.pcm_kernel_name.restype = ctypes.c_int
+    lib.pcm_profile_begin.argtypes = [_PTR, ctypes.c_uint64]
+    lib.pcm_profile_begin.restype = ctypes.c_int
+    lib.pcm_profile_end.argtypes = [_PTR, ctypes.c_void_p, ctypes.c_void_p]
+    lib.pcm_profile_end.restype = ctypes.c_int
     if path is None:
         _LIB = lib
     return lib
@@ -284,6 +290,34 @@ class DeviceSim(_LibSim):
 
     def synchronize(self):
         self._call("synchronize", self._h)
+
+    # ------------------------------------------------------------------ kernel timing
+    def profile_begin(self, kernels) -> None:
+        """Bracket every launch of the named kernels (see kernel_names()) with HIP events."""
+        names = kernel_names(self._lib)
+        mask = 0
+        for k in kernels:
+            mask |= 1 << names.index(k)
+        self._call("profile_begin", self._h, ctypes.c_uint64(mask))
+
+    def profile_end(self) -> dict:
+        """{kernel name: (total ms, launches)} for the kernels timed since profile_begin."""
+        names = kernel_names(self._lib)
+        ms = (ctypes.c_double * len(names))()
+        cnt = (ctypes.c_int64 * len(names))()
+        self._call("profile_end", self._h, ms, cnt)
+        return {names[k]: (ms[k], cnt[k]) for k in range(len(names)) if cnt[k] > 0}
+
+
+def kernel_names(lib=None):
+    lib = lib or load_library()
+    out = []
+    name = ctypes.c_char_p()
+    k = 0
+    while lib.pcm_kernel_name(k, ctypes.byref(name)) == abi.PCM_OK:
+        out.append(name.value.decode())
+        k += 1
+    return out
 
 
 class _Facade:

@@ -167,6 +167,21 @@ def test_fused_step(dims, real_bytes):
         dev.initialize_mesh()
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_neighbors_with_stale_mesh(dims):
+    """Neighbors rebuilt after the agents moved but before a new mesh update: the stencil is
+    taken around each agent's current voxel over the old buckets, as in the reference."""
+    mesh, st, p = parity.rich_population(dims, 1200, 50 + dims, 4, rates="rare")
+    dev, ora = parity.make_pair(mesh, st, seed=2)
+    for s in (dev, ora):
+        s.initialize_mesh()
+        s.update_cell_forces(p)
+        s.update_positions(p)
+        s.update_positions(p)
+        s.update_cell_neighbors(p)
+    parity.assert_same_neighbors(dev, ora, "stale mesh")
+
+
 def test_fused_equals_phases():
     """The fused step and the per-phase entry points give bit-identical states."""
     mesh, st, p = parity.rich_population(3, 1500, 33, 4, rates="busy")
