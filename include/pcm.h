@@ -169,6 +169,7 @@ int pcm_set_links(pcm_sim* sim, int32_t which, const int32_t* offsets, const int
 int pcm_get_links_size(pcm_sim* sim, int32_t which, int64_t* total);
 int pcm_get_links(pcm_sim* sim, int32_t which, int32_t* offsets, int32_t* indices);
 /* states.neighbors as built by update_cell_neighbors (empty for non-movable agents) */
+int pcm_set_neighbors(pcm_sim* sim, const int32_t* offsets, const int32_t* indices);
 int pcm_get_neighbors_size(pcm_sim* sim, int64_t* total);
 int pcm_get_neighbors(pcm_sim* sim, int32_t* offsets, int32_t* indices);
 /* environment::cells_in_mechanics_voxels: counts[voxel_count], agents concatenated in
@@ -177,6 +178,12 @@ int pcm_get_voxel_count(pcm_sim* sim, int64_t* voxel_count);
 int pcm_get_voxel_buckets(pcm_sim* sim, int32_t* counts, int32_t* agents);
 /* environment::deaths_count increments made on the device since creation */
 int pcm_get_deaths_count(pcm_sim* sim, int64_t* deaths);
+/* The removals made on the device since the last call, as the sequence of positions the
+ * reference's container.remove_at(i) received (containers_solver.cpp:81-123): replaying
+ * remove_at on a host container in this order reproduces the device's agent order, so host-
+ * only SoA members follow.  With positions == NULL only the length is returned; otherwise
+ * the log is copied (capacity >= length) and restarts. */
+int pcm_removal_log(pcm_sim* sim, int32_t* positions, int64_t capacity, int64_t* length);
 /* mechanics-step counter that keys the random stream (incremented by
  * pcm_update_cell_forces and by every pcm_step iteration) */
 int pcm_set_step_index(pcm_sim* sim, uint64_t step);
@@ -218,9 +225,9 @@ int pcm_synchronize(pcm_sim* sim);
 #define PCM_KERNEL_LIST(X)                                                                                             \
 	X(BEGIN_STEP) X(BIN_CLEAR) X(BIN_COUNT) X(BIN_CLAIM) X(BIN_SCATTER) X(BIN_ORDER) X(NEIGHBORS) X(FORCES)              \
 	X(MOTILITY) X(MEMBRANE) X(CONTRACT) X(POSITIONS) X(VELOCITY_FUSED) X(SPRING_FIRE) X(SPRING_ERASE)                   \
-	X(SPRING_COMPACT) X(SPRING_COLLECT) X(SPRING_RESOLVE) X(INTERACT_TENTATIVE) X(INTERACT_CHECK)                        \
-	X(INTERACT_APPLY) X(INTERACT_SERIAL) X(COMPACT_FLAG) X(COMPACT_PLAN) X(COMPACT_MOVE) X(COMPACT_LINKS)               \
-	X(COMPACT_RESTORE) X(COMPACT_DONE)
+	X(SPRING_COMPACT) X(SPRING_COLLECT) X(SPRING_RESOLVE) X(SPRING_APPEND) X(INTERACT_TENTATIVE) X(INTERACT_CHECK)       \
+	X(INTERACT_APPLY) X(INTERACT_SERIAL) X(COMPACT_FLAG) X(COMPACT_PLAN) X(COMPACT_APPLY) X(COMPACT_RESTORE)            \
+	X(COMPACT_DONE) X(FORCE_RECORDS)
 #define PCM_KERNEL_ENUM_(name) PCM_K_##name,
 enum pcm_kernel
 {
@@ -228,6 +235,10 @@ enum pcm_kernel
 };
 #undef PCM_KERNEL_ENUM_
 int pcm_kernel_name(int32_t kernel, const char** name);
+/* Work counters of the most recent step, 8 entries: spring attachment events, resolution
+ * rounds, resolution chunks, interaction candidates, interaction replay used (0/1), agents
+ * removed, occupied mechanics voxels, reserved. */
+int pcm_get_stats(pcm_sim* sim, int32_t* stats);
 int pcm_profile_begin(pcm_sim* sim, uint64_t kernel_mask);
 int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches);
 

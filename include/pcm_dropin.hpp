@@ -1,0 +1,470 @@
+// pcm_dropin.hpp — C++ drop-in for paraPhysiCell's mechanics solver (src/solver/).
+//
+// Provides the reference's solver classes by name — position_solver, containers_solver,
+// interactions_solver and the aggregate solver (src/solver/solver.h:9-17,
+// src/solver/host/position_solver.h:7-21, containers_solver.h:13-27,
+// interactions_solver.h:9-15) — templated on the environment type, each entry point
+// forwarding to libpcm.so through include/pcm.h.
+//
+// Calling convention, as in the reference: an entry point may be called by every thread of
+// an enclosing OpenMP team (the reference's orphaned `omp for`) or outside any team.  Inside
+// a team one thread makes the device call and all threads leave together (`omp single` and
+// its barrier).  Errors, which the reference never reports, throw std::runtime_error.
+//
+// Residency: the agent SoA lives on the GPU between mechanics phases.  Host code that reads
+// cell data calls pcm_dropin::sync_to_host(e) first; host code that writes it calls
+// pcm_dropin::host_modified(e) afterwards (the next device phase re-uploads).  Both mirror
+// simulator::sync_data_host for the diffusion solver (src/simulator.cpp:95-109).
+//
+// The integrating code supplies two functions for its environment type, found by
+// argument-dependent lookup:
+//   auto& pcm_cell_data(Env& e);              // the cell_data SoA (cell_data.h:162-196)
+//   void  pcm_host_remove_at(Env& e, int i);  // the container's remove_at (swap with last)
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+	#include <omp.h>
+#endif
+
+#include "pcm.h"
+
+namespace pcm_dropin {
+
+inline void check(int rc, const char* what)
+{
+	if (rc != PCM_OK)
+		throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + pcm_last_error());
+}
+
+// Run `f` on one thread of the current team (or directly outside a team); every thread
+// returns after it finished and every thread sees its exception.
+template <class F>
+void on_one_thread(F&& f)
+{
+#ifdef _OPENMP
+	if (omp_in_parallel())
+	{
+		static std::exception_ptr error;
+	#pragma omp single
+		{
+			error = nullptr;
+			try
+			{
+				f();
+			}
+			catch (...)
+			{
+				error = std::current_exception();
+			}
+		}
+		if (error)
+			std::rethrow_exception(error);
+		return;
+	}
+#endif
+	f();
+}
+
+// ---------------------------------------------------------------- SoA <-> field ids
+// Every device field with the reference's member that holds it (cell_data.h:14-196).
+template <class CellData, class F>
+void for_each_field(CellData& d, F&& f)
+{
+	f(PCM_F_POSITIONS, d.agent_data.positions);
+	f(PCM_F_VELOCITIES, d.velocities);
+	f(PCM_F_PREVIOUS_VELOCITIES, d.previous_velocities);
+	f(PCM_F_RADIUS, d.geometries.radius);
+	f(PCM_F_NUCLEAR_RADIUS, d.geometries.nuclear_radius);
+	f(PCM_F_SURFACE_AREA, d.geometries.surface_area);
+	f(PCM_F_VOLUME, d.agent_data.volumes);
+	f(PCM_F_FLUID, d.volumes.fluid);
+	f(PCM_F_SOLID, d.volumes.solid);
+	f(PCM_F_NUCLEAR, d.volumes.nuclear);
+	f(PCM_F_NUCLEAR_FLUID, d.volumes.nuclear_fluid);
+	f(PCM_F_NUCLEAR_SOLID, d.volumes.nuclear_solid);
+	f(PCM_F_CYTOPLASMIC, d.volumes.cytoplasmic);
+	f(PCM_F_CYTOPLASMIC_FLUID, d.volumes.cytoplasmic_fluid);
+	f(PCM_F_CYTOPLASMIC_SOLID, d.volumes.cytoplasmic_solid);
+	f(PCM_F_FLUID_FRACTION, d.volumes.fluid_fraction);
+	f(PCM_F_CYTOPLASMIC_TO_NUCLEAR_RATIO, d.volumes.cytoplasmic_to_nuclear_ratio);
+	f(PCM_F_TARGET_SOLID_CYTOPLASMIC, d.volumes.target_solid_cytoplasmic);
+	f(PCM_F_TARGET_SOLID_NUCLEAR, d.volumes.target_solid_nuclear);
+	f(PCM_F_CELL_CELL_ADHESION_STRENGTH, d.mechanics.cell_cell_adhesion_strength);
+	f(PCM_F_CELL_CELL_REPULSION_STRENGTH, d.mechanics.cell_cell_repulsion_strength);
+	f(PCM_F_CELL_BM_REPULSION_STRENGTH, d.mechanics.cell_BM_repulsion_strength);
+	f(PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE, d.mechanics.relative_maximum_adhesion_distance);
+	f(PCM_F_CELL_ADHESION_AFFINITIES, d.mechanics.cell_adhesion_affinities);
+	f(PCM_F_MAXIMUM_NUMBER_OF_ATTACHMENTS, d.mechanics.maximum_number_of_attachments);
+	f(PCM_F_ATTACHMENT_ELASTIC_CONSTANT, d.mechanics.attachment_elastic_constant);
+	f(PCM_F_ATTACHMENT_RATE, d.mechanics.attachment_rate);
+	f(PCM_F_DETACHMENT_RATE, d.mechanics.detachment_rate);
+	f(PCM_F_IS_MOTILE, d.motilities.is_motile);
+	f(PCM_F_PERSISTENCE_TIME, d.motilities.persistence_time);
+	f(PCM_F_MIGRATION_SPEED, d.motilities.migration_speed);
+	f(PCM_F_MIGRATION_BIAS_DIRECTION, d.motilities.migration_bias_direction);
+	f(PCM_F_MIGRATION_BIAS, d.motilities.migration_bias);
+	f(PCM_F_MOTILITY_VECTOR, d.motilities.motility_vector);
+	f(PCM_F_RESTRICT_TO_2D, d.motilities.restrict_to_2d);
+	f(PCM_F_DEAD, d.deaths.dead);
+	f(PCM_F_DEAD_PHAGOCYTOSIS_RATE, d.interactions.dead_phagocytosis_rate);
+	f(PCM_F_LIVE_PHAGOCYTOSIS_RATES, d.interactions.live_phagocytosis_rates);
+	f(PCM_F_DAMAGE_RATE, d.interactions.damage_rate);
+	f(PCM_F_ATTACK_RATES, d.interactions.attack_rates);
+	f(PCM_F_IMMUNOGENICITIES, d.interactions.immunogenicities);
+	f(PCM_F_FUSION_RATES, d.interactions.fusion_rates);
+	f(PCM_F_SIMPLE_PRESSURE, d.states.simple_pressure);
+	f(PCM_F_NUMBER_OF_NUCLEI, d.states.number_of_nuclei);
+	f(PCM_F_DAMAGE, d.states.damage);
+	f(PCM_F_TOTAL_ATTACK_TIME, d.states.total_attack_time);
+	f(PCM_F_CELL_DEFINITION_INDEX, d.cell_definition_indices);
+	f(PCM_F_IS_MOVABLE, d.is_movable);
+	f(PCM_F_FLAGS, d.flags);
+	f(PCM_F_INTERNALIZED_SUBSTRATES, d.agent_data.internalized_substrates);
+	f(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED, d.agent_data.fraction_transferred_when_ingested);
+}
+
+// vector<vector<index_t>> <-> CSR
+template <class Lists>
+void to_csr(const Lists& lists, int n, std::vector<int32_t>& off, std::vector<int32_t>& idx)
+{
+	off.assign(n + 1, 0);
+	idx.clear();
+	for (int i = 0; i < n; i++)
+	{
+		off[i] = (int32_t)idx.size();
+		for (auto j : lists[i])
+			idx.push_back((int32_t)j);
+	}
+	off[n] = (int32_t)idx.size();
+	if (idx.empty())
+		idx.push_back(0);
+}
+
+template <class Lists>
+void from_csr(Lists& lists, int n, const std::vector<int32_t>& off, const std::vector<int32_t>& idx)
+{
+	for (int i = 0; i < n; i++)
+		lists[i].assign(idx.begin() + off[i], idx.begin() + off[i + 1]);
+}
+
+// ---------------------------------------------------------------- residency
+struct residency
+{
+	pcm_sim* sim = nullptr;
+	int capacity = 0;
+	bool host_newer = true;    // the device copy must be refreshed before the next phase
+	bool device_newer = false; // the host copy must be refreshed before host code reads it
+	int64_t deaths_seen = 0;
+	uint64_t seed = 0;
+
+	~residency()
+	{
+		if (sim)
+			pcm_destroy(sim);
+	}
+};
+
+inline std::map<const void*, std::unique_ptr<residency>>& registry()
+{
+	static std::map<const void*, std::unique_ptr<residency>> r;
+	return r;
+}
+
+inline std::mutex& registry_mutex()
+{
+	static std::mutex m;
+	return m;
+}
+
+template <class Env>
+residency& state_of(Env& e)
+{
+	std::lock_guard<std::mutex> l(registry_mutex());
+	auto& slot = registry()[(const void*)&e];
+	if (!slot)
+		slot = std::make_unique<residency>();
+	return *slot;
+}
+
+// the random stream key of the device draws (the reference seeds std::mt19937,
+// src/random.cpp:27-46; the device stream is counter based)
+template <class Env>
+void set_seed(Env& e, uint64_t seed)
+{
+	state_of(e).seed = seed;
+}
+
+template <class Env>
+pcm_step_params params_of(Env& e)
+{
+	pcm_step_params p;
+	p.mechanics_time_step = (double)e.mechanics_time_step;
+	p.virtual_wall_at_domain_edges = e.virtual_wall_at_domain_edges ? 1 : 0;
+	p.automated_spring_adhesion = e.automated_spring_adhesion ? 1 : 0;
+	return p;
+}
+
+template <class Env>
+void upload(Env& e, residency& r)
+{
+	auto& d = pcm_cell_data(e);
+	const int n = (int)d.agents_count;
+	using real = std::remove_reference_t<decltype(d.agent_data.positions[0])>;
+	if (!r.sim || n > r.capacity)
+	{
+		if (r.sim)
+			check(pcm_destroy(r.sim), "pcm_destroy");
+		pcm_config c {};
+		c.mesh.dims = e.m.mesh.dims;
+		for (int k = 0; k < 3; k++)
+		{
+			c.mesh.bbox_min[k] = e.m.mesh.bounding_box_mins[k];
+			c.mesh.bbox_max[k] = e.m.mesh.bounding_box_maxs[k];
+			c.mesh.voxel[k] = e.mechanics_mesh.voxel_shape[k];
+		}
+		c.real_bytes = (int)sizeof(real);
+		c.cell_defs = e.cell_definitions_count;
+		c.substrates = e.m.substrates_count;
+		c.capacity = n < 1024 ? 2048 : 2 * n;
+		c.seed = r.seed;
+		check(pcm_create(&c, &r.sim), "pcm_create");
+		r.capacity = c.capacity;
+		r.deaths_seen = 0;
+	}
+	check(pcm_set_count(r.sim, n), "pcm_set_count");
+	for_each_field(d, [&](int id, auto& v) { check(pcm_set_field(r.sim, id, v.data()), "pcm_set_field"); });
+	std::vector<int32_t> off, idx;
+	to_csr(d.states.springs, n, off, idx);
+	check(pcm_set_links(r.sim, PCM_LINKS_SPRINGS, off.data(), idx.data()), "pcm_set_links");
+	to_csr(d.states.attached_cells, n, off, idx);
+	check(pcm_set_links(r.sim, PCM_LINKS_ATTACHED, off.data(), idx.data()), "pcm_set_links");
+	to_csr(d.states.neighbors, n, off, idx);
+	check(pcm_set_neighbors(r.sim, off.data(), idx.data()), "pcm_set_neighbors");
+	int64_t log_len = 0;
+	check(pcm_removal_log(r.sim, nullptr, 0, &log_len), "pcm_removal_log");
+	std::vector<int32_t> discard(log_len + 1);
+	check(pcm_removal_log(r.sim, discard.data(), log_len + 1, &log_len), "pcm_removal_log");
+	r.host_newer = false;
+}
+
+template <class Env>
+residency& on_device(Env& e)
+{
+	auto& r = state_of(e);
+	if (r.host_newer || !r.sim)
+		upload(e, r);
+	return r;
+}
+
+// Bring the host SoA up to date: replay the device's removals on the host container so
+// host-only members follow, then copy every device field, the lists and the buckets.
+template <class Env>
+void download(Env& e, residency& r)
+{
+	auto& d = pcm_cell_data(e);
+	int64_t log_len = 0;
+	check(pcm_removal_log(r.sim, nullptr, 0, &log_len), "pcm_removal_log");
+	std::vector<int32_t> log(log_len + 1);
+	check(pcm_removal_log(r.sim, log.data(), log_len + 1, &log_len), "pcm_removal_log");
+	for (int64_t k = 0; k < log_len; k++)
+		pcm_host_remove_at(e, log[k]);
+	int32_t n = 0;
+	check(pcm_get_count(r.sim, &n), "pcm_get_count");
+	if (n != (int32_t)d.agents_count)
+		throw std::runtime_error("pcm_dropin: host container and device disagree on the agent count");
+	for_each_field(d, [&](int id, auto& v) { check(pcm_get_field(r.sim, id, v.data()), "pcm_get_field"); });
+	auto fetch = [&](int which, auto& lists) {
+		int64_t total = 0;
+		check(pcm_get_links_size(r.sim, which, &total), "pcm_get_links_size");
+		std::vector<int32_t> off(n + 1), idx(total + 1);
+		check(pcm_get_links(r.sim, which, off.data(), idx.data()), "pcm_get_links");
+		from_csr(lists, n, off, idx);
+	};
+	fetch(PCM_LINKS_SPRINGS, d.states.springs);
+	fetch(PCM_LINKS_ATTACHED, d.states.attached_cells);
+	{
+		int64_t total = 0;
+		check(pcm_get_neighbors_size(r.sim, &total), "pcm_get_neighbors_size");
+		std::vector<int32_t> off(n + 1), idx(total + 1);
+		check(pcm_get_neighbors(r.sim, off.data(), idx.data()), "pcm_get_neighbors");
+		from_csr(d.states.neighbors, n, off, idx);
+	}
+	{
+		int64_t voxels = 0;
+		check(pcm_get_voxel_count(r.sim, &voxels), "pcm_get_voxel_count");
+		std::vector<int32_t> counts(voxels), agents(std::max<int64_t>(r.capacity, 1));
+		check(pcm_get_voxel_buckets(r.sim, counts.data(), agents.data()), "pcm_get_voxel_buckets");
+		int64_t t = 0;
+		for (int64_t v = 0; v < voxels; v++)
+		{
+			auto& b = e.cells_in_mechanics_voxels[v];
+			b.assign(agents.begin() + t, agents.begin() + t + counts[v]);
+			t += counts[v];
+		}
+	}
+	int64_t deaths = 0;
+	check(pcm_get_deaths_count(r.sim, &deaths), "pcm_get_deaths_count");
+	e.deaths_count += (decltype(e.deaths_count))(deaths - r.deaths_seen);
+	r.deaths_seen = deaths;
+	r.device_newer = false;
+}
+
+template <class Env>
+void sync_to_host(Env& e)
+{
+	on_one_thread([&] {
+		auto& r = state_of(e);
+		if (r.sim && r.device_newer)
+			download(e, r);
+	});
+}
+
+template <class Env>
+void host_modified(Env& e)
+{
+	on_one_thread([&] { state_of(e).host_newer = true; });
+}
+
+template <class Env>
+void release(Env& e)
+{
+	std::lock_guard<std::mutex> l(registry_mutex());
+	registry().erase((const void*)&e);
+}
+
+template <class Env, class Call>
+void run(Env& e, Call&& call)
+{
+	on_one_thread([&] {
+		auto& r = on_device(e);
+		call(r.sim);
+		r.device_newer = true;
+	});
+}
+
+// ---------------------------------------------------------------- reference classes
+template <class Env>
+class position_solver
+{
+public:
+	static void update_cell_forces(Env& e)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_update_cell_forces(s, &p), "pcm_update_cell_forces");
+		});
+	}
+
+	static void update_cell_neighbors(Env& e)
+	{
+		run(e, [&](pcm_sim* s) { check(pcm_update_cell_neighbors(s), "pcm_update_cell_neighbors"); });
+	}
+
+	static void update_motility(Env& e)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_update_motility(s, &p), "pcm_update_motility");
+		});
+	}
+
+	static void update_basement_membrane_interactions(Env& e)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_update_basement_membrane_interactions(s, &p), "pcm_update_basement_membrane_interactions");
+		});
+	}
+
+	static void update_spring_attachments(Env& e)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_update_spring_attachments(s, &p), "pcm_update_spring_attachments");
+		});
+	}
+
+	static void update_positions(Env& e)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_update_positions(s, &p), "pcm_update_positions");
+		});
+	}
+};
+
+template <class Env>
+class containers_solver
+{
+public:
+	// containers_solver::initialize: a fresh device handle for this environment
+	void initialize(Env& e)
+	{
+		on_one_thread([&] {
+			auto& r = state_of(e);
+			if (r.sim)
+			{
+				check(pcm_destroy(r.sim), "pcm_destroy");
+				r.sim = nullptr;
+			}
+			upload(e, r);
+		});
+	}
+
+	void update_mechanics_mesh(Env& e)
+	{
+		run(e, [&](pcm_sim* s) { check(pcm_update_mechanics_mesh(s), "pcm_update_mechanics_mesh"); });
+	}
+
+	void update_cell_container_for_mechanics(Env& e)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_update_cell_container_for_mechanics(s, &p), "pcm_update_cell_container_for_mechanics");
+		});
+	}
+};
+
+template <class Env>
+class interactions_solver
+{
+public:
+	void update_cell_cell_interactions(Env& e)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_update_cell_cell_interactions(s, &p), "pcm_update_cell_cell_interactions");
+		});
+	}
+};
+
+template <class Env>
+class solver
+{
+public:
+	position_solver<Env> position;
+	containers_solver<Env> containers;
+	interactions_solver<Env> interactions;
+
+	void initialize(Env& e) { containers.initialize(e); }
+
+	// `steps` whole mechanics steps (simulator.cpp:173-192) in one device submission
+	void simulate_mechanics(Env& e, int steps = 1)
+	{
+		run(e, [&](pcm_sim* s) {
+			const auto p = params_of(e);
+			check(pcm_step(s, &p, steps), "pcm_step");
+		});
+	}
+};
+
+} // namespace pcm_dropin

@@ -306,6 +306,15 @@ int oracle_get_links(void* h, int32_t which, int32_t* off, int32_t* idx)
 	return lists_get(cast(h)->lists(which), cast(h)->count(), off, idx);
 }
 
+int oracle_set_neighbors(void* h, const int32_t* off, const int32_t* idx)
+{
+	auto* o = cast(h);
+	auto& l = o->lists(2);
+	for (index_t i = 0; i < o->count(); i++)
+		l[i].assign(idx + off[i], idx + off[i + 1]);
+	return PCM_OK;
+}
+
 int oracle_get_neighbors_size(void* h, int64_t* total) { return lists_size(cast(h)->lists(2), cast(h)->count(), total); }
 int oracle_get_neighbors(void* h, int32_t* off, int32_t* idx) { return lists_get(cast(h)->lists(2), cast(h)->count(), off, idx); }
 

@@ -37,6 +37,7 @@ struct Scalars
 	int cursor;            // bucket allocation cursor of the voxel counting sort
 	int n_binned;          // agents binned by the last mesh update (for the sparse clear)
 	int att_events;        // spring attachment events collected this step
+	int att_actors;        // agents with attachment events this step
 	int cand_count;        // interaction candidate slots collected this step
 	int conflict;          // interaction speculation found a cross-agent dependency
 	int bad_count;         // agents the container update removes this step
@@ -45,7 +46,11 @@ struct Scalars
 	long long deaths;      // environment::deaths_count increments
 	int n_occ;             // occupied mechanics voxels of the last mesh update
 	int order_valid;       // bucket[0, n) is a permutation of the current agents
-	int pad[2];
+	int log_len;           // entries in the removal log (may exceed log_cap: log lost)
+	int pad;
+	// diagnostics of the last step (pcm_get_stats): attachment events, resolver rounds,
+	// resolver chunks, interaction candidates, speculation conflict, removals, occupied voxels
+	int stat[8];
 };
 
 template <class R>
@@ -58,7 +63,6 @@ struct SimCtx
 	int K;       // neighbor slots per agent
 	int L;       // spring slots per agent
 	int LA;      // attached-cell slots per agent
-	int ev_cap;  // attachment event buffer
 	int cand_cap; // interaction candidate buffer
 	int bad_cap;  // removal list buffer
 	unsigned long long seed;
@@ -97,10 +101,16 @@ struct SimCtx
 	// mechanics voxel grid
 	int *vcnt, *vstart, *vox_of, *vrank, *btmp, *bucket, *occ;
 	R* sorted; // 4 reals per bucket slot: x, y, z, rel*radius
+	R* frec;   // 8 reals per agent: x, y, z, radius, repulsion, adhesion, rel, type|movable<<30 bits
 
 	// spring scratch
 	uint32_t *fire, *erase;  // per agent slot masks (L <= 32)
-	int* ev;                 // attachment events: 3 ints (actor, slot, target) each
+	int* own_slot;           // per actor, K slots: neighbor slots of its attachment events
+	uint8_t* own_st;         // per actor, K slots: 0 undecided, 1 success, 2 failure
+	int* own_cnt;            // attachment events of an actor this step
+	int* inbox;              // per agent, K (actor, k) pairs: events that target it
+	int* in_cnt;             // inbox fill (zero outside the spring phase)
+	int* att_list;           // actors with attachment events
 	// interaction scratch
 	int* cand;               // candidate (actor, slot) pairs
 	int *n_rem_t, *n_atk_t;  // removal / attack events targeting an agent
@@ -110,6 +120,8 @@ struct SimCtx
 	uint8_t* acted;          // actor reached a lock at least once
 	// compaction scratch
 	int *bad, *remap, *seq, *move_src, *move_dst;
+	int* rm_log; // positions passed to remove_at, in the reference's order, since the last read
+	int log_cap;
 	// compaction copy plan: one entry per 4-byte word (or byte) of an agent's fields
 	int mv_n;
 	uint16_t mv_field[192];
@@ -178,6 +190,62 @@ __device__ __forceinline__ int wave_reserve(int* counter, int count)
 	return base + pre;
 }
 
+// -------------------------------------------------------------------------- force records
+__device__ __forceinline__ float bits_to_real(int b, float) { return __int_as_float(b); }
+__device__ __forceinline__ double bits_to_real(int b, double) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ int real_to_bits(float r) { return __float_as_int(r); }
+__device__ __forceinline__ int real_to_bits(double r) { return (int)__double_as_longlong(r); }
+
+template <class R>
+struct force_rec
+{
+	R p[3];
+	R r, rep, adh, rel;
+	int type;
+	bool movable;
+};
+
+template <class R, int D>
+__device__ __forceinline__ void write_force_record(const SimCtx<R>& c, int i)
+{
+	R* f = c.frec + (size_t)i * 8;
+	for (int k = 0; k < 3; k++)
+		f[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
+	f[3] = c.radius[i];
+	f[4] = c.repulsion[i];
+	f[5] = c.adhesion[i];
+	f[6] = c.rel_adh[i];
+	f[7] = bits_to_real(c.type[i] | (c.movable[i] ? (1 << 30) : 0), R(0));
+}
+
+template <class R>
+__device__ __forceinline__ force_rec<R> read_force_record(const SimCtx<R>& c, int j)
+{
+	force_rec<R> o;
+	const R* f = c.frec + (size_t)j * 8;
+	R v[8];
+	if constexpr (sizeof(R) == 4)
+	{
+		const float4 a = *(const float4*)f;
+		const float4 b = *(const float4*)(f + 4);
+		v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+	}
+	else
+	{
+		for (int k = 0; k < 8; k += 2)
+		{
+			const double2 a = *(const double2*)(f + k);
+			v[k] = a.x, v[k + 1] = a.y;
+		}
+	}
+	o.p[0] = v[0], o.p[1] = v[1], o.p[2] = v[2];
+	o.r = v[3], o.rep = v[4], o.adh = v[5], o.rel = v[6];
+	const int bits = real_to_bits(v[7]);
+	o.type = bits & ((1 << 30) - 1);
+	o.movable = (bits >> 30) & 1;
+	return o;
+}
+
 // -------------------------------------------------------------------------- helpers
 template <class R>
 __device__ __forceinline__ R real_max(R a, R b)
@@ -213,6 +281,38 @@ __device__ __forceinline__ R pair_distance(const R* a, const R* b)
 		const R s = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
 		return sqrt(s);
 	}
+}
+
+// sqrt(s) <= reach with the correctly rounded square root of the reference, decided
+// without the square root when s is clearly on one side: if s > reach^2 (1 + e) then
+// sqrt_rn(s) >= sqrt(s)(1 - u) > reach, and if s < reach^2 (1 - e) then
+// sqrt_rn(s) <= sqrt(s)(1 + u) < reach, for e = 1e-5 (float) / 1e-12 (double) far above the
+// rounding errors u of sqrt and of the products that form the thresholds.
+template <class R>
+__device__ __forceinline__ bool within_reach(R s, R reach)
+{
+	const R e = sizeof(R) == 4 ? (R)1e-5 : (R)1e-12;
+	const R r2 = reach * reach;
+	if (!(r2 >= (sizeof(R) == 4 ? (R)1.2e-38 : (R)2.3e-308))) // zero, subnormal or NaN threshold
+		return sqrt(s) <= reach;
+	if (s > r2 * ((R)1 + e))
+		return false;
+	if (s < r2 * ((R)1 - e))
+		return true;
+	return sqrt(s) <= reach;
+}
+
+// position_helper<D>::distance(a, b) <= reach, bit-exact with the reference's test
+template <int D, class R>
+__device__ __forceinline__ bool pair_within(const R* a, const R* b, R reach)
+{
+	if constexpr (D == 1)
+		return fabs(a[0] - b[0]) <= reach;
+	else if constexpr (D == 2)
+		return within_reach<R>((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]), reach);
+	else
+		return within_reach<R>(
+			(a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]), reach);
 }
 
 // voxel coordinate of a position: (index_t)((p - min) / voxel), clamped to the grid

@@ -12,9 +12,9 @@ namespace pcm {
 
 constexpr int BLOCK = 256;
 constexpr int RESOLVE_THREADS = 1024;
-constexpr int ATT_CHUNK = 2048;     // attachment events resolved per LDS pass
 constexpr int CAND_CHUNK = 4096;    // interaction candidates ordered per LDS pass
 constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
+constexpr int LANE_BUF = 8;         // appends a lane buffers before one wavefront-wide reservation
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -53,12 +53,20 @@ __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs)
 	}
 }
 
-template <class R, int D>
+// RESTORE: also return the container update's remap entries to the identity
+template <class R, int D, bool RESTORE>
 __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int i = gid();
 	const int n = c.sc->n;
+	if (RESTORE)
+	{
+		if (i < c.sc->bad_count)
+			c.remap[c.bad[i]] = c.bad[i];
+		if (i < c.sc->moves)
+			c.remap[c.move_src[i]] = c.move_src[i];
+	}
 	if (i == 0)
 		c.sc->n_binned = n;
 	if (i >= n)
@@ -71,12 +79,18 @@ __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs)
 // Each occupied voxel's first agent claims the voxel's bucket range; claims are summed
 // per block so a block issues one atomic for the bucket cursor and one for the list of
 // occupied voxels.
-template <class R>
+// RESET: also zero the container update's counters (their readers have finished)
+template <class R, bool RESET>
 __global__ void k_bin_claim(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int i = gid();
 	const int n = c.sc->n;
+	if (RESET && i == 0)
+	{
+		c.sc->bad_count = 0;
+		c.sc->moves = 0;
+	}
 	int amount = 0, v = -1;
 	if (i < n && c.vrank[i] == 0)
 	{
@@ -128,8 +142,19 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 	for (int k = 0; k < 3; k++)
 		out[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
 	out[3] = c.rel_adh[i] * c.radius[i];
+	write_force_record<R, D>(c, i);
 	if (i == 0)
 		c.sc->order_valid = 1;
+}
+
+// force records of the current agent state, when the mesh update's copy is stale
+template <class R, int D>
+__global__ void k_force_records(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i < c.sc->n)
+		write_force_record<R, D>(c, i);
 }
 
 // ------------------------------------------------------------------ neighbor lists
@@ -142,7 +167,11 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 // One wavefront per occupied voxel: the voxel's 3^d stencil buckets are staged once in
 // LDS in stencil order, then every agent of the voxel tests 64 candidates per iteration;
 // a ballot and a prefix popcount write the passing candidates in candidate order.
-constexpr int NBR_TILE = 1024;
+template <class R>
+constexpr int nbr_tile()
+{
+	return sizeof(R) == 4 ? 448 : 320; // candidates staged per wavefront
+}
 
 // One agent's list straight from HBM.  DIRECT: the buckets may predate the current
 // positions, so the stencil is centred on the agent's current voxel and candidates are
@@ -191,8 +220,7 @@ __device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
 					else
 						for (int d = 0; d < 4; d++)
 							q[d] = c.sorted[(size_t)k * 4 + d];
-					const R dist = pair_distance<D, R>(pi, q);
-					if (dist <= mi + q[3])
+					if (pair_within<D, R>(pi, q, mi + q[3]))
 					{
 						if (cnt < c.K)
 							out[cnt] = j;
@@ -215,8 +243,9 @@ template <class R, int D>
 __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	__shared__ R s_rec[NBR_TILE * 4];
-	__shared__ int s_idx[NBR_TILE];
+	constexpr int TILE = nbr_tile<R>();
+	__shared__ R s_rec[TILE * 4];
+	__shared__ int s_idx[TILE];
 	__shared__ int s_pre[28];
 	__shared__ int s_vs[27];
 	const int lane = threadIdx.x;
@@ -235,7 +264,7 @@ __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ 
 			{
 				const int nv = x + c.grid[0] * (y + c.grid[1] * z);
 				cnt = c.vcnt[nv];
-				vs = cnt ? c.vstart[nv] : 0;
+				vs = c.vstart[nv];
 			}
 		}
 		int C;
@@ -248,26 +277,53 @@ __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ 
 		if (lane == 0)
 			s_pre[27] = C;
 		__syncthreads();
-		const int m = c.vcnt[v], s0 = c.vstart[v];
-		if (C <= NBR_TILE)
+		const int m = s_pre[14] - s_pre[13]; // the centre voxel is stencil entry 13
+		if (C <= TILE)
 		{
-			for (int t = lane; t < C; t += 64)
+			// stage the stencil: four candidates per lane in flight
+			for (int t0 = 0; t0 < C; t0 += 256)
 			{
-				int sv = 0;
-				while (s_pre[sv + 1] <= t)
-					sv++;
-				const int g = s_vs[sv] + t - s_pre[sv];
-				s_idx[t] = c.bucket[g];
-				for (int k = 0; k < 4; k++)
-					s_rec[t * 4 + k] = c.sorted[(size_t)g * 4 + k];
+				int gi[4], jj[4];
+				R rec[4][4];
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+				{
+					const int t = t0 + lane + 64 * u;
+					gi[u] = -1;
+					if (t < C)
+					{
+						int sv = 0;
+						while (s_pre[sv + 1] <= t)
+							sv++;
+						gi[u] = s_vs[sv] + t - s_pre[sv];
+					}
+				}
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+					if (gi[u] >= 0)
+					{
+						jj[u] = c.bucket[gi[u]];
+						for (int k = 0; k < 4; k++)
+							rec[u][k] = c.sorted[(size_t)gi[u] * 4 + k];
+					}
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+					if (gi[u] >= 0)
+					{
+						const int t = t0 + lane + 64 * u;
+						s_idx[t] = jj[u];
+						for (int k = 0; k < 4; k++)
+							s_rec[t * 4 + k] = rec[u][k];
+					}
 			}
 			__syncthreads();
+			const int own0 = s_pre[13];
 			for (int a = 0; a < m; a++)
 			{
-				const int i = c.bucket[s0 + a];
+				const int i = s_idx[own0 + a];
 				R pi[4];
 				for (int k = 0; k < 4; k++)
-					pi[k] = c.sorted[(size_t)(s0 + a) * 4 + k];
+					pi[k] = s_rec[(own0 + a) * 4 + k];
 				int* out = c.nbr_idx + (size_t)i * c.K;
 				int base = 0;
 				for (int t0 = 0; t0 < C; t0 += 64)
@@ -281,7 +337,7 @@ __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ 
 						if (j != i)
 						{
 							const R* q = s_rec + t * 4;
-							pass = pair_distance<D, R>(pi, q) <= pi[3] + q[3];
+							pass = pair_within<D, R>(pi, q, pi[3] + q[3]);
 						}
 					}
 					const unsigned long long mask = __ballot(pass);
@@ -308,6 +364,7 @@ __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ 
 		else
 		{
 			// stencil larger than the LDS tile: one lane per agent, straight from HBM
+			const int s0 = c.vstart[v];
 			for (int a = lane; a < m; a += 64)
 				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
 		}
@@ -448,29 +505,134 @@ __global__ void k_positions(const SimCtx<R>* __restrict__ ctxs)
 	}
 }
 
+// solve_pair (position_solver.cpp:24-77) on force records; same operation order as
+// pair_force.  Returns the repulsion term that feeds simple_pressure.
+template <int D, class R>
+__device__ __forceinline__ R pair_force_rec(const force_rec<R>& a, const force_rec<R>& b, R aff_ab, R aff_ba,
+											  bool with_force, R* v)
+{
+	R diff[D];
+	for (int k = 0; k < D; k++)
+		diff[k] = a.p[k] - b.p[k];
+	R distance;
+	if constexpr (D == 1)
+		distance = fabs(diff[0]);
+	else if constexpr (D == 2)
+		distance = sqrt(diff[0] * diff[0] + diff[1] * diff[1]);
+	else
+		distance = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+	distance = real_max<R>(distance, (R)0.00001);
+	R rep = 1 - distance / (a.r + b.r);
+	rep = rep < 0 ? (R)0 : rep;
+	rep *= rep;
+	const R term = rep;
+	if (with_force)
+	{
+		rep *= sqrt(a.rep * b.rep);
+		const R reach = a.rel * a.r + b.rel * b.r;
+		R adh = 1 - distance / reach;
+		adh *= adh;
+		adh *= sqrt(a.adh * b.adh * aff_ab * aff_ba);
+		const R force = (rep - adh) / distance;
+		for (int k = 0; k < D; k++)
+			v[k] += force * diff[k];
+	}
+	return term;
+}
+
 // Forces, motility, basement membrane and spring contraction of one agent in the
-// reference's summation order (simulator.cpp:173-176), written to the velocity.
-template <class R, int D>
+// reference's summation order (simulator.cpp:173-176), written to the velocity.  Neighbor
+// and spring-partner data come from the force records, four partners loaded at a time.
+// CLEAR: also clear the voxel counts of the last mesh update (nothing reads them until
+// this step's mesh update)
+template <class R, int D, bool CLEAR>
 __global__ void k_velocity_fused(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int g = gid();
+	if (CLEAR)
+	{
+		if (g < c.sc->n_binned)
+			c.vcnt[c.vox_of[g]] = 0;
+		if (g == 0)
+		{
+			c.sc->cursor = 0;
+			c.sc->n_occ = 0;
+		}
+	}
 	if (g >= c.sc->n)
 		return;
 	const int i = agent_at<R>(c, g);
-	const bool mov = c.movable[i] != 0;
-	R p[D], v[D];
+	const force_rec<R> me = read_force_record<R>(c, i);
+	const bool mov = me.movable;
+	R v[D], p[D];
 	for (int k = 0; k < D; k++)
 	{
-		p[k] = c.pos[(size_t)i * D + k];
+		p[k] = me.p[k];
 		v[k] = c.vel[(size_t)i * D + k];
 	}
-	c.pressure[i] = forces_agent<D, R>(c, i, p, v);
+	const R* aff_row = c.affinity + (size_t)i * c.T;
+	const int cnt = c.nbr_cnt[i];
+	const int* nb = c.nbr_idx + (size_t)i * c.K;
+	R pressure = 0;
+	for (int s0 = 0; s0 < cnt; s0 += 4)
+	{
+		force_rec<R> other[4];
+		R aff_ba[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+			if (s0 + u < cnt)
+			{
+				const int j = nb[s0 + u];
+				other[u] = read_force_record<R>(c, j);
+				aff_ba[u] = c.affinity[(size_t)j * c.T + me.type];
+			}
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+			if (s0 + u < cnt)
+			{
+				const R term = pair_force_rec<D, R>(me, other[u], aff_row[other[u].type], aff_ba[u], mov, v);
+				const R t = term * pressure_coefficient<R>();
+				if (mov)
+					pressure += t;
+				if (other[u].movable)
+					pressure += t;
+			}
+	}
+	c.pressure[i] = pressure;
 	motility_agent<D, R>(c, i, c.sc->step, v);
 	if (mov && c.wall)
 		membrane_agent<D, R>(c, i, p, v);
 	if (mov && c.springs)
-		contract_agent<D, R>(c, i, p, v);
+	{
+		// spring_contract_function (position_solver.cpp:337-370)
+		const int ns = c.spr_cnt[i];
+		const int* sp = c.spr_idx + (size_t)i * c.L;
+		const R ki = c.elastic[i];
+		for (int s0 = 0; s0 < ns; s0 += 4)
+		{
+			force_rec<R> other[4];
+			R kj[4], aff_ba[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				if (s0 + u < ns)
+				{
+					const int j = sp[s0 + u];
+					other[u] = read_force_record<R>(c, j);
+					kj[u] = c.elastic[j];
+					aff_ba[u] = c.affinity[(size_t)j * c.T + me.type];
+				}
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				if (s0 + u < ns)
+				{
+					const R kij = ki * kj[u] * aff_row[other[u].type] * aff_ba[u];
+					const R adhesion = (R)sqrt((double)kij);
+					for (int k = 0; k < D; k++)
+						v[k] += adhesion * (other[u].p[k] - p[k]);
+				}
+		}
+	}
 	for (int k = 0; k < D; k++)
 		c.vel[(size_t)i * D + k] = v[k];
 }
@@ -504,69 +666,68 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
 			mask |= 1u << s;
 	}
 	c.fire[i] = mask;
+	c.erase[i] = 0;
 }
 
 template <class R>
 __global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int i = (int)(t / c.L), s = (int)(t % c.L);
 	const int n = c.sc->n;
 	if (i >= n)
 		return;
 	const int cnt = c.spr_cnt[i];
+	if (s >= cnt)
+		return;
 	const int* li = c.spr_idx + (size_t)i * c.L;
+	const int p = li[s];
+	if (p < 0 || p >= n)
+		return;
 	const uint32_t fi = c.fire[i];
-	uint32_t erased = 0;
-	for (int s = 0; s < cnt; s++)
+	if ((fi >> s) & 1u)
 	{
-		const int p = li[s];
-		if (p < 0 || p >= n)
-			continue;
-		const bool fired = (fi >> s) & 1u;
-		if (fired)
-		{
-			erased |= 1u << s;
-			continue;
-		}
-		const int* lp = c.spr_idx + (size_t)p * c.L;
-		const int cp = c.spr_cnt[p];
-		const uint32_t fp = c.fire[p];
-		if (i < p)
-		{
-			int f_i = 0, nf_rank = 0;
-			for (int t = 0; t < cnt; t++)
-				if (li[t] == p)
-				{
-					const bool ft = (fi >> t) & 1u;
-					f_i += ft;
-					if (!ft && t < s)
-						nf_rank++;
-				}
-			int g = 0, occ = 0;
-			for (int t = 0; t < cp; t++)
-				if (lp[t] == i)
-				{
-					occ++;
-					if (occ > f_i && ((fp >> t) & 1u))
-						g++;
-				}
-			if (nf_rank < g)
-				erased |= 1u << s;
-		}
-		else
-		{
-			int rank = 0;
-			for (int t = 0; t < s; t++)
-				rank += li[t] == p;
-			int f_p = 0;
-			for (int t = 0; t < cp; t++)
-				f_p += lp[t] == i && ((fp >> t) & 1u);
-			if (rank < f_p)
-				erased |= 1u << s;
-		}
+		atomicOr(&c.erase[i], 1u << s);
+		return;
 	}
-	c.erase[i] = erased;
+	const int* lp = c.spr_idx + (size_t)p * c.L;
+	const int cp = c.spr_cnt[p];
+	const uint32_t fp = c.fire[p];
+	bool erased = false;
+	if (i < p)
+	{
+		int f_i = 0, nf_rank = 0;
+		for (int u = 0; u < cnt; u++)
+			if (li[u] == p)
+			{
+				const bool fu = (fi >> u) & 1u;
+				f_i += fu;
+				if (!fu && u < s)
+					nf_rank++;
+			}
+		int g = 0, occ = 0;
+		for (int u = 0; u < cp; u++)
+			if (lp[u] == i)
+			{
+				occ++;
+				if (occ > f_i && ((fp >> u) & 1u))
+					g++;
+			}
+		erased = nf_rank < g;
+	}
+	else
+	{
+		int rank = 0;
+		for (int u = 0; u < s; u++)
+			rank += li[u] == p;
+		int f_p = 0;
+		for (int u = 0; u < cp; u++)
+			f_p += lp[u] == i && ((fp >> u) & 1u);
+		erased = rank < f_p;
+	}
+	if (erased)
+		atomicOr(&c.erase[i], 1u << s);
 }
 
 template <class R>
@@ -588,17 +749,15 @@ __global__ void k_spring_compact(const SimCtx<R>* __restrict__ ctxs)
 	c.spr_cnt[i] = w;
 }
 
-// Attachment candidates: every (i, slot) with j = neighbor > i whose draws succeed.
-// Events are buffered per lane and appended with one atomic per wavefront.
-constexpr int LANE_BUF = 8;
-
+// Attachment events: every (i, slot) with j = neighbor > i whose draws succeed.  An actor
+// keeps its events in slot order (own_slot); each target collects (actor, k) references in
+// an unordered inbox; actors with events are listed for the resolver.
 template <class R>
 __global__ void k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int g = gid();
-	int buf_s[LANE_BUF], buf_j[LANE_BUF];
-	int nb_ev = 0;
+	int n_ev = 0;
 	int i = -1;
 	if (g < c.sc->n)
 	{
@@ -608,50 +767,202 @@ __global__ void k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
 		const int ti = c.type[i];
 		const R rate_i = c.att_rate[i] * c.dt;
 		const unsigned long long step = c.sc->step;
-		for (int s = 0; s < cnt; s++)
+		for (int s0 = 0; s0 < cnt; s0 += 4)
 		{
-			const int j = nb[s];
+			int jj[4], tj[4];
+			R rate_j[4], aff_ji[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+				jj[u] = s0 + u < cnt ? nb[s0 + u] : -1;
+				if (jj[u] > i)
+				{
+					tj[u] = c.type[jj[u]];
+					rate_j[u] = c.att_rate[jj[u]];
+					aff_ji[u] = c.affinity[(size_t)jj[u] * c.T + ti];
+				}
+			}
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+			{
+			const int s = s0 + u;
+			const int j = jj[u];
 			if (j < i)
 				continue;
-			const R p_l = rate_i * c.affinity[(size_t)i * c.T + c.type[j]];
-			const R p_r = c.att_rate[j] * c.dt * c.affinity[(size_t)j * c.T + ti];
+			const R p_l = rate_i * c.affinity[(size_t)i * c.T + tj[u]];
+			const R p_r = rate_j[u] * c.dt * aff_ji[u];
 			const pcm_u32x4 b = pcm_block(c.seed, step, PCM_STREAM_ATTACH, (uint32_t)i, (uint32_t)s, 0u);
 			if (pcm_draw<R>::from(b, 0) <= p_l || pcm_draw<R>::from(b, 1) <= p_r)
 			{
-				if (nb_ev < LANE_BUF)
+				c.own_slot[(size_t)i * c.K + n_ev] = s;
+				c.own_st[(size_t)i * c.K + n_ev] = 0;
+				const int slot = atomicAdd(&c.in_cnt[j], 1);
+				if (slot < c.K)
 				{
-					buf_s[nb_ev] = s;
-					buf_j[nb_ev] = j;
-					nb_ev++;
+					c.inbox[2 * ((size_t)j * c.K + slot)] = i;
+					c.inbox[2 * ((size_t)j * c.K + slot) + 1] = n_ev;
 				}
 				else
-				{
-					const int e = atomicAdd(&c.sc->att_events, 1);
-					if (e < c.ev_cap)
-					{
-						c.ev[3 * e] = i;
-						c.ev[3 * e + 1] = s;
-						c.ev[3 * e + 2] = j;
-					}
-					else
-						atomicOr(&c.sc->status, ST_CAPACITY);
-				}
+					atomicOr(&c.sc->status, ST_CAPACITY);
+				n_ev++;
+			}
 			}
 		}
+		c.own_cnt[i] = n_ev;
 	}
-	const int base = wave_reserve(&c.sc->att_events, nb_ev);
-	for (int k = 0; k < nb_ev; k++)
+	const int base = wave_reserve(&c.sc->att_actors, n_ev > 0 ? 1 : 0);
+	if (n_ev > 0)
+		c.att_list[base] = i;
+	(void)wave_reserve(&c.sc->att_events, n_ev);
+}
+
+// Attachment resolution in the reference's order: event (i, j) succeeds iff, when the
+// single-thread loop reaches it (i ascending, neighbor order), both endpoints hold fewer
+// springs than their maximum.  The events before it at endpoint i are all of i's inbox
+// (their actors are smaller) and i's earlier own events; at endpoint j they are j's inbox
+// entries with a smaller actor.  Each endpoint's count is bounded below by the successes
+// and above by the successes plus the undecided among those events; rounds repeat until
+// every event is decided (the earliest undecided event is always decidable).
+template <class R>
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs)
+{
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	__shared__ int s_unknown;
+	if (c.sc->status)
+		return;
+	const int A = c.sc->att_actors;
+	const int E = c.sc->att_events;
+	volatile uint8_t* st = c.own_st;
+	int rounds = 0;
+	for (;; rounds++)
 	{
-		const int e = base + k;
-		if (e < c.ev_cap)
+		if (rounds > E + 1)
 		{
-			c.ev[3 * e] = i;
-			c.ev[3 * e + 1] = buf_s[k];
-			c.ev[3 * e + 2] = buf_j[k];
+			if (threadIdx.x == 0)
+				atomicOr(&c.sc->status, ST_CAPACITY);
+			return;
 		}
-		else
-			atomicOr(&c.sc->status, ST_CAPACITY);
+		if (threadIdx.x == 0)
+			s_unknown = 0;
+		__syncthreads();
+		for (int t = threadIdx.x; t < A; t += blockDim.x)
+		{
+			const int a = c.att_list[t];
+			const int na = c.own_cnt[a];
+			const int c0a = c.spr_cnt[a], maxa = c.max_att[a];
+			int in_lo = 0, in_hi = 0;
+			const int nin = min(c.in_cnt[a], c.K);
+			for (int q = 0; q < nin; q++)
+			{
+				const int x = c.inbox[2 * ((size_t)a * c.K + q)], kx = c.inbox[2 * ((size_t)a * c.K + q) + 1];
+				const int sx = st[(size_t)x * c.K + kx];
+				in_lo += sx == 1;
+				in_hi += sx != 2;
+			}
+			int own_lo = 0, own_hi = 0;
+			for (int k = 0; k < na; k++)
+			{
+				int sk = st[(size_t)a * c.K + k];
+				if (sk == 0)
+				{
+					const int lo_a = c0a + in_lo + own_lo, hi_a = c0a + in_hi + own_hi;
+					const int cond_a = lo_a >= maxa ? 2 : (hi_a < maxa ? 1 : 0);
+					const int j = c.nbr_idx[(size_t)a * c.K + c.own_slot[(size_t)a * c.K + k]];
+					int lo_j = c.spr_cnt[j], hi_j = lo_j;
+					const int maxj = c.max_att[j];
+					const int njin = min(c.in_cnt[j], c.K);
+					for (int q = 0; q < njin; q++)
+					{
+						const int x = c.inbox[2 * ((size_t)j * c.K + q)];
+						if (x >= a)
+							continue;
+						const int sx = st[(size_t)x * c.K + c.inbox[2 * ((size_t)j * c.K + q) + 1]];
+						lo_j += sx == 1;
+						hi_j += sx != 2;
+					}
+					const int cond_j = lo_j >= maxj ? 2 : (hi_j < maxj ? 1 : 0);
+					if (cond_a == 2 || cond_j == 2)
+						sk = 2;
+					else if (cond_a == 1 && cond_j == 1)
+						sk = 1;
+					if (sk != 0)
+						st[(size_t)a * c.K + k] = (uint8_t)sk;
+					else
+						atomicAdd(&s_unknown, 1);
+				}
+				own_lo += sk == 1;
+				own_hi += sk != 2;
+			}
+		}
+		__threadfence();
+		__syncthreads();
+		const int u = s_unknown;
+		__syncthreads();
+		if (u == 0)
+			break;
 	}
+	if (threadIdx.x == 0)
+	{
+		c.sc->stat[0] = E;
+		c.sc->stat[1] = rounds + 1;
+		c.sc->stat[2] = A;
+		c.sc->att_actors = 0;
+		c.sc->att_events = 0;
+	}
+}
+
+// Successful springs are appended to each endpoint's list in the order the reference
+// pushes them: events from smaller actors first (by actor), then the agent's own events in
+// neighbor order.
+template <class R>
+__global__ void k_spring_append(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int a = gid();
+	if (a >= c.sc->n)
+		return;
+	const int nin = min(c.in_cnt[a], c.K);
+	const int na = c.own_cnt[a];
+	if (nin + na == 0)
+		return;
+	int cnt = c.spr_cnt[a];
+	int* out = c.spr_idx + (size_t)a * c.L;
+	int last = -1;
+	for (int found = 0; found < nin; found++)
+	{
+		int best = 0x7fffffff, best_k = 0;
+		for (int q = 0; q < nin; q++)
+		{
+			const int x = c.inbox[2 * ((size_t)a * c.K + q)];
+			if (x > last && x < best)
+			{
+				best = x;
+				best_k = c.inbox[2 * ((size_t)a * c.K + q) + 1];
+			}
+		}
+		last = best;
+		if (c.own_st[(size_t)best * c.K + best_k] == 1)
+		{
+			if (cnt < c.L)
+				out[cnt] = best;
+			else
+				atomicOr(&c.sc->status, ST_LIST_OVERFLOW);
+			cnt++;
+		}
+	}
+	for (int k = 0; k < na; k++)
+		if (c.own_st[(size_t)a * c.K + k] == 1)
+		{
+			const int j = c.nbr_idx[(size_t)a * c.K + c.own_slot[(size_t)a * c.K + k]];
+			if (cnt < c.L)
+				out[cnt] = j;
+			else
+				atomicOr(&c.sc->status, ST_LIST_OVERFLOW);
+			cnt++;
+		}
+	c.spr_cnt[a] = min(cnt, c.L);
+	c.in_cnt[a] = 0;
+	c.own_cnt[a] = 0;
 }
 
 // Block-wide bitonic sort of `count` 64-bit keys (with 32-bit payloads) in shared memory.
@@ -727,164 +1038,6 @@ __device__ inline int block_chunk_end(const int* rec, int stride, int count, int
 	return a;
 }
 
-// Attachment resolution in the reference's order: event (i, j) succeeds iff, when it is
-// reached in (i ascending, neighbor order), both endpoints hold fewer springs than their
-// maximum.  Events are taken in actor ranges of at most ATT_CHUNK; within a range, each
-// event's endpoint counts are bounded below (successes so far) and above (successes plus
-// undecided) by the earlier events touching the endpoint, and the bounds are iterated until
-// every event is decided — the earliest undecided event is always decidable, so the loop
-// ends.  Successful springs are appended to both lists in event order.
-template <class R>
-__global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs)
-{
-	const SimCtx<R>& c = ctxs[blockIdx.y];
-	__shared__ unsigned long long ekey[ATT_CHUNK];
-	__shared__ int etgt[ATT_CHUNK];
-	__shared__ unsigned long long nkey[2 * ATT_CHUNK];
-	__shared__ int npos[2 * ATT_CHUNK];
-	__shared__ uint8_t est[ATT_CHUNK];
-	__shared__ int s_base[2 * ATT_CHUNK];
-	__shared__ int s_max[2 * ATT_CHUNK];
-	__shared__ int s_q0[ATT_CHUNK];
-	__shared__ int s_q1[ATT_CHUNK];
-	__shared__ int s_count, s_unknown, s_red;
-
-	if (c.sc->status)
-		return;
-	const int E = min(c.sc->att_events, c.ev_cap);
-	const int n = c.sc->n;
-	volatile int* spr_cnt = c.spr_cnt;
-	int lo = 0;
-	while (E > 0 && lo < n)
-	{
-		// choose the actor range [lo, hi); one actor never has more than K < ATT_CHUNK events
-		const int hi = block_chunk_end(c.ev, 3, E, lo, n, ATT_CHUNK, &s_red);
-		// gather the range's events
-		if (threadIdx.x == 0)
-			s_count = 0;
-		__syncthreads();
-		for (int e = threadIdx.x; e < E; e += blockDim.x)
-		{
-			const int a = c.ev[3 * e];
-			if (a >= lo && a < hi)
-			{
-				const int k = atomicAdd(&s_count, 1);
-				if (k < ATT_CHUNK)
-				{
-					ekey[k] = ((unsigned long long)a << 32) | (unsigned)c.ev[3 * e + 1];
-					etgt[k] = c.ev[3 * e + 2];
-				}
-			}
-		}
-		__syncthreads();
-		const int m = min(s_count, ATT_CHUNK);
-		if (s_count > ATT_CHUNK)
-		{
-			if (threadIdx.x == 0)
-				atomicOr(&c.sc->status, ST_CAPACITY);
-			return;
-		}
-		block_sort_pairs(ekey, etgt, m);
-		// endpoint entries keyed by (agent, event rank)
-		for (int r = threadIdx.x; r < m; r += blockDim.x)
-		{
-			const int actor = (int)(ekey[r] >> 32);
-			nkey[2 * r] = ((unsigned long long)actor << 32) | (unsigned)r;
-			nkey[2 * r + 1] = ((unsigned long long)etgt[r] << 32) | (unsigned)r;
-			npos[2 * r] = 2 * r;
-			npos[2 * r + 1] = 2 * r + 1;
-			est[r] = 0; // 0 undecided, 1 success, 2 failure
-		}
-		__syncthreads();
-		block_sort_pairs(nkey, npos, 2 * m);
-		// npos[q] now holds the (2 * rank + side) tag of sorted entry q.  Stage each
-		// endpoint's spring count and maximum, and each event's two entry positions.
-		for (int q = threadIdx.x; q < 2 * m; q += blockDim.x)
-		{
-			const int tag = npos[q];
-			const int agent = (int)(nkey[q] >> 32);
-			s_base[q] = spr_cnt[agent];
-			s_max[q] = c.max_att[agent];
-			if (tag & 1)
-				s_q1[tag >> 1] = q;
-			else
-				s_q0[tag >> 1] = q;
-		}
-		__syncthreads();
-		// rounds: each decides at least the earliest undecided event, so m + 1 rounds suffice
-		for (int round = 0;; round++)
-		{
-			if (round > m + 1)
-			{
-				if (threadIdx.x == 0)
-					atomicOr(&c.sc->status, ST_CAPACITY);
-				return;
-			}
-			if (threadIdx.x == 0)
-				s_unknown = 0;
-			__syncthreads();
-			for (int r = threadIdx.x; r < m; r += blockDim.x)
-			{
-				if (est[r] != 0)
-					continue;
-				int cond[2];
-				for (int side = 0; side < 2; side++)
-				{
-					const int q = side ? s_q1[r] : s_q0[r];
-					const unsigned agent = (unsigned)(nkey[q] >> 32);
-					int lo_c = s_base[q], hi_c = lo_c;
-					for (int w = q - 1; w >= 0 && (unsigned)(nkey[w] >> 32) == agent; w--)
-					{
-						const int rs = est[(int)(nkey[w] & 0xffffffffu)];
-						lo_c += rs == 1;
-						hi_c += rs != 2;
-					}
-					cond[side] = lo_c >= s_max[q] ? 2 : (hi_c < s_max[q] ? 1 : 0);
-				}
-				if (cond[0] == 2 || cond[1] == 2)
-					est[r] = 2;
-				else if (cond[0] == 1 && cond[1] == 1)
-					est[r] = 1;
-				else
-					atomicAdd(&s_unknown, 1);
-			}
-			__syncthreads();
-			const int u = s_unknown;
-			__syncthreads();
-			if (u == 0)
-				break;
-		}
-		// append successes per agent segment, in event order
-		for (int q = threadIdx.x; q < 2 * m; q += blockDim.x)
-		{
-			const int agent = (int)(nkey[q] >> 32);
-			if (q > 0 && (int)(nkey[q - 1] >> 32) == agent)
-				continue; // not a segment head
-			int cnt = s_base[q];
-			for (int w = q; w < 2 * m && (int)(nkey[w] >> 32) == agent; w++)
-			{
-				const int r = (int)(nkey[w] & 0xffffffffu);
-				if (est[r] != 1)
-					continue;
-				const int actor = (int)(ekey[r] >> 32);
-				const int other = actor == agent ? etgt[r] : actor;
-				if (cnt < c.L)
-					c.spr_idx[(size_t)agent * c.L + cnt] = other;
-				else
-					atomicOr(&c.sc->status, ST_LIST_OVERFLOW);
-				cnt++;
-			}
-			spr_cnt[agent] = min(cnt, c.L);
-		}
-		__threadfence();
-		__syncthreads();
-		lo = hi;
-	}
-	__syncthreads();
-	if (threadIdx.x == 0)
-		c.sc->att_events = 0;
-}
-
 // ------------------------------------------------------------------ cell-cell interactions
 // update_cell_cell_interactions_internal (interactions_solver.cpp:143-253).
 //
@@ -901,20 +1054,45 @@ enum : int
 	EV_FUSE = 2
 };
 
-template <class R>
+// AB2: first apply the agent's position update (update_positions precedes the interactions
+// and this pass reads no positions)
+template <class R, int D, bool AB2>
 __global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int g = gid();
 	const bool live_thread = g < c.sc->n;
 	const int i = live_thread ? agent_at<R>(c, g) : 0;
+	if (AB2 && live_thread && c.movable[i])
+	{
+		const R factor = (R)((double)c.dt * 1.5);
+		const R previous_factor = (R)((double)c.dt * -0.5);
+		for (int k = 0; k < D; k++)
+		{
+			const size_t o = (size_t)i * D + k;
+			const R v = c.vel[o];
+			c.pos[o] += v * factor + c.pvel[o] * previous_factor;
+			c.pvel[o] = v;
+			c.vel[o] = 0;
+		}
+	}
 	int buf_s[LANE_BUF];
 	int n_cand = 0;
 	int n_rem = 0;
 	bool acted = false;
 	if (live_thread)
 		c.atk_tgt[i] = -1;
+	// every branch compares a draw u in [0,1) with `<` against rate * dt, so an actor whose
+	// rates are all <= 0 never acts and its draws need not be made
+	bool any_rate = false;
 	if (live_thread && c.dead[i] != 1)
+	{
+		any_rate = c.dead_phago[i] > 0;
+		for (int t = 0; t < c.T; t++)
+			any_rate = any_rate || c.live_phago[(size_t)i * c.T + t] > 0 || c.attack[(size_t)i * c.T + t] > 0
+					   || c.fusion[(size_t)i * c.T + t] > 0;
+	}
+	if (any_rate)
 	{
 		const int cnt = visible_neighbors<R>(c, i);
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
@@ -923,17 +1101,36 @@ __global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 		const unsigned long long step = c.sc->step;
 		const bool self_removed = c.flags[i] == PCM_FLAG_TO_REMOVE;
 		bool attacked_once = false, phagocytosed_once = false, fused_once = false;
+		int jq[4], tq[4];
+		uint8_t dq[4], fq[4];
+		R immq[4];
 		for (int s = 0; s < cnt; s++)
 		{
-			const int j = nb[s];
-			const int tj = c.type[j];
-			const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
-			const bool dead_j = c.dead[j] == 1;
+			if ((s & 3) == 0)
+			{
+#pragma unroll
+				for (int q = 0; q < 4; q++)
+					if (s + q < cnt)
+					{
+						const int jn = nb[s + q];
+						jq[q] = jn;
+						tq[q] = c.type[jn];
+						dq[q] = c.dead[jn];
+						fq[q] = c.flags[jn];
+						immq[q] = c.immuno[(size_t)jn * c.T + ti];
+					}
+			}
+			const int j = jq[s & 3];
+			const int tj = tq[s & 3];
+			const bool dead_j = dq[s & 3] == 1;
+			const R p_dead = c.dead_phago[i] * dt;
 			const R p_live = c.live_phago[(size_t)i * c.T + tj] * dt;
-			const R p_attack = c.attack[(size_t)i * c.T + tj] * c.immuno[(size_t)j * c.T + ti] * dt;
+			const R p_attack = c.attack[(size_t)i * c.T + tj] * immq[s & 3] * dt;
 			const R p_fuse = c.fusion[(size_t)i * c.T + tj] * dt;
-			const bool candidate =
-				dead_j ? (u < c.dead_phago[i] * dt) : (u < p_live || u < p_attack || u < p_fuse);
+			if (dead_j ? !(p_dead > 0) : !(p_live > 0 || p_attack > 0 || p_fuse > 0))
+				continue; // no branch can fire: u < threshold <= 0 is false for every u
+			const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
+			const bool candidate = dead_j ? (u < p_dead) : (u < p_live || u < p_attack || u < p_fuse);
 			if (!candidate)
 				continue;
 			if (n_cand < LANE_BUF)
@@ -950,7 +1147,7 @@ __global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 					atomicOr(&c.sc->status, ST_CAPACITY);
 			}
 
-			const bool target_removed = c.flags[j] == PCM_FLAG_TO_REMOVE;
+			const bool target_removed = fq[s & 3] == PCM_FLAG_TO_REMOVE;
 			int kind = -1; // -1 none, 0 ingest, 1 attack, 2 fuse
 			if (dead_j)
 				kind = 0;
@@ -1019,13 +1216,34 @@ __global__ void k_interact_check(const SimCtx<R>* __restrict__ ctxs)
 		c.sc->conflict = 1;
 }
 
-template <class R>
+// out of the box of the microenvironment mesh (containers_solver.cpp:89-96)
+template <class R, int D>
+__device__ __forceinline__ bool out_of_bounds(const SimCtx<R>& c, int i)
+{
+	bool out = false;
+	for (int k = 0; k < D; k++)
+	{
+		const R p = c.pos[(size_t)i * D + k];
+		if (p < (R)c.bmin[k] || p > (R)c.bmax[k])
+			out = true;
+	}
+	return out;
+}
+
+// FLAG: also decide, from the agent's final state, whether the container update removes
+// it, and append it to the removal list (the speculation is exact here, so an agent with
+// a removal event targeting it is removed)
+template <class R, int D, bool FLAG>
 __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n || c.sc->conflict)
+	if (c.sc->conflict)
 		return;
+	const int i = gid();
+	const bool live_thread = i < c.sc->n;
+	bool removed = false;
+	if (live_thread)
+	{
 	const int nr = c.rem_cnt[i];
 	for (int k = 0; k < nr; k++)
 	{
@@ -1059,23 +1277,39 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 			last = best;
 		}
 	}
+	if (FLAG)
+		removed = c.n_rem_t[i] > 0 || c.flags[i] == PCM_FLAG_TO_REMOVE || out_of_bounds<R, D>(c, i);
 	c.n_rem_t[i] = 0;
 	c.n_atk_t[i] = 0;
+	}
+	if (FLAG)
+	{
+		const int k = wave_reserve(&c.sc->bad_count, removed ? 1 : 0);
+		if (removed)
+			c.bad[k] = i;
+	}
 }
 
 // Exact replay of the reference loop over the candidates, in (actor, slot) order, when the
 // speculative pass found a dependency.  Candidates are ordered CAND_CHUNK at a time.
 template <class R>
+__device__ void compact_plan_block(const SimCtx<R>& c, int* sb);
+
+// PLAN: then build the removal list when the replay ran (the apply pass built it otherwise)
+// and plan the container update in the same block.
+template <class R, int D, bool PLAN>
 __global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCtx<R>* __restrict__ ctxs)
 {
 	const SimCtx<R>& c = ctxs[blockIdx.y];
 	__shared__ unsigned long long key[CAND_CHUNK];
 	__shared__ int val[CAND_CHUNK];
 	__shared__ int s_count, s_red;
+	__shared__ int sb[PLAN ? BAD_CHUNK : 1];
 	if (c.sc->status)
 		return;
 	const int C = min(c.sc->cand_count, c.cand_cap);
-	if (c.sc->conflict && C > 0)
+	const bool replay = c.sc->conflict != 0;
+	if (replay && C > 0)
 	{
 		const int n = c.sc->n;
 		const R dt = c.dt;
@@ -1196,8 +1430,35 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCt
 	__syncthreads();
 	if (threadIdx.x == 0)
 	{
+		c.sc->stat[3] = C;
+		c.sc->stat[4] = c.sc->conflict;
 		c.sc->conflict = 0;
 		c.sc->cand_count = 0;
+	}
+	if (PLAN)
+	{
+		if (replay)
+		{
+			// the replay's writes are read back by other wavefronts: bypass the L1
+			const int n = c.sc->n;
+			const volatile uint8_t* flags = c.flags;
+			const volatile R* pos = c.pos;
+			for (int i = threadIdx.x; i < n; i += blockDim.x)
+			{
+				bool out = flags[i] == PCM_FLAG_TO_REMOVE;
+				for (int k = 0; k < D; k++)
+				{
+					const R p = pos[(size_t)i * D + k];
+					if (p < (R)c.bmin[k] || p > (R)c.bmax[k])
+						out = true;
+				}
+				if (out)
+					c.bad[atomicAdd(&c.sc->bad_count, 1)] = i;
+			}
+		}
+		__threadfence();
+		__syncthreads();
+		compact_plan_block<R>(c, sb);
 	}
 }
 
@@ -1258,13 +1519,11 @@ __device__ inline void block_sort_ints(int* a, int count)
 }
 
 template <class R>
-__global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R>* __restrict__ ctxs)
+__device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
 {
-	const SimCtx<R>& c = ctxs[blockIdx.y];
-	__shared__ int sb[BAD_CHUNK];
-	if (c.sc->status)
-		return;
-	const int b = c.sc->bad_count;
+	// the removal list may have been filled earlier in the same kernel: read past the L1
+	const int b = *(volatile int*)&c.sc->bad_count;
+	const volatile int* bad_list = c.bad;
 	if (b == 0)
 		return;
 	// ascending list of removed agents: in LDS when it fits, else sorted in place in HBM
@@ -1272,7 +1531,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R
 	if (b <= BAD_CHUNK)
 	{
 		for (int t = threadIdx.x; t < b; t += blockDim.x)
-			sb[t] = c.bad[t];
+			sb[t] = bad_list[t];
 		__syncthreads();
 		block_sort_ints(sb, b);
 	}
@@ -1296,6 +1555,12 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R
 	// walk the tail [n1, n0) downwards: good agents fill holes, bad ones get their
 	// position in the removal sequence
 	int q = 0;
+	int log = c.sc->log_len;
+	auto log_removal = [&](int position) {
+		if (log < c.log_cap)
+			c.rm_log[log] = position;
+		log++;
+	};
 	int tb = b - 1; // index of the largest not-yet-visited bad tail agent
 	int top = n0;   // agents >= top have been consumed
 	int moves = 0;
@@ -1303,6 +1568,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R
 	{
 		const int hole = list[h];
 		c.remap[hole] = -1 - q++;
+		log_removal(hole);
 		// consume tail agents from top-1 down until a good one fills the hole
 		for (;;)
 		{
@@ -1315,6 +1581,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R
 			if (tb >= holes && list[tb] == t)
 			{
 				c.remap[t] = -1 - q++;
+				log_removal(hole); // moved into the hole, then removed there
 				tb--;
 				continue;
 			}
@@ -1329,57 +1596,30 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R
 	if (top > n1)
 	{
 		c.remap[n1] = -1 - q++;
+		log_removal(n1);
 		for (int t = top - 1; t > n1; t--)
+		{
 			c.remap[t] = -1 - q++;
+			log_removal(n1); // each in turn becomes the last agent, moves into n1, is removed
+		}
 	}
+	c.sc->log_len = log;
 	c.sc->moves = moves;
+	c.sc->stat[5] = b;
+	c.sc->stat[6] = c.sc->n_occ;
 	c.sc->n = n1;
 	c.sc->deaths += deaths;
 	c.sc->order_valid = 0;
 }
 
-// move every field (and the list rows) of each surviving tail agent into its hole: one
-// wavefront per move, its lanes spread over the words of the copy plan and the rows
 template <class R>
-__global__ void k_compact_move(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R>* __restrict__ ctxs)
 {
-	PCM_PROLOGUE(c)
-	const int lane = lane_id();
-	const int waves = gridDim.x * (blockDim.x >> 6);
-	const int moves = c.sc->moves;
-	for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
-	{
-		const int src = c.move_src[k], dst = c.move_dst[k];
-		for (int u = lane; u < c.mv_n; u += 64)
-		{
-			const int f = c.mv_field[u];
-			const int nb = c.fbytes[f];
-			if (c.mv_bytes[u] == 4)
-			{
-				const uint32_t* sp = (const uint32_t*)((const uint8_t*)c.fptr[f] + (size_t)src * nb);
-				uint32_t* dp = (uint32_t*)((uint8_t*)c.fptr[f] + (size_t)dst * nb);
-				dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
-			}
-			else
-			{
-				const uint8_t* sp = (const uint8_t*)c.fptr[f] + (size_t)src * nb;
-				uint8_t* dp = (uint8_t*)c.fptr[f] + (size_t)dst * nb;
-				dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
-			}
-		}
-		for (int t = lane; t < c.L; t += 64)
-			c.spr_idx[(size_t)dst * c.L + t] = c.spr_idx[(size_t)src * c.L + t];
-		for (int t = lane; t < c.LA; t += 64)
-			c.atc_idx[(size_t)dst * c.LA + t] = c.atc_idx[(size_t)src * c.LA + t];
-		for (int t = lane; t < c.K; t += 64)
-			c.nbr_idx[(size_t)dst * c.K + t] = c.nbr_idx[(size_t)src * c.K + t];
-		if (lane == 0)
-		{
-			c.spr_cnt[dst] = c.spr_cnt[src];
-			c.atc_cnt[dst] = c.atc_cnt[src];
-			c.nbr_cnt[dst] = c.nbr_cnt[src];
-		}
-	}
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	__shared__ int sb[BAD_CHUNK];
+	if (c.sc->status)
+		return;
+	compact_plan_block<R>(c, sb);
 }
 
 // remove_attached (swap with back) for removed partners in removal order, then
@@ -1434,12 +1674,62 @@ __device__ inline void edit_links(int* list, int* cnt_ptr, const int* remap)
 	*cnt_ptr = cnt;
 }
 
+// The container update's data movement, in one launch: the first MOVE_BLOCKS blocks move
+// every field and list row of each surviving tail agent into its hole (one wavefront per
+// move, lanes over the words of the copy plan; the moved lists are edited by the mover),
+// the other blocks edit the link lists of the survivors that stay in place.
+constexpr int MOVE_BLOCKS = 16;
+
 template <class R>
-__global__ void k_compact_links(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (c.sc->bad_count == 0 || i >= c.sc->n)
+	if (c.sc->bad_count == 0)
+		return;
+	if (blockIdx.x < MOVE_BLOCKS)
+	{
+		const int lane = lane_id();
+		const int waves = MOVE_BLOCKS * (blockDim.x >> 6);
+		const int moves = c.sc->moves;
+		for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
+		{
+			const int src = c.move_src[k], dst = c.move_dst[k];
+			for (int u = lane; u < c.mv_n; u += 64)
+			{
+				const int f = c.mv_field[u];
+				const int nb = c.fbytes[f];
+				if (c.mv_bytes[u] == 4)
+				{
+					const uint32_t* sp = (const uint32_t*)((const uint8_t*)c.fptr[f] + (size_t)src * nb);
+					uint32_t* dp = (uint32_t*)((uint8_t*)c.fptr[f] + (size_t)dst * nb);
+					dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
+				}
+				else
+				{
+					const uint8_t* sp = (const uint8_t*)c.fptr[f] + (size_t)src * nb;
+					uint8_t* dp = (uint8_t*)c.fptr[f] + (size_t)dst * nb;
+					dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
+				}
+			}
+			for (int t = lane; t < c.K; t += 64)
+				c.nbr_idx[(size_t)dst * c.K + t] = c.nbr_idx[(size_t)src * c.K + t];
+			if (lane == 0)
+			{
+				c.nbr_cnt[dst] = c.nbr_cnt[src];
+				for (int t = 0; t < c.L; t++)
+					c.spr_idx[(size_t)dst * c.L + t] = c.spr_idx[(size_t)src * c.L + t];
+				c.spr_cnt[dst] = c.spr_cnt[src];
+				for (int t = 0; t < c.LA; t++)
+					c.atc_idx[(size_t)dst * c.LA + t] = c.atc_idx[(size_t)src * c.LA + t];
+				c.atc_cnt[dst] = c.atc_cnt[src];
+				edit_links(c.spr_idx + (size_t)dst * c.L, c.spr_cnt + dst, c.remap);
+				edit_links(c.atc_idx + (size_t)dst * c.LA, c.atc_cnt + dst, c.remap);
+			}
+		}
+		return;
+	}
+	const int i = (blockIdx.x - MOVE_BLOCKS) * blockDim.x + threadIdx.x;
+	if (i >= c.sc->n || c.remap[i] < 0) // holes receive their moved agent's edited lists
 		return;
 	edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
 	edit_links(c.atc_idx + (size_t)i * c.LA, c.atc_cnt + i, c.remap);

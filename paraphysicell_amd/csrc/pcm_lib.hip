@@ -74,6 +74,7 @@ struct pcm_sim
 	virtual int set_links(int which, const int32_t* off, const int32_t* idx) = 0;
 	virtual int get_links(int which, int32_t* off, int32_t* idx, int64_t* total) = 0;
 	virtual int get_neighbors(int32_t* off, int32_t* idx, int64_t* total) = 0;
+	virtual int set_neighbors(const int32_t* off, const int32_t* idx) = 0;
 	virtual int get_buckets(int32_t* counts, int32_t* agents) = 0;
 	virtual long long voxel_count() const = 0;
 	virtual int get_deaths(int64_t* d) = 0;
@@ -82,6 +83,8 @@ struct pcm_sim
 	virtual int phase(int which, const pcm_step_params* p) = 0;
 	virtual int step(const pcm_step_params* p, int steps) = 0;
 	virtual int synchronize() = 0;
+	virtual int removal_log(int32_t* positions, int64_t capacity, int64_t* length) = 0;
+	virtual int stats(int32_t* out) = 0;
 	virtual int profile_begin(uint64_t mask) = 0;
 	virtual int profile_end(double* total_ms, int64_t* launches) = 0;
 };
@@ -202,6 +205,10 @@ public:
 		PCM_TRY(alloc_t(&h.bucket, cap));
 		PCM_TRY(alloc_t(&h.occ, cap));
 		PCM_TRY(alloc_t(&h.sorted, (size_t)cap * 4));
+		PCM_TRY(alloc_t(&h.frec, (size_t)cap * 8));
+		PCM_TRY(alloc_t(&h.own_cnt, cap));
+		PCM_TRY(alloc_t(&h.in_cnt, cap));
+		PCM_TRY(alloc_t(&h.att_list, cap));
 		PCM_TRY(alloc_t(&h.fire, cap));
 		PCM_TRY(alloc_t(&h.erase, cap));
 		PCM_TRY(alloc_t(&h.n_rem_t, cap));
@@ -215,6 +222,8 @@ public:
 		PCM_TRY(alloc_t(&h.remap, cap));
 		PCM_TRY(alloc_t(&h.move_src, cap));
 		PCM_TRY(alloc_t(&h.move_dst, cap));
+		h.log_cap = (int)std::min<long long>(0x7ffffff0LL, 4LL * cap + 64);
+		PCM_TRY(alloc_t(&h.rm_log, (size_t)h.log_cap));
 		(void)dims;
 
 		std::vector<int> ident(cap);
@@ -238,14 +247,17 @@ public:
 		s.n = n;
 		s.status = 0;
 		s.att_events = 0;
+		s.att_actors = 0;
 		s.cand_count = 0;
 		s.conflict = 0;
 		s.bad_count = 0;
 		s.moves = 0;
 		s.order_valid = 0;
+		s.log_len = 0;
 		PCM_HIP(hipMemcpy(h_.sc, &s, sizeof(s), hipMemcpyHostToDevice));
 		n_launch_ = n;
 		mesh_fresh_ = false;
+		records_fresh_ = false;
 		std::vector<int> ident(std::max(n, 1));
 		for (int i = 0; i < n; i++)
 			ident[i] = i;
@@ -286,6 +298,7 @@ public:
 		PCM_HIP(hipMemcpy(h_.fptr[f], src, (size_t)n * h_.fbytes[f], hipMemcpyHostToDevice));
 		if (f == PCM_F_POSITIONS || f == PCM_F_RADIUS || f == PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE)
 			mesh_fresh_ = false;
+		records_fresh_ = false;
 		return PCM_OK;
 	}
 
@@ -404,6 +417,58 @@ public:
 		return PCM_OK;
 	}
 
+	// Host neighbor lists (states.neighbors).  A non-movable agent's reference list is
+	// empty; its device list (which feeds simple_pressure) is rebuilt as the movable agents
+	// that list it, in ascending order.
+	int set_neighbors(const int32_t* off, const int32_t* idx) override
+	{
+		if (!off || !idx)
+			return fail(PCM_ERR_ARGUMENT, "null neighbor lists");
+		PCM_HIP(hipStreamSynchronize(stream_));
+		const int n = n_launch_;
+		std::vector<uint8_t> mov(std::max(n, 1));
+		if (n > 0)
+			PCM_HIP(hipMemcpy(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
+		std::vector<std::vector<int>> lists(std::max(n, 1));
+		for (int i = 0; i < n; i++)
+		{
+			if (off[i + 1] < off[i])
+				return fail(PCM_ERR_ARGUMENT, "offsets must be non-decreasing");
+			if (!mov[i])
+				continue;
+			for (int s = off[i]; s < off[i + 1]; s++)
+			{
+				const int j = idx[s];
+				if (j < 0 || j >= n || j == i)
+					return fail(PCM_ERR_ARGUMENT, "neighbor entry is not another agent's index");
+				lists[i].push_back(j);
+			}
+		}
+		for (int i = 0; i < n; i++)
+			if (mov[i])
+				for (int s = off[i]; s < off[i + 1]; s++)
+					if (!mov[idx[s]])
+						lists[idx[s]].push_back(i);
+		size_t longest = 0;
+		for (int i = 0; i < n; i++)
+			longest = std::max(longest, lists[i].size());
+		if ((int)longest > h_.K)
+			PCM_TRY(grow_neighbors((int)longest));
+		const int K = h_.K;
+		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * K, -1);
+		for (int i = 0; i < n; i++)
+		{
+			cnt[i] = (int)lists[i].size();
+			std::copy(lists[i].begin(), lists[i].end(), rows.begin() + (size_t)i * K);
+		}
+		if (n > 0)
+		{
+			PCM_HIP(hipMemcpy(h_.nbr_cnt, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+			PCM_HIP(hipMemcpy(h_.nbr_idx, rows.data(), sizeof(int) * (size_t)n * K, hipMemcpyHostToDevice));
+		}
+		return PCM_OK;
+	}
+
 	long long voxel_count() const override { return h_.vcount; }
 
 	int get_buckets(int32_t* counts, int32_t* agents) override
@@ -489,6 +554,34 @@ public:
 		return fail(PCM_ERR_STATE, "mechanics step did not converge after repeated neighbor-capacity growth");
 	}
 
+	// positions of the removals since the last read, then the log restarts
+	int removal_log(int32_t* positions, int64_t capacity, int64_t* length) override
+	{
+		PCM_TRY(read_scalars());
+		const long long len = sc_.log_len;
+		*length = len;
+		if (!positions)
+			return PCM_OK;
+		if (len > h_.log_cap)
+			return fail(PCM_ERR_CAPACITY, "removal log overflowed: read it at least every " +
+											   std::to_string(h_.log_cap) + " removals");
+		if (capacity < len)
+			return fail(PCM_ERR_ARGUMENT, "output buffer smaller than the removal log");
+		if (len > 0)
+			PCM_HIP(hipMemcpy(positions, h_.rm_log, sizeof(int) * len, hipMemcpyDeviceToHost));
+		sc_.log_len = 0;
+		PCM_HIP(hipMemcpy(&h_.sc->log_len, &sc_.log_len, sizeof(int), hipMemcpyHostToDevice));
+		return PCM_OK;
+	}
+
+	int stats(int32_t* out) override
+	{
+		PCM_TRY(read_scalars());
+		for (int k = 0; k < 8; k++)
+			out[k] = sc_.stat[k];
+		return PCM_OK;
+	}
+
 	int synchronize() override
 	{
 		PCM_TRY(read_scalars());
@@ -506,6 +599,8 @@ private:
 	std::vector<void*> allocations_;
 	int n_launch_ = 0; // agents at the last upload: an upper bound of the device count
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
+	bool records_fresh_ = false; // force records describe the current agent fields
+	int binned_launch_ = 1 << 30; // launch bound of the last mesh update (>= agents it binned)
 
 	int width(int kind) const
 	{
@@ -612,8 +707,9 @@ private:
 		const size_t cap = std::max(1, h_.cap);
 		const size_t K = h_.K;
 		PCM_TRY(alloc_t(&h_.nbr_idx, cap * K));
-		h_.ev_cap = (int)std::min<size_t>(0x7ffffff0, cap * K / 2 + 16);
-		PCM_TRY(alloc_t(&h_.ev, (size_t)h_.ev_cap * 3));
+		PCM_TRY(alloc_t(&h_.own_slot, cap * K));
+		PCM_TRY(alloc_t(&h_.own_st, cap * K));
+		PCM_TRY(alloc_t(&h_.inbox, cap * K * 2));
 		h_.cand_cap = (int)std::min<size_t>(0x7ffffff0, cap * K + 16);
 		PCM_TRY(alloc_t(&h_.cand, (size_t)h_.cand_cap * 2));
 		return PCM_OK;
@@ -623,7 +719,9 @@ private:
 	{
 		PCM_HIP(hipStreamSynchronize(stream_));
 		release(h_.nbr_idx);
-		release(h_.ev);
+		release(h_.own_slot);
+		release(h_.own_st);
+		release(h_.inbox);
 		release(h_.cand);
 		h_.K = next_pow2(std::max(needed, 2 * h_.K));
 		PCM_TRY(alloc_neighbor_storage());
@@ -823,13 +921,18 @@ private:
 		}
 	}
 
-	template <int D>
+	// FUSED: the clear ran in this step's velocity kernel and the container update's
+	// remap restore and counter reset ride on the count and claim kernels
+	template <int D, bool FUSED = false>
 	int mesh_kernels()
 	{
 		mesh_fresh_ = true;
-		PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
-		PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D>));
-		PCM_TRY(per_agent(PCM_K_BIN_CLAIM, k_bin_claim<R>));
+		records_fresh_ = true;
+		binned_launch_ = n_launch_;
+		if (!FUSED)
+			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
+		PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D, FUSED>));
+		PCM_TRY(per_agent(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>));
 		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R>));
 		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 	}
@@ -837,18 +940,26 @@ private:
 	int spring_update_kernels()
 	{
 		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R>));
-		PCM_TRY(per_agent(PCM_K_SPRING_ERASE, k_spring_erase<R>));
+		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
+					   std::max(1, (int)(((long long)n_launch_ * h_.L + BLOCK - 1) / BLOCK)), BLOCK));
 		PCM_TRY(per_agent(PCM_K_SPRING_COMPACT, k_spring_compact<R>));
 		PCM_TRY(per_agent(PCM_K_SPRING_COLLECT, k_spring_collect<R>));
-		return launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS);
+		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
+		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
 	}
 
+	template <int D>
 	int interaction_kernels()
 	{
-		PCM_TRY(per_agent(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R>));
+		PCM_TRY(per_agent(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, false>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
-		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R>));
-		return launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R>, 1, RESOLVE_THREADS);
+		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, false>));
+		return launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, false>, 1, RESOLVE_THREADS);
+	}
+
+	int compact_apply_kernel()
+	{
+		return launch(PCM_K_COMPACT_APPLY, k_compact_apply<R>, MOVE_BLOCKS + grid_agents(), BLOCK);
 	}
 
 	template <int D>
@@ -856,8 +967,7 @@ private:
 	{
 		PCM_TRY(per_agent(PCM_K_COMPACT_FLAG, k_compact_flag<R, D>));
 		PCM_TRY(launch(PCM_K_COMPACT_PLAN, k_compact_plan<R>, 1, RESOLVE_THREADS));
-		PCM_TRY(launch(PCM_K_COMPACT_MOVE, k_compact_move<R>, 64, BLOCK));
-		PCM_TRY(per_agent(PCM_K_COMPACT_LINKS, k_compact_links<R>));
+		PCM_TRY(compact_apply_kernel());
 		PCM_TRY(per_agent(PCM_K_COMPACT_RESTORE, k_compact_restore<R>));
 		return launch(PCM_K_COMPACT_DONE, k_compact_done<R>, 1, 64);
 	}
@@ -865,6 +975,8 @@ private:
 	template <int D>
 	int phase_d(int which)
 	{
+		if (which != PH_MESH && which != PH_NEIGHBORS)
+			records_fresh_ = false;
 		switch (which)
 		{
 			case PH_MESH:
@@ -896,7 +1008,7 @@ private:
 				return per_agent(PCM_K_POSITIONS, k_positions<R, D>);
 			case PH_INTERACTIONS:
 				mesh_fresh_ = false; // fusion moves agents, ingestion changes radii
-				return interaction_kernels();
+				return interaction_kernels<D>();
 			case PH_CONTAINER:
 				mesh_fresh_ = false;
 				return container_kernels<D>();
@@ -913,11 +1025,23 @@ private:
 		PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 		if (h_.springs)
 			PCM_TRY(spring_update_kernels());
-		PCM_TRY(per_agent(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D>));
-		PCM_TRY(per_agent(PCM_K_POSITIONS, k_positions<R, D>));
-		PCM_TRY(interaction_kernels());
-		PCM_TRY(container_kernels<D>());
-		PCM_TRY(mesh_kernels<D>());
+		if (!records_fresh_)
+			PCM_TRY(per_agent(PCM_K_FORCE_RECORDS, k_force_records<R, D>));
+		// the previous mesh update's voxel counts are cleared by the velocity kernel when its
+		// grid covers every agent that update binned
+		const bool fold_clear = binned_launch_ <= n_launch_;
+		if (fold_clear)
+			PCM_TRY(per_agent(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, true>));
+		else
+			PCM_TRY(per_agent(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, false>));
+		PCM_TRY(per_agent(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, true>));
+		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
+		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, true>));
+		PCM_TRY(launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, true>, 1, RESOLVE_THREADS));
+		PCM_TRY(compact_apply_kernel());
+		if (!fold_clear)
+			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
+		PCM_TRY((mesh_kernels<D, true>()));
 		return neighbors_d<D>();
 	}
 
@@ -1045,6 +1169,12 @@ int pcm_get_links(pcm_sim* sim, int32_t which, int32_t* offsets, int32_t* indice
 	return sim->get_links(which, offsets, indices, &t);
 }
 
+int pcm_set_neighbors(pcm_sim* sim, const int32_t* offsets, const int32_t* indices)
+{
+	PCM_NEED(sim);
+	return sim->set_neighbors(offsets, indices);
+}
+
 int pcm_get_neighbors_size(pcm_sim* sim, int64_t* total)
 {
 	PCM_NEED(sim);
@@ -1140,6 +1270,22 @@ int pcm_synchronize(pcm_sim* sim)
 {
 	PCM_NEED(sim);
 	return sim->synchronize();
+}
+
+int pcm_removal_log(pcm_sim* sim, int32_t* positions, int64_t capacity, int64_t* length)
+{
+	PCM_NEED(sim);
+	if (!length)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->removal_log(positions, capacity, length);
+}
+
+int pcm_get_stats(pcm_sim* sim, int32_t* stats)
+{
+	PCM_NEED(sim);
+	if (!stats)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->stats(stats);
 }
 
 int pcm_kernel_name(int32_t kernel, const char** name)

@@ -41,6 +41,7 @@ def _declare(lib, prefix: str):
         "set_links": [P, ctypes.c_int32, P, P],
         "get_links_size": [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)],
         "get_links": [P, ctypes.c_int32, P, P],
+        "set_neighbors": [P, P, P],
         "get_neighbors_size": [P, ctypes.POINTER(ctypes.c_int64)],
         "get_neighbors": [P, P, P],
         "get_voxel_count": [P, ctypes.POINTER(ctypes.c_int64)],
@@ -138,6 +139,12 @@ class _LibSim:
             if idx.size == 0:
                 idx = np.zeros(1, np.int32)
             self._call("set_links", self._h, which, off.ctypes.data, idx.ctypes.data)
+        if state.neighbors is not None:
+            off = np.ascontiguousarray(state.neighbors[0], dtype=np.int32)
+            idx = np.ascontiguousarray(state.neighbors[1], dtype=np.int32)
+            if idx.size == 0:
+                idx = np.zeros(1, np.int32)
+            self._call("set_neighbors", self._h, off.ctypes.data, idx.ctypes.data)
 
     def get_field(self, name: str) -> np.ndarray:
         ftype, wkind = abi.FIELDS[abi.FIELD_ID[name]][1:]
@@ -154,6 +161,7 @@ class _LibSim:
             st.arrays[name] = self.get_field(name)
         for which in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED):
             st.links[which] = self.links(which)
+        st.neighbors = self.neighbors()
         return st
 
     def links(self, which: int):
@@ -274,6 +282,8 @@ def load_library(path: str | None = None):
     lib.pcm_profile_begin.restype = ctypes.c_int
     lib.pcm_profile_end.argtypes = [_PTR, ctypes.c_void_p, ctypes.c_void_p]
     lib.pcm_profile_end.restype = ctypes.c_int
+    lib.pcm_get_stats.argtypes = [_PTR, ctypes.c_void_p]
+    lib.pcm_get_stats.restype = ctypes.c_int
     if path is None:
         _LIB = lib
     return lib
@@ -290,6 +300,15 @@ class DeviceSim(_LibSim):
 
     def synchronize(self):
         self._call("synchronize", self._h)
+
+    STAT_NAMES = ("attach_events", "resolve_rounds", "resolve_chunks", "interaction_candidates",
+                  "interaction_replay", "removed", "occupied_voxels", "reserved")
+
+    def stats(self) -> dict:
+        """Work counters of the most recent step."""
+        out = (ctypes.c_int32 * 8)()
+        self._call("get_stats", self._h, out)
+        return dict(zip(self.STAT_NAMES, list(out)))
 
     # ------------------------------------------------------------------ kernel timing
     def profile_begin(self, kernels) -> None:

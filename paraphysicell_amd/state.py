@@ -27,6 +27,8 @@ class AgentState:
         self.arrays["origin"][:] = np.arange(self.count, dtype=np.int32)
         empty = (np.zeros(self.count + 1, np.int32), np.zeros(0, np.int32))
         self.links = {abi.LINKS_SPRINGS: empty, abi.LINKS_ATTACHED: (empty[0].copy(), empty[1].copy())}
+        # states.neighbors as CSR, or None when the state carries no lists
+        self.neighbors = None
 
     # ------------------------------------------------------------------ access
     @property

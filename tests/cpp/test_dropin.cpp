@@ -1,0 +1,308 @@
+// Drop-in test: the reference's solver classes replaced by pcm_dropin (libpcm.so on the GPU),
+// driven like test/test.cpp drives the reference (calls made by every thread of an OpenMP
+// team), compared with the oracle solver on an identical environment.
+//
+// Exit status 0 when every case passes.  Run by tests/test_dropin.py on a GPU box.
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+#include "../../include/pcm_dropin.hpp"
+#include "../../oracle/pcm_oracle.hpp"
+
+namespace pcm_oracle {
+// customization points of the drop-in for the oracle's environment stand-in
+template <class R>
+cell_data<R>& pcm_cell_data(environment<R>& e)
+{
+	return e.data;
+}
+
+template <class R>
+void pcm_host_remove_at(environment<R>& e, int i)
+{
+	e.remove_at(i);
+}
+} // namespace pcm_oracle
+
+using namespace pcm_oracle;
+
+static int failures = 0;
+
+#define EXPECT(cond, ...)                                                                                              \
+	do                                                                                                                 \
+	{                                                                                                                  \
+		if (!(cond))                                                                                                   \
+		{                                                                                                              \
+			failures++;                                                                                                \
+			std::printf("FAIL %s:%d: ", __FILE__, __LINE__);                                                           \
+			std::printf(__VA_ARGS__);                                                                                  \
+			std::printf("\n");                                                                                         \
+		}                                                                                                              \
+	} while (0)
+
+static bool float_eq(double a, double b)
+{
+	const float fa = (float)a, fb = (float)b;
+	if (fa == fb)
+		return true;
+	int32_t ia, ib;
+	std::memcpy(&ia, &fa, 4);
+	std::memcpy(&ib, &fb, 4);
+	if (ia < 0)
+		ia = (int32_t)0x80000000 - ia;
+	if (ib < 0)
+		ib = (int32_t)0x80000000 - ib;
+	return std::llabs((long long)ia - ib) <= 4;
+}
+
+// host_velocity_solver.simple (test/test.cpp:141-191) through the drop-in
+template <class R>
+void velocity_simple(int dims)
+{
+	pcm_mesh_desc md {};
+	md.dims = dims;
+	for (int k = 0; k < 3; k++)
+	{
+		md.bbox_min[k] = 0;
+		md.bbox_max[k] = 80;
+		md.voxel[k] = 30;
+	}
+	environment<R> e(md, 3, 2);
+	e.resize(3);
+	auto& d = e.data;
+	for (int i = 0; i < 3; i++)
+	{
+		d.is_movable[i] = 1;
+		d.states.number_of_nuclei[i] = 1;
+	}
+	const int offsets[2] = { 0, 10 };
+	for (int c = 0; c < 2; c++)
+	{
+		const int o = offsets[c];
+		for (int k = 0; k < dims; k++)
+			d.agent_data.positions[c * dims + k] = (R)(1 + o + k);
+		d.cell_definition_indices[c] = c;
+		d.geometries.radius[c] = (R)(4 + o);
+		d.mechanics.cell_cell_adhesion_strength[c] = (R)(5 + o);
+		d.mechanics.cell_cell_repulsion_strength[c] = (R)(6 + o);
+		d.mechanics.relative_maximum_adhesion_distance[c] = (R)(7 + o);
+		for (int t = 0; t < 3; t++)
+			d.mechanics.cell_adhesion_affinities[c * 3 + t] = (R)(8 + o + t);
+	}
+	for (int k = 0; k < dims; k++)
+		d.agent_data.positions[2 * dims + k] = 70;
+
+	pcm_dropin::containers_solver<environment<R>> cs;
+	cs.initialize(e);
+#pragma omp parallel
+	{
+		cs.update_mechanics_mesh(e);
+		pcm_dropin::position_solver<environment<R>>::update_cell_neighbors(e);
+		pcm_dropin::position_solver<environment<R>>::update_cell_forces(e);
+	}
+	pcm_dropin::sync_to_host(e);
+	const double expected = dims == 3 ? 55.61362893236258 : (dims == 2 ? 69.556601146394243 : 100.15967659502964);
+	for (int k = 0; k < dims; k++)
+	{
+		EXPECT(float_eq(d.velocities[k], expected), "simple dims %d: v0 %g", dims, (double)d.velocities[k]);
+		EXPECT(float_eq(d.velocities[dims + k], -expected), "simple dims %d: v1 %g", dims,
+			   (double)d.velocities[dims + k]);
+		EXPECT(d.velocities[2 * dims + k] == 0, "simple dims %d: v2", dims);
+	}
+	pcm_dropin::release(e);
+}
+
+// a dense random population with springs, interactions and removals
+template <class R>
+void fill_population(environment<R>& e, int n, unsigned seed)
+{
+	std::mt19937 g(seed);
+	std::uniform_real_distribution<double> u(0.0, 1.0);
+	const int D = e.m.mesh.dims;
+	e.resize(n);
+	auto& d = e.data;
+	for (int i = 0; i < n; i++)
+	{
+		for (int k = 0; k < D; k++)
+		{
+			d.agent_data.positions[i * D + k] = (R)(-80 + 160 * u(g));
+			d.velocities[i * D + k] = (R)(u(g) - 0.5);
+			d.previous_velocities[i * D + k] = (R)(u(g) - 0.5);
+			d.motilities.migration_bias_direction[i * D + k] = (R)(u(g) - 0.5);
+		}
+		d.geometries.radius[i] = (R)(7 + 3 * u(g));
+		d.geometries.nuclear_radius[i] = 5;
+		d.agent_data.volumes[i] = 2494;
+		d.volumes.fluid[i] = 1870;
+		d.volumes.solid[i] = 624;
+		d.volumes.nuclear[i] = 540;
+		d.volumes.nuclear_fluid[i] = 405;
+		d.volumes.nuclear_solid[i] = 135;
+		d.volumes.cytoplasmic[i] = 1954;
+		d.volumes.cytoplasmic_fluid[i] = 1465;
+		d.volumes.cytoplasmic_solid[i] = 489;
+		d.mechanics.cell_cell_adhesion_strength[i] = (R)0.4;
+		d.mechanics.cell_cell_repulsion_strength[i] = 10;
+		d.mechanics.cell_BM_repulsion_strength[i] = 100;
+		d.mechanics.relative_maximum_adhesion_distance[i] = (R)1.25;
+		for (int t = 0; t < e.cell_definitions_count; t++)
+		{
+			d.mechanics.cell_adhesion_affinities[i * e.cell_definitions_count + t] = (R)(0.5 + u(g));
+			d.interactions.attack_rates[i * e.cell_definitions_count + t] = (R)(u(g) < 0.3 ? 1.0 : 0.0);
+			d.interactions.immunogenicities[i * e.cell_definitions_count + t] = 1;
+			d.interactions.fusion_rates[i * e.cell_definitions_count + t] = (R)(u(g) < 0.1 ? 0.5 : 0.0);
+		}
+		d.mechanics.maximum_number_of_attachments[i] = 4;
+		d.mechanics.attachment_elastic_constant[i] = (R)0.02;
+		d.mechanics.attachment_rate[i] = (R)(2 * u(g));
+		d.mechanics.detachment_rate[i] = (R)(0.5 * u(g));
+		d.motilities.is_motile[i] = u(g) < 0.8;
+		d.motilities.persistence_time[i] = (R)(0.2 + u(g));
+		d.motilities.migration_speed[i] = 1;
+		d.motilities.migration_bias[i] = (R)(u(g) < 0.3 ? u(g) : 0.0);
+		d.deaths.dead[i] = u(g) < 0.05;
+		d.interactions.dead_phagocytosis_rate[i] = (R)(2 * u(g));
+		d.interactions.damage_rate[i] = 1;
+		d.states.number_of_nuclei[i] = 1;
+		d.cell_definition_indices[i] = (int)(u(g) * e.cell_definitions_count);
+		d.is_movable[i] = u(g) < 0.9;
+		d.flags[i] = cell_state_flag::none;
+	}
+	// a few agents leave the box during the first step
+	for (int i = 0; i < n / 100 + 1; i++)
+	{
+		d.agent_data.positions[i * D] = (R)(e.m.mesh.bounding_box_maxs[0] - 0.5);
+		d.velocities[i * D] = 50;
+	}
+}
+
+template <class R>
+double rel_err(const std::vector<R>& a, const std::vector<R>& b, size_t count)
+{
+	double worst = 0;
+	for (size_t k = 0; k < count; k++)
+		worst = std::max(worst, std::abs((double)a[k] - (double)b[k]) / std::max(std::abs((double)b[k]), 1.0));
+	return worst;
+}
+
+template <class R>
+void population_steps(int dims, int n, int steps, bool fused)
+{
+	const double tol = sizeof(R) == 4 ? 1e-5 : 1e-12;
+	pcm_mesh_desc md {};
+	md.dims = dims;
+	for (int k = 0; k < 3; k++)
+	{
+		md.bbox_min[k] = -150;
+		md.bbox_max[k] = 150;
+		md.voxel[k] = 30;
+	}
+	environment<R> ref(md, 2, 1), dev(md, 2, 1);
+	fill_population(ref, n, 42 + dims);
+	fill_population(dev, n, 42 + dims);
+	for (auto* e : { &ref, &dev })
+	{
+		e->mechanics_time_step = (R)0.1;
+		e->virtual_wall_at_domain_edges = true;
+		e->automated_spring_adhesion = true;
+	}
+	ref.rng.mode = rng_mode::keyed;
+	ref.rng.seed = 99;
+	pcm_dropin::set_seed(dev, 99);
+
+	solver<R> oracle;
+	oracle.initialize(ref);
+	oracle.update_mechanics_mesh(ref);
+	oracle.update_cell_neighbors(ref);
+
+	pcm_dropin::solver<environment<R>> s;
+	s.initialize(dev);
+	s.containers.update_mechanics_mesh(dev);
+	s.position.update_cell_neighbors(dev);
+
+	for (int step = 0; step < steps; step++)
+	{
+		oracle.step(ref);
+		if (fused)
+			s.simulate_mechanics(dev, 1);
+		else
+		{
+#pragma omp parallel
+			{
+				s.position.update_cell_forces(dev);
+				s.position.update_motility(dev);
+				s.position.update_basement_membrane_interactions(dev);
+				s.position.update_spring_attachments(dev);
+				s.position.update_positions(dev);
+				s.interactions.update_cell_cell_interactions(dev);
+				s.containers.update_cell_container_for_mechanics(dev);
+				s.containers.update_mechanics_mesh(dev);
+				s.position.update_cell_neighbors(dev);
+			}
+		}
+		pcm_dropin::sync_to_host(dev);
+		auto& a = dev.data;
+		auto& b = ref.data;
+		EXPECT(a.agents_count == b.agents_count, "dims %d step %d: count %d vs %d", dims, step, a.agents_count,
+			   b.agents_count);
+		if (a.agents_count != b.agents_count)
+		{
+			pcm_dropin::release(dev);
+			return;
+		}
+		const size_t nn = (size_t)b.agents_count;
+		EXPECT(rel_err(a.agent_data.positions, b.agent_data.positions, nn * dims) <= tol, "dims %d step %d positions",
+			   dims, step);
+		EXPECT(rel_err(a.previous_velocities, b.previous_velocities, nn * dims) <= tol, "dims %d step %d velocities",
+			   dims, step);
+		EXPECT(rel_err(a.states.simple_pressure, b.states.simple_pressure, nn) <= tol, "dims %d step %d pressure", dims,
+			   step);
+		EXPECT(rel_err(a.states.damage, b.states.damage, nn) <= tol, "dims %d step %d damage", dims, step);
+		EXPECT(rel_err(a.agent_data.volumes, b.agent_data.volumes, nn) <= tol, "dims %d step %d volumes", dims, step);
+		bool lists_equal = true;
+		for (size_t i = 0; i < nn; i++)
+			lists_equal = lists_equal && a.states.neighbors[i] == b.states.neighbors[i]
+						  && a.states.springs[i] == b.states.springs[i]
+						  && a.states.attached_cells[i] == b.states.attached_cells[i];
+		EXPECT(lists_equal, "dims %d step %d: neighbor/spring lists differ", dims, step);
+		bool buckets_equal = dev.cells_in_mechanics_voxels == ref.cells_in_mechanics_voxels;
+		EXPECT(buckets_equal, "dims %d step %d: voxel buckets differ", dims, step);
+		EXPECT(dev.deaths_count == ref.deaths_count, "dims %d step %d: deaths %d vs %d", dims, step, dev.deaths_count,
+			   ref.deaths_count);
+		// continue from identical states: the oracle's state is the next step's input
+		std::vector<const void*> src(PCM_FIELD_COUNT, nullptr);
+		std::vector<size_t> bytes(PCM_FIELD_COUNT, 0);
+		for_each_field(b, dims, ref.cell_definitions_count, ref.m.substrates_count, [&](int id, auto& v, index_t) {
+			src[id] = v.data();
+			bytes[id] = v.size() * sizeof(v[0]);
+		});
+		for_each_field(a, dims, ref.cell_definitions_count, ref.m.substrates_count, [&](int id, auto& v, index_t) {
+			std::memcpy((void*)v.data(), src[id], std::min(bytes[id], v.size() * sizeof(v[0])));
+		});
+		a.states.springs = b.states.springs;
+		a.states.attached_cells = b.states.attached_cells;
+		pcm_dropin::host_modified(dev);
+	}
+	std::printf("population dims %d %s %s: %d agents after %d steps, %d deaths\n", dims,
+				sizeof(R) == 4 ? "FP32" : "FP64", fused ? "fused" : "phases", dev.data.agents_count, steps,
+				dev.deaths_count);
+	pcm_dropin::release(dev);
+}
+
+int main()
+{
+	for (int dims = 1; dims <= 3; dims++)
+	{
+		velocity_simple<float>(dims);
+		velocity_simple<double>(dims);
+	}
+	for (int dims = 2; dims <= 3; dims++)
+	{
+		population_steps<float>(dims, dims == 3 ? 1500 : 700, 4, false);
+		population_steps<float>(dims, dims == 3 ? 1500 : 700, 4, true);
+		population_steps<double>(dims, dims == 3 ? 1500 : 700, 3, false);
+	}
+	std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
+	return failures ? 1 : 0;
+}
