@@ -70,6 +70,20 @@ def time_cpu_baseline(seconds: float, threads: int, n_agents: int):
     return agent_steps / (ms / 1000.0), steps, ms
 
 
+def copy_calibration(mib: int) -> None:
+    """A device-to-device copy of ``mib`` MiB, launched 5 times: read mib MiB, write mib MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,7 +93,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS or all host cores")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (5 launches): the reference point that "
+                         "scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (every simulation here fits one "
@@ -147,7 +168,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"3D mechanics step, {args.agents} agents in a 380 um ball, 2 cell types 70/30, springs "
-                        "+ attack + dead phagocytosis, BM on, dt_mech 0.1 min (configs[2])",
+                        "+ attack + dead phagocytosis, BM on, dt_mech 0.1 min",
             "agents": args.agents,
             "agents_at_end": n_after,
             "dims": 3,

@@ -408,7 +408,8 @@ __device__ __forceinline__ void membrane_push(R position, R wall, R sign, R radi
 // update_motility_single (position_solver.cpp:115-145) for one agent.  The 3D walk is
 // planar because the reference passes the restrict_to_2d array pointer as the bool.
 template <int D, class R>
-__device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsigned long long step, R* v)
+__device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsigned long long step, R* v,
+											   bool writer = true)
 {
 	if (c.motile[i] == 0)
 		return;
@@ -448,7 +449,8 @@ __device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsign
 		for (int k = 0; k < D; k++)
 		{
 			m[k] = length > zero_threshold<R>() ? m[k] * speed / length : (R)0;
-			mv[k] = m[k];
+			if (writer)
+				mv[k] = m[k];
 		}
 	}
 	for (int k = 0; k < D; k++)

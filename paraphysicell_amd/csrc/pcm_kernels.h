@@ -15,6 +15,8 @@ constexpr int RESOLVE_THREADS = 1024;
 constexpr int CAND_CHUNK = 4096;    // interaction candidates ordered per LDS pass
 constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
 constexpr int LANE_BUF = 8;         // appends a lane buffers before one wavefront-wide reservation
+constexpr int VELOCITY_LANES = 8;   // lanes that share one agent in the velocity kernel
+constexpr int NBR_WAVES = 4;        // wavefronts that share one voxel in the neighbor kernel
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -240,7 +242,7 @@ __device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
 }
 
 template <class R, int D>
-__global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(64 * NBR_WAVES) k_neighbors(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	constexpr int TILE = nbr_tile<R>();
@@ -248,7 +250,9 @@ __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ 
 	__shared__ int s_idx[TILE];
 	__shared__ int s_pre[28];
 	__shared__ int s_vs[27];
-	const int lane = threadIdx.x;
+	const int tid = threadIdx.x;
+	const int lane = tid & 63;
+	const int wave = tid >> 6;
 	const int n_occ = c.sc->n_occ;
 	const unsigned long long below = (1ull << lane) - 1ull;
 	for (int w = blockIdx.x; w < n_occ; w += gridDim.x)
@@ -256,69 +260,58 @@ __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ 
 		const int v = c.occ[w];
 		int co[3];
 		voxel_coords<R>(c, v, co);
-		int cnt = 0, vs = 0;
-		if (lane < 27)
+		if (wave == 0)
 		{
-			const int x = co[0] + lane % 3 - 1, y = co[1] + (lane / 3) % 3 - 1, z = co[2] + lane / 9 - 1;
-			if (x >= 0 && x < c.grid[0] && y >= 0 && y < c.grid[1] && z >= 0 && z < c.grid[2])
+			int cnt = 0, vs = 0;
+			if (lane < 27)
 			{
-				const int nv = x + c.grid[0] * (y + c.grid[1] * z);
-				cnt = c.vcnt[nv];
-				vs = c.vstart[nv];
+				const int x = co[0] + lane % 3 - 1, y = co[1] + (lane / 3) % 3 - 1, z = co[2] + lane / 9 - 1;
+				if (x >= 0 && x < c.grid[0] && y >= 0 && y < c.grid[1] && z >= 0 && z < c.grid[2])
+				{
+					const int nv = x + c.grid[0] * (y + c.grid[1] * z);
+					cnt = c.vcnt[nv];
+					vs = c.vstart[nv];
+				}
 			}
+			int C;
+			const int pre = wave_exclusive_scan(cnt, C);
+			if (lane < 27)
+			{
+				s_pre[lane] = pre;
+				s_vs[lane] = vs;
+			}
+			if (lane == 0)
+				s_pre[27] = C;
 		}
-		int C;
-		const int pre = wave_exclusive_scan(cnt, C);
-		if (lane < 27)
-		{
-			s_pre[lane] = pre;
-			s_vs[lane] = vs;
-		}
-		if (lane == 0)
-			s_pre[27] = C;
 		__syncthreads();
+		const int C = s_pre[27];
 		const int m = s_pre[14] - s_pre[13]; // the centre voxel is stencil entry 13
 		if (C <= TILE)
 		{
-			// stage the stencil: four candidates per lane in flight
-			for (int t0 = 0; t0 < C; t0 += 256)
+			// stage the stencil in stencil order; a candidate's stencil entry is the number
+			// of entry starts at or below its position
+			int pre[28];
+#pragma unroll
+			for (int k = 0; k < 28; k++)
+				pre[k] = s_pre[k];
+			for (int t = tid; t < C; t += 64 * NBR_WAVES)
 			{
-				int gi[4], jj[4];
-				R rec[4][4];
+				int sv = 0;
 #pragma unroll
-				for (int u = 0; u < 4; u++)
-				{
-					const int t = t0 + lane + 64 * u;
-					gi[u] = -1;
-					if (t < C)
-					{
-						int sv = 0;
-						while (s_pre[sv + 1] <= t)
-							sv++;
-						gi[u] = s_vs[sv] + t - s_pre[sv];
-					}
-				}
-#pragma unroll
-				for (int u = 0; u < 4; u++)
-					if (gi[u] >= 0)
-					{
-						jj[u] = c.bucket[gi[u]];
-						for (int k = 0; k < 4; k++)
-							rec[u][k] = c.sorted[(size_t)gi[u] * 4 + k];
-					}
-#pragma unroll
-				for (int u = 0; u < 4; u++)
-					if (gi[u] >= 0)
-					{
-						const int t = t0 + lane + 64 * u;
-						s_idx[t] = jj[u];
-						for (int k = 0; k < 4; k++)
-							s_rec[t * 4 + k] = rec[u][k];
-					}
+				for (int k = 1; k < 27; k++)
+					sv += pre[k] <= t;
+				const int g = s_vs[sv] + t - s_pre[sv];
+				const int j = c.bucket[g];
+				R rec[4];
+				for (int k = 0; k < 4; k++)
+					rec[k] = c.sorted[(size_t)g * 4 + k];
+				s_idx[t] = j;
+				for (int k = 0; k < 4; k++)
+					s_rec[t * 4 + k] = rec[k];
 			}
 			__syncthreads();
-			const int own0 = s_pre[13];
-			for (int a = 0; a < m; a++)
+			const int own0 = pre[13];
+			for (int a = wave; a < m; a += NBR_WAVES)
 			{
 				const int i = s_idx[own0 + a];
 				R pi[4];
@@ -365,7 +358,7 @@ __global__ void __launch_bounds__(64) k_neighbors(const SimCtx<R>* __restrict__ 
 		{
 			// stencil larger than the LDS tile: one lane per agent, straight from HBM
 			const int s0 = c.vstart[v];
-			for (int a = lane; a < m; a += 64)
+			for (int a = tid; a < m; a += 64 * NBR_WAVES)
 				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
 		}
 		__syncthreads();
@@ -635,6 +628,113 @@ __global__ void k_velocity_fused(const SimCtx<R>* __restrict__ ctxs)
 	}
 	for (int k = 0; k < D; k++)
 		c.vel[(size_t)i * D + k] = v[k];
+}
+
+// k_velocity_fused with G lanes per agent: lane s of a group evaluates list slots s, s+G,
+// ... (gather + pair arithmetic), then every lane of the group adds the slot terms in
+// list order, read through lane shuffles, so sums are rounded exactly as the serial
+// reference sums them.  Motility and the membrane run redundantly on the group's lanes
+// (lane 0 writes), spring contraction is spread over lanes like the neighbor loop.
+template <class R, int D, bool CLEAR, int G>
+__global__ void __launch_bounds__(BLOCK) k_velocity_lanes(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int t = gid();
+	if (CLEAR)
+	{
+		if (t < c.sc->n_binned)
+			c.vcnt[c.vox_of[t]] = 0;
+		if (t == 0)
+		{
+			c.sc->cursor = 0;
+			c.sc->n_occ = 0;
+		}
+	}
+	const int g = t / G;
+	const int lane = t & (G - 1);
+	if (g >= c.sc->n)
+		return;
+	const int i = agent_at<R>(c, g);
+	const force_rec<R> me = read_force_record<R>(c, i);
+	const bool mov = me.movable;
+	R v[D], p[D];
+	for (int k = 0; k < D; k++)
+	{
+		p[k] = me.p[k];
+		v[k] = c.vel[(size_t)i * D + k];
+	}
+	const R* aff_row = c.affinity + (size_t)i * c.T;
+	const int cnt = c.nbr_cnt[i];
+	const int* nb = c.nbr_idx + (size_t)i * c.K;
+	const R coefficient = pressure_coefficient<R>();
+	R pressure = 0;
+	for (int base = 0; base < cnt; base += G)
+	{
+		const int s = base + lane;
+		R f[D], mine = 0, theirs = 0;
+		for (int k = 0; k < D; k++)
+			f[k] = 0;
+		if (s < cnt)
+		{
+			const int j = nb[s];
+			const force_rec<R> other = read_force_record<R>(c, j);
+			const R aff_ba = c.affinity[(size_t)j * c.T + me.type];
+			R zero[D];
+			for (int k = 0; k < D; k++)
+				zero[k] = -(R)0;
+			// the pair's velocity term alone: -0 + x is exactly x for every x
+			const R term = pair_force_rec<D, R>(me, other, aff_row[other.type], aff_ba, mov, zero);
+			for (int k = 0; k < D; k++)
+				f[k] = zero[k];
+			const R tp = term * coefficient;
+			mine = mov ? tp : (R)0;
+			theirs = other.movable ? tp : (R)0;
+		}
+		const int m = min(G, cnt - base);
+		for (int u = 0; u < m; u++)
+		{
+			if (mov)
+				for (int k = 0; k < D; k++)
+					v[k] += __shfl(f[k], u, G);
+			pressure += __shfl(mine, u, G);
+			pressure += __shfl(theirs, u, G);
+		}
+	}
+	if (lane == 0)
+		c.pressure[i] = pressure;
+	motility_agent<D, R>(c, i, c.sc->step, v, lane == 0);
+	if (mov && c.wall)
+		membrane_agent<D, R>(c, i, p, v);
+	if (mov && c.springs)
+	{
+		// spring_contract_function (position_solver.cpp:337-370)
+		const int ns = c.spr_cnt[i];
+		const int* sp = c.spr_idx + (size_t)i * c.L;
+		const R ki = c.elastic[i];
+		for (int base = 0; base < ns; base += G)
+		{
+			const int s = base + lane;
+			R f[D];
+			for (int k = 0; k < D; k++)
+				f[k] = 0;
+			if (s < ns)
+			{
+				const int j = sp[s];
+				const force_rec<R> other = read_force_record<R>(c, j);
+				const R kij = ki * c.elastic[j] * aff_row[other.type] * c.affinity[(size_t)j * c.T + me.type];
+				const R adhesion = (R)sqrt((double)kij);
+				for (int k = 0; k < D; k++)
+					f[k] = adhesion * (other.p[k] - p[k]);
+			}
+			const int m = min(G, ns - base);
+			for (int u = 0; u < m; u++)
+				for (int k = 0; k < D; k++)
+					v[k] += __shfl(f[k], u, G);
+		}
+	}
+	if (lane == 0)
+		for (int k = 0; k < D; k++)
+			c.vel[(size_t)i * D + k] = v[k];
 }
 
 // ------------------------------------------------------------------ springs

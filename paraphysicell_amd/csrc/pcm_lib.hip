@@ -902,7 +902,7 @@ private:
 	int neighbors_d()
 	{
 		if (mesh_fresh_)
-			return launch(PCM_K_NEIGHBORS, k_neighbors<R, D>, neighbor_blocks(), 64);
+			return launch(PCM_K_NEIGHBORS, k_neighbors<R, D>, neighbor_blocks(), 64 * NBR_WAVES);
 		// the voxel buckets predate position changes: search each agent's stencil around
 		// its current position, as the reference does with stale buckets
 		return per_agent(PCM_K_NEIGHBORS, k_neighbors_direct<R, D>);
@@ -1030,10 +1030,11 @@ private:
 		// the previous mesh update's voxel counts are cleared by the velocity kernel when its
 		// grid covers every agent that update binned
 		const bool fold_clear = binned_launch_ <= n_launch_;
+		const int vblocks = std::max(1, (int)(((long long)n_launch_ * VELOCITY_LANES + BLOCK - 1) / BLOCK));
 		if (fold_clear)
-			PCM_TRY(per_agent(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, true>));
+			PCM_TRY(launch(PCM_K_VELOCITY_FUSED, k_velocity_lanes<R, D, true, VELOCITY_LANES>, vblocks, BLOCK));
 		else
-			PCM_TRY(per_agent(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, false>));
+			PCM_TRY(launch(PCM_K_VELOCITY_FUSED, k_velocity_lanes<R, D, false, VELOCITY_LANES>, vblocks, BLOCK));
 		PCM_TRY(per_agent(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, true>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, true>));
