@@ -34,10 +34,9 @@ struct Scalars
 	int n;                 // agents_count
 	int status;            // ST_* bits
 	unsigned long long step; // mechanics step index (keys the random stream)
-	int cursor;            // bucket allocation cursor of the voxel counting sort
+	unsigned long long claim; // voxel claims of the mesh update: agents << 32 | voxels (zero between updates)
 	int n_binned;          // agents binned by the last mesh update (for the sparse clear)
-	int att_events;        // spring attachment events collected this step
-	int att_actors;        // agents with attachment events this step
+	int pad0;
 	int cand_count;        // interaction candidate slots collected this step
 	int conflict;          // interaction speculation found a cross-agent dependency
 	int bad_count;         // agents the container update removes this step
@@ -110,7 +109,9 @@ struct SimCtx
 	int* own_cnt;            // attachment events of an actor this step
 	int* inbox;              // per agent, K (actor, k) pairs: events that target it
 	int* in_cnt;             // inbox fill (zero outside the spring phase)
-	int* att_list;           // actors with attachment events
+	int* att_list;           // actors with attachment events, a fixed segment per collect block
+	int* blk_cnt;            // actors listed in each collect block's segment
+	int* att_c;              // the actors of all segments, compacted by the resolver
 	// interaction scratch
 	int* cand;               // candidate (actor, slot) pairs
 	int *n_rem_t, *n_atk_t;  // removal / attack events targeting an agent
@@ -405,21 +406,45 @@ __device__ __forceinline__ void membrane_push(R position, R wall, R sign, R radi
 	velocity += rep * distance;
 }
 
+// Inputs of update_motility_single for one agent, loaded ahead of use.
+template <int D, class R>
+struct motility_in
+{
+	bool motile;
+	R m[D], dir[D];
+	R persistence, bias, speed;
+};
+
+template <int D, class R>
+__device__ __forceinline__ motility_in<D, R> load_motility(const SimCtx<R>& c, int i)
+{
+	motility_in<D, R> in;
+	in.motile = c.motile[i] != 0;
+	for (int k = 0; k < D; k++)
+	{
+		in.m[k] = c.motility[(size_t)i * D + k];
+		in.dir[k] = c.bias_dir[(size_t)i * D + k];
+	}
+	in.persistence = c.persistence[i];
+	in.bias = c.bias[i];
+	in.speed = c.speed[i];
+	return in;
+}
+
 // update_motility_single (position_solver.cpp:115-145) for one agent.  The 3D walk is
 // planar because the reference passes the restrict_to_2d array pointer as the bool.
 template <int D, class R>
-__device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsigned long long step, R* v,
-											   bool writer = true)
+__device__ __forceinline__ void motility_apply(const SimCtx<R>& c, int i, unsigned long long step,
+											   const motility_in<D, R>& in, R* v, bool writer)
 {
-	if (c.motile[i] == 0)
+	if (!in.motile)
 		return;
-	R* mv = c.motility + (size_t)i * D;
 	R m[D];
 	for (int k = 0; k < D; k++)
-		m[k] = mv[k];
+		m[k] = in.m[k];
 	const pcm_u32x4 blk = pcm_block(c.seed, step, PCM_STREAM_MOTILITY, (uint32_t)i, 0u, 0u);
 	const R u0 = pcm_draw<R>::from(blk, 0);
-	if (u0 < c.dt / c.persistence[i])
+	if (u0 < c.dt / in.persistence)
 	{
 		R walk[D];
 		const R u1 = pcm_draw<R>::from(blk, 1);
@@ -434,10 +459,9 @@ __device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsign
 			if constexpr (D == 3)
 				walk[2] = 0;
 		}
-		const R b = c.bias[i];
-		const R* dir = c.bias_dir + (size_t)i * D;
+		const R b = in.bias;
 		for (int k = 0; k < D; k++)
-			m[k] = (1 - b) * walk[k] + b * dir[k];
+			m[k] = (1 - b) * walk[k] + b * in.dir[k];
 		R length;
 		if constexpr (D == 1)
 			length = fabs(m[0]);
@@ -445,19 +469,37 @@ __device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsign
 			length = sqrt(m[0] * m[0] + m[1] * m[1]);
 		else
 			length = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
-		const R speed = c.speed[i];
 		for (int k = 0; k < D; k++)
 		{
-			m[k] = length > zero_threshold<R>() ? m[k] * speed / length : (R)0;
+			m[k] = length > zero_threshold<R>() ? m[k] * in.speed / length : (R)0;
 			if (writer)
-				mv[k] = m[k];
+				c.motility[(size_t)i * D + k] = m[k];
 		}
 	}
 	for (int k = 0; k < D; k++)
 		v[k] += m[k];
 }
 
+template <int D, class R>
+__device__ __forceinline__ void motility_agent(const SimCtx<R>& c, int i, unsigned long long step, R* v)
+{
+	if (c.motile[i] == 0)
+		return;
+	motility_apply<D, R>(c, i, step, load_motility<D, R>(c, i), v, true);
+}
+
 // position_helper<D>::update_membrane_velocities (solver_helper.h:103-110, 184-193, 288-299)
+template <int D, class R>
+__device__ __forceinline__ void membrane_apply(const SimCtx<R>& c, R r, R s, const R* p, R* v)
+{
+	for (int k = 0; k < D; k++)
+	{
+		const int upper = (D == 1) ? c.bmax[1] : c.bmax[k];
+		membrane_push<R>(p[k], (R)c.bmin[k], (R)1, r, s, v[k]);
+		membrane_push<R>(p[k], (R)upper, (R)-1, r, s, v[k]);
+	}
+}
+
 template <int D, class R>
 __device__ __forceinline__ void membrane_agent(const SimCtx<R>& c, int i, const R* p, R* v)
 {

@@ -16,9 +16,14 @@ constexpr int CAND_CHUNK = 4096;    // interaction candidates ordered per LDS pa
 constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
 constexpr int LANE_BUF = 8;         // appends a lane buffers before one wavefront-wide reservation
 constexpr int VELOCITY_LANES = 8;   // lanes that share one agent in the velocity kernel
+constexpr int VELOCITY_ROUNDS = 2;  // list slots per lane gathered together
+constexpr int EVENT_LANES = 8;      // lanes that share one agent in the event-drawing kernels
 constexpr int NBR_WAVES = 4;        // wavefronts that share one voxel in the neighbor kernel
+constexpr int CLAIM_THREADS = 1024; // block of the voxel claim kernel
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+
 
 #define PCM_PROLOGUE(c)                                                                                                \
 	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
@@ -49,7 +54,6 @@ __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs)
 		c.vcnt[c.vox_of[i]] = 0;
 	if (i == 0)
 	{
-		c.sc->cursor = 0;
 		c.sc->n_occ = 0;
 		c.sc->order_valid = 0;
 	}
@@ -78,12 +82,13 @@ __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs)
 	c.vrank[i] = atomicAdd(&c.vcnt[v], 1);
 }
 
-// Each occupied voxel's first agent claims the voxel's bucket range; claims are summed
-// per block so a block issues one atomic for the bucket cursor and one for the list of
-// occupied voxels.
+// Each occupied voxel's first agent claims the voxel's bucket range.  Claims are summed
+// per block of CLAIM_THREADS so that a block issues one 64-bit atomic on a packed counter
+// (agents in the high word, occupied voxels in the low word): device-scope atomics on one
+// cache line serialise across the XCDs, so there are as few of them as possible.
 // RESET: also zero the container update's counters (their readers have finished)
 template <class R, bool RESET>
-__global__ void k_bin_claim(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int i = gid();
@@ -102,17 +107,16 @@ __global__ void k_bin_claim(const SimCtx<R>* __restrict__ ctxs)
 	int total_amount, total_voxels;
 	const int pre_amount = block_exclusive_scan(amount, total_amount);
 	const int pre_voxels = block_exclusive_scan(amount > 0 ? 1 : 0, total_voxels);
-	__shared__ int s_base[2];
+	__shared__ unsigned long long s_base;
 	if (threadIdx.x == 0)
-	{
-		s_base[0] = total_amount ? atomicAdd(&c.sc->cursor, total_amount) : 0;
-		s_base[1] = total_voxels ? atomicAdd(&c.sc->n_occ, total_voxels) : 0;
-	}
+		s_base = total_voxels
+					 ? atomicAdd(&c.sc->claim, ((unsigned long long)total_amount << 32) | (unsigned)total_voxels)
+					 : 0ull;
 	__syncthreads();
 	if (amount > 0)
 	{
-		c.vstart[v] = s_base[0] + pre_amount;
-		c.occ[s_base[1] + pre_voxels] = v;
+		c.vstart[v] = (int)(s_base >> 32) + pre_amount;
+		c.occ[(int)(s_base & 0xffffffffull) + pre_voxels] = v;
 	}
 }
 
@@ -123,6 +127,12 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs)
 	const int i = gid();
 	if (i >= c.sc->n)
 		return;
+	if (i == 0)
+	{
+		// the claims have finished
+		c.sc->n_occ = (int)(c.sc->claim & 0xffffffffull);
+		c.sc->claim = 0;
+	}
 	c.btmp[c.vstart[c.vox_of[i]] + c.vrank[i]] = i;
 }
 
@@ -534,202 +544,152 @@ __device__ __forceinline__ R pair_force_rec(const force_rec<R>& a, const force_r
 }
 
 // Forces, motility, basement membrane and spring contraction of one agent in the
-// reference's summation order (simulator.cpp:173-176), written to the velocity.  Neighbor
-// and spring-partner data come from the force records, four partners loaded at a time.
+// reference's summation order (simulator.cpp:173-176), written to the velocity; neighbor
+// and spring-partner data come from the force records.  G lanes share an agent: lane s of
+// a group evaluates list slots s, s+G, ... (gather + pair arithmetic), then every lane of
+// the group adds the slot terms in list order, read through lane shuffles, so sums are
+// rounded exactly as the serial reference sums them.  Motility and the membrane run redundantly on the group's lanes
+// (lane 0 writes), spring contraction is spread over lanes like the neighbor loop.
 // CLEAR: also clear the voxel counts of the last mesh update (nothing reads them until
 // this step's mesh update)
-template <class R, int D, bool CLEAR>
-__global__ void k_velocity_fused(const SimCtx<R>* __restrict__ ctxs)
-{
-	PCM_PROLOGUE(c)
-	const int g = gid();
-	if (CLEAR)
-	{
-		if (g < c.sc->n_binned)
-			c.vcnt[c.vox_of[g]] = 0;
-		if (g == 0)
-		{
-			c.sc->cursor = 0;
-			c.sc->n_occ = 0;
-		}
-	}
-	if (g >= c.sc->n)
-		return;
-	const int i = agent_at<R>(c, g);
-	const force_rec<R> me = read_force_record<R>(c, i);
-	const bool mov = me.movable;
-	R v[D], p[D];
-	for (int k = 0; k < D; k++)
-	{
-		p[k] = me.p[k];
-		v[k] = c.vel[(size_t)i * D + k];
-	}
-	const R* aff_row = c.affinity + (size_t)i * c.T;
-	const int cnt = c.nbr_cnt[i];
-	const int* nb = c.nbr_idx + (size_t)i * c.K;
-	R pressure = 0;
-	for (int s0 = 0; s0 < cnt; s0 += 4)
-	{
-		force_rec<R> other[4];
-		R aff_ba[4];
-#pragma unroll
-		for (int u = 0; u < 4; u++)
-			if (s0 + u < cnt)
-			{
-				const int j = nb[s0 + u];
-				other[u] = read_force_record<R>(c, j);
-				aff_ba[u] = c.affinity[(size_t)j * c.T + me.type];
-			}
-#pragma unroll
-		for (int u = 0; u < 4; u++)
-			if (s0 + u < cnt)
-			{
-				const R term = pair_force_rec<D, R>(me, other[u], aff_row[other[u].type], aff_ba[u], mov, v);
-				const R t = term * pressure_coefficient<R>();
-				if (mov)
-					pressure += t;
-				if (other[u].movable)
-					pressure += t;
-			}
-	}
-	c.pressure[i] = pressure;
-	motility_agent<D, R>(c, i, c.sc->step, v);
-	if (mov && c.wall)
-		membrane_agent<D, R>(c, i, p, v);
-	if (mov && c.springs)
-	{
-		// spring_contract_function (position_solver.cpp:337-370)
-		const int ns = c.spr_cnt[i];
-		const int* sp = c.spr_idx + (size_t)i * c.L;
-		const R ki = c.elastic[i];
-		for (int s0 = 0; s0 < ns; s0 += 4)
-		{
-			force_rec<R> other[4];
-			R kj[4], aff_ba[4];
-#pragma unroll
-			for (int u = 0; u < 4; u++)
-				if (s0 + u < ns)
-				{
-					const int j = sp[s0 + u];
-					other[u] = read_force_record<R>(c, j);
-					kj[u] = c.elastic[j];
-					aff_ba[u] = c.affinity[(size_t)j * c.T + me.type];
-				}
-#pragma unroll
-			for (int u = 0; u < 4; u++)
-				if (s0 + u < ns)
-				{
-					const R kij = ki * kj[u] * aff_row[other[u].type] * aff_ba[u];
-					const R adhesion = (R)sqrt((double)kij);
-					for (int k = 0; k < D; k++)
-						v[k] += adhesion * (other[u].p[k] - p[k]);
-				}
-		}
-	}
-	for (int k = 0; k < D; k++)
-		c.vel[(size_t)i * D + k] = v[k];
-}
-
-// k_velocity_fused with G lanes per agent: lane s of a group evaluates list slots s, s+G,
-// ... (gather + pair arithmetic), then every lane of the group adds the slot terms in
-// list order, read through lane shuffles, so sums are rounded exactly as the serial
-// reference sums them.  Motility and the membrane run redundantly on the group's lanes
-// (lane 0 writes), spring contraction is spread over lanes like the neighbor loop.
 template <class R, int D, bool CLEAR, int G>
-__global__ void __launch_bounds__(BLOCK) k_velocity_lanes(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
+	constexpr int B = VELOCITY_ROUNDS;
 	const int t = gid();
-	if (CLEAR)
-	{
-		if (t < c.sc->n_binned)
-			c.vcnt[c.vox_of[t]] = 0;
-		if (t == 0)
-		{
-			c.sc->cursor = 0;
-			c.sc->n_occ = 0;
-		}
-	}
+	if (CLEAR && t < c.sc->n_binned)
+		c.vcnt[c.vox_of[t]] = 0;
 	const int g = t / G;
 	const int lane = t & (G - 1);
 	if (g >= c.sc->n)
 		return;
 	const int i = agent_at<R>(c, g);
+	// every agent-level input is requested before the first dependent gather
 	const force_rec<R> me = read_force_record<R>(c, i);
-	const bool mov = me.movable;
-	R v[D], p[D];
-	for (int k = 0; k < D; k++)
-	{
-		p[k] = me.p[k];
-		v[k] = c.vel[(size_t)i * D + k];
-	}
-	const R* aff_row = c.affinity + (size_t)i * c.T;
 	const int cnt = c.nbr_cnt[i];
 	const int* nb = c.nbr_idx + (size_t)i * c.K;
+	int jn[B];
+#pragma unroll
+	for (int u = 0; u < B; u++)
+		jn[u] = u * G + lane < c.K ? nb[u * G + lane] : -1; // in the row; used only below cnt
+	R v[D];
+	for (int k = 0; k < D; k++)
+		v[k] = c.vel[(size_t)i * D + k];
+	const motility_in<D, R> mot = load_motility<D, R>(c, i);
+	const R bm_strength = c.bm_repulsion[i];
+	const bool with_springs = c.springs != 0;
+	const int ns = with_springs ? c.spr_cnt[i] : 0;
+	const R ki = with_springs ? c.elastic[i] : (R)0;
+	const int* sp = c.spr_idx + (size_t)i * c.L;
+	int js[B];
+#pragma unroll
+	for (int u = 0; u < B; u++)
+		js[u] = with_springs && u * G + lane < c.L ? sp[u * G + lane] : -1;
+
+	const bool mov = me.movable;
+	R p[D];
+	for (int k = 0; k < D; k++)
+		p[k] = me.p[k];
+	const R* aff_row = c.affinity + (size_t)i * c.T;
 	const R coefficient = pressure_coefficient<R>();
 	R pressure = 0;
-	for (int base = 0; base < cnt; base += G)
+	for (int base = 0; base < cnt; base += B * G)
 	{
-		const int s = base + lane;
-		R f[D], mine = 0, theirs = 0;
-		for (int k = 0; k < D; k++)
-			f[k] = 0;
-		if (s < cnt)
+		R f[B][D], mine[B], theirs[B];
+		if (base > 0)
 		{
-			const int j = nb[s];
-			const force_rec<R> other = read_force_record<R>(c, j);
-			const R aff_ba = c.affinity[(size_t)j * c.T + me.type];
-			R zero[D];
-			for (int k = 0; k < D; k++)
-				zero[k] = -(R)0;
-			// the pair's velocity term alone: -0 + x is exactly x for every x
-			const R term = pair_force_rec<D, R>(me, other, aff_row[other.type], aff_ba, mov, zero);
-			for (int k = 0; k < D; k++)
-				f[k] = zero[k];
-			const R tp = term * coefficient;
-			mine = mov ? tp : (R)0;
-			theirs = other.movable ? tp : (R)0;
+#pragma unroll
+			for (int u = 0; u < B; u++)
+				jn[u] = base + u * G + lane < cnt ? nb[base + u * G + lane] : -1;
 		}
-		const int m = min(G, cnt - base);
-		for (int u = 0; u < m; u++)
+		force_rec<R> other[B];
+		R aff_ba[B];
+#pragma unroll
+		for (int u = 0; u < B; u++)
+			if (base + u * G + lane < cnt)
+			{
+				other[u] = read_force_record<R>(c, jn[u]);
+				aff_ba[u] = c.affinity[(size_t)jn[u] * c.T + me.type];
+			}
+#pragma unroll
+		for (int u = 0; u < B; u++)
 		{
-			if (mov)
-				for (int k = 0; k < D; k++)
-					v[k] += __shfl(f[k], u, G);
-			pressure += __shfl(mine, u, G);
-			pressure += __shfl(theirs, u, G);
+			for (int k = 0; k < D; k++)
+				f[u][k] = -(R)0; // -0 + x is exactly x for every x
+			mine[u] = 0;
+			theirs[u] = 0;
+			if (base + u * G + lane < cnt)
+			{
+				const R term = pair_force_rec<D, R>(me, other[u], aff_row[other[u].type], aff_ba[u], mov, f[u]);
+				const R tp = term * coefficient;
+				mine[u] = mov ? tp : (R)0;
+				theirs[u] = other[u].movable ? tp : (R)0;
+			}
+		}
+		// the slot terms in list order
+#pragma unroll
+		for (int u = 0; u < B; u++)
+		{
+			const int m = min(G, cnt - base - u * G);
+			for (int src = 0; src < m; src++)
+			{
+				if (mov)
+					for (int k = 0; k < D; k++)
+						v[k] += __shfl(f[u][k], src, G);
+				pressure += __shfl(mine[u], src, G);
+				pressure += __shfl(theirs[u], src, G);
+			}
 		}
 	}
 	if (lane == 0)
 		c.pressure[i] = pressure;
-	motility_agent<D, R>(c, i, c.sc->step, v, lane == 0);
+	motility_apply<D, R>(c, i, c.sc->step, mot, v, lane == 0);
 	if (mov && c.wall)
-		membrane_agent<D, R>(c, i, p, v);
-	if (mov && c.springs)
+		membrane_apply<D, R>(c, me.r, bm_strength, p, v);
+	if (mov && with_springs)
 	{
 		// spring_contract_function (position_solver.cpp:337-370)
-		const int ns = c.spr_cnt[i];
-		const int* sp = c.spr_idx + (size_t)i * c.L;
-		const R ki = c.elastic[i];
-		for (int base = 0; base < ns; base += G)
+		for (int base = 0; base < ns; base += B * G)
 		{
-			const int s = base + lane;
-			R f[D];
-			for (int k = 0; k < D; k++)
-				f[k] = 0;
-			if (s < ns)
+			if (base > 0)
 			{
-				const int j = sp[s];
-				const force_rec<R> other = read_force_record<R>(c, j);
-				const R kij = ki * c.elastic[j] * aff_row[other.type] * c.affinity[(size_t)j * c.T + me.type];
-				const R adhesion = (R)sqrt((double)kij);
-				for (int k = 0; k < D; k++)
-					f[k] = adhesion * (other.p[k] - p[k]);
+#pragma unroll
+				for (int u = 0; u < B; u++)
+					js[u] = base + u * G + lane < ns ? sp[base + u * G + lane] : -1;
 			}
-			const int m = min(G, ns - base);
-			for (int u = 0; u < m; u++)
+			R f[B][D];
+			force_rec<R> other[B];
+			R kj[B], aff_ba[B];
+#pragma unroll
+			for (int u = 0; u < B; u++)
+				if (base + u * G + lane < ns)
+				{
+					other[u] = read_force_record<R>(c, js[u]);
+					kj[u] = c.elastic[js[u]];
+					aff_ba[u] = c.affinity[(size_t)js[u] * c.T + me.type];
+				}
+#pragma unroll
+			for (int u = 0; u < B; u++)
+			{
 				for (int k = 0; k < D; k++)
-					v[k] += __shfl(f[k], u, G);
+					f[u][k] = 0;
+				if (base + u * G + lane < ns)
+				{
+					const R kij = ki * kj[u] * aff_row[other[u].type] * aff_ba[u];
+					const R adhesion = (R)sqrt((double)kij);
+					for (int k = 0; k < D; k++)
+						f[u][k] = adhesion * (other[u].p[k] - p[k]);
+				}
+			}
+#pragma unroll
+			for (int u = 0; u < B; u++)
+			{
+				const int m = min(G, ns - base - u * G);
+				for (int src = 0; src < m; src++)
+					for (int k = 0; k < D; k++)
+						v[k] += __shfl(f[u][k], src, G);
+			}
 		}
 	}
 	if (lane == 0)
@@ -791,9 +751,11 @@ __global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs)
 		atomicOr(&c.erase[i], 1u << s);
 		return;
 	}
+	const uint32_t fp = c.fire[p];
+	if (fp == 0)
+		return; // both rules below need a fired entry in p's list
 	const int* lp = c.spr_idx + (size_t)p * c.L;
 	const int cp = c.spr_cnt[p];
-	const uint32_t fp = c.fire[p];
 	bool erased = false;
 	if (i < p)
 	{
@@ -851,12 +813,17 @@ __global__ void k_spring_compact(const SimCtx<R>* __restrict__ ctxs)
 
 // Attachment events: every (i, slot) with j = neighbor > i whose draws succeed.  An actor
 // keeps its events in slot order (own_slot); each target collects (actor, k) references in
-// an unordered inbox; actors with events are listed for the resolver.
-template <class R>
-__global__ void k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
+// an unordered inbox; actors with events are listed for the resolver.  G lanes share an
+// agent: lane l draws for slots l, l+G, ...; a ballot ranks the events in slot order.
+template <class R, int G>
+__global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int g = gid();
+	const int t = gid();
+	const int g = t / G;
+	const int lane = t & (G - 1);
+	const int group_shift = (int)(threadIdx.x & 63) & ~(G - 1);
+	const unsigned long long group_mask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
 	int n_ev = 0;
 	int i = -1;
 	if (g < c.sc->n)
@@ -866,54 +833,54 @@ __global__ void k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
 		const int ti = c.type[i];
 		const R rate_i = c.att_rate[i] * c.dt;
+		const R* aff_i = c.affinity + (size_t)i * c.T;
 		const unsigned long long step = c.sc->step;
-		for (int s0 = 0; s0 < cnt; s0 += 4)
+		for (int base = 0; base < cnt; base += G)
 		{
-			int jj[4], tj[4];
-			R rate_j[4], aff_ji[4];
-#pragma unroll
-			for (int u = 0; u < 4; u++)
+			const int s = base + lane;
+			bool event = false;
+			int j = -1;
+			if (s < cnt)
 			{
-				jj[u] = s0 + u < cnt ? nb[s0 + u] : -1;
-				if (jj[u] > i)
+				j = nb[s];
+				if (j > i)
 				{
-					tj[u] = c.type[jj[u]];
-					rate_j[u] = c.att_rate[jj[u]];
-					aff_ji[u] = c.affinity[(size_t)jj[u] * c.T + ti];
+					const R p_l = rate_i * aff_i[c.type[j]];
+					const R p_r = c.att_rate[j] * c.dt * c.affinity[(size_t)j * c.T + ti];
+					const pcm_u32x4 b = pcm_block(c.seed, step, PCM_STREAM_ATTACH, (uint32_t)i, (uint32_t)s, 0u);
+					event = pcm_draw<R>::from(b, 0) <= p_l || pcm_draw<R>::from(b, 1) <= p_r;
 				}
 			}
-#pragma unroll
-			for (int u = 0; u < 4; u++)
+			const unsigned long long m = (__ballot(event) >> group_shift) & group_mask;
+			if (event)
 			{
-			const int s = s0 + u;
-			const int j = jj[u];
-			if (j < i)
-				continue;
-			const R p_l = rate_i * c.affinity[(size_t)i * c.T + tj[u]];
-			const R p_r = rate_j[u] * c.dt * aff_ji[u];
-			const pcm_u32x4 b = pcm_block(c.seed, step, PCM_STREAM_ATTACH, (uint32_t)i, (uint32_t)s, 0u);
-			if (pcm_draw<R>::from(b, 0) <= p_l || pcm_draw<R>::from(b, 1) <= p_r)
-			{
-				c.own_slot[(size_t)i * c.K + n_ev] = s;
-				c.own_st[(size_t)i * c.K + n_ev] = 0;
+				const int k = n_ev + __popcll(m & ((1ull << lane) - 1ull));
+				c.own_slot[(size_t)i * c.K + k] = s;
+				c.own_st[(size_t)i * c.K + k] = 0;
 				const int slot = atomicAdd(&c.in_cnt[j], 1);
 				if (slot < c.K)
 				{
 					c.inbox[2 * ((size_t)j * c.K + slot)] = i;
-					c.inbox[2 * ((size_t)j * c.K + slot) + 1] = n_ev;
+					c.inbox[2 * ((size_t)j * c.K + slot) + 1] = k;
 				}
 				else
 					atomicOr(&c.sc->status, ST_CAPACITY);
-				n_ev++;
 			}
-			}
+			n_ev += __popcll(m);
 		}
-		c.own_cnt[i] = n_ev;
+		if (lane == 0)
+			c.own_cnt[i] = n_ev;
 	}
-	const int base = wave_reserve(&c.sc->att_actors, n_ev > 0 ? 1 : 0);
-	if (n_ev > 0)
-		c.att_list[base] = i;
-	(void)wave_reserve(&c.sc->att_events, n_ev);
+	// actors are listed per block, with no device-wide counter (same-address atomics from
+	// every XCD serialise): block b's actors fill att_list[b * APB, b * APB + blk_cnt[b])
+	constexpr int APB = BLOCK / G;
+	const bool lead = lane == 0 && n_ev > 0;
+	int total;
+	const int rank = block_exclusive_scan(lead ? 1 : 0, total);
+	if (lead)
+		c.att_list[blockIdx.x * APB + rank] = i;
+	if (threadIdx.x == 0)
+		c.blk_cnt[blockIdx.x] = total;
 }
 
 // Attachment resolution in the reference's order: event (i, j) succeeds iff, when the
@@ -930,8 +897,27 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 	__shared__ int s_unknown;
 	if (c.sc->status)
 		return;
-	const int A = c.sc->att_actors;
-	const int E = c.sc->att_events;
+	// compact the collect blocks' actor segments into att_c
+	constexpr int APB = BLOCK / EVENT_LANES;
+	const int nb = (c.sc->n + APB - 1) / APB;
+	int A = 0, my_events = 0;
+	for (int b0 = 0; b0 < nb; b0 += blockDim.x)
+	{
+		const int b = b0 + threadIdx.x;
+		const int m = b < nb ? c.blk_cnt[b] : 0;
+		int total;
+		const int pre = block_exclusive_scan(m, total);
+		for (int k = 0; k < m; k++)
+		{
+			const int a = c.att_list[b * APB + k];
+			c.att_c[A + pre + k] = a;
+			my_events += c.own_cnt[a];
+		}
+		A += total;
+	}
+	int E;
+	(void)block_exclusive_scan(my_events, E);
+	__syncthreads();
 	volatile uint8_t* st = c.own_st;
 	int rounds = 0;
 	for (;; rounds++)
@@ -947,7 +933,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 		__syncthreads();
 		for (int t = threadIdx.x; t < A; t += blockDim.x)
 		{
-			const int a = c.att_list[t];
+			const int a = c.att_c[t];
 			const int na = c.own_cnt[a];
 			const int c0a = c.spr_cnt[a], maxa = c.max_att[a];
 			int in_lo = 0, in_hi = 0;
@@ -1006,8 +992,6 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 		c.sc->stat[0] = E;
 		c.sc->stat[1] = rounds + 1;
 		c.sc->stat[2] = A;
-		c.sc->att_actors = 0;
-		c.sc->att_events = 0;
 	}
 }
 
@@ -1155,42 +1139,44 @@ enum : int
 };
 
 // AB2: first apply the agent's position update (update_positions precedes the interactions
-// and this pass reads no positions)
-template <class R, int D, bool AB2>
-__global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
+// and this pass reads no positions).  G lanes share an agent: lane l loads and draws for
+// slots l, l+G, ...; the rare successful draws are then walked in slot order by the whole
+// group (the per-actor state is group-uniform), reading the slot data through shuffles.
+template <class R, int D, bool AB2, int G>
+__global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
-	const int g = gid();
-	const bool live_thread = g < c.sc->n;
-	const int i = live_thread ? agent_at<R>(c, g) : 0;
-	if (AB2 && live_thread && c.movable[i])
+	const int t = gid();
+	const int g = t / G;
+	const int lane = t & (G - 1);
+	const int group_shift = (int)(threadIdx.x & 63) & ~(G - 1);
+	const unsigned long long group_mask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+	if (g >= c.sc->n)
+		return;
+	const int i = agent_at<R>(c, g);
+	if (AB2 && lane < D && c.movable[i])
 	{
 		const R factor = (R)((double)c.dt * 1.5);
 		const R previous_factor = (R)((double)c.dt * -0.5);
-		for (int k = 0; k < D; k++)
-		{
-			const size_t o = (size_t)i * D + k;
-			const R v = c.vel[o];
-			c.pos[o] += v * factor + c.pvel[o] * previous_factor;
-			c.pvel[o] = v;
-			c.vel[o] = 0;
-		}
+		const size_t o = (size_t)i * D + lane;
+		const R v = c.vel[o];
+		c.pos[o] += v * factor + c.pvel[o] * previous_factor;
+		c.pvel[o] = v;
+		c.vel[o] = 0;
 	}
-	int buf_s[LANE_BUF];
-	int n_cand = 0;
+	if (lane == 0)
+		c.atk_tgt[i] = -1;
 	int n_rem = 0;
 	bool acted = false;
-	if (live_thread)
-		c.atk_tgt[i] = -1;
 	// every branch compares a draw u in [0,1) with `<` against rate * dt, so an actor whose
 	// rates are all <= 0 never acts and its draws need not be made
 	bool any_rate = false;
-	if (live_thread && c.dead[i] != 1)
+	if (c.dead[i] != 1)
 	{
 		any_rate = c.dead_phago[i] > 0;
-		for (int t = 0; t < c.T; t++)
-			any_rate = any_rate || c.live_phago[(size_t)i * c.T + t] > 0 || c.attack[(size_t)i * c.T + t] > 0
-					   || c.fusion[(size_t)i * c.T + t] > 0;
+		for (int k = 0; k < c.T; k++)
+			any_rate = any_rate || c.live_phago[(size_t)i * c.T + k] > 0 || c.attack[(size_t)i * c.T + k] > 0
+					   || c.fusion[(size_t)i * c.T + k] > 0;
 	}
 	if (any_rate)
 	{
@@ -1200,104 +1186,102 @@ __global__ void k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
 		const R dt = c.dt;
 		const unsigned long long step = c.sc->step;
 		const bool self_removed = c.flags[i] == PCM_FLAG_TO_REMOVE;
-		bool attacked_once = false, phagocytosed_once = false, fused_once = false;
-		int jq[4], tq[4];
-		uint8_t dq[4], fq[4];
-		R immq[4];
-		for (int s = 0; s < cnt; s++)
+		const R p_dead = c.dead_phago[i] * dt;
+		bool attacked_once = false, phagocytosed_once = false, fused_once = false, stop = false;
+		for (int base = 0; base < cnt && !stop; base += G)
 		{
-			if ((s & 3) == 0)
+			const int s = base + lane;
+			int j = -1;
+			// bit 0 candidate, 1 dead target, 2 u < p_live, 3 u < p_attack, 4 u < p_fuse,
+			// 5 target flagged for removal
+			int bits = 0;
+			if (s < cnt)
 			{
-#pragma unroll
-				for (int q = 0; q < 4; q++)
-					if (s + q < cnt)
-					{
-						const int jn = nb[s + q];
-						jq[q] = jn;
-						tq[q] = c.type[jn];
-						dq[q] = c.dead[jn];
-						fq[q] = c.flags[jn];
-						immq[q] = c.immuno[(size_t)jn * c.T + ti];
-					}
-			}
-			const int j = jq[s & 3];
-			const int tj = tq[s & 3];
-			const bool dead_j = dq[s & 3] == 1;
-			const R p_dead = c.dead_phago[i] * dt;
-			const R p_live = c.live_phago[(size_t)i * c.T + tj] * dt;
-			const R p_attack = c.attack[(size_t)i * c.T + tj] * immq[s & 3] * dt;
-			const R p_fuse = c.fusion[(size_t)i * c.T + tj] * dt;
-			if (dead_j ? !(p_dead > 0) : !(p_live > 0 || p_attack > 0 || p_fuse > 0))
-				continue; // no branch can fire: u < threshold <= 0 is false for every u
-			const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
-			const bool candidate = dead_j ? (u < p_dead) : (u < p_live || u < p_attack || u < p_fuse);
-			if (!candidate)
-				continue;
-			if (n_cand < LANE_BUF)
-				buf_s[n_cand++] = s;
-			else
-			{
-				const int k = atomicAdd(&c.sc->cand_count, 1);
-				if (k < c.cand_cap)
+				j = nb[s];
+				const int tj = c.type[j];
+				const bool dead_j = c.dead[j] == 1;
+				const R p_live = c.live_phago[(size_t)i * c.T + tj] * dt;
+				const R p_attack = c.attack[(size_t)i * c.T + tj] * c.immuno[(size_t)j * c.T + ti] * dt;
+				const R p_fuse = c.fusion[(size_t)i * c.T + tj] * dt;
+				// no branch can fire when its threshold is <= 0: u < threshold is false
+				if (dead_j ? p_dead > 0 : (p_live > 0 || p_attack > 0 || p_fuse > 0))
 				{
-					c.cand[2 * k] = i;
-					c.cand[2 * k + 1] = s;
+					const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
+					const bool lt_live = u < p_live, lt_attack = u < p_attack, lt_fuse = u < p_fuse;
+					const bool candidate = dead_j ? (u < p_dead) : (lt_live || lt_attack || lt_fuse);
+					if (candidate)
+						bits = 1 | (dead_j ? 2 : 0) | (lt_live ? 4 : 0) | (lt_attack ? 8 : 0) | (lt_fuse ? 16 : 0)
+							   | (c.flags[j] == PCM_FLAG_TO_REMOVE ? 32 : 0);
 				}
-				else
-					atomicOr(&c.sc->status, ST_CAPACITY);
 			}
-
-			const bool target_removed = fq[s & 3] == PCM_FLAG_TO_REMOVE;
-			int kind = -1; // -1 none, 0 ingest, 1 attack, 2 fuse
-			if (dead_j)
-				kind = 0;
-			else if (!phagocytosed_once && u < p_live)
-				kind = 0;
-			else if (!attacked_once && u < p_attack)
-				kind = 1;
-			else if (!fused_once && u < p_fuse)
-				kind = 2;
-			if (kind < 0)
-				continue;
-			acted = true;
-			if (self_removed)
-				break;
-			if (target_removed)
-				continue;
-			if (kind == 1)
+			unsigned long long m = (__ballot(bits != 0) >> group_shift) & group_mask;
+			while (m)
 			{
-				c.atk_tgt[i] = j;
-				atomicAdd(&c.n_atk_t[j], 1);
-				attacked_once = true;
-				continue;
+				const int src = __ffsll((long long)m) - 1;
+				m &= m - 1;
+				const int b = __shfl(bits, src, G);
+				const int jj = __shfl(j, src, G);
+				const int ss = base + src;
+				if (lane == 0)
+				{
+					const int k = atomicAdd(&c.sc->cand_count, 1);
+					if (k < c.cand_cap)
+					{
+						c.cand[2 * k] = i;
+						c.cand[2 * k + 1] = ss;
+					}
+					else
+						atomicOr(&c.sc->status, ST_CAPACITY);
+				}
+				int kind = -1; // -1 none, 0 ingest, 1 attack, 2 fuse
+				if (b & 2)
+					kind = 0;
+				else if (!phagocytosed_once && (b & 4))
+					kind = 0;
+				else if (!attacked_once && (b & 8))
+					kind = 1;
+				else if (!fused_once && (b & 16))
+					kind = 2;
+				if (kind < 0)
+					continue;
+				acted = true;
+				if (self_removed)
+				{
+					stop = true;
+					break;
+				}
+				if (b & 32)
+					continue;
+				if (kind == 1)
+				{
+					if (lane == 0)
+					{
+						c.atk_tgt[i] = jj;
+						atomicAdd(&c.n_atk_t[jj], 1);
+					}
+					attacked_once = true;
+					continue;
+				}
+				if (!(b & 2))
+				{
+					if (kind == 0)
+						phagocytosed_once = true;
+					else
+						fused_once = true;
+				}
+				if (lane == 0)
+				{
+					atomicAdd(&c.n_rem_t[jj], 1);
+					if (n_rem < 4)
+						c.rem_ev[4 * i + n_rem] = ss | ((kind == 2 ? EV_FUSE : EV_INGEST) << 24);
+					else
+						c.sc->conflict = 1;
+				}
+				n_rem++;
 			}
-			if (!dead_j)
-			{
-				if (kind == 0)
-					phagocytosed_once = true;
-				else
-					fused_once = true;
-			}
-			atomicAdd(&c.n_rem_t[j], 1);
-			if (n_rem < 4)
-				c.rem_ev[4 * i + n_rem] = s | ((kind == 2 ? EV_FUSE : EV_INGEST) << 24);
-			else
-				c.sc->conflict = 1;
-			n_rem++;
 		}
 	}
-	const int base = wave_reserve(&c.sc->cand_count, n_cand);
-	for (int k = 0; k < n_cand; k++)
-	{
-		if (base + k < c.cand_cap)
-		{
-			c.cand[2 * (base + k)] = i;
-			c.cand[2 * (base + k) + 1] = buf_s[k];
-		}
-		else
-			atomicOr(&c.sc->status, ST_CAPACITY);
-	}
-	if (live_thread)
+	if (lane == 0)
 	{
 		c.rem_cnt[i] = n_rem;
 		c.acted[i] = acted;

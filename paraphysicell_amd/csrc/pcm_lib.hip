@@ -208,7 +208,10 @@ public:
 		PCM_TRY(alloc_t(&h.frec, (size_t)cap * 8));
 		PCM_TRY(alloc_t(&h.own_cnt, cap));
 		PCM_TRY(alloc_t(&h.in_cnt, cap));
-		PCM_TRY(alloc_t(&h.att_list, cap));
+		// one segment of BLOCK / EVENT_LANES actors per collect block
+		PCM_TRY(alloc_t(&h.att_list, (size_t)cap + BLOCK));
+		PCM_TRY(alloc_t(&h.blk_cnt, (size_t)cap / (BLOCK / EVENT_LANES) + 2));
+		PCM_TRY(alloc_t(&h.att_c, cap));
 		PCM_TRY(alloc_t(&h.fire, cap));
 		PCM_TRY(alloc_t(&h.erase, cap));
 		PCM_TRY(alloc_t(&h.n_rem_t, cap));
@@ -246,13 +249,12 @@ public:
 		PCM_HIP(hipMemcpy(&s, h_.sc, sizeof(s), hipMemcpyDeviceToHost));
 		s.n = n;
 		s.status = 0;
-		s.att_events = 0;
-		s.att_actors = 0;
 		s.cand_count = 0;
 		s.conflict = 0;
 		s.bad_count = 0;
 		s.moves = 0;
 		s.order_valid = 0;
+		s.claim = 0;
 		s.log_len = 0;
 		PCM_HIP(hipMemcpy(h_.sc, &s, sizeof(s), hipMemcpyHostToDevice));
 		n_launch_ = n;
@@ -830,6 +832,13 @@ private:
 		return launch(tag, kernel, grid_agents(), BLOCK);
 	}
 
+	// `lanes` threads per agent
+	template <class Kern>
+	int per_lanes(int tag, Kern kernel, int lanes)
+	{
+		return launch(tag, kernel, std::max(1, (int)(((long long)n_launch_ * lanes + BLOCK - 1) / BLOCK)), BLOCK);
+	}
+
 	struct prof_record
 	{
 		int tag;
@@ -932,7 +941,8 @@ private:
 		if (!FUSED)
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 		PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D, FUSED>));
-		PCM_TRY(per_agent(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>));
+		PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (n_launch_ + CLAIM_THREADS - 1) / CLAIM_THREADS),
+					   CLAIM_THREADS));
 		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R>));
 		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 	}
@@ -943,7 +953,7 @@ private:
 		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
 					   std::max(1, (int)(((long long)n_launch_ * h_.L + BLOCK - 1) / BLOCK)), BLOCK));
 		PCM_TRY(per_agent(PCM_K_SPRING_COMPACT, k_spring_compact<R>));
-		PCM_TRY(per_agent(PCM_K_SPRING_COLLECT, k_spring_collect<R>));
+		PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, EVENT_LANES>, EVENT_LANES));
 		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
 		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
 	}
@@ -951,7 +961,7 @@ private:
 	template <int D>
 	int interaction_kernels()
 	{
-		PCM_TRY(per_agent(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, false>));
+		PCM_TRY(per_lanes(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, false, EVENT_LANES>, EVENT_LANES));
 		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, false>));
 		return launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, false>, 1, RESOLVE_THREADS);
@@ -1030,12 +1040,11 @@ private:
 		// the previous mesh update's voxel counts are cleared by the velocity kernel when its
 		// grid covers every agent that update binned
 		const bool fold_clear = binned_launch_ <= n_launch_;
-		const int vblocks = std::max(1, (int)(((long long)n_launch_ * VELOCITY_LANES + BLOCK - 1) / BLOCK));
 		if (fold_clear)
-			PCM_TRY(launch(PCM_K_VELOCITY_FUSED, k_velocity_lanes<R, D, true, VELOCITY_LANES>, vblocks, BLOCK));
+			PCM_TRY(per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, true, VELOCITY_LANES>, VELOCITY_LANES));
 		else
-			PCM_TRY(launch(PCM_K_VELOCITY_FUSED, k_velocity_lanes<R, D, false, VELOCITY_LANES>, vblocks, BLOCK));
-		PCM_TRY(per_agent(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, true>));
+			PCM_TRY(per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, false, VELOCITY_LANES>, VELOCITY_LANES));
+		PCM_TRY(per_lanes(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, true, EVENT_LANES>, EVENT_LANES));
 		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, true>));
 		PCM_TRY(launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, true>, 1, RESOLVE_THREADS));
