@@ -16,9 +16,9 @@ constexpr int CAND_CHUNK = 4096;    // interaction candidates ordered per LDS pa
 constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
 constexpr int LANE_BUF = 8;         // appends a lane buffers before one wavefront-wide reservation
 constexpr int VELOCITY_LANES = 8;   // lanes that share one agent in the velocity kernel
-constexpr int VELOCITY_ROUNDS = 2;  // list slots per lane gathered together
+constexpr int VELOCITY_ROUNDS = 1;  // list slots per lane gathered together
 constexpr int EVENT_LANES = 8;      // lanes that share one agent in the event-drawing kernels
-constexpr int NBR_WAVES = 4;        // wavefronts that share one voxel in the neighbor kernel
+constexpr int NBR_WAVES = 2;        // wavefronts that share one voxel in the neighbor kernel
 constexpr int CLAIM_THREADS = 1024; // block of the voxel claim kernel
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -299,17 +299,15 @@ __global__ void __launch_bounds__(64 * NBR_WAVES) k_neighbors(const SimCtx<R>* _
 		if (C <= TILE)
 		{
 			// stage the stencil in stencil order; a candidate's stencil entry is the number
-			// of entry starts at or below its position
-			int pre[28];
-#pragma unroll
-			for (int k = 0; k < 28; k++)
-				pre[k] = s_pre[k];
+			// of entry starts at or below its position (lane k of every wavefront holds the
+			// start of entry k; read back with readlane, no LDS round trip per entry)
+			const int lane_pre = s_pre[lane < 28 ? lane : 27];
 			for (int t = tid; t < C; t += 64 * NBR_WAVES)
 			{
 				int sv = 0;
 #pragma unroll
 				for (int k = 1; k < 27; k++)
-					sv += pre[k] <= t;
+					sv += __builtin_amdgcn_readlane(lane_pre, k) <= t;
 				const int g = s_vs[sv] + t - s_pre[sv];
 				const int j = c.bucket[g];
 				R rec[4];
@@ -320,7 +318,7 @@ __global__ void __launch_bounds__(64 * NBR_WAVES) k_neighbors(const SimCtx<R>* _
 					s_rec[t * 4 + k] = rec[k];
 			}
 			__syncthreads();
-			const int own0 = pre[13];
+			const int own0 = __builtin_amdgcn_readlane(lane_pre, 13);
 			for (int a = wave; a < m; a += NBR_WAVES)
 			{
 				const int i = s_idx[own0 + a];
@@ -792,13 +790,11 @@ __global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs)
 		atomicOr(&c.erase[i], 1u << s);
 }
 
+// std::remove of the erased entries (position_solver.cpp:404-411); runs in the attachment
+// collection kernel, which reads neighbor lists only
 template <class R>
-__global__ void k_spring_compact(const SimCtx<R>* __restrict__ ctxs)
+__device__ __forceinline__ void spring_compact_agent(const SimCtx<R>& c, int i)
 {
-	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n)
-		return;
 	const uint32_t e = c.erase[i];
 	if (e == 0)
 		return;
@@ -829,6 +825,8 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 	if (g < c.sc->n)
 	{
 		i = agent_at<R>(c, g);
+		if (lane == 0)
+			spring_compact_agent<R>(c, i);
 		const int cnt = visible_neighbors<R>(c, i);
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
 		const int ti = c.type[i];
@@ -897,26 +895,27 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 	__shared__ int s_unknown;
 	if (c.sc->status)
 		return;
-	// compact the collect blocks' actor segments into att_c
+	// compact the collect blocks' actor segments into att_c: thread t takes a contiguous
+	// range of segments
 	constexpr int APB = BLOCK / EVENT_LANES;
 	const int nb = (c.sc->n + APB - 1) / APB;
-	int A = 0, my_events = 0;
-	for (int b0 = 0; b0 < nb; b0 += blockDim.x)
+	const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+	const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
+	int mine = 0;
+	for (int b = b0; b < b1; b++)
+		mine += c.blk_cnt[b];
+	int A;
+	int at = block_exclusive_scan(mine, A);
+	for (int b = b0; b < b1; b++)
 	{
-		const int b = b0 + threadIdx.x;
-		const int m = b < nb ? c.blk_cnt[b] : 0;
-		int total;
-		const int pre = block_exclusive_scan(m, total);
+		const int m = c.blk_cnt[b];
 		for (int k = 0; k < m; k++)
-		{
-			const int a = c.att_list[b * APB + k];
-			c.att_c[A + pre + k] = a;
-			my_events += c.own_cnt[a];
-		}
-		A += total;
+			c.att_c[at++] = c.att_list[b * APB + k];
 	}
-	int E;
-	(void)block_exclusive_scan(my_events, E);
+	const long long E = (long long)A * c.K; // bound on the events
+	__shared__ int s_events;
+	if (threadIdx.x == 0)
+		s_events = 0;
 	__syncthreads();
 	volatile uint8_t* st = c.own_st;
 	int rounds = 0;
@@ -935,6 +934,8 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 		{
 			const int a = c.att_c[t];
 			const int na = c.own_cnt[a];
+			if (rounds == 0)
+				atomicAdd(&s_events, na);
 			const int c0a = c.spr_cnt[a], maxa = c.max_att[a];
 			int in_lo = 0, in_hi = 0;
 			const int nin = min(c.in_cnt[a], c.K);
@@ -989,7 +990,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 	}
 	if (threadIdx.x == 0)
 	{
-		c.sc->stat[0] = E;
+		c.sc->stat[0] = s_events;
 		c.sc->stat[1] = rounds + 1;
 		c.sc->stat[2] = A;
 	}

@@ -952,7 +952,6 @@ private:
 		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R>));
 		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
 					   std::max(1, (int)(((long long)n_launch_ * h_.L + BLOCK - 1) / BLOCK)), BLOCK));
-		PCM_TRY(per_agent(PCM_K_SPRING_COMPACT, k_spring_compact<R>));
 		PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, EVENT_LANES>, EVENT_LANES));
 		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
 		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
