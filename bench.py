@@ -50,6 +50,21 @@ def kernel_bytes(kernel: str, d: int, r: int, T: int, k: float, s: float) -> flo
     return float(table.get(kernel, 0.0))
 
 
+def measured_traffic(tag: str):
+    """HBM bytes per launch of kernel `tag` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (copy-calibrated), or None when the summary has no row for it."""
+    path = os.path.join(ROOT, "profiles", "mechanics_step_88k_3d_f32_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cols) == 5 and cols[1] == tag:
+                    return float(cols[4])
+    except OSError:
+        return None
+    return None
+
+
 def time_cpu_baseline(seconds: float, threads: int, n_agents: int):
     """Oracle in OpenMP mode (reference RNG) on the host: bounded sample of cfg2 steps."""
     from oracle.pyoracle import OracleSim
@@ -183,7 +198,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(dominant) if args.agents == 88700 else None,
+            "traffic_source": "profiles/mechanics_step_88k_3d_f32_hbm_traffic.md (rocprofv3 FETCH_SIZE + "
+                              "WRITE_SIZE per launch, copy-calibrated)",
             "bytes_per_agent": round(bytes_per_agent, 1),
             "avg_launch_us": round(avg_ms * 1000.0, 2),
             "launches": launches,

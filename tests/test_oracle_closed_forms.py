@@ -1,0 +1,562 @@
+"""Pins of the oracle (the CPU restatement of the reference solver) that the reference's own
+tests leave open: closed forms, invariants, brute force on tiny inputs, saturating-rate
+special cases and draw statistics.  Each test states the property it checks and the
+reference code that implies it; none re-evaluates the oracle's own formula in the same
+operation order, so a dropped term, a wrong sign, index or operand fails one of them.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import OracleSim
+from paraphysicell_amd import abi
+from paraphysicell_amd.state import AgentState
+
+PRESSURE_COEFFICIENT = 36.64504274775163  # position_solver.cpp:13
+
+
+def _sim(mesh, st, seed=7):
+    sim = OracleSim(mesh, st.real_bytes, st.defs, st.subs, capacity=max(st.count, 1), seed=seed)
+    sim.upload(st)
+    return sim
+
+
+def _state(dims, n, real_bytes=8, defs=1, subs=1, box=100, seed=0):
+    mesh = abi.MeshSpec(dims, (-box,) * 3, (box,) * 3)
+    return mesh, AgentState(mesh, real_bytes, defs, subs, n).defaults()
+
+
+def _lists(off, idx):
+    return [idx[off[i]:off[i + 1]].tolist() for i in range(len(off) - 1)]
+
+
+def _close(a, b, tol):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.all(np.abs(a - b) <= tol * np.maximum(np.abs(b), 1))
+
+
+# ------------------------------------------------------------------ motility (A8)
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_motility_full_bias_is_normalized_bias_direction(dims):
+    """migration_bias = 1 and dt/persistence > 1: every motile agent redraws and its
+    motility vector is bias_direction * speed / |bias_direction| (0 when |dir| <= 1e-16);
+    the velocity receives it (position_solver.cpp:115-145, solver_helper.h:250-286)."""
+    n = 64
+    mesh, st = _state(dims, n)
+    rng = np.random.default_rng(1)
+    dirs = rng.normal(size=(n, dims))
+    dirs[5] = 0.0  # zero direction -> zero vector
+    speed = rng.uniform(0.5, 3.0, n)
+    st["migration_bias_direction"] = dirs.ravel()
+    st["migration_bias"] = 1.0
+    st["persistence_time"] = 0.05
+    st["migration_speed"] = speed
+    st["is_motile"] = 1
+    st["velocities"] = 0.0
+    sim = _sim(mesh, st)
+    sim.update_motility(abi.Params(dt=0.1))
+    m = sim.get_field("motility_vector").reshape(n, dims)
+    length = np.linalg.norm(dirs, axis=1)
+    expected = np.where(length[:, None] > 1e-16, dirs / np.maximum(length, 1e-300)[:, None] * speed[:, None], 0.0)
+    assert _close(m, expected, 1e-12)
+    assert _close(sim.get_field("velocities").reshape(n, dims), expected, 1e-12)
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_motility_random_walk_has_speed_length(dims):
+    """migration_bias = 0: |m| = speed; the 3D walk lies in the xy plane because the
+    reference passes the restrict_to_2d array, not its element, as the flag
+    (position_solver.cpp:131, solver_helper.h:253-258); 2D angles are uniform."""
+    n = 4000
+    mesh, st = _state(dims, n)
+    speed = np.random.default_rng(2).uniform(0.5, 2.0, n)
+    st["migration_bias"] = 0.0
+    st["persistence_time"] = 0.05
+    st["migration_speed"] = speed
+    st["is_motile"] = 1
+    sim = _sim(mesh, st)
+    sim.update_motility(abi.Params(dt=0.1))
+    m = sim.get_field("motility_vector").reshape(n, dims)
+    assert _close(np.linalg.norm(m, axis=1), speed, 1e-12)
+    if dims == 3:
+        assert np.all(m[:, 2] == 0.0)
+    if dims >= 2:
+        unit = m[:, :2] / speed[:, None]
+        assert abs(unit[:, 0].mean()) < 4 / np.sqrt(n) and abs(unit[:, 1].mean()) < 4 / np.sqrt(n)
+        # uniform angle: the second moments of a uniform direction are 1/2
+        assert abs((unit[:, 0] ** 2).mean() - 0.5) < 4 / np.sqrt(n)
+    else:
+        assert abs((m[:, 0] > 0).mean() - 0.5) < 4 * 0.5 / np.sqrt(n)
+
+
+def test_motility_redraw_rate_and_persistence():
+    """A motile agent redraws with probability dt / persistence_time (strict `<` against a
+    uniform draw, position_solver.cpp:127); otherwise it keeps its motility vector, and in
+    both cases adds it to the velocity.  Non-motile agents are untouched, movable or not
+    (motility is not gated by is_movable)."""
+    n = 20000
+    mesh, st = _state(2, n)
+    st["migration_bias"] = 0.0
+    st["persistence_time"] = 0.1 / 0.3
+    st["migration_speed"] = 1.0
+    st["motility_vector"] = np.tile([7.0, -3.0], n)
+    st["velocities"] = np.tile([0.5, 0.25], n)
+    st["is_motile"] = (np.arange(n) % 10 != 0).astype(np.uint8)
+    st["is_movable"] = (np.arange(n) % 3 != 0).astype(np.uint8)
+    sim = _sim(mesh, st)
+    sim.update_motility(abi.Params(dt=0.1))
+    m = sim.get_field("motility_vector").reshape(n, 2)
+    v = sim.get_field("velocities").reshape(n, 2)
+    motile = st["is_motile"] == 1
+    kept = np.all(m == [7.0, -3.0], axis=1)
+    assert np.all(kept[~motile]) and np.all(v[~motile] == [0.5, 0.25])
+    frac = 1.0 - kept[motile].mean()
+    assert abs(frac - 0.3) < 5 * np.sqrt(0.3 * 0.7 / motile.sum())
+    assert _close(v[motile], m[motile] + [0.5, 0.25], 1e-12)
+
+
+# ------------------------------------------------------------------ basement membrane (A9)
+def _membrane_expected(p, lo, hi, r, s):
+    """v += sign * S * (1 - d / r)^2 * d for each of the two walls, d = max(|wall - x|, 1e-5),
+    no push beyond one radius (solver_helper.h:14-28)."""
+    out = 0.0
+    for wall, sign in ((lo, 1.0), (hi, -1.0)):
+        d = max(abs(wall - p), 1e-5)
+        out += sign * s * max(0.0, 1 - d / r) ** 2 * d
+    return out
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_membrane_closed_form(dims):
+    mins, maxs = (-100, -120, -140), (100, 120, 140)
+    mesh = abi.MeshSpec(dims, mins, maxs)
+    n = 300
+    st = AgentState(mesh, 8, 1, 1, n).defaults()
+    rng = np.random.default_rng(3)
+    r = rng.uniform(5, 10, n)
+    s = rng.uniform(1, 200, n)
+    pos = np.zeros((n, dims))
+    upper = [maxs[1] if dims == 1 else maxs[k] for k in range(dims)]  # 1D reads bbox max[1]
+    for k in range(dims):
+        side = rng.integers(0, 3, n)
+        pos[:, k] = np.where(side == 0, mins[k] + rng.uniform(0, 12, n),
+                             np.where(side == 1, upper[k] - rng.uniform(0, 12, n), rng.uniform(-50, 50, n)))
+    st["positions"] = pos.ravel()
+    st["radius"] = r
+    st["cell_BM_repulsion_strength"] = s
+    st["velocities"] = 0.0
+    st["is_movable"] = (np.arange(n) % 7 != 0).astype(np.uint8)
+    sim = _sim(mesh, st)
+    sim.update_basement_membrane_interactions(abi.Params(virtual_wall=True))
+    v = sim.get_field("velocities").reshape(n, dims)
+    exp = np.zeros((n, dims))
+    for i in range(n):
+        if st["is_movable"][i]:
+            for k in range(dims):
+                exp[i, k] = _membrane_expected(pos[i, k], mins[k], upper[k], r[i], s[i])
+    assert _close(v, exp, 1e-12)
+    # the wall flag off leaves velocities alone
+    sim2 = _sim(mesh, st)
+    sim2.update_basement_membrane_interactions(abi.Params(virtual_wall=False))
+    assert np.all(sim2.get_field("velocities") == 0.0)
+
+
+# ------------------------------------------------------------------ positions (A11)
+@pytest.mark.parametrize("real_bytes", [4, 8])
+def test_adams_bashforth_closed_form(real_bytes):
+    """Movable agents: x += 1.5 dt v - 0.5 dt v_prev, v_prev = v, v = 0; others keep all
+    three (position_solver.cpp:481-504)."""
+    n, dims = 500, 3
+    mesh, st = _state(dims, n, real_bytes)
+    rng = np.random.default_rng(4)
+    x, v, vp = (rng.normal(0, 20, (n, dims)) for _ in range(3))
+    st["positions"], st["velocities"], st["previous_velocities"] = x.ravel(), v.ravel(), vp.ravel()
+    mov = (rng.random(n) < 0.8).astype(np.uint8)
+    st["is_movable"] = mov
+    sim = _sim(mesh, st)
+    sim.update_positions(abi.Params(dt=0.1))
+    real = np.float32 if real_bytes == 4 else np.float64
+    x0, v0, vp0 = (st.view(f).astype(np.float64) for f in ("positions", "velocities", "previous_velocities"))
+    exp = np.where(mov[:, None] == 1, x0 + 0.15 * v0 - 0.05 * vp0, x0)
+    tol = 1e-5 if real_bytes == 4 else 1e-12
+    assert _close(sim.get_field("positions").reshape(n, dims), exp, tol)
+    assert np.array_equal(sim.get_field("previous_velocities").reshape(n, dims),
+                          np.where(mov[:, None] == 1, v0, vp0).astype(real))
+    assert np.array_equal(sim.get_field("velocities").reshape(n, dims),
+                          np.where(mov[:, None] == 1, 0.0, v0).astype(real))
+
+
+# ------------------------------------------------------------------ forces and pressure (A6, A7)
+def _binned(mesh, pos):
+    g = mesh.grid_shape()
+    c = [np.clip(np.floor((pos[:, k] - mesh.bbox_min[k]) / mesh.voxel[k]).astype(int), 0, g[k] - 1)
+         if k < pos.shape[1] else np.zeros(len(pos), int) for k in range(3)]
+    return c, c[0] + g[0] * (c[1] + g[1] * c[2])
+
+
+def _random_cluster(dims, n, seed, real_bytes=8, defs=2):
+    mesh, st = _state(dims, n, real_bytes, defs=defs, box=120)
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-70, 70, (n, dims))
+    # keep every pair away from the adhesion threshold and from voxel faces so that
+    # membership does not hinge on the last bit
+    st["positions"] = pos.ravel()
+    st["radius"] = rng.uniform(6, 11, n)
+    st["relative_maximum_adhesion_distance"] = rng.uniform(1.0, 1.4, n)
+    st["cell_cell_repulsion_strength"] = rng.uniform(5, 15, n)
+    st["cell_cell_adhesion_strength"] = rng.uniform(0.2, 0.8, n)
+    st["cell_adhesion_affinities"] = rng.uniform(0.5, 1.5, n * defs)
+    st["cell_definition_index"] = rng.integers(0, defs, n)
+    st["is_movable"] = (rng.random(n) < 0.85).astype(np.uint8)
+    st["velocities"] = 0.0
+    return mesh, st
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_neighbor_lists_brute_force(dims):
+    """For movable i, the list holds every j != i in the 3^d stencil of i's voxel (z, y, x
+    order, each bucket in descending index) within rel_i r_i + rel_j r_j; non-movable
+    agents get empty lists (position_solver.cpp:79-97, 180-195; common_solver.h:77-105;
+    the descending bucket order of containers_solver.cpp:44-56 run on one thread)."""
+    n = 400
+    mesh, st = _random_cluster(dims, n, 11 + dims)
+    sim = _sim(mesh, st)
+    sim.initialize_mesh()
+    got = _lists(*sim.neighbors())
+    pos = st.view("positions").astype(np.float64)
+    reach = st["relative_maximum_adhesion_distance"] * st["radius"]
+    c, vid = _binned(mesh, pos)
+    g = mesh.grid_shape()
+    members = {}
+    for i in range(n - 1, -1, -1):
+        members.setdefault(int(vid[i]), []).append(i)
+    for i in range(n):
+        if not st["is_movable"][i]:
+            assert got[i] == []
+            continue
+        exp = []
+        for z in (-1, 0, 1):
+            for y in (-1, 0, 1):
+                for x in (-1, 0, 1):
+                    cc = (c[0][i] + x, c[1][i] + y, c[2][i] + z)
+                    if not all(0 <= cc[k] < g[k] for k in range(3)):
+                        continue
+                    for j in members.get(cc[0] + g[0] * (cc[1] + g[1] * cc[2]), []):
+                        if j != i and np.linalg.norm(pos[i] - pos[j]) <= reach[i] + reach[j]:
+                            exp.append(j)
+        assert got[i] == exp, i
+
+
+def test_voxel_buckets_brute_force():
+    """Bucket v lists exactly the agents whose voxel is v, linearised x + gx (y + gy z),
+    in descending index (containers_solver.cpp:15-56 on one thread; test/test.cpp:322-331)."""
+    n = 700
+    mesh, st = _random_cluster(3, n, 5)
+    sim = _sim(mesh, st)
+    sim.update_mechanics_mesh()
+    counts, agents = sim.buckets()
+    _, vid = _binned(mesh, st.view("positions").astype(np.float64))
+    start = np.concatenate([[0], np.cumsum(counts)])
+    for v in range(len(counts)):
+        members = sorted(np.nonzero(vid == v)[0].tolist(), reverse=True)
+        assert agents[start[v]:start[v + 1]].tolist() == members
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_simple_pressure_closed_form(dims):
+    """p_k = c * sum over pairs (k, j) in each other's stencil and within adhesion reach of
+    max(0, 1 - d / (r_k + r_j))^2 * ([k movable] + [j movable]): both members of a pair add
+    the term to both pressures (position_solver.cpp:13, 32-76, 158-178)."""
+    n = 300
+    mesh, st = _random_cluster(dims, n, 21 + dims)
+    sim = _sim(mesh, st)
+    sim.initialize_mesh()
+    sim.update_cell_forces(abi.Params())
+    p = sim.get_field("simple_pressure")
+    pos = st.view("positions").astype(np.float64)
+    r = st["radius"].astype(np.float64)
+    reach = st["relative_maximum_adhesion_distance"] * r
+    c, _ = _binned(mesh, pos)
+    mov = st["is_movable"].astype(np.float64)
+    exp = np.zeros(n)
+    for k in range(n):
+        for j in range(n):
+            if j == k or max(abs(c[a][k] - c[a][j]) for a in range(3)) > 1:
+                continue
+            d = max(np.linalg.norm(pos[k] - pos[j]), 1e-5)
+            if d > reach[k] + reach[j]:
+                continue
+            exp[k] += PRESSURE_COEFFICIENT * max(0.0, 1 - d / (r[k] + r[j])) ** 2 * (mov[k] + mov[j])
+    assert _close(p, exp, 1e-12)
+
+
+# ------------------------------------------------------------------ springs (A10)
+def _with_springs(st, lists):
+    st.set_links(abi.LINKS_SPRINGS, lists)
+    return st
+
+
+def test_spring_contraction_closed_form():
+    """Rates zero: the lists stay and every movable i gets
+    sum_j sqrt(k_i k_j aff[i, t_j] aff[j, t_i]) (x_j - x_i) in slot order
+    (position_solver.cpp:337-370)."""
+    n, dims = 60, 3
+    mesh, st = _state(dims, n, defs=2)
+    rng = np.random.default_rng(9)
+    pos = rng.uniform(-50, 50, (n, dims))
+    st["positions"] = pos.ravel()
+    st["attachment_elastic_constant"] = rng.uniform(0.01, 0.2, n)
+    st["cell_adhesion_affinities"] = rng.uniform(0.5, 1.5, 2 * n)
+    st["cell_definition_index"] = rng.integers(0, 2, n)
+    st["attachment_rate"] = 0.0
+    st["detachment_rate"] = 0.0
+    st["is_movable"] = (rng.random(n) < 0.8).astype(np.uint8)
+    st["velocities"] = 0.0
+    lists = [[] for _ in range(n)]
+    for _ in range(120):
+        a, b = rng.choice(n, 2, replace=False)
+        if len(lists[a]) < 10 and len(lists[b]) < 10:
+            lists[a].append(int(b))
+            lists[b].append(int(a))
+    sim = _sim(mesh, _with_springs(st, lists))
+    sim.update_spring_attachments(abi.Params(springs=True))
+    assert _lists(*sim.links(abi.LINKS_SPRINGS)) == lists
+    k = st["attachment_elastic_constant"]
+    aff = st.view("cell_adhesion_affinities")
+    t = st["cell_definition_index"]
+    exp = np.zeros((n, dims))
+    for i in range(n):
+        if st["is_movable"][i]:
+            for j in lists[i]:
+                exp[i] += np.sqrt(k[i] * k[j] * aff[i, t[j]] * aff[j, t[i]]) * (pos[j] - pos[i])
+    assert _close(sim.get_field("velocities").reshape(n, dims), exp, 1e-12)
+
+
+def test_spring_detach_saturating():
+    """detachment_rate * dt >= 1 fires every draw (`<=`, position_solver.cpp:386): every
+    spring touching such an agent goes, springs between two zero-rate agents stay."""
+    n = 40
+    mesh, st = _state(3, n)
+    rng = np.random.default_rng(10)
+    st["positions"] = rng.uniform(-60, 60, 3 * n)
+    hot = rng.random(n) < 0.4
+    st["detachment_rate"] = np.where(hot, 20.0, 0.0)
+    st["attachment_rate"] = 0.0
+    lists = [[] for _ in range(n)]
+    for _ in range(60):
+        a, b = rng.choice(n, 2, replace=False)
+        if len(lists[a]) < 8 and len(lists[b]) < 8:
+            lists[a].append(int(b))
+            lists[b].append(int(a))
+    sim = _sim(mesh, _with_springs(st, lists))
+    sim.update_spring_attachments(abi.Params(springs=True))
+    got = _lists(*sim.links(abi.LINKS_SPRINGS))
+    exp = [[j for j in lists[i] if not hot[i] and not hot[j]] for i in range(n)]
+    assert got == exp
+
+
+def test_spring_attach_saturating_is_greedy_in_reference_order():
+    """attachment_rate * dt * affinity >= 1 succeeds on every draw: attachments are made in
+    (i ascending, neighbor order) for j > i while both ends are below their maximum, with
+    no duplicate check against existing springs (position_solver.cpp:413-448)."""
+    n = 120
+    mesh, st = _random_cluster(3, n, 12, defs=1)
+    rng = np.random.default_rng(13)
+    st["attachment_rate"] = 50.0
+    st["detachment_rate"] = 0.0
+    st["cell_adhesion_affinities"] = 1.0
+    st["maximum_number_of_attachments"] = rng.integers(0, 5, n)
+    lists = [[] for _ in range(n)]
+    # one pre-existing spring between two agents that will be neighbors
+    sim0 = _sim(mesh, st)
+    sim0.initialize_mesh()
+    nb = _lists(*sim0.neighbors())
+    a = next(i for i in range(n) if any(j > i for j in nb[i]) and st["maximum_number_of_attachments"][i] > 1)
+    b = next(j for j in nb[a] if j > a)
+    st["maximum_number_of_attachments"][b] = max(2, st["maximum_number_of_attachments"][b])
+    lists[a].append(b)
+    lists[b].append(a)
+    sim = _sim(mesh, _with_springs(st, lists))
+    sim.initialize_mesh()
+    sim.update_spring_attachments(abi.Params(springs=True))
+    got = _lists(*sim.links(abi.LINKS_SPRINGS))
+    cap = st["maximum_number_of_attachments"]
+    exp = [list(l) for l in lists]
+    for i in range(n):
+        for j in nb[i]:
+            if j > i and len(exp[i]) < cap[i] and len(exp[j]) < cap[j]:
+                exp[i].append(j)
+                exp[j].append(i)
+    assert got == exp
+    assert got[a].count(b) == 2  # the duplicate of the existing spring
+
+
+def test_spring_detach_gives_each_spring_two_chances():
+    """One draw per (agent, slot) with probability q: a spring survives only if neither
+    endpoint's draw fires, (1 - q)^2 (position_solver.cpp:380-402)."""
+    pairs = 6000
+    n = 2 * pairs
+    mesh, st = _state(3, n, box=2000)
+    rng = np.random.default_rng(14)
+    st["positions"] = rng.uniform(-1500, 1500, 3 * n)
+    q = 0.2
+    st["detachment_rate"] = q / 0.1
+    st["attachment_rate"] = 0.0
+    lists = [[i + 1] if i % 2 == 0 else [i - 1] for i in range(n)]
+    sim = _sim(mesh, _with_springs(st, lists))
+    sim.update_spring_attachments(abi.Params(dt=0.1, springs=True))
+    got = _lists(*sim.links(abi.LINKS_SPRINGS))
+    survive = np.mean([len(got[2 * p]) == 1 for p in range(pairs)])
+    assert all(len(got[2 * p]) == len(got[2 * p + 1]) for p in range(pairs))
+    assert abs(survive - (1 - q) ** 2) < 5 * np.sqrt(0.64 * 0.36 / pairs)
+
+
+def test_spring_attach_probability():
+    """Isolated neighbor pairs i < j attach with probability 1 - (1 - p_l)(1 - p_r),
+    p_l = rate_i dt aff[i, t_j], p_r = rate_j dt aff[j, t_i] (position_solver.cpp:425-436)."""
+    pairs = 5000
+    n = 2 * pairs
+    mesh, st = _state(3, n, box=3000)
+    rng = np.random.default_rng(15)
+    centres = np.stack(np.meshgrid(*(np.arange(18) * 150 - 1300,) * 3, indexing="ij"), -1).reshape(-1, 3)[:pairs]
+    pos = np.repeat(centres, 2, axis=0).astype(np.float64)
+    pos[1::2, 0] += 10.0
+    st["positions"] = pos.ravel()
+    st["attachment_rate"] = np.tile([0.3 / 0.1, 0.2 / 0.1], pairs)
+    st["detachment_rate"] = 0.0
+    sim = _sim(mesh, st)
+    sim.initialize_mesh()
+    sim.update_spring_attachments(abi.Params(dt=0.1, springs=True))
+    got = _lists(*sim.links(abi.LINKS_SPRINGS))
+    frac = np.mean([len(got[2 * p]) == 1 for p in range(pairs)])
+    expected = 1 - 0.7 * 0.8
+    assert abs(frac - expected) < 5 * np.sqrt(expected * (1 - expected) / pairs)
+
+
+# ------------------------------------------------------------------ interactions (A12)
+def test_ingest_conserves_volume_and_sets_geometry():
+    """Phagocytosis of a dead cell: the eater's total, fluid and solid volumes grow by the
+    prey's, the parts stay consistent, the radius is the sphere radius of the new volume,
+    internalized substrates grow by prey * fraction, and the prey is flagged
+    (interactions_solver.cpp:8-26, 58-67, 89-96, 98-112)."""
+    mesh = abi.MeshSpec(3, (0, 0, 0), (100, 100, 100))
+    st = AgentState(mesh, 8, 1, 2, 2).defaults()
+    st["positions"] = np.array([40, 40, 40, 45, 40, 40], np.float64)
+    st["dead"] = np.array([0, 1], np.uint8)
+    st["dead_phagocytosis_rate"] = np.array([1e9, 0.0])
+    st["volume"] = np.array([2494.0, 1800.0])
+    st["fluid"] = np.array([1870.5, 1300.0])
+    st["solid"] = np.array([623.5, 500.0])
+    st["internalized_substrates"] = np.array([1.0, 2.0, 3.0, 5.0])
+    st["fraction_transferred_when_ingested"] = np.array([0.0, 0.0, 0.5, 0.25])
+    st.set_links(abi.LINKS_SPRINGS, [[], []])
+    before = {f: st[f].copy() for f in ("volume", "fluid", "solid", "cytoplasmic", "cytoplasmic_fluid",
+                                          "cytoplasmic_solid")}
+    sim = _sim(mesh, st)
+    sim.initialize_mesh()
+    sim.update_cell_cell_interactions(abi.Params())
+    after = {f: sim.get_field(f) for f in before}
+    assert sim.get_field("flags")[1] == abi.FLAG_TO_REMOVE
+    assert np.isclose(after["volume"][0], before["volume"][0] + before["fluid"][1] + before["solid"][1], rtol=1e-14)
+    assert np.isclose(after["fluid"][0] + after["solid"][0], before["fluid"][0] + before["solid"][0]
+                      + before["fluid"][1] + before["solid"][1], rtol=1e-14)
+    assert np.isclose(after["cytoplasmic_fluid"][0] - before["cytoplasmic_fluid"][0], before["fluid"][1], rtol=1e-14)
+    assert np.isclose(after["cytoplasmic_solid"][0] - before["cytoplasmic_solid"][0], before["solid"][1], rtol=1e-14)
+    v = after["volume"][0]
+    assert np.isclose(sim.get_field("radius")[0], (3 * v / (4 * np.pi)) ** (1 / 3), rtol=1e-14)
+    assert np.isclose(sim.get_field("surface_area")[0], 4 * np.pi * sim.get_field("radius")[0] ** 2, rtol=1e-14)
+    assert np.allclose(sim.get_field("internalized_substrates")[:2], [1.0 + 3.0 * 0.5, 2.0 + 5.0 * 0.25], rtol=1e-14)
+
+
+def test_fusion_conserves_volume_and_centre_of_volume():
+    """Fusion: the fused cell sits at the volume-weighted mean position, total and nuclear
+    volumes add, the nuclei add, substrates add (interactions_solver.cpp:28-56, 69-87, 114-133)."""
+    mesh = abi.MeshSpec(3, (0, 0, 0), (100, 100, 100))
+    st = AgentState(mesh, 8, 2, 1, 2).defaults()
+    st["positions"] = np.array([40, 41, 42, 46, 43, 40], np.float64)
+    st["cell_definition_index"] = np.array([0, 1], np.int32)
+    st.view("fusion_rates")[0, :] = [0, 1e9]
+    st["volume"] = np.array([2494.0, 1500.0])
+    st["nuclear"] = np.array([540.0, 300.0])
+    st["number_of_nuclei"] = np.array([1, 2], np.int32)
+    st["internalized_substrates"] = np.array([1.5, 2.5])
+    sim = _sim(mesh, st)
+    sim.initialize_mesh()
+    sim.update_cell_cell_interactions(abi.Params())
+    x = sim.get_field("positions").reshape(2, 3)
+    exp = (np.array([40, 41, 42]) * 2494.0 + np.array([46, 43, 40]) * 1500.0) / (2494.0 + 1500.0)
+    assert np.allclose(x[0], exp, rtol=1e-14)
+    assert sim.get_field("flags")[1] == abi.FLAG_TO_REMOVE
+    assert sim.get_field("number_of_nuclei")[0] == 3
+    assert np.isclose(sim.get_field("internalized_substrates")[0], 4.0, rtol=1e-14)
+    nuc = sim.get_field("nuclear")[0]
+    assert np.isclose(nuc, st["nuclear_fluid"][0] + st["nuclear_solid"][0] + st["nuclear_fluid"][1]
+                      + st["nuclear_solid"][1], rtol=1e-14)
+    tot = sim.get_field("volume")[0]
+    assert np.isclose(tot, nuc + sim.get_field("cytoplasmic")[0], rtol=1e-14)
+
+
+def test_attack_probability():
+    """An attacker with one live target attacks with probability attack * immunogenicity * dt
+    (one draw per neighbor, strict `<`, interactions_solver.cpp:167, 208-231)."""
+    pairs = 5000
+    n = 2 * pairs
+    mesh, st = _state(3, n, defs=2, box=3000)
+    centres = np.stack(np.meshgrid(*(np.arange(18) * 150 - 1300,) * 3, indexing="ij"), -1).reshape(-1, 3)[:pairs]
+    pos = np.repeat(centres, 2, axis=0).astype(np.float64)
+    pos[1::2, 0] += 10.0
+    st["positions"] = pos.ravel()
+    st["cell_definition_index"] = np.tile([0, 1], pairs).astype(np.int32)
+    st.view("attack_rates")[0::2, 1] = 2.5
+    st.view("immunogenicities")[1::2, 0] = 0.8
+    sim = _sim(mesh, st)
+    sim.initialize_mesh()
+    sim.update_cell_cell_interactions(abi.Params(dt=0.1))
+    damaged = sim.get_field("damage")[1::2] > 0
+    p = 2.5 * 0.8 * 0.1
+    assert abs(damaged.mean() - p) < 5 * np.sqrt(p * (1 - p) / pairs)
+    assert np.allclose(sim.get_field("damage")[1::2][damaged], 0.1)  # damage_rate 1 * dt
+    assert np.all(sim.get_field("damage")[0::2] == 0)
+
+
+# ------------------------------------------------------------------ container update (A13)
+def test_compaction_closed_form_permutation():
+    """Removing flagged and out-of-box agents one at a time by swap-with-last, ascending,
+    leaves: survivors below N' in place; holes below N', ascending, filled by the survivors
+    at or above N' in descending index (containers_solver.cpp:81-138).  Deaths count only
+    flagged removals; spring partners are renamed."""
+    n = 300
+    mesh, st = _state(3, n, box=100)
+    rng = np.random.default_rng(16)
+    pos = rng.uniform(-90, 90, (n, 3))
+    out = rng.random(n) < 0.05
+    pos[out, 0] = 150.0
+    flagged = rng.random(n) < 0.2
+    st["positions"] = pos.ravel()
+    st["flags"] = np.where(flagged, abi.FLAG_TO_REMOVE, abi.FLAG_NONE).astype(np.uint8)
+    st["origin"] = np.arange(n, dtype=np.int32)
+    lists = [[] for _ in range(n)]
+    for _ in range(400):
+        a, b = rng.choice(n, 2, replace=False)
+        if len(lists[a]) < 8 and len(lists[b]) < 8:
+            lists[a].append(int(b))
+            lists[b].append(int(a))
+    sim = _sim(mesh, _with_springs(st, lists))
+    sim.update_cell_container_for_mechanics(abi.Params())
+    removed = out | flagged
+    kept = np.nonzero(~removed)[0]
+    n2 = len(kept)
+    holes = [i for i in range(n2) if removed[i]]
+    tail = sorted([i for i in kept if i >= n2], reverse=True)
+    perm = np.arange(n2)
+    for h, t in zip(holes, tail):
+        perm[h] = t
+    assert sim.count == n2
+    assert np.array_equal(sim.get_field("origin"), perm)
+    assert sim.deaths_count == int(flagged.sum())
+    new_of = {int(o): k for k, o in enumerate(perm)}
+    got = _lists(*sim.links(abi.LINKS_SPRINGS))
+    for k, o in enumerate(perm):
+        exp = sorted(new_of[j] for j in lists[o] if not removed[j])
+        assert sorted(got[k]) == exp
