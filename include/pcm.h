@@ -217,6 +217,24 @@ int pcm_step(pcm_sim* sim, const pcm_step_params* params, int32_t steps);
 int pcm_synchronize(pcm_sim* sim);
 
 /* ---------------------------------------------------------------- kernel timing */
+/* ---------------------------------------------------------------- ensembles
+ * No counterpart in the reference, which advances one simulation per process: a batch
+ * advances many simulations together for ensemble and serving workloads.  Members of
+ * one dimensionality share every kernel launch of the step (one launch covers all of
+ * them, simulations of different sizes included); the 1D, 2D and 3D groups run on
+ * separate streams.  A member keeps its handle and device state: its own calls keep
+ * working (they run on the group's stream) and pcm_destroy retires it first.  Results
+ * are the same as stepping each simulation alone. */
+typedef struct pcm_batch pcm_batch;
+int pcm_batch_create(int32_t real_bytes, pcm_batch** out);
+int pcm_batch_destroy(pcm_batch* batch);
+int pcm_batch_admit(pcm_batch* batch, pcm_sim* sim);
+int pcm_batch_retire(pcm_batch* batch, pcm_sim* sim);
+int pcm_batch_size(pcm_batch* batch, int32_t* sims);
+/* every member advances `steps` mechanics steps under the same parameters */
+int pcm_batch_step(pcm_batch* batch, const pcm_step_params* params, int32_t steps);
+int pcm_batch_synchronize(pcm_batch* batch);
+
 /* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,
  * simulator_measurer.h:5-33), at kernel granularity: while a profile is open, every launch
  * of a kernel whose bit is set in `kernel_mask` is bracketed by HIP events on the handle's

@@ -39,6 +39,14 @@ __global__ void k_begin_step(const SimCtx<R>* __restrict__ ctxs)
 		c.sc->step++;
 }
 
+// every member's scalars into one array (a batch reads all of them with one copy)
+template <class R>
+__global__ void k_gather_scalars(const SimCtx<R>* __restrict__ ctxs, Scalars* __restrict__ out)
+{
+	if (threadIdx.x == 0)
+		out[blockIdx.y] = *ctxs[blockIdx.y].sc;
+}
+
 // ------------------------------------------------------------------ counting sort into voxels
 // containers_solver::update_mechanics_mesh (containers_solver.cpp:15-56).  The reference's
 // single-thread scatter (fetch_sub from the voxel count) leaves every bucket in descending

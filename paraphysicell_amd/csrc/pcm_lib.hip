@@ -89,6 +89,16 @@ struct pcm_sim
 	virtual int profile_end(double* total_ms, int64_t* launches) = 0;
 };
 
+struct pcm_batch
+{
+	virtual ~pcm_batch() = default;
+	virtual int admit(pcm_sim* sim) = 0;
+	virtual int retire(pcm_sim* sim) = 0;
+	virtual int size(int32_t* sims) = 0;
+	virtual int step(const pcm_step_params* p, int steps) = 0;
+	virtual int synchronize() = 0;
+};
+
 namespace {
 
 enum phase_id
@@ -600,6 +610,14 @@ private:
 	hipStream_t stream_ = nullptr;
 	std::vector<void*> allocations_;
 	int n_launch_ = 0; // agents at the last upload: an upper bound of the device count
+	// launch target: a batch points these at its context array for one segmented launch
+	// over all its simulations (grid.y = simulation); grid bounds then cover every member
+	const SimCtx<R>* launch_ctx_ = nullptr; // nullptr: this simulation's own context
+	int launch_y_ = 1;
+	int grid_n_ = 0; // agent bound of grid sizes (0: n_launch_)
+	int grid_L_ = 0; // spring-slot bound of grid sizes (0: h_.L)
+	uint64_t ctx_version_ = 0; // increments whenever the device context changes
+	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
 	bool records_fresh_ = false; // force records describe the current agent fields
 	int binned_launch_ = 1 << 30; // launch bound of the last mesh update (>= agents it binned)
@@ -762,6 +780,7 @@ private:
 
 	int push_ctx()
 	{
+		ctx_version_++;
 		PCM_HIP(hipMemcpyAsync(d_, &h_, sizeof(h_), hipMemcpyHostToDevice, stream_));
 		// the host copy may change before the async copy runs
 		PCM_HIP(hipStreamSynchronize(stream_));
@@ -804,7 +823,9 @@ private:
 		return PCM_OK;
 	}
 
-	int grid_agents() const { return std::max(1, (n_launch_ + BLOCK - 1) / BLOCK); }
+	int gn() const { return grid_n_ ? grid_n_ : n_launch_; }
+	int gl() const { return grid_L_ ? grid_L_ : h_.L; }
+	int grid_agents() const { return std::max(1, (gn() + BLOCK - 1) / BLOCK); }
 
 	template <class Kern>
 	int launch(int tag, Kern kernel, int blocks, int threads)
@@ -816,7 +837,8 @@ private:
 			PCM_TRY(take_events(ev));
 			PCM_HIP(hipEventRecord(ev[0], stream_));
 		}
-		hipLaunchKernelGGL(kernel, dim3(blocks, 1), dim3(threads), 0, stream_, (const SimCtx<R>*)d_);
+		hipLaunchKernelGGL(kernel, dim3(blocks, launch_y_), dim3(threads), 0, stream_,
+						   launch_ctx_ ? launch_ctx_ : (const SimCtx<R>*)d_);
 		PCM_HIP(hipGetLastError());
 		if (timed)
 		{
@@ -836,7 +858,7 @@ private:
 	template <class Kern>
 	int per_lanes(int tag, Kern kernel, int lanes)
 	{
-		return launch(tag, kernel, std::max(1, (int)(((long long)n_launch_ * lanes + BLOCK - 1) / BLOCK)), BLOCK);
+		return launch(tag, kernel, std::max(1, (int)(((long long)gn() * lanes + BLOCK - 1) / BLOCK)), BLOCK);
 	}
 
 	struct prof_record
@@ -905,7 +927,7 @@ public:
 
 private:
 
-	int neighbor_blocks() const { return std::max(1, std::min(n_launch_, 16384)); }
+	int neighbor_blocks() const { return std::max(1, std::min(gn(), 16384)); }
 
 	template <int D>
 	int neighbors_d()
@@ -941,7 +963,7 @@ private:
 		if (!FUSED)
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 		PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D, FUSED>));
-		PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (n_launch_ + CLAIM_THREADS - 1) / CLAIM_THREADS),
+		PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
 					   CLAIM_THREADS));
 		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R>));
 		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
@@ -951,7 +973,7 @@ private:
 	{
 		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R>));
 		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
-					   std::max(1, (int)(((long long)n_launch_ * h_.L + BLOCK - 1) / BLOCK)), BLOCK));
+					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK));
 		PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, EVENT_LANES>, EVENT_LANES));
 		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
 		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
@@ -1029,16 +1051,15 @@ private:
 	// forces, motility and the membrane do not read springs; velocity contributions are
 	// then summed in the reference's order by the fused velocity kernel.
 	template <int D>
-	int step_d()
+	int step_d(bool records_fresh, bool fold_clear)
 	{
 		PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 		if (h_.springs)
 			PCM_TRY(spring_update_kernels());
-		if (!records_fresh_)
+		if (!records_fresh)
 			PCM_TRY(per_agent(PCM_K_FORCE_RECORDS, k_force_records<R, D>));
-		// the previous mesh update's voxel counts are cleared by the velocity kernel when its
-		// grid covers every agent that update binned
-		const bool fold_clear = binned_launch_ <= n_launch_;
+		// fold_clear: the previous mesh update's voxel counts are cleared by the velocity
+		// kernel, whose grid covers every agent that update binned
 		if (fold_clear)
 			PCM_TRY(per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, true, VELOCITY_LANES>, VELOCITY_LANES));
 		else
@@ -1054,17 +1075,284 @@ private:
 		return neighbors_d<D>();
 	}
 
-	int dispatch_step()
+	int dispatch_step() { return dispatch_step_flags(records_fresh_, binned_launch_ <= n_launch_); }
+
+	int dispatch_step_flags(bool records_fresh, bool fold_clear)
 	{
 		switch (h_.dims)
 		{
 			case 1:
-				return step_d<1>();
+				return step_d<1>(records_fresh, fold_clear);
 			case 2:
-				return step_d<2>();
+				return step_d<2>(records_fresh, fold_clear);
 			default:
-				return step_d<3>();
+				return step_d<3>(records_fresh, fold_clear);
 		}
+	}
+
+public:
+	// ---------------------------------------------------------------- batch membership
+	hipStream_t stream() const { return stream_; }
+	void set_stream(hipStream_t s) { stream_ = s; }
+	const SimCtx<R>& host_ctx() const { return h_; }
+	uint64_t ctx_version() const { return ctx_version_; }
+	int dims() const { return h_.dims; }
+	int launch_bound() const { return n_launch_; }
+	int spring_slots() const { return h_.L; }
+	bool records_fresh() const { return records_fresh_; }
+	bool binned_within(int grid_n) const { return binned_launch_ <= grid_n; }
+	void*& batch() { return batch_; }
+	int apply_params(const pcm_step_params& p) { return set_params(p); }
+	void set_scalars(const Scalars& sc) { sc_ = sc; }
+	const Scalars& scalars() const { return sc_; }
+
+	// `steps` fused steps of every simulation in `ctxs` (this one is the batch's first
+	// member), as one launch per kernel with grid.y = m
+	int run_segmented(const SimCtx<R>* ctxs, int m, int grid_n, int grid_L, bool records_fresh, bool fold_clear,
+					  int steps)
+	{
+		launch_ctx_ = ctxs;
+		launch_y_ = m;
+		grid_n_ = grid_n;
+		grid_L_ = grid_L;
+		int rc = PCM_OK;
+		for (int k = 0; k < steps && rc == PCM_OK; k++)
+		{
+			rc = dispatch_step_flags(records_fresh, fold_clear);
+			records_fresh = true;
+			fold_clear = true;
+		}
+		launch_ctx_ = nullptr;
+		launch_y_ = 1;
+		grid_n_ = 0;
+		grid_L_ = 0;
+		return rc;
+	}
+
+	// after segmented steps: the mesh, records and voxel counts are this step's
+	void mark_stepped()
+	{
+		mesh_fresh_ = true;
+		records_fresh_ = true;
+		binned_launch_ = n_launch_;
+	}
+
+	// a member whose segmented steps stopped on neighbor overflow: grow, rebuild, then
+	// finish its remaining steps alone
+	int finish_steps(unsigned long long target)
+	{
+		PCM_TRY(read_scalars());
+		for (int guard = 0; guard < 64 && sc_.step < target; guard++)
+		{
+			if (sc_.status)
+				PCM_TRY(recover());
+			const long long remaining = (long long)(target - sc_.step);
+			for (long long s = 0; s < remaining; s++)
+				PCM_TRY(dispatch_step());
+			PCM_TRY(read_scalars());
+		}
+		if (sc_.status)
+			PCM_TRY(recover());
+		if (sc_.step < target)
+			return fail(PCM_ERR_STATE, "mechanics step did not converge after repeated neighbor-capacity growth");
+		return PCM_OK;
+	}
+};
+
+// ---------------------------------------------------------------------------- batches
+// Simulations of one real type advance together: members with the same dimensionality
+// form a group whose kernels are launched once for all members (grid.y = member, each
+// block reads its member's context); the groups of 1D, 2D and 3D members run on their
+// own streams so their launches overlap.  Members keep their device state and handles;
+// while admitted their calls run on the group's stream.
+template <class R>
+class batch_impl final : public pcm_batch
+{
+public:
+	~batch_impl() override
+	{
+		for (auto& g : groups_)
+		{
+			if (g.stream)
+				(void)hipStreamSynchronize(g.stream);
+			for (auto& mb : g.sims)
+			{
+				mb.sim->set_stream(mb.own);
+				mb.sim->batch() = nullptr;
+			}
+			if (g.stream)
+			{
+				(void)hipStreamSynchronize(g.stream);
+				(void)hipStreamDestroy(g.stream);
+			}
+			if (g.d_ctx)
+				(void)hipFree(g.d_ctx);
+			if (g.d_sc)
+				(void)hipFree(g.d_sc);
+		}
+	}
+
+	int admit(pcm_sim* base) override
+	{
+		auto* sim = dynamic_cast<sim_impl<R>*>(base);
+		if (!sim)
+			return fail(PCM_ERR_ARGUMENT, "simulation of another real type than the batch");
+		if (sim->batch())
+			return fail(PCM_ERR_STATE, "simulation already belongs to a batch");
+		group& g = groups_[sim->dims() - 1];
+		if (!g.stream)
+			PCM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+		PCM_HIP(hipStreamSynchronize(sim->stream()));
+		member m;
+		m.sim = sim;
+		m.own = sim->stream();
+		sim->set_stream(g.stream);
+		sim->batch() = static_cast<pcm_batch*>(this);
+		g.sims.push_back(m);
+		return PCM_OK;
+	}
+
+	int retire(pcm_sim* base) override
+	{
+		for (auto& g : groups_)
+			for (size_t k = 0; k < g.sims.size(); k++)
+				if (g.sims[k].sim == base)
+				{
+					PCM_HIP(hipStreamSynchronize(g.stream));
+					g.sims[k].sim->set_stream(g.sims[k].own);
+					g.sims[k].sim->batch() = nullptr;
+					g.sims.erase(g.sims.begin() + (long)k);
+					return PCM_OK;
+				}
+		return fail(PCM_ERR_ARGUMENT, "simulation is not a member of this batch");
+	}
+
+	int size(int32_t* sims) override
+	{
+		int32_t total = 0;
+		for (auto& g : groups_)
+			total += (int32_t)g.sims.size();
+		*sims = total;
+		return PCM_OK;
+	}
+
+	int step(const pcm_step_params* p, int steps) override
+	{
+		if (!p || steps < 0)
+			return fail(PCM_ERR_ARGUMENT, "null params or negative step count");
+		// launch every group, then collect: the groups' streams run concurrently
+		for (int d = 0; d < 3; d++)
+			PCM_TRY(launch_group(groups_[d], d + 1, *p, steps));
+		for (auto& g : groups_)
+			PCM_TRY(collect_group(g, steps));
+		return PCM_OK;
+	}
+
+	int synchronize() override
+	{
+		for (auto& g : groups_)
+			if (g.stream)
+				PCM_HIP(hipStreamSynchronize(g.stream));
+		return PCM_OK;
+	}
+
+private:
+	struct member
+	{
+		sim_impl<R>* sim;
+		hipStream_t own;
+		unsigned long long start = 0;
+	};
+	struct group
+	{
+		std::vector<member> sims;
+		hipStream_t stream = nullptr;
+		SimCtx<R>* d_ctx = nullptr;
+		Scalars* d_sc = nullptr;
+		int cap = 0;
+		std::vector<SimCtx<R>> h_ctx;
+		std::vector<Scalars> h_sc;
+	};
+	group groups_[3];
+
+	int reserve(group& g, int m)
+	{
+		if (m <= g.cap)
+			return PCM_OK;
+		const int cap = std::max(m, 2 * g.cap);
+		if (g.d_ctx)
+			PCM_HIP(hipFree(g.d_ctx));
+		if (g.d_sc)
+			PCM_HIP(hipFree(g.d_sc));
+		g.d_ctx = nullptr;
+		g.d_sc = nullptr;
+		PCM_HIP(hipMalloc((void**)&g.d_ctx, sizeof(SimCtx<R>) * cap));
+		PCM_HIP(hipMalloc((void**)&g.d_sc, sizeof(Scalars) * cap));
+		g.cap = cap;
+		return PCM_OK;
+	}
+
+	// every member's scalars in one copy
+	int gather(group& g)
+	{
+		const int m = (int)g.sims.size();
+		hipLaunchKernelGGL(k_gather_scalars<R>, dim3(1, m), dim3(64), 0, g.stream, (const SimCtx<R>*)g.d_ctx, g.d_sc);
+		PCM_HIP(hipGetLastError());
+		g.h_sc.resize(m);
+		PCM_HIP(hipMemcpyAsync(g.h_sc.data(), g.d_sc, sizeof(Scalars) * m, hipMemcpyDeviceToHost, g.stream));
+		PCM_HIP(hipStreamSynchronize(g.stream));
+		return PCM_OK;
+	}
+
+	int launch_group(group& g, int dims, const pcm_step_params& p, int steps)
+	{
+		const int m = (int)g.sims.size();
+		if (m == 0 || steps == 0)
+			return PCM_OK;
+		PCM_TRY(reserve(g, m));
+		int grid_n = 1, grid_L = 1;
+		bool records_fresh = true;
+		for (auto& mb : g.sims)
+		{
+			PCM_TRY(mb.sim->apply_params(p));
+			grid_n = std::max(grid_n, mb.sim->launch_bound());
+			grid_L = std::max(grid_L, mb.sim->spring_slots());
+			records_fresh = records_fresh && mb.sim->records_fresh();
+		}
+		bool fold_clear = true;
+		for (auto& mb : g.sims)
+			fold_clear = fold_clear && mb.sim->binned_within(grid_n);
+		g.h_ctx.resize(m);
+		for (int k = 0; k < m; k++)
+			g.h_ctx[k] = g.sims[k].sim->host_ctx();
+		PCM_HIP(hipMemcpyAsync(g.d_ctx, g.h_ctx.data(), sizeof(SimCtx<R>) * m, hipMemcpyHostToDevice, g.stream));
+		PCM_TRY(gather(g));
+		for (int k = 0; k < m; k++)
+		{
+			if (g.h_sc[k].status)
+				return fail(PCM_ERR_STATE, "a batch member's device status word is set");
+			g.sims[k].start = g.h_sc[k].step;
+		}
+		(void)dims;
+		return g.sims[0].sim->run_segmented(g.d_ctx, m, grid_n, grid_L, records_fresh, fold_clear, steps);
+	}
+
+	int collect_group(group& g, int steps)
+	{
+		const int m = (int)g.sims.size();
+		if (m == 0 || steps == 0)
+			return PCM_OK;
+		PCM_TRY(gather(g));
+		for (int k = 0; k < m; k++)
+		{
+			auto* sim = g.sims[k].sim;
+			sim->mark_stepped();
+			sim->set_scalars(g.h_sc[k]);
+			const unsigned long long target = g.sims[k].start + (unsigned long long)steps;
+			if (g.h_sc[k].status || g.h_sc[k].step != target)
+				PCM_TRY(sim->finish_steps(target));
+		}
+		return PCM_OK;
 	}
 };
 
@@ -1121,8 +1409,69 @@ int pcm_create(const pcm_config* config, pcm_sim** out)
 
 int pcm_destroy(pcm_sim* sim)
 {
+	if (!sim)
+		return PCM_OK;
+	if (auto* s4 = dynamic_cast<sim_impl<float>*>(sim); s4 && s4->batch())
+		PCM_TRY(((pcm_batch*)s4->batch())->retire(sim));
+	if (auto* s8 = dynamic_cast<sim_impl<double>*>(sim); s8 && s8->batch())
+		PCM_TRY(((pcm_batch*)s8->batch())->retire(sim));
 	delete sim;
 	return PCM_OK;
+}
+
+int pcm_batch_create(int32_t real_bytes, pcm_batch** out)
+{
+	if (!out)
+		return fail(PCM_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	if (real_bytes == 4)
+		*out = new batch_impl<float>();
+	else if (real_bytes == 8)
+		*out = new batch_impl<double>();
+	else
+		return fail(PCM_ERR_ARGUMENT, "real_bytes must be 4 or 8");
+	return PCM_OK;
+}
+
+int pcm_batch_destroy(pcm_batch* batch)
+{
+	delete batch;
+	return PCM_OK;
+}
+
+int pcm_batch_admit(pcm_batch* batch, pcm_sim* sim)
+{
+	if (!batch || !sim)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return batch->admit(sim);
+}
+
+int pcm_batch_retire(pcm_batch* batch, pcm_sim* sim)
+{
+	if (!batch || !sim)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return batch->retire(sim);
+}
+
+int pcm_batch_size(pcm_batch* batch, int32_t* sims)
+{
+	if (!batch || !sims)
+		return fail(PCM_ERR_ARGUMENT, "null argument");
+	return batch->size(sims);
+}
+
+int pcm_batch_step(pcm_batch* batch, const pcm_step_params* params, int32_t steps)
+{
+	if (!batch)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return batch->step(params, steps);
+}
+
+int pcm_batch_synchronize(pcm_batch* batch)
+{
+	if (!batch)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return batch->synchronize();
 }
 
 #define PCM_NEED(sim)                                                                                                  \

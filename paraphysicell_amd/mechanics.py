@@ -284,6 +284,14 @@ def load_library(path: str | None = None):
     lib.pcm_profile_end.restype = ctypes.c_int
     lib.pcm_get_stats.argtypes = [_PTR, ctypes.c_void_p]
     lib.pcm_get_stats.restype = ctypes.c_int
+    for name, args in (("batch_create", [ctypes.c_int32, ctypes.POINTER(_PTR)]), ("batch_destroy", [_PTR]),
+                       ("batch_admit", [_PTR, _PTR]), ("batch_retire", [_PTR, _PTR]),
+                       ("batch_size", [_PTR, ctypes.POINTER(ctypes.c_int32)]),
+                       ("batch_step", [_PTR, ctypes.POINTER(abi.StepParams), ctypes.c_int32]),
+                       ("batch_synchronize", [_PTR])):
+        fn = getattr(lib, "pcm_" + name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
     if path is None:
         _LIB = lib
     return lib
@@ -326,6 +334,54 @@ class DeviceSim(_LibSim):
         cnt = (ctypes.c_int64 * len(names))()
         self._call("profile_end", self._h, ms, cnt)
         return {names[k]: (ms[k], cnt[k]) for k in range(len(names)) if cnt[k] > 0}
+
+
+class Batch:
+    """Simulations advanced together (pcm_batch_*): one launch per kernel covers every
+    member of a dimensionality, whatever its size; results equal stepping each alone."""
+
+    def __init__(self, real_bytes: int = 4, lib=None):
+        self._lib = lib or load_library()
+        self._h = _PTR()
+        self._check(self._lib.pcm_batch_create(int(real_bytes), ctypes.byref(self._h)))
+        self.members: list = []
+
+    def _check(self, rc):
+        if rc != abi.PCM_OK:
+            raise PcmError(rc, self._lib.pcm_last_error().decode())
+
+    def admit(self, sim: "DeviceSim") -> None:
+        self._check(self._lib.pcm_batch_admit(self._h, sim._h))
+        self.members.append(sim)
+
+    def retire(self, sim: "DeviceSim") -> None:
+        self._check(self._lib.pcm_batch_retire(self._h, sim._h))
+        self.members.remove(sim)
+
+    @property
+    def size(self) -> int:
+        n = ctypes.c_int32()
+        self._check(self._lib.pcm_batch_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def step(self, params=None, steps: int = 1) -> None:
+        p = (params or abi.Params()).to_c()
+        self._check(self._lib.pcm_batch_step(self._h, ctypes.byref(p), int(steps)))
+
+    def synchronize(self) -> None:
+        self._check(self._lib.pcm_batch_synchronize(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.pcm_batch_destroy(self._h)
+            self._h = _PTR()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def kernel_names(lib=None):

@@ -72,3 +72,16 @@ def test_argument_errors(lib):
     cfg = abi.make_config(abi.MeshSpec(3, (0, 0, 0), (9, 9, 9)), 2, 1, 1, 8, 0)
     assert lib.pcm_create(ctypes.byref(cfg), ctypes.byref(h)) in (abi.PCM_ERR_ARGUMENT, abi.PCM_ERR_DEVICE)
     assert lib.pcm_step(None, None, 1) == abi.PCM_ERR_ARGUMENT
+
+
+def test_batch_argument_errors(lib):
+    b = ctypes.c_void_p()
+    assert lib.pcm_batch_create(3, ctypes.byref(b)) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_create(4, ctypes.byref(b)) == abi.PCM_OK
+    n = ctypes.c_int32(-1)
+    assert lib.pcm_batch_size(b, ctypes.byref(n)) == abi.PCM_OK and n.value == 0
+    assert lib.pcm_batch_admit(b, None) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_retire(b, None) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_step(b, None, 1) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_step(None, None, 1) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_destroy(b) == abi.PCM_OK
