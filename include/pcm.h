@@ -234,6 +234,9 @@ int pcm_batch_size(pcm_batch* batch, int32_t* sims);
 /* every member advances `steps` mechanics steps under the same parameters */
 int pcm_batch_step(pcm_batch* batch, const pcm_step_params* params, int32_t steps);
 int pcm_batch_synchronize(pcm_batch* batch);
+/* enable != 0: each group's step is captured once as a hipGraph over all member slots and
+ * replayed per step (joining and leaving rewrite slots without a new capture) */
+int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable);
 
 /* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,
  * simulator_measurer.h:5-33), at kernel granularity: while a profile is open, every launch

@@ -885,7 +885,8 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 	const int rank = block_exclusive_scan(lead ? 1 : 0, total);
 	if (lead)
 		c.att_list[blockIdx.x * APB + rank] = i;
-	if (threadIdx.x == 0)
+	// only blocks that cover agents own a segment (a batch's grid covers its largest member)
+	if (threadIdx.x == 0 && (int)blockIdx.x * APB < c.sc->n)
 		c.blk_cnt[blockIdx.x] = total;
 }
 

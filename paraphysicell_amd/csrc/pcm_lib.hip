@@ -1160,11 +1160,16 @@ public:
 };
 
 // ---------------------------------------------------------------------------- batches
-// Simulations of one real type advance together: members with the same dimensionality
-// form a group whose kernels are launched once for all members (grid.y = member, each
-// block reads its member's context); the groups of 1D, 2D and 3D members run on their
-// own streams so their launches overlap.  Members keep their device state and handles;
-// while admitted their calls run on the group's stream.
+// Simulations of one real type advance together.  Members with the same dimensionality
+// form a group whose kernels are launched once for all of them: the group's context
+// array has one slot per member (grid.y = slot, each block reads its slot's context),
+// and free slots hold an empty placeholder simulation whose blocks exit at once.  The
+// 1D, 2D and 3D groups run on their own streams, so their launches overlap.  With graphs
+// on, one step of a group is captured once (a hipGraph of the step's launches over all
+// slots) and replayed each step; joining or leaving only rewrites context slots, and a
+// new capture is needed only when the slot count, the grid bound or the step parameters
+// change.  Members keep their device state and handles; while admitted, their own calls
+// run on the group's stream.
 template <class R>
 class batch_impl final : public pcm_batch
 {
@@ -1175,16 +1180,19 @@ public:
 		{
 			if (g.stream)
 				(void)hipStreamSynchronize(g.stream);
-			for (auto& mb : g.sims)
-			{
-				mb.sim->set_stream(mb.own);
-				mb.sim->batch() = nullptr;
-			}
+			for (auto& mb : g.slots)
+				if (mb.sim)
+				{
+					mb.sim->set_stream(mb.own);
+					mb.sim->batch() = nullptr;
+				}
+			if (g.exec)
+				(void)hipGraphExecDestroy(g.exec);
+			if (g.placeholder)
+				g.placeholder->set_stream(g.placeholder_own);
+			g.placeholder.reset();
 			if (g.stream)
-			{
-				(void)hipStreamSynchronize(g.stream);
 				(void)hipStreamDestroy(g.stream);
-			}
 			if (g.d_ctx)
 				(void)hipFree(g.d_ctx);
 			if (g.d_sc)
@@ -1200,28 +1208,34 @@ public:
 		if (sim->batch())
 			return fail(PCM_ERR_STATE, "simulation already belongs to a batch");
 		group& g = groups_[sim->dims() - 1];
-		if (!g.stream)
-			PCM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+		PCM_TRY(ensure_group(g, sim->dims()));
 		PCM_HIP(hipStreamSynchronize(sim->stream()));
+		PCM_HIP(hipStreamSynchronize(g.stream));
 		member m;
 		m.sim = sim;
 		m.own = sim->stream();
 		sim->set_stream(g.stream);
 		sim->batch() = static_cast<pcm_batch*>(this);
-		g.sims.push_back(m);
+		for (auto& slot : g.slots)
+			if (!slot.sim)
+			{
+				slot = m;
+				return PCM_OK;
+			}
+		g.slots.push_back(m);
 		return PCM_OK;
 	}
 
 	int retire(pcm_sim* base) override
 	{
 		for (auto& g : groups_)
-			for (size_t k = 0; k < g.sims.size(); k++)
-				if (g.sims[k].sim == base)
+			for (auto& slot : g.slots)
+				if (slot.sim == base)
 				{
 					PCM_HIP(hipStreamSynchronize(g.stream));
-					g.sims[k].sim->set_stream(g.sims[k].own);
-					g.sims[k].sim->batch() = nullptr;
-					g.sims.erase(g.sims.begin() + (long)k);
+					slot.sim->set_stream(slot.own);
+					slot.sim->batch() = nullptr;
+					slot = member {};
 					return PCM_OK;
 				}
 		return fail(PCM_ERR_ARGUMENT, "simulation is not a member of this batch");
@@ -1231,7 +1245,8 @@ public:
 	{
 		int32_t total = 0;
 		for (auto& g : groups_)
-			total += (int32_t)g.sims.size();
+			for (auto& slot : g.slots)
+				total += slot.sim != nullptr;
 		*sims = total;
 		return PCM_OK;
 	}
@@ -1241,8 +1256,8 @@ public:
 		if (!p || steps < 0)
 			return fail(PCM_ERR_ARGUMENT, "null params or negative step count");
 		// launch every group, then collect: the groups' streams run concurrently
-		for (int d = 0; d < 3; d++)
-			PCM_TRY(launch_group(groups_[d], d + 1, *p, steps));
+		for (auto& g : groups_)
+			PCM_TRY(launch_group(g, *p, steps));
 		for (auto& g : groups_)
 			PCM_TRY(collect_group(g, steps));
 		return PCM_OK;
@@ -1256,30 +1271,71 @@ public:
 		return PCM_OK;
 	}
 
+	void use_graphs(bool on) { graphs_ = on; }
+
 private:
 	struct member
 	{
-		sim_impl<R>* sim;
-		hipStream_t own;
+		sim_impl<R>* sim = nullptr;
+		hipStream_t own = nullptr;
 		unsigned long long start = 0;
+	};
+	struct capture_key
+	{
+		int slots = 0, grid_n = 0, grid_L = 0, springs = -1;
+		bool operator==(const capture_key&) const = default;
 	};
 	struct group
 	{
-		std::vector<member> sims;
+		std::vector<member> slots;
+		std::unique_ptr<sim_impl<R>> placeholder; // the empty simulation of free slots; launches the group
+		hipStream_t placeholder_own = nullptr;    // its own stream, restored before it is destroyed
 		hipStream_t stream = nullptr;
 		SimCtx<R>* d_ctx = nullptr;
 		Scalars* d_sc = nullptr;
 		int cap = 0;
+		hipGraphExec_t exec = nullptr;
+		capture_key key;
 		std::vector<SimCtx<R>> h_ctx;
 		std::vector<Scalars> h_sc;
+		bool active = false; // launched by the current step call
 	};
 	group groups_[3];
+	bool graphs_ = false;
+
+	int ensure_group(group& g, int dims)
+	{
+		if (g.placeholder)
+			return PCM_OK;
+		PCM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+		pcm_config c {};
+		c.mesh.dims = dims;
+		for (int k = 0; k < 3; k++)
+		{
+			c.mesh.bbox_min[k] = 0;
+			c.mesh.bbox_max[k] = 1;
+			c.mesh.voxel[k] = 1;
+		}
+		c.real_bytes = (int)sizeof(R);
+		c.cell_defs = 1;
+		c.substrates = 0;
+		c.capacity = 1;
+		auto ph = std::make_unique<sim_impl<R>>(c);
+		PCM_TRY(ph->init());
+		PCM_TRY(ph->set_count(0));
+		PCM_HIP(hipStreamSynchronize(ph->stream()));
+		g.placeholder_own = ph->stream();
+		ph->set_stream(g.stream);
+		g.placeholder = std::move(ph);
+		return PCM_OK;
+	}
 
 	int reserve(group& g, int m)
 	{
 		if (m <= g.cap)
 			return PCM_OK;
 		const int cap = std::max(m, 2 * g.cap);
+		PCM_HIP(hipStreamSynchronize(g.stream));
 		if (g.d_ctx)
 			PCM_HIP(hipFree(g.d_ctx));
 		if (g.d_sc)
@@ -1289,13 +1345,17 @@ private:
 		PCM_HIP(hipMalloc((void**)&g.d_ctx, sizeof(SimCtx<R>) * cap));
 		PCM_HIP(hipMalloc((void**)&g.d_sc, sizeof(Scalars) * cap));
 		g.cap = cap;
+		if (g.exec)
+		{
+			PCM_HIP(hipGraphExecDestroy(g.exec));
+			g.exec = nullptr;
+		}
 		return PCM_OK;
 	}
 
-	// every member's scalars in one copy
-	int gather(group& g)
+	// every slot's scalars in one copy
+	int gather(group& g, int m)
 	{
-		const int m = (int)g.sims.size();
 		hipLaunchKernelGGL(k_gather_scalars<R>, dim3(1, m), dim3(64), 0, g.stream, (const SimCtx<R>*)g.d_ctx, g.d_sc);
 		PCM_HIP(hipGetLastError());
 		g.h_sc.resize(m);
@@ -1304,51 +1364,101 @@ private:
 		return PCM_OK;
 	}
 
-	int launch_group(group& g, int dims, const pcm_step_params& p, int steps)
+	static int round_up_pow2(int x)
 	{
-		const int m = (int)g.sims.size();
-		if (m == 0 || steps == 0)
+		int p = 1;
+		while (p < x)
+			p <<= 1;
+		return p;
+	}
+
+	int launch_group(group& g, const pcm_step_params& p, int steps)
+	{
+		g.active = false;
+		int members = 0;
+		for (auto& slot : g.slots)
+			members += slot.sim != nullptr;
+		if (members == 0 || steps == 0)
 			return PCM_OK;
+		g.active = true;
+		const int m = (int)g.slots.size();
 		PCM_TRY(reserve(g, m));
 		int grid_n = 1, grid_L = 1;
 		bool records_fresh = true;
-		for (auto& mb : g.sims)
-		{
-			PCM_TRY(mb.sim->apply_params(p));
-			grid_n = std::max(grid_n, mb.sim->launch_bound());
-			grid_L = std::max(grid_L, mb.sim->spring_slots());
-			records_fresh = records_fresh && mb.sim->records_fresh();
-		}
+		PCM_TRY(g.placeholder->apply_params(p));
+		for (auto& slot : g.slots)
+			if (slot.sim)
+			{
+				PCM_TRY(slot.sim->apply_params(p));
+				grid_n = std::max(grid_n, slot.sim->launch_bound());
+				grid_L = std::max(grid_L, slot.sim->spring_slots());
+				records_fresh = records_fresh && slot.sim->records_fresh();
+			}
+		grid_n = round_up_pow2(grid_n);
 		bool fold_clear = true;
-		for (auto& mb : g.sims)
-			fold_clear = fold_clear && mb.sim->binned_within(grid_n);
+		for (auto& slot : g.slots)
+			if (slot.sim)
+				fold_clear = fold_clear && slot.sim->binned_within(grid_n);
 		g.h_ctx.resize(m);
 		for (int k = 0; k < m; k++)
-			g.h_ctx[k] = g.sims[k].sim->host_ctx();
+			g.h_ctx[k] = g.slots[k].sim ? g.slots[k].sim->host_ctx() : g.placeholder->host_ctx();
 		PCM_HIP(hipMemcpyAsync(g.d_ctx, g.h_ctx.data(), sizeof(SimCtx<R>) * m, hipMemcpyHostToDevice, g.stream));
-		PCM_TRY(gather(g));
+		PCM_TRY(gather(g, m));
 		for (int k = 0; k < m; k++)
+			if (g.slots[k].sim)
+			{
+				if (g.h_sc[k].status)
+					return fail(PCM_ERR_STATE, "a batch member's device status word is set");
+				g.slots[k].start = g.h_sc[k].step;
+			}
+		auto* leader = g.placeholder.get();
+		int done = 0;
+		if (!records_fresh || !fold_clear)
 		{
-			if (g.h_sc[k].status)
-				return fail(PCM_ERR_STATE, "a batch member's device status word is set");
-			g.sims[k].start = g.h_sc[k].step;
+			// the first step refreshes records or clears counts for some member: no graph
+			PCM_TRY(leader->run_segmented(g.d_ctx, m, grid_n, grid_L, records_fresh, fold_clear, 1));
+			done = 1;
 		}
-		(void)dims;
-		return g.sims[0].sim->run_segmented(g.d_ctx, m, grid_n, grid_L, records_fresh, fold_clear, steps);
+		if (!graphs_)
+			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, true, true, steps - done);
+		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs };
+		if (!g.exec || !(key == g.key))
+		{
+			if (g.exec)
+			{
+				PCM_HIP(hipGraphExecDestroy(g.exec));
+				g.exec = nullptr;
+			}
+			hipGraph_t graph = nullptr;
+			PCM_HIP(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+			const int rc = leader->run_segmented(g.d_ctx, m, grid_n, grid_L, true, true, 1);
+			const hipError_t end = hipStreamEndCapture(g.stream, &graph);
+			PCM_TRY(rc);
+			PCM_HIP(end);
+			const hipError_t inst = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+			PCM_HIP(hipGraphDestroy(graph));
+			PCM_HIP(inst);
+			g.key = key;
+		}
+		for (int k = done; k < steps; k++)
+			PCM_HIP(hipGraphLaunch(g.exec, g.stream));
+		return PCM_OK;
 	}
 
 	int collect_group(group& g, int steps)
 	{
-		const int m = (int)g.sims.size();
-		if (m == 0 || steps == 0)
+		if (!g.active)
 			return PCM_OK;
-		PCM_TRY(gather(g));
+		const int m = (int)g.slots.size();
+		PCM_TRY(gather(g, m));
 		for (int k = 0; k < m; k++)
 		{
-			auto* sim = g.sims[k].sim;
+			auto* sim = g.slots[k].sim;
+			if (!sim)
+				continue;
 			sim->mark_stepped();
 			sim->set_scalars(g.h_sc[k]);
-			const unsigned long long target = g.sims[k].start + (unsigned long long)steps;
+			const unsigned long long target = g.slots[k].start + (unsigned long long)steps;
 			if (g.h_sc[k].status || g.h_sc[k].step != target)
 				PCM_TRY(sim->finish_steps(target));
 		}
@@ -1472,6 +1582,17 @@ int pcm_batch_synchronize(pcm_batch* batch)
 	if (!batch)
 		return fail(PCM_ERR_ARGUMENT, "null handle");
 	return batch->synchronize();
+}
+
+int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable)
+{
+	if (!batch)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	if (auto* b4 = dynamic_cast<batch_impl<float>*>(batch))
+		b4->use_graphs(enable != 0);
+	else if (auto* b8 = dynamic_cast<batch_impl<double>*>(batch))
+		b8->use_graphs(enable != 0);
+	return PCM_OK;
 }
 
 #define PCM_NEED(sim)                                                                                                  \

@@ -288,7 +288,7 @@ def load_library(path: str | None = None):
                        ("batch_admit", [_PTR, _PTR]), ("batch_retire", [_PTR, _PTR]),
                        ("batch_size", [_PTR, ctypes.POINTER(ctypes.c_int32)]),
                        ("batch_step", [_PTR, ctypes.POINTER(abi.StepParams), ctypes.c_int32]),
-                       ("batch_synchronize", [_PTR])):
+                       ("batch_synchronize", [_PTR]), ("batch_use_graphs", [_PTR, ctypes.c_int32])):
         fn = getattr(lib, "pcm_" + name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
@@ -340,10 +340,11 @@ class Batch:
     """Simulations advanced together (pcm_batch_*): one launch per kernel covers every
     member of a dimensionality, whatever its size; results equal stepping each alone."""
 
-    def __init__(self, real_bytes: int = 4, lib=None):
+    def __init__(self, real_bytes: int = 4, graphs: bool = False, lib=None):
         self._lib = lib or load_library()
         self._h = _PTR()
         self._check(self._lib.pcm_batch_create(int(real_bytes), ctypes.byref(self._h)))
+        self._check(self._lib.pcm_batch_use_graphs(self._h, 1 if graphs else 0))
         self.members: list = []
 
     def _check(self, rc):

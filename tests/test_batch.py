@@ -47,16 +47,17 @@ def _member(spec, real_bytes=4):
 SPECS = [(2, 40, 1), (3, 300, 2), (2, 500, 3), (3, 64, 4), (3, 900, 5)]
 
 
+@pytest.mark.parametrize("graphs", [False, True])
 @pytest.mark.parametrize("real_bytes", [4, 8])
-def test_batch_equals_alone(real_bytes):
-    print(f"batch vs alone, real_bytes {real_bytes}", flush=True)
+def test_batch_equals_alone(real_bytes, graphs):
+    print(f"batch vs alone, real_bytes {real_bytes}, graphs {graphs}", flush=True)
     alone = []
     for spec in SPECS:
         sim, params = _member(spec, real_bytes)
         sim.step(params, 12)
         alone.append(_state_of(sim))
         sim.close()
-    batch = Batch(real_bytes)
+    batch = Batch(real_bytes, graphs=graphs)
     sims = []
     for spec in SPECS:
         sim, params = _member(spec, real_bytes)
@@ -64,10 +65,36 @@ def test_batch_equals_alone(real_bytes):
         sims.append(sim)
     assert batch.size == len(SPECS)
     batch.step(params, 5)
+    # eager work on a member between replays (a read of its state)
+    _ = sims[1].get_field("positions")
     batch.step(params, 7)
     batch.synchronize()
     for spec, sim, ref in zip(SPECS, sims, alone):
         _assert_equal(_state_of(sim), ref, f"member {spec}")
+    batch.close()
+
+
+def test_batch_graph_replays_match_alone_each_chunk():
+    """Every graph replay chunk agrees with the simulations stepped alone, with eager reads
+    and synchronisation between the replays."""
+    print("graph replays vs alone", flush=True)
+    specs = SPECS[:3]
+    refs = []
+    for spec in specs:
+        sim, params = _member(spec)
+        refs.append(sim)
+    batch = Batch(4, graphs=True)
+    sims = []
+    for spec in specs:
+        sim, params = _member(spec)
+        batch.admit(sim)
+        sims.append(sim)
+    for chunk in (3, 1, 4, 2):
+        batch.step(params, chunk)
+        batch.synchronize()
+        for spec, sim, ref in zip(specs, sims, refs):
+            ref.step(params, chunk)
+            _assert_equal(_state_of(sim), _state_of(ref), f"member {spec} after chunk {chunk}")
     batch.close()
 
 
@@ -82,7 +109,7 @@ def test_batch_join_and_leave():
         sim.close()
     a, params = _member((3, 400, 11))
     b, _ = _member((2, 300, 12))
-    batch = Batch(4)
+    batch = Batch(4, graphs=True)
     batch.admit(a)
     batch.step(params, 4)
     batch.admit(b)
