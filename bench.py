@@ -99,6 +99,81 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
 
 
+def ensemble_leg(steps: int):
+    """64 replicates (seeds 0-63, 500-20000 agents log-uniform, 2D even / 3D odd seeds, the
+    900-agent 2D workload's density and coefficients), `steps` steps each: one batch with
+    hipGraph-captured steps against the same simulations launched one at a time."""
+    from paraphysicell_amd import synth
+    from paraphysicell_amd.serving import run_ensemble
+
+    sizes = synth.config3_sizes()
+    members = []
+    params = None
+    for s_, n in enumerate(sizes):
+        mesh, st, params = synth.config3_member(s_, n)
+        members.append((mesh, st, s_))
+    t_b, as_b, sims = run_ensemble(members, params, steps, batched=True, graphs=True)
+    for sim in sims:
+        sim.close()
+    t_s, as_s, sims = run_ensemble(members, params, steps, batched=False)
+    for sim in sims:
+        sim.close()
+    return {
+        "workload": f"{len(sizes)} replicates, 500-20000 agents log-uniform (total {sum(sizes)}), 2D/3D alternating, "
+                    f"900-agent-2D density and coefficients, {steps} steps each, FP32",
+        "value": round(as_b / t_b, 1),
+        "unit": "agent-steps/s",
+        "ms_per_step": round(t_b * 1000.0 / steps, 4),
+        "sequential_value": round(as_s / t_s, 1),
+        "speedup_vs_sequential": round((as_b / t_b) / (as_s / t_s), 2),
+        "launch": "one batch, hipGraph-captured step per dims group (grid.y = simulation)",
+    }
+
+
+def serving_leg(count: int, rate: float, steps: int):
+    """Poisson arrivals at `rate`/s of `count` requests (1k-200k agents log-uniform, 2D/3D at
+    equal odds, the 88.7k-agent workload's physics), `steps` steps each, served by
+    continuous batching under a 200 GB HBM budget; the same requests saturating the server
+    and run one at a time give the throughput comparison."""
+    import numpy as np
+
+    from paraphysicell_amd import synth
+    from paraphysicell_amd.serving import Request, Server, poisson_arrivals, run_alone
+
+    arrivals = poisson_arrivals(count, rate, seed=4)
+    specs = [synth.serving_request(k, seed=4) for k in range(count)]
+
+    def requests(times):
+        return [Request(k, float(times[k]), mesh, st, params, steps, seed=1000 + k)
+                for k, (mesh, st, params) in enumerate(specs)]
+
+    server = Server(budget_bytes=200e9, chunk=10, graphs=True)
+    done = server.run(requests(arrivals), realtime=True)
+    served_rate = server.agent_steps / server.elapsed
+    lat = np.array([q.latency for q in done]) * 1000.0
+    server.close()
+    sat = Server(budget_bytes=200e9, chunk=10, graphs=True)
+    sat.run(requests(np.zeros(count)), realtime=False)
+    sat_rate = sat.agent_steps / sat.elapsed
+    sat.close()
+    t_alone, as_alone = run_alone(requests(np.zeros(count)))
+    alone_rate = as_alone / t_alone
+    agents = [st.count for _, st, _ in specs]
+    return {
+        "workload": f"{count} requests, Poisson {rate}/s, {min(agents)}-{max(agents)} agents (mean "
+                    f"{int(np.mean(agents))}), 2D/3D, springs + attack + dead phagocytosis, {steps} steps each, FP32, "
+                    "200 GB HBM budget, chunks of 10 steps",
+        "throughput": round(served_rate, 1),
+        "unit": "agent-steps/s",
+        "p50_latency_ms": round(float(np.percentile(lat, 50)), 2),
+        "p99_latency_ms": round(float(np.percentile(lat, 99)), 2),
+        "saturated_throughput": round(sat_rate, 1),
+        "one_at_a_time_throughput": round(alone_rate, 1),
+        "speedup_vs_one_at_a_time": round(sat_rate / alone_rate, 2),
+        "latency": "arrival to result (upload, mesh build, steps, read-back included)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +183,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS or all host cores")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-replicate ensemble leg")
+    ap.add_argument("--ensemble-steps", type=int, default=600)
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-requests leg")
+    ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 launches): the reference point that "
                          "scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
@@ -222,6 +301,10 @@ def main():
                       f"oracle with the reference's mt19937 draws, {cpu_ms / 1000.0:.1f} s",
             "gpu_over_cpu": round(value / cpu_value, 1),
         }
+    if not args.no_ensemble:
+        result["ensemble"] = ensemble_leg(args.ensemble_steps)
+    if not args.no_serving:
+        result["serving"] = serving_leg(args.serving_requests, 20.0, 100)
     print(json.dumps(result), flush=True)
     del np
 

@@ -1,0 +1,191 @@
+"""Ensembles and a served request loop over batches (pcm_batch_*).
+
+``run_ensemble`` advances a list of simulations together (one segmented launch per kernel
+for all members of a dimensionality) and, for comparison, one at a time.
+
+``Server`` is a continuous-batching request loop.  Each request is one new simulation and
+a number of mechanics steps.  A request that has arrived is admitted as soon as the HBM
+budget allows: its state is uploaded, its mesh built, and it joins the live batch between
+two step chunks, while the other requests keep advancing.  A finished request leaves the
+batch, its result is read back, and its device memory is freed.  Latency runs from arrival
+to result.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .mechanics import Batch, DeviceSim
+
+
+def footprint_bytes(mesh: abi.MeshSpec, real_bytes: int, defs: int, subs: int, capacity: int) -> int:
+    """Device bytes of one simulation: agent fields, fixed-stride lists and step scratch per
+    agent, plus two ints per mechanics voxel."""
+    d, r = mesh.dims, real_bytes
+    fields = sum(abi.field_width(w, d, defs, subs) * (r if t == abi.T_REAL else (4 if t == abi.T_I32 else 1))
+                 for _, t, w in abi.FIELDS)
+    k = {1: 8, 2: 16, 3: 32}[d]
+    per_agent = fields + 4 * (k + 12 + 1 + 2 * k + k) + k + 12 * r + 160
+    g = mesh.grid_shape()
+    return int(capacity * per_agent + 8 * g[0] * g[1] * g[2] + (1 << 16))
+
+
+@dataclass
+class Request:
+    key: int
+    arrival: float  # seconds after the start of the run
+    mesh: abi.MeshSpec
+    state: object
+    params: abi.Params
+    steps: int
+    seed: int = 0
+    # filled by the server
+    sim: DeviceSim | None = None
+    remaining: int = 0
+    need: int = 0
+    admitted: float = float("nan")
+    finished: float = float("nan")
+    result: dict = field(default_factory=dict)
+
+    @property
+    def agents(self) -> int:
+        return self.state.count
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def _start(req: Request) -> DeviceSim:
+    st = req.state
+    sim = DeviceSim(req.mesh, st.real_bytes, st.defs, st.subs, capacity=st.count, seed=req.seed)
+    sim.upload(st)
+    sim.initialize_mesh()
+    return sim
+
+
+def _result(sim: DeviceSim) -> dict:
+    return {"count": sim.count, "positions": sim.get_field("positions"), "velocities": sim.get_field("velocities"),
+            "deaths": sim.deaths_count}
+
+
+class Server:
+    """Continuous batching over one pcm batch (hipGraph-captured steps by default)."""
+
+    def __init__(self, budget_bytes: float = 200e9, chunk: int = 10, graphs: bool = True):
+        self.budget = budget_bytes
+        self.chunk = chunk
+        self.batch = Batch(4, graphs=graphs)
+        self.used = 0
+        self.agent_steps = 0
+
+    def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
+        """Serve `requests` (sorted by arrival).  With realtime, a request only becomes
+        visible at its arrival time; without, all are present at once (saturation)."""
+        pending = sorted(requests, key=lambda q: q.arrival)
+        live: list[Request] = []
+        done: list[Request] = []
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        while pending or live:
+            # admit every arrived request that fits the budget, in arrival order
+            while pending and (not realtime or pending[0].arrival <= now()):
+                q = pending[0]
+                need = footprint_bytes(q.mesh, q.state.real_bytes, q.state.defs, q.state.subs, q.agents)
+                if live and self.used + need > self.budget:
+                    break
+                pending.pop(0)
+                q.sim = _start(q)
+                q.remaining = q.steps
+                q.admitted = now()
+                q.need = need
+                self.used += need
+                self.batch.admit(q.sim)
+                live.append(q)
+            if not live:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
+                continue
+            k = min(self.chunk, min(q.remaining for q in live))
+            before = sum(q.sim.count for q in live)
+            self.batch.step(live[0].params, k)
+            self.agent_steps += before * k
+            for q in list(live):
+                q.remaining -= k
+                if q.remaining == 0:
+                    self.batch.retire(q.sim)
+                    q.result = _result(q.sim)
+                    q.sim.close()
+                    q.sim = None
+                    q.finished = now()
+                    self.used -= q.need
+                    live.remove(q)
+                    done.append(q)
+        self.elapsed = now()
+        return done
+
+    def close(self):
+        self.batch.close()
+
+
+def run_alone(requests: list[Request]) -> tuple[float, int]:
+    """Each request on its own, one after another: (seconds, agent-steps)."""
+    t0 = time.perf_counter()
+    agent_steps = 0
+    for q in requests:
+        sim = _start(q)
+        agent_steps += sim.count * q.steps
+        sim.step(q.params, q.steps)
+        q.result = _result(sim)
+        sim.close()
+    return time.perf_counter() - t0, agent_steps
+
+
+def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
+    """Arrival times of a Poisson stream (exponential gaps, seeded)."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, count))
+
+
+def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, graphs: bool = True):
+    """Advance every (mesh, state, seed) member `steps` steps; returns (seconds, agent-steps,
+    sims).  The timed region covers the steps only: states are resident before it starts."""
+    import torch
+
+    sims = []
+    for mesh, st, seed in members:
+        sim = DeviceSim(mesh, st.real_bytes, st.defs, st.subs, capacity=st.count, seed=seed)
+        sim.upload(st)
+        sim.initialize_mesh()
+        sims.append(sim)
+    agents = sum(s.count for s in sims)
+    batch = None
+    if batched:
+        batch = Batch(4, graphs=graphs)
+        for s in sims:
+            batch.admit(s)
+        batch.step(params, 1)  # warm-up: first-step refresh and graph capture
+        batch.synchronize()
+    else:
+        for s in sims:
+            s.step(params, 1)
+            s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if batched:
+        batch.step(params, steps)
+        batch.synchronize()
+    else:
+        for s in sims:
+            s.step(params, steps)
+        for s in sims:
+            s.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    agent_steps = sum(s.count for s in sims) * 0.5 * steps + agents * 0.5 * steps
+    if batch is not None:
+        batch.close()
+    return elapsed, agent_steps, sims

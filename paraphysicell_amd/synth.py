@@ -86,7 +86,9 @@ def config3_sizes(count: int = 64, lo: int = 500, hi: int = 20000, seed: int = 0
 
 
 def config3_member(s: int, n: int, real_bytes: int = 4):
-    """Replicate s of the ensemble: 2D if s is even, else 3D, at configs[0]'s density."""
+    """Replicate s of the 64-member ensemble: 2D if s is even, else 3D, at the density of the
+    900-agent 2D workload (disc of 400 um; the 3D density is its 3/2 power), in a box of
+    2.5 radii rounded up to 20 um, with that workload's coefficients (motile, wall on)."""
     dims = 2 if s % 2 == 0 else 3
     rho2 = 900 / (math.pi * 400.0 ** 2)
     if dims == 2:
@@ -96,3 +98,36 @@ def config3_member(s: int, n: int, real_bytes: int = 4):
         radius = (3 * n / (4 * math.pi * rho3)) ** (1 / 3)
     box = int(math.ceil(2.5 * radius / 20.0) * 20)
     return ball_population(dims, n, radius, box, s, real_bytes, motile=True, wall=True)
+
+
+def serving_request(k: int, seed: int = 0):
+    """Request k of the served workload: log-uniform 1k-200k agents, 2D or 3D at equal odds,
+    with the 88.7k-agent workload's physics (two types, springs, attack, dead phagocytosis).
+    3D populations keep that workload's density (ball of 380 um scaled by (N/88700)^(1/3));
+    2D ones use the 900-agent 2D workload's density (disc of 400 um scaled by
+    (N/900)^(1/2)), since a disc at the 3D count per area would overlap every cell."""
+    rng = np.random.default_rng([seed, k])
+    n = int(round(math.exp(rng.uniform(math.log(1000), math.log(200000)))))
+    dims = 2 if rng.uniform() < 0.5 else 3
+    if dims == 3:
+        ball = 380.0 * (n / 88700) ** (1 / 3)
+    else:
+        ball = 400.0 * math.sqrt(n / 900)
+    box = int(math.ceil(2.5 * ball / 20.0) * 20)
+    mesh, st, params = ball_population(dims, n, ball, box, int(rng.integers(1 << 30)), 4, defs=2, subs=1,
+                                       motile=True, wall=True, springs=True)
+    types = np.zeros(n, np.int32)
+    types[rng.permutation(n)[: int(round(0.3 * n))]] = 1
+    st["cell_definition_index"] = types
+    st["attachment_elastic_constant"] = 0.05
+    st["attachment_rate"] = 0.01
+    st["detachment_rate"] = 0.001
+    st["maximum_number_of_attachments"] = 12
+    st.view("attack_rates")[types == 1, 0] = 0.005
+    st["damage_rate"] = 1.0
+    st["dead_phagocytosis_rate"] = np.where(types == 1, 0.01, 0.0)
+    dead = np.zeros(n, np.uint8)
+    dead[rng.permutation(n)[: int(round(0.01 * n))]] = 1
+    st["dead"] = dead
+    st["is_motile"] = np.where(dead == 1, 0, 1)
+    return mesh, st, params
