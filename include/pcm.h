@@ -235,7 +235,10 @@ int pcm_batch_size(pcm_batch* batch, int32_t* sims);
 int pcm_batch_step(pcm_batch* batch, const pcm_step_params* params, int32_t steps);
 int pcm_batch_synchronize(pcm_batch* batch);
 /* enable != 0: each group's step is captured once as a hipGraph over all member slots and
- * replayed per step (joining and leaving rewrite slots without a new capture) */
+ * replayed per step (joining and leaving rewrite slots without a new capture).  A capture
+ * happens inside pcm_batch_step when none matches; no other host thread may make
+ * synchronous device calls (pcm_create, uploads, reads) meanwhile, since those invalidate
+ * it: disable graphs for steps that overlap such work. */
 int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable);
 
 /* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,

@@ -372,6 +372,9 @@ class Batch:
     def synchronize(self) -> None:
         self._check(self._lib.pcm_batch_synchronize(self._h))
 
+    def use_graphs(self, on: bool) -> None:
+        self._check(self._lib.pcm_batch_use_graphs(self._h, 1 if on else 0))
+
     def close(self) -> None:
         if self._h:
             self._lib.pcm_batch_destroy(self._h)
