@@ -217,6 +217,25 @@ int pcm_step(pcm_sim* sim, const pcm_step_params* params, int32_t steps);
 int pcm_synchronize(pcm_sim* sim);
 
 /* ---------------------------------------------------------------- kernel timing */
+/* ---------------------------------------------------------------- launch shape
+ * No counterpart in the reference (whose loops have no launch shape).  The lane-parallel
+ * kernels of the step can run with several shapes, all giving bit-identical results:
+ * lanes per agent of the force/velocity kernel and of the attachment and interaction
+ * kernels (4, 8 or 16), and wavefronts per occupied voxel of the neighbor search (1, 2 or
+ * 4).  The default is {8, 8, 2}. */
+typedef struct pcm_tuning
+{
+	int32_t velocity_lanes;
+	int32_t event_lanes;
+	int32_t neighbor_waves;
+} pcm_tuning;
+int pcm_set_tuning(pcm_sim* sim, const pcm_tuning* tuning);
+int pcm_get_tuning(pcm_sim* sim, pcm_tuning* tuning);
+/* Autotuner: times each candidate shape on the simulation's own steps (results are the
+ * same for every shape) and keeps, per kernel family, the fastest.  Advances the
+ * simulation by 1 + 3 * steps_per_candidate steps; writes the chosen shape if `chosen`. */
+int pcm_autotune(pcm_sim* sim, const pcm_step_params* params, int32_t steps_per_candidate, pcm_tuning* chosen);
+
 /* ---------------------------------------------------------------- ensembles
  * No counterpart in the reference, which advances one simulation per process: a batch
  * advances many simulations together for ensemble and serving workloads.  Members of
@@ -240,6 +259,9 @@ int pcm_batch_synchronize(pcm_batch* batch);
  * synchronous device calls (pcm_create, uploads, reads) meanwhile, since those invalidate
  * it: disable graphs for steps that overlap such work. */
 int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable);
+/* pcm_autotune for every dimensionality group of the batch, on the batch's own steps (one
+ * shape per group, used by its segmented launches); members advance 1 + 3 * steps. */
+int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t steps_per_candidate);
 
 /* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,
  * simulator_measurer.h:5-33), at kernel granularity: while a profile is open, every launch

@@ -114,6 +114,19 @@ class StepParams(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """pcm_tuning: launch shape of the lane-parallel kernels (results do not depend on it)."""
+
+    _fields_ = [
+        ("velocity_lanes", ctypes.c_int32),
+        ("event_lanes", ctypes.c_int32),
+        ("neighbor_waves", ctypes.c_int32),
+    ]
+
+    def as_tuple(self):
+        return (self.velocity_lanes, self.event_lanes, self.neighbor_waves)
+
+
 @dataclass
 class MeshSpec:
     """Microenvironment box (integer microns) and the mechanics voxel shape."""

@@ -15,10 +15,8 @@ constexpr int RESOLVE_THREADS = 1024;
 constexpr int CAND_CHUNK = 4096;    // interaction candidates ordered per LDS pass
 constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
 constexpr int LANE_BUF = 8;         // appends a lane buffers before one wavefront-wide reservation
-constexpr int VELOCITY_LANES = 8;   // lanes that share one agent in the velocity kernel
 constexpr int VELOCITY_ROUNDS = 1;  // list slots per lane gathered together
-constexpr int EVENT_LANES = 8;      // lanes that share one agent in the event-drawing kernels
-constexpr int NBR_WAVES = 2;        // wavefronts that share one voxel in the neighbor kernel
+constexpr int MAX_EVENT_LANES = 16; // largest lane count of the event-drawing kernels
 constexpr int CLAIM_THREADS = 1024; // block of the voxel claim kernel
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -259,8 +257,8 @@ __device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
 	c.nbr_cnt[i] = cnt;
 }
 
-template <class R, int D>
-__global__ void __launch_bounds__(64 * NBR_WAVES) k_neighbors(const SimCtx<R>* __restrict__ ctxs)
+template <class R, int D, int W>
+__global__ void __launch_bounds__(64 * W) k_neighbors(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	constexpr int TILE = nbr_tile<R>();
@@ -310,7 +308,7 @@ __global__ void __launch_bounds__(64 * NBR_WAVES) k_neighbors(const SimCtx<R>* _
 			// of entry starts at or below its position (lane k of every wavefront holds the
 			// start of entry k; read back with readlane, no LDS round trip per entry)
 			const int lane_pre = s_pre[lane < 28 ? lane : 27];
-			for (int t = tid; t < C; t += 64 * NBR_WAVES)
+			for (int t = tid; t < C; t += 64 * W)
 			{
 				int sv = 0;
 #pragma unroll
@@ -327,7 +325,7 @@ __global__ void __launch_bounds__(64 * NBR_WAVES) k_neighbors(const SimCtx<R>* _
 			}
 			__syncthreads();
 			const int own0 = __builtin_amdgcn_readlane(lane_pre, 13);
-			for (int a = wave; a < m; a += NBR_WAVES)
+			for (int a = wave; a < m; a += W)
 			{
 				const int i = s_idx[own0 + a];
 				R pi[4];
@@ -374,7 +372,7 @@ __global__ void __launch_bounds__(64 * NBR_WAVES) k_neighbors(const SimCtx<R>* _
 		{
 			// stencil larger than the LDS tile: one lane per agent, straight from HBM
 			const int s0 = c.vstart[v];
-			for (int a = tid; a < m; a += 64 * NBR_WAVES)
+			for (int a = tid; a < m; a += 64 * W)
 				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
 		}
 		__syncthreads();
@@ -897,16 +895,16 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 // entries with a smaller actor.  Each endpoint's count is bounded below by the successes
 // and above by the successes plus the undecided among those events; rounds repeat until
 // every event is decided (the earliest undecided event is always decidable).
-template <class R>
+template <class R, int G>
 __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs)
 {
 	const SimCtx<R>& c = ctxs[blockIdx.y];
 	__shared__ int s_unknown;
 	if (c.sc->status)
 		return;
-	// compact the collect blocks' actor segments into att_c: thread t takes a contiguous
-	// range of segments
-	constexpr int APB = BLOCK / EVENT_LANES;
+	// compact the collect blocks' actor segments into att_c (the collect kernel ran with G
+	// lanes per agent): thread t takes a contiguous range of segments
+	constexpr int APB = BLOCK / G;
 	const int nb = (c.sc->n + APB - 1) / APB;
 	const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
 	const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);

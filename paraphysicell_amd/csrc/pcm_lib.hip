@@ -87,6 +87,9 @@ struct pcm_sim
 	virtual int stats(int32_t* out) = 0;
 	virtual int profile_begin(uint64_t mask) = 0;
 	virtual int profile_end(double* total_ms, int64_t* launches) = 0;
+	virtual int set_tuning(const pcm_tuning* t) = 0;
+	virtual int get_tuning(pcm_tuning* t) = 0;
+	virtual int autotune(const pcm_step_params* p, int steps, pcm_tuning* chosen) = 0;
 };
 
 struct pcm_batch
@@ -97,6 +100,7 @@ struct pcm_batch
 	virtual int size(int32_t* sims) = 0;
 	virtual int step(const pcm_step_params* p, int steps) = 0;
 	virtual int synchronize() = 0;
+	virtual int autotune(const pcm_step_params* p, int steps) = 0;
 };
 
 namespace {
@@ -218,9 +222,9 @@ public:
 		PCM_TRY(alloc_t(&h.frec, (size_t)cap * 8));
 		PCM_TRY(alloc_t(&h.own_cnt, cap));
 		PCM_TRY(alloc_t(&h.in_cnt, cap));
-		// one segment of BLOCK / EVENT_LANES actors per collect block
+		// one segment of BLOCK / lanes actors per collect block (lanes <= MAX_EVENT_LANES)
 		PCM_TRY(alloc_t(&h.att_list, (size_t)cap + BLOCK));
-		PCM_TRY(alloc_t(&h.blk_cnt, (size_t)cap / (BLOCK / EVENT_LANES) + 2));
+		PCM_TRY(alloc_t(&h.blk_cnt, (size_t)cap / (BLOCK / MAX_EVENT_LANES) + 2));
 		PCM_TRY(alloc_t(&h.att_c, cap));
 		PCM_TRY(alloc_t(&h.fire, cap));
 		PCM_TRY(alloc_t(&h.erase, cap));
@@ -617,6 +621,7 @@ private:
 	int grid_n_ = 0; // agent bound of grid sizes (0: n_launch_)
 	int grid_L_ = 0; // spring-slot bound of grid sizes (0: h_.L)
 	uint64_t ctx_version_ = 0; // increments whenever the device context changes
+	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
 	bool records_fresh_ = false; // force records describe the current agent fields
@@ -933,7 +938,17 @@ private:
 	int neighbors_d()
 	{
 		if (mesh_fresh_)
-			return launch(PCM_K_NEIGHBORS, k_neighbors<R, D>, neighbor_blocks(), 64 * NBR_WAVES);
+		{
+			switch (tune_.neighbor_waves)
+			{
+				case 1:
+					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 1>, neighbor_blocks(), 64);
+				case 4:
+					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 4>, neighbor_blocks(), 256);
+				default:
+					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 2>, neighbor_blocks(), 128);
+			}
+		}
 		// the voxel buckets predate position changes: search each agent's stencil around
 		// its current position, as the reference does with stale buckets
 		return per_agent(PCM_K_NEIGHBORS, k_neighbors_direct<R, D>);
@@ -969,20 +984,60 @@ private:
 		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 	}
 
+	template <int D, bool CLEAR>
+	int launch_velocity()
+	{
+		switch (tune_.velocity_lanes)
+		{
+			case 4:
+				return per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, CLEAR, 4>, 4);
+			case 16:
+				return per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, CLEAR, 16>, 16);
+			default:
+				return per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, CLEAR, 8>, 8);
+		}
+	}
+
+	template <int D, bool AB2>
+	int launch_tentative()
+	{
+		switch (tune_.event_lanes)
+		{
+			case 4:
+				return per_lanes(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, AB2, 4>, 4);
+			case 16:
+				return per_lanes(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, AB2, 16>, 16);
+			default:
+				return per_lanes(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, AB2, 8>, 8);
+		}
+	}
+
 	int spring_update_kernels()
 	{
 		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R>));
 		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
 					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK));
-		PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, EVENT_LANES>, EVENT_LANES));
-		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
+		switch (tune_.event_lanes)
+		{
+			case 4:
+				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 4>, 4));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R, 4>, 1, RESOLVE_THREADS));
+				break;
+			case 16:
+				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 16>, 16));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R, 16>, 1, RESOLVE_THREADS));
+				break;
+			default:
+				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 8>, 8));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R, 8>, 1, RESOLVE_THREADS));
+		}
 		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
 	}
 
 	template <int D>
 	int interaction_kernels()
 	{
-		PCM_TRY(per_lanes(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, false, EVENT_LANES>, EVENT_LANES));
+		PCM_TRY((launch_tentative<D, false>()));
 		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, false>));
 		return launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, false>, 1, RESOLVE_THREADS);
@@ -1061,10 +1116,10 @@ private:
 		// fold_clear: the previous mesh update's voxel counts are cleared by the velocity
 		// kernel, whose grid covers every agent that update binned
 		if (fold_clear)
-			PCM_TRY(per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, true, VELOCITY_LANES>, VELOCITY_LANES));
+			PCM_TRY((launch_velocity<D, true>()));
 		else
-			PCM_TRY(per_lanes(PCM_K_VELOCITY_FUSED, k_velocity_fused<R, D, false, VELOCITY_LANES>, VELOCITY_LANES));
-		PCM_TRY(per_lanes(PCM_K_INTERACT_TENTATIVE, k_interact_tentative<R, D, true, EVENT_LANES>, EVENT_LANES));
+			PCM_TRY((launch_velocity<D, false>()));
+		PCM_TRY((launch_tentative<D, true>()));
 		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, true>));
 		PCM_TRY(launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, true>, 1, RESOLVE_THREADS));
@@ -1105,6 +1160,78 @@ public:
 	int apply_params(const pcm_step_params& p) { return set_params(p); }
 	void set_scalars(const Scalars& sc) { sc_ = sc; }
 	const Scalars& scalars() const { return sc_; }
+
+	pcm_tuning tuning() const { return tune_; }
+
+	static bool valid_tuning(const pcm_tuning& t)
+	{
+		auto one_of = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
+		return one_of(t.velocity_lanes, 4, 8, 16) && one_of(t.event_lanes, 4, 8, 16) && one_of(t.neighbor_waves, 1, 2, 4);
+	}
+
+	int set_tuning(const pcm_tuning* t) override
+	{
+		if (!t || !valid_tuning(*t))
+			return fail(PCM_ERR_ARGUMENT, "tuning values must be 4, 8 or 16 lanes and 1, 2 or 4 wavefronts");
+		tune_ = *t;
+		return PCM_OK;
+	}
+
+	int get_tuning(pcm_tuning* t) override
+	{
+		if (!t)
+			return fail(PCM_ERR_ARGUMENT, "null tuning");
+		*t = tune_;
+		return PCM_OK;
+	}
+
+	// The launch shapes give bit-identical results, so the tuner times real steps: each
+	// candidate shape runs `steps` steps with its kernels bracketed by events, and each
+	// kernel family keeps its fastest shape.  The simulation advances 1 + 3 * steps steps.
+	// `run` advances this simulation (or the batch it leads) by some steps.
+	template <class Run>
+	int autotune_with(int steps, Run&& run, pcm_tuning* chosen)
+	{
+		static const pcm_tuning candidates[3] = { { 4, 4, 1 }, { 8, 8, 2 }, { 16, 16, 4 } };
+		const uint64_t mask = (1ull << PCM_K_VELOCITY_FUSED) | (1ull << PCM_K_SPRING_COLLECT)
+							  | (1ull << PCM_K_SPRING_RESOLVE) | (1ull << PCM_K_INTERACT_TENTATIVE)
+							  | (1ull << PCM_K_NEIGHBORS);
+		PCM_TRY(run(1)); // warm-up with the current shape
+		double vel[3], ev[3], nb[3];
+		std::vector<double> ms(PCM_KERNEL_COUNT);
+		std::vector<int64_t> cnt(PCM_KERNEL_COUNT);
+		for (int k = 0; k < 3; k++)
+		{
+			tune_ = candidates[k];
+			PCM_TRY(profile_begin(mask));
+			PCM_TRY(run(steps));
+			PCM_TRY(profile_end(ms.data(), cnt.data()));
+			auto avg = [&](int tag) { return cnt[tag] ? ms[tag] / (double)cnt[tag] : 0.0; };
+			vel[k] = avg(PCM_K_VELOCITY_FUSED);
+			ev[k] = avg(PCM_K_SPRING_COLLECT) + avg(PCM_K_SPRING_RESOLVE) + avg(PCM_K_INTERACT_TENTATIVE);
+			nb[k] = avg(PCM_K_NEIGHBORS);
+		}
+		auto best = [](const double* t) {
+			int b = 1; // the default shape wins ties
+			for (int k = 0; k < 3; k++)
+				if (t[k] < t[b])
+					b = k;
+			return b;
+		};
+		tune_.velocity_lanes = candidates[best(vel)].velocity_lanes;
+		tune_.event_lanes = candidates[best(ev)].event_lanes;
+		tune_.neighbor_waves = candidates[best(nb)].neighbor_waves;
+		if (chosen)
+			*chosen = tune_;
+		return PCM_OK;
+	}
+
+	int autotune(const pcm_step_params* p, int steps, pcm_tuning* chosen) override
+	{
+		if (!p || steps < 1)
+			return fail(PCM_ERR_ARGUMENT, "null params or no steps per candidate");
+		return autotune_with(steps, [&](int k) { return step(p, k); }, chosen);
+	}
 
 	// `steps` fused steps of every simulation in `ctxs` (this one is the batch's first
 	// member), as one launch per kernel with grid.y = m
@@ -1273,6 +1400,33 @@ public:
 
 	void use_graphs(bool on) { graphs_ = on; }
 
+	// tune each group's launch shape on its real steps (graphs are off while tuning)
+	int autotune(const pcm_step_params* p, int steps) override
+	{
+		if (!p || steps < 1)
+			return fail(PCM_ERR_ARGUMENT, "null params or no steps per candidate");
+		const bool graphs = graphs_;
+		graphs_ = false;
+		int rc = PCM_OK;
+		for (auto& g : groups_)
+		{
+			bool any = false;
+			for (auto& slot : g.slots)
+				any = any || slot.sim != nullptr;
+			if (!any || rc != PCM_OK)
+				continue;
+			rc = g.placeholder->autotune_with(
+				steps,
+				[&](int k) {
+					PCM_TRY(launch_group(g, *p, k));
+					return collect_group(g, k);
+				},
+				nullptr);
+		}
+		graphs_ = graphs;
+		return rc;
+	}
+
 private:
 	struct member
 	{
@@ -1283,6 +1437,7 @@ private:
 	struct capture_key
 	{
 		int slots = 0, grid_n = 0, grid_L = 0, springs = -1;
+		int velocity_lanes = 0, event_lanes = 0, neighbor_waves = 0;
 		bool operator==(const capture_key&) const = default;
 	};
 	struct group
@@ -1421,7 +1576,9 @@ private:
 		}
 		if (!graphs_)
 			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, true, true, steps - done);
-		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs };
+		const pcm_tuning t = leader->tuning();
+		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs, t.velocity_lanes, t.event_lanes,
+								t.neighbor_waves };
 		if (!g.exec || !(key == g.key))
 		{
 			if (g.exec)
@@ -1582,6 +1739,34 @@ int pcm_batch_synchronize(pcm_batch* batch)
 	if (!batch)
 		return fail(PCM_ERR_ARGUMENT, "null handle");
 	return batch->synchronize();
+}
+
+int pcm_set_tuning(pcm_sim* sim, const pcm_tuning* tuning)
+{
+	if (!sim)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return sim->set_tuning(tuning);
+}
+
+int pcm_get_tuning(pcm_sim* sim, pcm_tuning* tuning)
+{
+	if (!sim)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return sim->get_tuning(tuning);
+}
+
+int pcm_autotune(pcm_sim* sim, const pcm_step_params* params, int32_t steps_per_candidate, pcm_tuning* chosen)
+{
+	if (!sim)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return sim->autotune(params, steps_per_candidate, chosen);
+}
+
+int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t steps_per_candidate)
+{
+	if (!batch)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return batch->autotune(params, steps_per_candidate);
 }
 
 int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable)

@@ -288,7 +288,12 @@ def load_library(path: str | None = None):
                        ("batch_admit", [_PTR, _PTR]), ("batch_retire", [_PTR, _PTR]),
                        ("batch_size", [_PTR, ctypes.POINTER(ctypes.c_int32)]),
                        ("batch_step", [_PTR, ctypes.POINTER(abi.StepParams), ctypes.c_int32]),
-                       ("batch_synchronize", [_PTR]), ("batch_use_graphs", [_PTR, ctypes.c_int32])):
+                       ("batch_synchronize", [_PTR]), ("batch_use_graphs", [_PTR, ctypes.c_int32]),
+                       ("batch_autotune", [_PTR, ctypes.POINTER(abi.StepParams), ctypes.c_int32]),
+                       ("set_tuning", [_PTR, ctypes.POINTER(abi.Tuning)]),
+                       ("get_tuning", [_PTR, ctypes.POINTER(abi.Tuning)]),
+                       ("autotune", [_PTR, ctypes.POINTER(abi.StepParams), ctypes.c_int32,
+                                     ctypes.POINTER(abi.Tuning)])):
         fn = getattr(lib, "pcm_" + name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
@@ -308,6 +313,25 @@ class DeviceSim(_LibSim):
 
     def synchronize(self):
         self._call("synchronize", self._h)
+
+    # ------------------------------------------------------------------ launch shape
+    @property
+    def tuning(self) -> tuple:
+        t = abi.Tuning()
+        self._call("get_tuning", self._h, ctypes.byref(t))
+        return t.as_tuple()
+
+    @tuning.setter
+    def tuning(self, value) -> None:
+        t = abi.Tuning(*value)
+        self._call("set_tuning", self._h, ctypes.byref(t))
+
+    def autotune(self, params=None, steps_per_candidate: int = 5) -> tuple:
+        """Pick the launch shape on this simulation's own steps (advances 1 + 3k steps)."""
+        p = (params or abi.Params()).to_c()
+        t = abi.Tuning()
+        self._call("autotune", self._h, ctypes.byref(p), int(steps_per_candidate), ctypes.byref(t))
+        return t.as_tuple()
 
     STAT_NAMES = ("attach_events", "resolve_rounds", "resolve_chunks", "interaction_candidates",
                   "interaction_replay", "removed", "occupied_voxels", "reserved")
@@ -374,6 +398,11 @@ class Batch:
 
     def use_graphs(self, on: bool) -> None:
         self._check(self._lib.pcm_batch_use_graphs(self._h, 1 if on else 0))
+
+    def autotune(self, params=None, steps_per_candidate: int = 5) -> None:
+        """Pick each dimensionality group's launch shape on the batch's own steps."""
+        p = (params or abi.Params()).to_c()
+        self._check(self._lib.pcm_batch_autotune(self._h, ctypes.byref(p), int(steps_per_candidate)))
 
     def close(self) -> None:
         if self._h:

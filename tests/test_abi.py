@@ -85,3 +85,10 @@ def test_batch_argument_errors(lib):
     assert lib.pcm_batch_step(b, None, 1) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_batch_step(None, None, 1) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_batch_destroy(b) == abi.PCM_OK
+
+
+def test_tuning_argument_errors(lib):
+    t = abi.Tuning(8, 8, 2)
+    assert lib.pcm_set_tuning(None, ctypes.byref(t)) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_autotune(None, None, 1, None) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_autotune(None, None, 1) == abi.PCM_ERR_ARGUMENT

@@ -158,3 +158,52 @@ def test_batch_member_neighbor_overflow():
     _assert_equal(_state_of(sim), want, "dense member")
     _assert_equal(_state_of(other), other_want, "other member")
     batch.close()
+
+
+SHAPES = [(4, 4, 1), (8, 8, 2), (16, 16, 4), (4, 16, 2), (16, 8, 1)]
+
+
+@pytest.mark.parametrize("real_bytes", [4, 8])
+def test_launch_shapes_give_identical_results(real_bytes):
+    """Every launch shape of the lane-parallel kernels gives bit-identical states."""
+    print(f"launch shapes, real_bytes {real_bytes}", flush=True)
+    want = None
+    for shape in SHAPES:
+        sim, params = _member((3, 400, 31), real_bytes)
+        sim.tuning = shape
+        assert sim.tuning == shape
+        sim.step(params, 6)
+        got = _state_of(sim)
+        sim.close()
+        if want is None:
+            want = got
+        else:
+            _assert_equal(got, want, f"shape {shape}")
+
+
+def test_autotune_keeps_results():
+    """Tuning on the simulation's own steps changes nothing but the launch shape."""
+    print("autotune", flush=True)
+    ref, params = _member((3, 600, 32))
+    ref.step(params, 1 + 3 * 2 + 4)
+    sim, _ = _member((3, 600, 32))
+    chosen = sim.autotune(params, 2)
+    assert sim.tuning == chosen
+    sim.step(params, 4)
+    _assert_equal(_state_of(sim), _state_of(ref), "autotuned")
+    batch = Batch(4, graphs=True)
+    refs = []
+    sims = []
+    for spec in SPECS[:3]:
+        r, _ = _member(spec)
+        r.step(params, 1 + 3 * 2 + 3)
+        refs.append(r)
+        s, _ = _member(spec)
+        batch.admit(s)
+        sims.append(s)
+    batch.autotune(params, 2)
+    batch.step(params, 3)
+    batch.synchronize()
+    for spec, s, r in zip(SPECS[:3], sims, refs):
+        _assert_equal(_state_of(s), _state_of(r), f"batch autotuned member {spec}")
+    batch.close()
