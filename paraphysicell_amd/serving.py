@@ -150,9 +150,11 @@ def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, count))
 
 
-def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, graphs: bool = True):
+def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, graphs: bool = True,
+                 autotune: bool = True):
     """Advance every (mesh, state, seed) member `steps` steps; returns (seconds, agent-steps,
-    sims).  The timed region covers the steps only: states are resident before it starts."""
+    sims).  The timed region covers the steps only: states are resident and the launch
+    shape tuned (batch groups, or each simulation alone) before it starts."""
     import torch
 
     sims = []
@@ -167,10 +169,14 @@ def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, 
         batch = Batch(4, graphs=graphs)
         for s in sims:
             batch.admit(s)
+        if autotune:
+            batch.autotune(params, 2)  # launch shape per dims group, on real steps
         batch.step(params, 1)  # warm-up: first-step refresh and graph capture
         batch.synchronize()
     else:
         for s in sims:
+            if autotune:
+                s.autotune(params, 2)
             s.step(params, 1)
             s.synchronize()
     torch.cuda.synchronize()
