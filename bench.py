@@ -227,10 +227,10 @@ def main():
     dev.upload(st)
     dev.initialize_mesh()
 
-    # warm-up steps: the autotuner picks the launch shape on real steps (1 + 3 * 3 of
+    # warm-up steps: the autotuner picks the launch shape on real steps (1 + 6 * 2 of
     # them), then the kernel profile of the last few steps picks the dominant kernel
-    tuned = dev.autotune(params, 3) if args.warmup >= 10 else dev.tuning
-    dev.step(params, max(args.warmup - 10 - 5, 0))
+    tuned = dev.autotune(params, 2) if args.warmup >= 13 else dev.tuning
+    dev.step(params, max(args.warmup - (13 if args.warmup >= 13 else 0) - 5, 0))
     from paraphysicell_amd.mechanics import kernel_names
 
     names = kernel_names()

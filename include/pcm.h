@@ -232,8 +232,9 @@ typedef struct pcm_tuning
 int pcm_set_tuning(pcm_sim* sim, const pcm_tuning* tuning);
 int pcm_get_tuning(pcm_sim* sim, pcm_tuning* tuning);
 /* Autotuner: times each candidate shape on the simulation's own steps (results are the
- * same for every shape) and keeps, per kernel family, the fastest.  Advances the
- * simulation by 1 + 3 * steps_per_candidate steps; writes the chosen shape if `chosen`. */
+ * same for every shape), twice in opposite orders, and keeps, per kernel family, the
+ * fastest.  Advances the simulation by 1 + 6 * steps_per_candidate steps; writes the
+ * chosen shape if `chosen`. */
 int pcm_autotune(pcm_sim* sim, const pcm_step_params* params, int32_t steps_per_candidate, pcm_tuning* chosen);
 
 /* ---------------------------------------------------------------- ensembles
@@ -260,7 +261,7 @@ int pcm_batch_synchronize(pcm_batch* batch);
  * it: disable graphs for steps that overlap such work. */
 int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable);
 /* pcm_autotune for every dimensionality group of the batch, on the batch's own steps (one
- * shape per group, used by its segmented launches); members advance 1 + 3 * steps. */
+ * shape per group, used by its segmented launches); members advance 1 + 6 * steps. */
 int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t steps_per_candidate);
 
 /* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,

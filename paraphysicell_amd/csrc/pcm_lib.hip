@@ -1186,9 +1186,10 @@ public:
 	}
 
 	// The launch shapes give bit-identical results, so the tuner times real steps: each
-	// candidate shape runs `steps` steps with its kernels bracketed by events, and each
-	// kernel family keeps its fastest shape.  The simulation advances 1 + 3 * steps steps.
-	// `run` advances this simulation (or the batch it leads) by some steps.
+	// candidate shape runs `steps` steps with its kernels bracketed by events, in two
+	// rounds of opposite order (the better round counts), and each kernel family keeps its
+	// fastest shape.  The simulation advances 1 + 6 * steps steps.  `run` advances this
+	// simulation (or the batch it leads) by some steps.
 	template <class Run>
 	int autotune_with(int steps, Run&& run, pcm_tuning* chosen)
 	{
@@ -1197,19 +1198,22 @@ public:
 							  | (1ull << PCM_K_SPRING_RESOLVE) | (1ull << PCM_K_INTERACT_TENTATIVE)
 							  | (1ull << PCM_K_NEIGHBORS);
 		PCM_TRY(run(1)); // warm-up with the current shape
-		double vel[3], ev[3], nb[3];
+		double vel[3] = { 1e300, 1e300, 1e300 }, ev[3] = { 1e300, 1e300, 1e300 }, nb[3] = { 1e300, 1e300, 1e300 };
 		std::vector<double> ms(PCM_KERNEL_COUNT);
 		std::vector<int64_t> cnt(PCM_KERNEL_COUNT);
-		for (int k = 0; k < 3; k++)
+		static const int order[6] = { 0, 1, 2, 2, 1, 0 };
+		for (int r = 0; r < 6; r++)
 		{
+			const int k = order[r];
 			tune_ = candidates[k];
 			PCM_TRY(profile_begin(mask));
 			PCM_TRY(run(steps));
 			PCM_TRY(profile_end(ms.data(), cnt.data()));
 			auto avg = [&](int tag) { return cnt[tag] ? ms[tag] / (double)cnt[tag] : 0.0; };
-			vel[k] = avg(PCM_K_VELOCITY_FUSED);
-			ev[k] = avg(PCM_K_SPRING_COLLECT) + avg(PCM_K_SPRING_RESOLVE) + avg(PCM_K_INTERACT_TENTATIVE);
-			nb[k] = avg(PCM_K_NEIGHBORS);
+			vel[k] = std::min(vel[k], avg(PCM_K_VELOCITY_FUSED));
+			ev[k] = std::min(ev[k], avg(PCM_K_SPRING_COLLECT) + avg(PCM_K_SPRING_RESOLVE)
+										+ avg(PCM_K_INTERACT_TENTATIVE));
+			nb[k] = std::min(nb[k], avg(PCM_K_NEIGHBORS));
 		}
 		auto best = [](const double* t) {
 			int b = 1; // the default shape wins ties

@@ -327,7 +327,7 @@ class DeviceSim(_LibSim):
         self._call("set_tuning", self._h, ctypes.byref(t))
 
     def autotune(self, params=None, steps_per_candidate: int = 5) -> tuple:
-        """Pick the launch shape on this simulation's own steps (advances 1 + 3k steps)."""
+        """Pick the launch shape on this simulation's own steps (advances 1 + 6k steps)."""
         p = (params or abi.Params()).to_c()
         t = abi.Tuning()
         self._call("autotune", self._h, ctypes.byref(p), int(steps_per_candidate), ctypes.byref(t))

@@ -185,7 +185,7 @@ def test_autotune_keeps_results():
     """Tuning on the simulation's own steps changes nothing but the launch shape."""
     print("autotune", flush=True)
     ref, params = _member((3, 600, 32))
-    ref.step(params, 1 + 3 * 2 + 4)
+    ref.step(params, 1 + 6 * 2 + 4)
     sim, _ = _member((3, 600, 32))
     chosen = sim.autotune(params, 2)
     assert sim.tuning == chosen
@@ -196,7 +196,7 @@ def test_autotune_keeps_results():
     sims = []
     for spec in SPECS[:3]:
         r, _ = _member(spec)
-        r.step(params, 1 + 3 * 2 + 3)
+        r.step(params, 1 + 6 * 2 + 3)
         refs.append(r)
         s, _ = _member(spec)
         batch.admit(s)
