@@ -258,7 +258,8 @@ __device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
 }
 
 template <class R, int D, int W>
-__global__ void __launch_bounds__(64 * W) k_neighbors(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))) k_neighbors(
+	const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	constexpr int TILE = nbr_tile<R>();
@@ -569,7 +570,9 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	if (g >= c.sc->n)
 		return;
 	const int i = agent_at<R>(c, g);
-	// every agent-level input is requested before the first dependent gather
+	// the agent's record, velocity, count and first list slots are requested together;
+	// the motility, membrane and spring inputs are loaded where they are used, which keeps
+	// the register count (and so the occupancy) of the neighbor loop low
 	const force_rec<R> me = read_force_record<R>(c, i);
 	const int cnt = c.nbr_cnt[i];
 	const int* nb = c.nbr_idx + (size_t)i * c.K;
@@ -580,16 +583,6 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	R v[D];
 	for (int k = 0; k < D; k++)
 		v[k] = c.vel[(size_t)i * D + k];
-	const motility_in<D, R> mot = load_motility<D, R>(c, i);
-	const R bm_strength = c.bm_repulsion[i];
-	const bool with_springs = c.springs != 0;
-	const int ns = with_springs ? c.spr_cnt[i] : 0;
-	const R ki = with_springs ? c.elastic[i] : (R)0;
-	const int* sp = c.spr_idx + (size_t)i * c.L;
-	int js[B];
-#pragma unroll
-	for (int u = 0; u < B; u++)
-		js[u] = with_springs && u * G + lane < c.L ? sp[u * G + lane] : -1;
 
 	const bool mov = me.movable;
 	R p[D];
@@ -648,20 +641,22 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	}
 	if (lane == 0)
 		c.pressure[i] = pressure;
-	motility_apply<D, R>(c, i, c.sc->step, mot, v, lane == 0);
+	if (c.motile[i] != 0)
+		motility_apply<D, R>(c, i, c.sc->step, load_motility<D, R>(c, i), v, lane == 0);
 	if (mov && c.wall)
-		membrane_apply<D, R>(c, me.r, bm_strength, p, v);
-	if (mov && with_springs)
+		membrane_apply<D, R>(c, me.r, c.bm_repulsion[i], p, v);
+	if (mov && c.springs)
 	{
 		// spring_contract_function (position_solver.cpp:337-370)
+		const int ns = c.spr_cnt[i];
+		const R ki = c.elastic[i];
+		const int* sp = c.spr_idx + (size_t)i * c.L;
 		for (int base = 0; base < ns; base += B * G)
 		{
-			if (base > 0)
-			{
+			int js[B];
 #pragma unroll
-				for (int u = 0; u < B; u++)
-					js[u] = base + u * G + lane < ns ? sp[base + u * G + lane] : -1;
-			}
+			for (int u = 0; u < B; u++)
+				js[u] = base + u * G + lane < ns ? sp[base + u * G + lane] : -1;
 			R f[B][D];
 			force_rec<R> other[B];
 			R kj[B], aff_ba[B];
