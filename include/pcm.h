@@ -216,13 +216,13 @@ int pcm_step(pcm_sim* sim, const pcm_step_params* params, int32_t steps);
 /* Wait for queued device work; reports PCM_ERR_STATE if a step aborted. */
 int pcm_synchronize(pcm_sim* sim);
 
-/* ---------------------------------------------------------------- kernel timing */
 /* ---------------------------------------------------------------- launch shape
  * No counterpart in the reference (whose loops have no launch shape).  The lane-parallel
  * kernels of the step can run with several shapes, all giving bit-identical results:
  * lanes per agent of the force/velocity kernel and of the attachment and interaction
- * kernels (4, 8 or 16), and wavefronts per occupied voxel of the neighbor search (1, 2 or
- * 4).  The default is {8, 8, 2}. */
+ * kernels (4, 8 or 16), and the neighbor search as 1, 2 or 4 wavefronts per occupied voxel
+ * sharing an LDS copy of its stencil, or (0) one wavefront per voxel holding the stencil in
+ * registers.  The default is {8, 8, 2}. */
 typedef struct pcm_tuning
 {
 	int32_t velocity_lanes;
@@ -264,6 +264,7 @@ int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable);
  * shape per group, used by its segmented launches); members advance 1 + 6 * steps. */
 int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t steps_per_candidate);
 
+/* ---------------------------------------------------------------- kernel timing */
 /* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,
  * simulator_measurer.h:5-33), at kernel granularity: while a profile is open, every launch
  * of a kernel whose bit is set in `kernel_mask` is bracketed by HIP events on the handle's

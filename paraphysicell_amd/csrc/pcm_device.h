@@ -104,7 +104,7 @@ struct SimCtx
 
 	// spring scratch
 	uint32_t *fire, *erase;  // per agent slot masks (L <= 32)
-	int* own_slot;           // per actor, K slots: neighbor slots of its attachment events
+	int* own_tgt;            // per actor, K slots: partners of its attachment events, in slot order
 	uint8_t* own_st;         // per actor, K slots: 0 undecided, 1 success, 2 failure
 	int* own_cnt;            // attachment events of an actor this step
 	int* inbox;              // per agent, K (actor, k) pairs: events that target it

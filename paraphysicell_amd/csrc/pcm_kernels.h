@@ -17,6 +17,7 @@ constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
 constexpr int LANE_BUF = 8;         // appends a lane buffers before one wavefront-wide reservation
 constexpr int VELOCITY_ROUNDS = 1;  // list slots per lane gathered together
 constexpr int MAX_EVENT_LANES = 16; // largest lane count of the event-drawing kernels
+constexpr int NBR_REG_CHUNKS = 5;   // 64-candidate chunks a wavefront keeps in registers
 constexpr int CLAIM_THREADS = 1024; // block of the voxel claim kernel
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -377,6 +378,142 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))
 				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
 		}
 		__syncthreads();
+	}
+}
+
+// Broadcast a lane's value to the whole wavefront (the lane index is uniform).
+__device__ __forceinline__ float bcast(float x, int src)
+{
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+__device__ __forceinline__ double bcast(double x, int src)
+{
+	const long long b = __double_as_longlong(x);
+	const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+	const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ int bcast(int x, int src) { return __builtin_amdgcn_readlane(x, src); }
+
+// element u of a register array, u uniform (a register array cannot be indexed at run time)
+template <class T, int N>
+__device__ __forceinline__ T pick(const T (&a)[N], int u)
+{
+	T r = a[0];
+#pragma unroll
+	for (int k = 1; k < N; k++)
+		if (u == k)
+			r = a[k];
+	return r;
+}
+
+// One wavefront per occupied voxel with the whole 3^d stencil held in registers: lane l
+// holds candidates l, l+64, ... of the stencil order.  Each agent of the centre voxel is
+// broadcast from the lane holding it and tested against every candidate; a ballot and a
+// prefix popcount per 64 candidates write the passing ones in candidate order.  No LDS
+// and no barrier, so residency is bounded by registers only.  A stencil of more than
+// NBR_REG_CHUNKS * 64 candidates takes the per-lane path.
+template <class R, int D>
+__global__ void __launch_bounds__(64) k_neighbors_regs(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	constexpr int NC = NBR_REG_CHUNKS;
+	const int lane = (int)threadIdx.x;
+	const int n_occ = c.sc->n_occ;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	for (int w = blockIdx.x; w < n_occ; w += gridDim.x)
+	{
+		const int v = c.occ[w];
+		int co[3];
+		voxel_coords<R>(c, v, co);
+		int cnt = 0, vs = 0;
+		if (lane < 27)
+		{
+			const int x = co[0] + lane % 3 - 1, y = co[1] + (lane / 3) % 3 - 1, z = co[2] + lane / 9 - 1;
+			if (x >= 0 && x < c.grid[0] && y >= 0 && y < c.grid[1] && z >= 0 && z < c.grid[2])
+			{
+				const int nv = x + c.grid[0] * (y + c.grid[1] * z);
+				cnt = c.vcnt[nv];
+				vs = c.vstart[nv];
+			}
+		}
+		int C;
+		const int pre = wave_exclusive_scan(cnt, C); // lane k: first stencil position of entry k
+		const int own0 = bcast(pre, 13);             // the centre voxel is stencil entry 13
+		const int m = bcast(pre, 14) - own0;
+		if (C > NC * 64)
+		{
+			const int s0 = c.vstart[v];
+			for (int a = lane; a < m; a += 64)
+				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
+			continue;
+		}
+		R q[NC][4];
+		int qj[NC];
+#pragma unroll
+		for (int u = 0; u < NC; u++)
+		{
+			const int t = u * 64 + lane;
+			// the stencil entry of position t and its bucket start, read across lanes with
+			// every lane active (a lane shuffle from an inactive lane is undefined)
+			int sv = 0;
+#pragma unroll
+			for (int k = 1; k < 27; k++)
+				sv += bcast(pre, k) <= t;
+			const int entry_start = __shfl(vs, sv, 64), entry_first = __shfl(pre, sv, 64);
+			qj[u] = -1;
+			for (int k = 0; k < 4; k++)
+				q[u][k] = 0;
+			if (t < C)
+			{
+				const int g = entry_start + t - entry_first;
+				qj[u] = c.bucket[g];
+				for (int k = 0; k < 4; k++)
+					q[u][k] = c.sorted[(size_t)g * 4 + k];
+			}
+		}
+		for (int a = 0; a < m; a++)
+		{
+			const int ta = own0 + a;
+			const int ua = ta >> 6, la = ta & 63;
+			const int i = bcast(pick(qj, ua), la);
+			R pi[4];
+			for (int k = 0; k < 4; k++)
+			{
+				R col[NC];
+#pragma unroll
+				for (int u = 0; u < NC; u++)
+					col[u] = q[u][k];
+				pi[k] = bcast(pick(col, ua), la);
+			}
+			int* out = c.nbr_idx + (size_t)i * c.K;
+			int base = 0;
+#pragma unroll
+			for (int u = 0; u < NC; u++)
+			{
+				if (u * 64 >= C)
+					break;
+				const bool pass = qj[u] >= 0 && qj[u] != i && pair_within<D, R>(pi, q[u], pi[3] + q[u][3]);
+				const unsigned long long mask = __ballot(pass);
+				if (pass)
+				{
+					const int slot = base + __popcll(mask & below);
+					if (slot < c.K)
+						out[slot] = qj[u];
+				}
+				base += __popcll(mask);
+			}
+			if (lane == 0)
+			{
+				if (base > c.K)
+				{
+					atomicOr(&c.sc->status, ST_NEIGHBOR_OVERFLOW);
+					atomicMax(&c.sc->max_neighbors, base);
+					base = c.K;
+				}
+				c.nbr_cnt[i] = base;
+			}
+		}
 	}
 }
 
@@ -809,7 +946,7 @@ __device__ __forceinline__ void spring_compact_agent(const SimCtx<R>& c, int i)
 }
 
 // Attachment events: every (i, slot) with j = neighbor > i whose draws succeed.  An actor
-// keeps its events in slot order (own_slot); each target collects (actor, k) references in
+// keeps its events' partners in slot order (own_tgt); each target collects (actor, k) references in
 // an unordered inbox; actors with events are listed for the resolver.  G lanes share an
 // agent: lane l draws for slots l, l+G, ...; a ballot ranks the events in slot order.
 template <class R, int G>
@@ -854,7 +991,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 			if (event)
 			{
 				const int k = n_ev + __popcll(m & ((1ull << lane) - 1ull));
-				c.own_slot[(size_t)i * c.K + k] = s;
+				c.own_tgt[(size_t)i * c.K + k] = j;
 				c.own_st[(size_t)i * c.K + k] = 0;
 				const int slot = atomicAdd(&c.in_cnt[j], 1);
 				if (slot < c.K)
@@ -883,6 +1020,57 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 		c.blk_cnt[blockIdx.x] = total;
 }
 
+// One actor's undecided attachment events, in its event order, against the states decided
+// so far (0 undecided, 1 success, 2 failure).  Undecided events left count into `unknown`.
+template <class R>
+__device__ __forceinline__ void resolve_actor(const SimCtx<R>& c, volatile uint8_t* st, int a, int na, int& unknown)
+{
+	const int c0a = c.spr_cnt[a], maxa = c.max_att[a];
+	int in_lo = 0, in_hi = 0;
+	const int nin = min(c.in_cnt[a], c.K);
+	for (int q = 0; q < nin; q++)
+	{
+		const int x = c.inbox[2 * ((size_t)a * c.K + q)], kx = c.inbox[2 * ((size_t)a * c.K + q) + 1];
+		const int sx = st[(size_t)x * c.K + kx];
+		in_lo += sx == 1;
+		in_hi += sx != 2;
+	}
+	int own_lo = 0, own_hi = 0;
+	for (int k = 0; k < na; k++)
+	{
+		int sk = st[(size_t)a * c.K + k];
+		if (sk == 0)
+		{
+			const int lo_a = c0a + in_lo + own_lo, hi_a = c0a + in_hi + own_hi;
+			const int cond_a = lo_a >= maxa ? 2 : (hi_a < maxa ? 1 : 0);
+			const int j = c.own_tgt[(size_t)a * c.K + k];
+			int lo_j = c.spr_cnt[j], hi_j = lo_j;
+			const int maxj = c.max_att[j];
+			const int njin = min(c.in_cnt[j], c.K);
+			for (int q = 0; q < njin; q++)
+			{
+				const int x = c.inbox[2 * ((size_t)j * c.K + q)];
+				if (x >= a)
+					continue;
+				const int sx = st[(size_t)x * c.K + c.inbox[2 * ((size_t)j * c.K + q) + 1]];
+				lo_j += sx == 1;
+				hi_j += sx != 2;
+			}
+			const int cond_j = lo_j >= maxj ? 2 : (hi_j < maxj ? 1 : 0);
+			if (cond_a == 2 || cond_j == 2)
+				sk = 2;
+			else if (cond_a == 1 && cond_j == 1)
+				sk = 1;
+			if (sk != 0)
+				st[(size_t)a * c.K + k] = (uint8_t)sk;
+			else
+				atomicAdd(&unknown, 1);
+		}
+		own_lo += sk == 1;
+		own_hi += sk != 2;
+	}
+}
+
 // Attachment resolution in the reference's order: event (i, j) succeeds iff, when the
 // single-thread loop reaches it (i ascending, neighbor order), both endpoints hold fewer
 // springs than their maximum.  The events before it at endpoint i are all of i's inbox
@@ -894,7 +1082,7 @@ template <class R, int G>
 __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs)
 {
 	const SimCtx<R>& c = ctxs[blockIdx.y];
-	__shared__ int s_unknown;
+	__shared__ int s_unknown, s_events;
 	if (c.sc->status)
 		return;
 	// compact the collect blocks' actor segments into att_c (the collect kernel ran with G
@@ -915,7 +1103,6 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 			c.att_c[at++] = c.att_list[b * APB + k];
 	}
 	const long long E = (long long)A * c.K; // bound on the events
-	__shared__ int s_events;
 	if (threadIdx.x == 0)
 		s_events = 0;
 	__syncthreads();
@@ -938,50 +1125,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 			const int na = c.own_cnt[a];
 			if (rounds == 0)
 				atomicAdd(&s_events, na);
-			const int c0a = c.spr_cnt[a], maxa = c.max_att[a];
-			int in_lo = 0, in_hi = 0;
-			const int nin = min(c.in_cnt[a], c.K);
-			for (int q = 0; q < nin; q++)
-			{
-				const int x = c.inbox[2 * ((size_t)a * c.K + q)], kx = c.inbox[2 * ((size_t)a * c.K + q) + 1];
-				const int sx = st[(size_t)x * c.K + kx];
-				in_lo += sx == 1;
-				in_hi += sx != 2;
-			}
-			int own_lo = 0, own_hi = 0;
-			for (int k = 0; k < na; k++)
-			{
-				int sk = st[(size_t)a * c.K + k];
-				if (sk == 0)
-				{
-					const int lo_a = c0a + in_lo + own_lo, hi_a = c0a + in_hi + own_hi;
-					const int cond_a = lo_a >= maxa ? 2 : (hi_a < maxa ? 1 : 0);
-					const int j = c.nbr_idx[(size_t)a * c.K + c.own_slot[(size_t)a * c.K + k]];
-					int lo_j = c.spr_cnt[j], hi_j = lo_j;
-					const int maxj = c.max_att[j];
-					const int njin = min(c.in_cnt[j], c.K);
-					for (int q = 0; q < njin; q++)
-					{
-						const int x = c.inbox[2 * ((size_t)j * c.K + q)];
-						if (x >= a)
-							continue;
-						const int sx = st[(size_t)x * c.K + c.inbox[2 * ((size_t)j * c.K + q) + 1]];
-						lo_j += sx == 1;
-						hi_j += sx != 2;
-					}
-					const int cond_j = lo_j >= maxj ? 2 : (hi_j < maxj ? 1 : 0);
-					if (cond_a == 2 || cond_j == 2)
-						sk = 2;
-					else if (cond_a == 1 && cond_j == 1)
-						sk = 1;
-					if (sk != 0)
-						st[(size_t)a * c.K + k] = (uint8_t)sk;
-					else
-						atomicAdd(&s_unknown, 1);
-				}
-				own_lo += sk == 1;
-				own_hi += sk != 2;
-			}
+			resolve_actor<R>(c, st, a, na, s_unknown);
 		}
 		__threadfence();
 		__syncthreads();
@@ -1040,7 +1184,7 @@ __global__ void k_spring_append(const SimCtx<R>* __restrict__ ctxs)
 	for (int k = 0; k < na; k++)
 		if (c.own_st[(size_t)a * c.K + k] == 1)
 		{
-			const int j = c.nbr_idx[(size_t)a * c.K + c.own_slot[(size_t)a * c.K + k]];
+			const int j = c.own_tgt[(size_t)a * c.K + k];
 			if (cnt < c.L)
 				out[cnt] = j;
 			else

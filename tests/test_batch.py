@@ -160,16 +160,17 @@ def test_batch_member_neighbor_overflow():
     batch.close()
 
 
-SHAPES = [(4, 4, 1), (8, 8, 2), (16, 16, 4), (4, 16, 2), (16, 8, 1)]
+SHAPES = [(4, 4, 1), (8, 8, 2), (16, 16, 4), (4, 16, 2), (16, 8, 1), (8, 8, 0), (4, 4, 0)]
 
 
+@pytest.mark.parametrize("spec", [(3, 400, 31), (2, 300, 33)])
 @pytest.mark.parametrize("real_bytes", [4, 8])
-def test_launch_shapes_give_identical_results(real_bytes):
+def test_launch_shapes_give_identical_results(real_bytes, spec):
     """Every launch shape of the lane-parallel kernels gives bit-identical states."""
-    print(f"launch shapes, real_bytes {real_bytes}", flush=True)
+    print(f"launch shapes, real_bytes {real_bytes}, {spec}", flush=True)
     want = None
     for shape in SHAPES:
-        sim, params = _member((3, 400, 31), real_bytes)
+        sim, params = _member(spec, real_bytes)
         sim.tuning = shape
         assert sim.tuning == shape
         sim.step(params, 6)
