@@ -221,8 +221,7 @@ int pcm_synchronize(pcm_sim* sim);
  * kernels of the step can run with several shapes, all giving bit-identical results:
  * lanes per agent of the force/velocity kernel and of the attachment and interaction
  * kernels (4, 8 or 16), and the neighbor search as 1, 2 or 4 wavefronts per occupied voxel
- * sharing an LDS copy of its stencil, or (0) one wavefront per voxel holding the stencil in
- * registers.  The default is {8, 8, 2}. */
+ * sharing an LDS copy of its stencil.  The default is {8, 8, 2}. */
 typedef struct pcm_tuning
 {
 	int32_t velocity_lanes;

@@ -17,7 +17,6 @@ constexpr int BAD_CHUNK = 8192;     // removed agents ordered in LDS
 constexpr int LANE_BUF = 8;         // appends a lane buffers before one wavefront-wide reservation
 constexpr int VELOCITY_ROUNDS = 1;  // list slots per lane gathered together
 constexpr int MAX_EVENT_LANES = 16; // largest lane count of the event-drawing kernels
-constexpr int NBR_REG_CHUNKS = 5;   // 64-candidate chunks a wavefront keeps in registers
 constexpr int CLAIM_THREADS = 1024; // block of the voxel claim kernel
 
 __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -378,142 +377,6 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))
 				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
 		}
 		__syncthreads();
-	}
-}
-
-// Broadcast a lane's value to the whole wavefront (the lane index is uniform).
-__device__ __forceinline__ float bcast(float x, int src)
-{
-	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
-}
-__device__ __forceinline__ double bcast(double x, int src)
-{
-	const long long b = __double_as_longlong(x);
-	const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
-	const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ int bcast(int x, int src) { return __builtin_amdgcn_readlane(x, src); }
-
-// element u of a register array, u uniform (a register array cannot be indexed at run time)
-template <class T, int N>
-__device__ __forceinline__ T pick(const T (&a)[N], int u)
-{
-	T r = a[0];
-#pragma unroll
-	for (int k = 1; k < N; k++)
-		if (u == k)
-			r = a[k];
-	return r;
-}
-
-// One wavefront per occupied voxel with the whole 3^d stencil held in registers: lane l
-// holds candidates l, l+64, ... of the stencil order.  Each agent of the centre voxel is
-// broadcast from the lane holding it and tested against every candidate; a ballot and a
-// prefix popcount per 64 candidates write the passing ones in candidate order.  No LDS
-// and no barrier, so residency is bounded by registers only.  A stencil of more than
-// NBR_REG_CHUNKS * 64 candidates takes the per-lane path.
-template <class R, int D>
-__global__ void __launch_bounds__(64) k_neighbors_regs(const SimCtx<R>* __restrict__ ctxs)
-{
-	PCM_PROLOGUE(c)
-	constexpr int NC = NBR_REG_CHUNKS;
-	const int lane = (int)threadIdx.x;
-	const int n_occ = c.sc->n_occ;
-	const unsigned long long below = (1ull << lane) - 1ull;
-	for (int w = blockIdx.x; w < n_occ; w += gridDim.x)
-	{
-		const int v = c.occ[w];
-		int co[3];
-		voxel_coords<R>(c, v, co);
-		int cnt = 0, vs = 0;
-		if (lane < 27)
-		{
-			const int x = co[0] + lane % 3 - 1, y = co[1] + (lane / 3) % 3 - 1, z = co[2] + lane / 9 - 1;
-			if (x >= 0 && x < c.grid[0] && y >= 0 && y < c.grid[1] && z >= 0 && z < c.grid[2])
-			{
-				const int nv = x + c.grid[0] * (y + c.grid[1] * z);
-				cnt = c.vcnt[nv];
-				vs = c.vstart[nv];
-			}
-		}
-		int C;
-		const int pre = wave_exclusive_scan(cnt, C); // lane k: first stencil position of entry k
-		const int own0 = bcast(pre, 13);             // the centre voxel is stencil entry 13
-		const int m = bcast(pre, 14) - own0;
-		if (C > NC * 64)
-		{
-			const int s0 = c.vstart[v];
-			for (int a = lane; a < m; a += 64)
-				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
-			continue;
-		}
-		R q[NC][4];
-		int qj[NC];
-#pragma unroll
-		for (int u = 0; u < NC; u++)
-		{
-			const int t = u * 64 + lane;
-			// the stencil entry of position t and its bucket start, read across lanes with
-			// every lane active (a lane shuffle from an inactive lane is undefined)
-			int sv = 0;
-#pragma unroll
-			for (int k = 1; k < 27; k++)
-				sv += bcast(pre, k) <= t;
-			const int entry_start = __shfl(vs, sv, 64), entry_first = __shfl(pre, sv, 64);
-			qj[u] = -1;
-			for (int k = 0; k < 4; k++)
-				q[u][k] = 0;
-			if (t < C)
-			{
-				const int g = entry_start + t - entry_first;
-				qj[u] = c.bucket[g];
-				for (int k = 0; k < 4; k++)
-					q[u][k] = c.sorted[(size_t)g * 4 + k];
-			}
-		}
-		for (int a = 0; a < m; a++)
-		{
-			const int ta = own0 + a;
-			const int ua = ta >> 6, la = ta & 63;
-			const int i = bcast(pick(qj, ua), la);
-			R pi[4];
-			for (int k = 0; k < 4; k++)
-			{
-				R col[NC];
-#pragma unroll
-				for (int u = 0; u < NC; u++)
-					col[u] = q[u][k];
-				pi[k] = bcast(pick(col, ua), la);
-			}
-			int* out = c.nbr_idx + (size_t)i * c.K;
-			int base = 0;
-#pragma unroll
-			for (int u = 0; u < NC; u++)
-			{
-				if (u * 64 >= C)
-					break;
-				const bool pass = qj[u] >= 0 && qj[u] != i && pair_within<D, R>(pi, q[u], pi[3] + q[u][3]);
-				const unsigned long long mask = __ballot(pass);
-				if (pass)
-				{
-					const int slot = base + __popcll(mask & below);
-					if (slot < c.K)
-						out[slot] = qj[u];
-				}
-				base += __popcll(mask);
-			}
-			if (lane == 0)
-			{
-				if (base > c.K)
-				{
-					atomicOr(&c.sc->status, ST_NEIGHBOR_OVERFLOW);
-					atomicMax(&c.sc->max_neighbors, base);
-					base = c.K;
-				}
-				c.nbr_cnt[i] = base;
-			}
-		}
 	}
 }
 

@@ -941,8 +941,6 @@ private:
 		{
 			switch (tune_.neighbor_waves)
 			{
-				case 0:
-					return launch(PCM_K_NEIGHBORS, k_neighbors_regs<R, D>, neighbor_blocks(), 64);
 				case 1:
 					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 1>, neighbor_blocks(), 64);
 				case 4:
@@ -1168,8 +1166,7 @@ public:
 	static bool valid_tuning(const pcm_tuning& t)
 	{
 		auto one_of = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-		return one_of(t.velocity_lanes, 4, 8, 16) && one_of(t.event_lanes, 4, 8, 16)
-			   && (t.neighbor_waves == 0 || one_of(t.neighbor_waves, 1, 2, 4));
+		return one_of(t.velocity_lanes, 4, 8, 16) && one_of(t.event_lanes, 4, 8, 16) && one_of(t.neighbor_waves, 1, 2, 4);
 	}
 
 	int set_tuning(const pcm_tuning* t) override
@@ -1196,7 +1193,7 @@ public:
 	template <class Run>
 	int autotune_with(int steps, Run&& run, pcm_tuning* chosen)
 	{
-		static const pcm_tuning candidates[3] = { { 4, 4, 0 }, { 8, 8, 2 }, { 16, 16, 4 } };
+		static const pcm_tuning candidates[3] = { { 4, 4, 1 }, { 8, 8, 2 }, { 16, 16, 4 } };
 		const uint64_t mask = (1ull << PCM_K_VELOCITY_FUSED) | (1ull << PCM_K_SPRING_COLLECT)
 							  | (1ull << PCM_K_SPRING_RESOLVE) | (1ull << PCM_K_INTERACT_TENTATIVE)
 							  | (1ull << PCM_K_NEIGHBORS);

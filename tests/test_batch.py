@@ -160,7 +160,7 @@ def test_batch_member_neighbor_overflow():
     batch.close()
 
 
-SHAPES = [(4, 4, 1), (8, 8, 2), (16, 16, 4), (4, 16, 2), (16, 8, 1), (8, 8, 0), (4, 4, 0)]
+SHAPES = [(4, 4, 1), (8, 8, 2), (16, 16, 4), (4, 16, 2), (16, 8, 1)]
 
 
 @pytest.mark.parametrize("spec", [(3, 400, 31), (2, 300, 33)])
