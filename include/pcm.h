@@ -100,7 +100,8 @@ extern "C" {
 	X(FLAGS, PCM_T_U8, PCM_W_ONE)                                 /* flags (cell_state_flag)         */            \
 	X(INTERNALIZED_SUBSTRATES, PCM_T_REAL, PCM_W_SUBS)            /* agent_data.internalized_substrates */         \
 	X(FRACTION_TRANSFERRED_WHEN_INGESTED, PCM_T_REAL, PCM_W_SUBS) /* agent_data.fraction_transferred_... */        \
-	X(ORIGIN, PCM_T_I32, PCM_W_ONE) /* index of the agent in the last host upload; moves with the agent */
+	X(ORIGIN, PCM_T_I32, PCM_W_ONE) /* index of the agent in the last host upload; moves with the agent */          \
+	X(CONTACT_FUNCTION, PCM_T_U8, PCM_W_ONE) /* functions.contact_function as a PCM_CONTACT_* code */
 
 #define PCM_FIELD_ENUM_(name, type, width) PCM_F_##name,
 enum pcm_field
@@ -113,6 +114,12 @@ enum pcm_field
 #define PCM_FLAG_NONE 0
 #define PCM_FLAG_TO_REMOVE 1
 #define PCM_FLAG_TO_DIVIDE 2
+
+/* PCM_F_CONTACT_FUNCTION codes.  The reference keeps a std::function per cell
+ * (src/cell_functions.h:26) that evaluate_interactions (src/simulator.cpp:65-82) calls for
+ * every attached cell; the device knows the one model the reference ships. */
+#define PCM_CONTACT_NONE 0             /* contact_function == nullptr */
+#define PCM_CONTACT_STANDARD_ELASTIC 1 /* standard_elastic_contact_function (standard_models.cpp:643-654) */
 
 /* link lists */
 #define PCM_LINKS_SPRINGS 0  /* states.springs        */
@@ -159,7 +166,9 @@ int pcm_field_info(int32_t field, int32_t* type, int32_t* width_kind, const char
 int pcm_version(void);
 
 /* ---------------------------------------------------------------- data movement */
-/* cell_data::agents_count.  Setting the count resets ORIGIN to the identity. */
+/* cell_data::agents_count.  Setting the count resets ORIGIN to the identity, empties the
+ * spring, attached-cell and neighbor lists and clears CONTACT_FUNCTION of agents
+ * [0, count); upload fields and lists after it. */
 int pcm_set_count(pcm_sim* sim, int32_t count);
 int pcm_get_count(pcm_sim* sim, int32_t* count);
 int pcm_set_field(pcm_sim* sim, int32_t field, const void* host);
@@ -194,6 +203,10 @@ int pcm_get_step_index(pcm_sim* sim, uint64_t* step);
 int pcm_update_mechanics_mesh(pcm_sim* sim);
 /* position_solver::update_cell_neighbors (src/solver/host/position_solver.cpp:262-286) */
 int pcm_update_cell_neighbors(pcm_sim* sim);
+/* evaluate_interactions (src/simulator.cpp:65-82): every agent whose contact function is
+ * PCM_CONTACT_STANDARD_ELASTIC adds sqrt(k_i k_j a_i[t_j] a_j[t_i]) (x_j - x_i) to its
+ * velocity for each attached cell j, in list order (standard_models.cpp:643-654). */
+int pcm_evaluate_interactions(pcm_sim* sim, const pcm_step_params* params);
 /* position_solver::update_cell_forces (src/solver/host/position_solver.cpp:233-260) */
 int pcm_update_cell_forces(pcm_sim* sim, const pcm_step_params* params);
 /* position_solver::update_motility (src/solver/host/position_solver.cpp:288-314) */
@@ -209,9 +222,10 @@ int pcm_update_cell_cell_interactions(pcm_sim* sim, const pcm_step_params* param
 /* containers_solver::update_cell_container_for_mechanics (containers_solver.cpp:140-147) */
 int pcm_update_cell_container_for_mechanics(pcm_sim* sim, const pcm_step_params* params);
 
-/* The mechanics step of simulator::simulate_mechanics (src/simulator.cpp:173-192), run
- * `steps` times without host round trips: forces, motility, basement membrane, springs,
- * positions, cell-cell interactions, container compaction, mechanics mesh, neighbors. */
+/* The mechanics step of simulator::simulate_mechanics (src/simulator.cpp:171-192), run
+ * `steps` times without host round trips: contact functions, forces, motility, basement
+ * membrane, springs, positions, cell-cell interactions, container compaction, mechanics
+ * mesh, neighbors. */
 int pcm_step(pcm_sim* sim, const pcm_step_params* params, int32_t steps);
 /* Wait for queued device work; reports PCM_ERR_STATE if a step aborted. */
 int pcm_synchronize(pcm_sim* sim);
@@ -274,7 +288,7 @@ int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t 
 	X(MOTILITY) X(MEMBRANE) X(CONTRACT) X(POSITIONS) X(VELOCITY_FUSED) X(SPRING_FIRE) X(SPRING_ERASE)                   \
 	X(SPRING_COMPACT) X(SPRING_COLLECT) X(SPRING_RESOLVE) X(SPRING_APPEND) X(INTERACT_TENTATIVE) X(INTERACT_CHECK)       \
 	X(INTERACT_APPLY) X(INTERACT_SERIAL) X(COMPACT_FLAG) X(COMPACT_PLAN) X(COMPACT_APPLY) X(COMPACT_RESTORE)            \
-	X(COMPACT_DONE) X(FORCE_RECORDS)
+	X(COMPACT_DONE) X(FORCE_RECORDS) X(CONTACT)
 #define PCM_KERNEL_ENUM_(name) PCM_K_##name,
 enum pcm_kernel
 {

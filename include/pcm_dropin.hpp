@@ -20,6 +20,9 @@
 // argument-dependent lookup:
 //   auto& pcm_cell_data(Env& e);              // the cell_data SoA (cell_data.h:162-196)
 //   void  pcm_host_remove_at(Env& e, int i);  // the container's remove_at (swap with last)
+// and optionally a third, when cells carry contact functions (cell_functions.h:26):
+//   void  pcm_contact_functions(Env& e, std::vector<uint8_t>& codes); // PCM_CONTACT_* per agent
+// Without it every agent uploads PCM_CONTACT_NONE.
 #pragma once
 
 #include <cstdint>
@@ -214,6 +217,9 @@ pcm_step_params params_of(Env& e)
 }
 
 template <class Env>
+concept has_contact_functions = requires(Env& e, std::vector<uint8_t>& codes) { pcm_contact_functions(e, codes); };
+
+template <class Env>
 void upload(Env& e, residency& r)
 {
 	auto& d = pcm_cell_data(e);
@@ -242,6 +248,12 @@ void upload(Env& e, residency& r)
 	}
 	check(pcm_set_count(r.sim, n), "pcm_set_count");
 	for_each_field(d, [&](int id, auto& v) { check(pcm_set_field(r.sim, id, v.data()), "pcm_set_field"); });
+	if constexpr (has_contact_functions<Env>)
+	{
+		std::vector<uint8_t> codes(std::max(n, 1), PCM_CONTACT_NONE);
+		pcm_contact_functions(e, codes);
+		check(pcm_set_field(r.sim, PCM_F_CONTACT_FUNCTION, codes.data()), "pcm_set_field");
+	}
 	std::vector<int32_t> off, idx;
 	to_csr(d.states.springs, n, off, idx);
 	check(pcm_set_links(r.sim, PCM_LINKS_SPRINGS, off.data(), idx.data()), "pcm_set_links");
@@ -352,6 +364,17 @@ void run(Env& e, Call&& call)
 }
 
 // ---------------------------------------------------------------- reference classes
+// evaluate_interactions (simulator.cpp:65-82): the contact functions over attached cells,
+// which the reference runs right before update_cell_forces.  simulate_mechanics includes it.
+template <class Env>
+void evaluate_interactions(Env& e)
+{
+	run(e, [&](pcm_sim* s) {
+		const auto p = params_of(e);
+		check(pcm_evaluate_interactions(s, &p), "pcm_evaluate_interactions");
+	});
+}
+
 template <class Env>
 class position_solver
 {
@@ -457,7 +480,8 @@ public:
 
 	void initialize(Env& e) { containers.initialize(e); }
 
-	// `steps` whole mechanics steps (simulator.cpp:173-192) in one device submission
+	// `steps` whole mechanics steps (simulator.cpp:171-192, contact functions included) in
+	// one device submission
 	void simulate_mechanics(Env& e, int steps = 1)
 	{
 		run(e, [&](pcm_sim* s) {

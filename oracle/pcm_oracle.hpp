@@ -186,6 +186,8 @@ struct cell_data
 	std::vector<std::uint8_t> is_movable;
 	std::vector<cell_state_flag> flags;
 	std::vector<index_t> origin; // identity in the last upload (bookkeeping of the harness)
+	// cell_functions.h:26 contact_function, as a PCM_CONTACT_* code per agent
+	std::vector<std::uint8_t> contact_function;
 
 	index_t& agents_count = agent_data.agents_count;
 
@@ -249,6 +251,7 @@ void for_each_field(cell_data<R>& d, index_t dims, index_t defs, index_t subs, F
 	f(PCM_F_INTERNALIZED_SUBSTRATES, d.agent_data.internalized_substrates, subs);
 	f(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED, d.agent_data.fraction_transferred_when_ingested, subs);
 	f(PCM_F_ORIGIN, d.origin, 1);
+	f(PCM_F_CONTACT_FUNCTION, d.contact_function, 1);
 }
 
 // ------------------------------------------------------------------ random service
@@ -601,6 +604,34 @@ public:
 							  d.mechanics.cell_cell_adhesion_strength.data(),
 							  d.mechanics.relative_maximum_adhesion_distance.data(),
 							  d.mechanics.cell_adhesion_affinities.data(), d.cell_definition_indices.data());
+		}
+	}
+
+	// evaluate_interactions (src/simulator.cpp:65-82): for an agent with a contact function,
+	// the function runs once per attached cell, in list order.  The one model the reference
+	// ships, standard_elastic_contact_function (standard_models.cpp:643-654), pulls the agent
+	// towards the other with the spring constant of the automated springs.
+	void evaluate_interactions(env_t& e)
+	{
+		auto& d = e.data;
+		const index_t dims = e.mechanics_mesh.dims;
+		const index_t defs = e.cell_definitions_count;
+		const R* k = d.mechanics.attachment_elastic_constant.data();
+		const R* aff = d.mechanics.cell_adhesion_affinities.data();
+		const index_t* type = d.cell_definition_indices.data();
+		const R* x = d.agent_data.positions.data();
+		R* v = d.velocities.data();
+#pragma omp for
+		for (index_t i = 0; i < d.agents_count; i++)
+		{
+			if (d.contact_function[i] != PCM_CONTACT_STANDARD_ELASTIC)
+				continue;
+			for (const index_t j : d.states.attached_cells[i])
+			{
+				const R adhesion = std::sqrt(k[i] * k[j] * aff[i * defs + type[j]] * aff[j * defs + type[i]]);
+				for (index_t c = 0; c < dims; c++)
+					v[i * dims + c] += adhesion * (x[j * dims + c] - x[i * dims + c]);
+			}
 		}
 	}
 
@@ -1047,9 +1078,10 @@ public:
 			remove_single(e, i);
 	}
 
-	// simulator::simulate_mechanics, mechanics part (src/simulator.cpp:173-192)
+	// simulator::simulate_mechanics, mechanics part (src/simulator.cpp:171-192)
 	void step(env_t& e)
 	{
+		evaluate_interactions(e);
 		update_cell_forces(e);
 		update_motility(e);
 		update_basement_membrane_interactions(e);

@@ -4,6 +4,7 @@
 // C-ABI (include/pcm.h) name for name with an `oracle_` prefix so that a test can drive
 // the oracle and the device library with the same host arrays.  Only tests/, bench.py's
 // cpu_baseline leg and __graft_entry__.smoke() load this library.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <memory>
@@ -51,7 +52,8 @@ enum phase_id
 	PH_SPRINGS,
 	PH_POSITIONS,
 	PH_INTERACTIONS,
-	PH_CONTAINER
+	PH_CONTAINER,
+	PH_CONTACT
 };
 
 template <class R>
@@ -156,6 +158,9 @@ struct oracle_impl final : oracle_base
 				break;
 			case PH_CONTAINER:
 				s.update_cell_container_for_mechanics(env);
+				break;
+			case PH_CONTACT:
+				s.evaluate_interactions(env);
 				break;
 		}
 	}
@@ -268,8 +273,24 @@ int oracle_set_links(void* h, int32_t which, const int32_t* off, const int32_t* 
 	auto* o = cast(h);
 	if (which != PCM_LINKS_SPRINGS && which != PCM_LINKS_ATTACHED)
 		return fail(PCM_ERR_ARGUMENT, "bad link list id");
+	// The reference's removal (remove_attached, containers_solver.cpp:58-69) finds the removed
+	// agent in each partner's list and writes through the iterator without checking it: lists
+	// that are not symmetric (j appears in i's list as often as i in j's, never i in its own)
+	// are undefined behaviour there, so the oracle refuses them.
+	const index_t n = o->count();
+	for (index_t i = 0; i < n; i++)
+		for (int32_t s = off[i]; s < off[i + 1]; s++)
+		{
+			const index_t j = idx[s];
+			if (j < 0 || j >= n || j == i)
+				return fail(PCM_ERR_ARGUMENT, "link list entry out of range or naming its own agent");
+			const auto mine = std::count(idx + off[i], idx + off[i + 1], j);
+			const auto theirs = std::count(idx + off[j], idx + off[j + 1], i);
+			if (mine != theirs)
+				return fail(PCM_ERR_ARGUMENT, "link lists are not symmetric");
+		}
 	auto& l = o->lists(which);
-	for (index_t i = 0; i < o->count(); i++)
+	for (index_t i = 0; i < n; i++)
 		l[i].assign(idx + off[i], idx + off[i + 1]);
 	return PCM_OK;
 }
@@ -373,6 +394,7 @@ ORACLE_PHASE(update_spring_attachments, PH_SPRINGS)
 ORACLE_PHASE(update_positions, PH_POSITIONS)
 ORACLE_PHASE(update_cell_cell_interactions, PH_INTERACTIONS)
 ORACLE_PHASE(update_cell_container_for_mechanics, PH_CONTAINER)
+ORACLE_PHASE(evaluate_interactions, PH_CONTACT)
 
 int oracle_step(void* h, const pcm_step_params* p, int32_t steps)
 {

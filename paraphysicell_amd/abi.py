@@ -82,7 +82,10 @@ FIELDS = [
     ("internalized_substrates", T_REAL, W_SUBS),
     ("fraction_transferred_when_ingested", T_REAL, W_SUBS),
     ("origin", T_I32, W_ONE),
+    ("contact_function", T_U8, W_ONE),
 ]
+CONTACT_NONE = 0  # PCM_CONTACT_NONE
+CONTACT_STANDARD_ELASTIC = 1  # PCM_CONTACT_STANDARD_ELASTIC
 FIELD_ID = {name: i for i, (name, _, _) in enumerate(FIELDS)}
 
 

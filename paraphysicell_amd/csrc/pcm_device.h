@@ -69,6 +69,7 @@ struct SimCtx
 	R dt;
 	int wall;
 	int springs;
+	int contacts; // some uploaded agent has a contact function
 	Scalars* sc;
 
 	// agent fields (include/pcm.h order); fptr/fbytes give the generic view
@@ -92,6 +93,7 @@ struct SimCtx
 	uint8_t* flags;
 	R *internal, *frac_ingest;
 	int* origin;
+	uint8_t* contact; // PCM_CONTACT_* code per agent
 
 	// link lists
 	int *spr_cnt, *spr_idx, *atc_cnt, *atc_idx;
@@ -510,6 +512,37 @@ __device__ __forceinline__ void membrane_agent(const SimCtx<R>& c, int i, const 
 		const int upper = (D == 1) ? c.bmax[1] : c.bmax[k];
 		membrane_push<R>(p[k], (R)c.bmin[k], (R)1, r, s, v[k]);
 		membrane_push<R>(p[k], (R)upper, (R)-1, r, s, v[k]);
+	}
+}
+
+// standard_elastic_contact_function (standard_models.cpp:643-654) of agent i towards j:
+// the pull sqrt(k_i k_j a_i[t_j] a_j[t_i]) (x_j - x_i), the product taken left to right.
+template <int D, class R>
+__device__ __forceinline__ R contact_strength(const SimCtx<R>& c, R ki, const R* aff_row_i, int ti, int j, int tj)
+{
+	const R kij = ki * c.elastic[j] * aff_row_i[tj] * c.affinity[(size_t)j * c.T + ti];
+	return (R)sqrt((double)kij); // correctly rounded in R either way
+}
+
+// evaluate_interactions (simulator.cpp:65-82) for one agent, one thread: the attached cells
+// in list order
+template <int D, class R>
+__device__ __forceinline__ void contact_agent(const SimCtx<R>& c, int i, R* v)
+{
+	const int na = c.atc_cnt[i];
+	const int* row = c.atc_idx + (size_t)i * c.LA;
+	const R ki = c.elastic[i];
+	const int ti = c.type[i];
+	const R* aff_row = c.affinity + (size_t)i * c.T;
+	R pi[D];
+	for (int k = 0; k < D; k++)
+		pi[k] = c.pos[(size_t)i * D + k];
+	for (int s = 0; s < na; s++)
+	{
+		const int j = row[s];
+		const R a = contact_strength<D, R>(c, ki, aff_row, ti, j, c.type[j]);
+		for (int k = 0; k < D; k++)
+			v[k] += a * (c.pos[(size_t)j * D + k] - pi[k]);
 	}
 }
 

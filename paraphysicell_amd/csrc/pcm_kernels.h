@@ -442,6 +442,23 @@ __global__ void k_forces(const SimCtx<R>* __restrict__ ctxs)
 			c.vel[(size_t)i * D + k] = v[k];
 }
 
+// evaluate_interactions (simulator.cpp:65-82): agents whose contact function is the standard
+// elastic one; there is no movable gating, as in the reference
+template <class R, int D>
+__global__ void k_contact(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || c.contact[i] != PCM_CONTACT_STANDARD_ELASTIC)
+		return;
+	R v[D];
+	for (int k = 0; k < D; k++)
+		v[k] = c.vel[(size_t)i * D + k];
+	contact_agent<D, R>(c, i, v);
+	for (int k = 0; k < D; k++)
+		c.vel[(size_t)i * D + k] = v[k];
+}
+
 template <class R, int D>
 __global__ void k_motility(const SimCtx<R>* __restrict__ ctxs)
 {
@@ -588,6 +605,32 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	R p[D];
 	for (int k = 0; k < D; k++)
 		p[k] = me.p[k];
+	if (c.contacts && c.contact[i] == PCM_CONTACT_STANDARD_ELASTIC)
+	{
+		// evaluate_interactions (simulator.cpp:65-82) runs before the forces: one attached
+		// cell per lane, the terms summed in list order
+		const int na = c.atc_cnt[i];
+		const int* row = c.atc_idx + (size_t)i * c.LA;
+		const R ki = c.elastic[i];
+		const R* aff_i = c.affinity + (size_t)i * c.T;
+		for (int base = 0; base < na; base += G)
+		{
+			R f[D];
+			for (int k = 0; k < D; k++)
+				f[k] = 0;
+			if (base + lane < na)
+			{
+				const force_rec<R> o = read_force_record<R>(c, row[base + lane]);
+				const R a = contact_strength<D, R>(c, ki, aff_i, me.type, row[base + lane], o.type);
+				for (int k = 0; k < D; k++)
+					f[k] = a * (o.p[k] - p[k]);
+			}
+			const int m = min(G, na - base);
+			for (int src = 0; src < m; src++)
+				for (int k = 0; k < D; k++)
+					v[k] += __shfl(f[k], src, G);
+		}
+	}
 	const R* aff_row = c.affinity + (size_t)i * c.T;
 	const R coefficient = pressure_coefficient<R>();
 	R pressure = 0;

@@ -115,7 +115,8 @@ enum phase_id
 	PH_SPRINGS,
 	PH_POSITIONS,
 	PH_INTERACTIONS,
-	PH_CONTAINER
+	PH_CONTAINER,
+	PH_CONTACT
 };
 
 template <class R>
@@ -283,6 +284,12 @@ public:
 			PCM_HIP(hipMemset(h_.spr_cnt, 0, sizeof(int) * n));
 			PCM_HIP(hipMemset(h_.atc_cnt, 0, sizeof(int) * n));
 			PCM_HIP(hipMemset(h_.nbr_cnt, 0, sizeof(int) * n));
+			PCM_HIP(hipMemset(h_.contact, 0, n));
+		}
+		if (h_.contacts)
+		{
+			h_.contacts = 0;
+			PCM_TRY(push_ctx());
 		}
 		return PCM_OK;
 	}
@@ -310,6 +317,22 @@ public:
 				return fail(PCM_ERR_UNSUPPORTED, "maximum_number_of_attachments above 32 is not supported");
 			if (mx > h_.L)
 				PCM_TRY(grow_springs(mx));
+		}
+		if (f == PCM_F_CONTACT_FUNCTION)
+		{
+			const uint8_t* code = (const uint8_t*)src;
+			int any = 0;
+			for (int i = 0; i < n; i++)
+			{
+				if (code[i] > PCM_CONTACT_STANDARD_ELASTIC)
+					return fail(PCM_ERR_UNSUPPORTED, "unknown contact function code");
+				any |= code[i];
+			}
+			if (any != h_.contacts)
+			{
+				h_.contacts = any;
+				PCM_TRY(push_ctx());
+			}
 		}
 		PCM_HIP(hipMemcpy(h_.fptr[f], src, (size_t)n * h_.fbytes[f], hipMemcpyHostToDevice));
 		if (f == PCM_F_POSITIONS || f == PCM_F_RADIUS || f == PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE)
@@ -725,6 +748,7 @@ private:
 		h.internal = (R*)F(PCM_F_INTERNALIZED_SUBSTRATES);
 		h.frac_ingest = (R*)F(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED);
 		h.origin = (int*)F(PCM_F_ORIGIN);
+		h.contact = (uint8_t*)F(PCM_F_CONTACT_FUNCTION);
 	}
 
 	int alloc_neighbor_storage()
@@ -1075,6 +1099,10 @@ private:
 					PCM_TRY(recover());
 				return PCM_OK;
 			}
+			case PH_CONTACT:
+				if (!h_.contacts)
+					return PCM_OK;
+				return per_agent(PCM_K_CONTACT, k_contact<R, D>);
 			case PH_FORCES:
 				PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 				return per_agent(PCM_K_FORCES, k_forces<R, D>);
@@ -1920,6 +1948,7 @@ int pcm_update_cell_neighbors(pcm_sim* sim)
 		return sim->phase(ID, params);                                                                                 \
 	}
 
+PCM_PHASE_ENTRY(pcm_evaluate_interactions, PH_CONTACT)
 PCM_PHASE_ENTRY(pcm_update_cell_forces, PH_FORCES)
 PCM_PHASE_ENTRY(pcm_update_motility, PH_MOTILITY)
 PCM_PHASE_ENTRY(pcm_update_basement_membrane_interactions, PH_MEMBRANE)

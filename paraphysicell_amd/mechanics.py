@@ -51,6 +51,7 @@ def _declare(lib, prefix: str):
         "get_step_index": [P, ctypes.POINTER(ctypes.c_uint64)],
         "update_mechanics_mesh": None,
         "update_cell_neighbors": None,
+        "evaluate_interactions": [P, ctypes.POINTER(abi.StepParams)],
         "update_cell_forces": [P, ctypes.POINTER(abi.StepParams)],
         "update_motility": [P, ctypes.POINTER(abi.StepParams)],
         "update_basement_membrane_interactions": [P, ctypes.POINTER(abi.StepParams)],
@@ -218,6 +219,10 @@ class _LibSim:
     def update_cell_neighbors(self, params=None):
         self._phase("update_cell_neighbors", params)
 
+    def evaluate_interactions(self, params=None):
+        """Contact functions over the attached cells (src/simulator.cpp:65-82)."""
+        self._phase("evaluate_interactions", params)
+
     def update_cell_forces(self, params=None):
         self._phase("update_cell_forces", params)
 
@@ -249,8 +254,8 @@ class _LibSim:
         self._call("step", self._h, ctypes.byref(p), int(steps))
 
     def run_phases(self, params=None):
-        """The mechanics step as the sequence of per-phase calls (src/simulator.cpp:173-192)."""
-        for name in ("update_cell_forces", "update_motility", "update_basement_membrane_interactions",
+        """The mechanics step as the sequence of per-phase calls (src/simulator.cpp:171-192)."""
+        for name in ("evaluate_interactions", "update_cell_forces", "update_motility", "update_basement_membrane_interactions",
                      "update_spring_attachments", "update_positions", "update_cell_cell_interactions",
                      "update_cell_container_for_mechanics", "update_mechanics_mesh", "update_cell_neighbors"):
             getattr(self, name)(params)
@@ -445,6 +450,8 @@ class Solver:
                                       "update_positions"])
         self.containers = _Facade(sim, ["update_mechanics_mesh", "update_cell_container_for_mechanics"])
         self.interactions = _Facade(sim, ["update_cell_cell_interactions"])
+        # the free function evaluate_interactions(environment&) of src/simulator.cpp:65-82
+        self.evaluate_interactions = sim.evaluate_interactions
 
     def initialize(self):
         self.sim.initialize_mesh()

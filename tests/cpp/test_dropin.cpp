@@ -23,6 +23,13 @@ void pcm_host_remove_at(environment<R>& e, int i)
 {
 	e.remove_at(i);
 }
+
+template <class R>
+void pcm_contact_functions(environment<R>& e, std::vector<uint8_t>& codes)
+{
+	for (index_t i = 0; i < e.data.agents_count; i++)
+		codes[i] = e.data.contact_function[i];
+}
 } // namespace pcm_oracle
 
 using namespace pcm_oracle;
@@ -169,6 +176,24 @@ void fill_population(environment<R>& e, int n, unsigned seed)
 		d.is_movable[i] = u(g) < 0.9;
 		d.flags[i] = cell_state_flag::none;
 	}
+	// symmetric attached cells (cell::attach_cells, cell.cpp:219-223) on some agents, and
+	// standard elastic contact functions on half of them
+	for (int i = 0; i < n; i++)
+	{
+		d.contact_function[i] = u(g) < 0.5 ? PCM_CONTACT_STANDARD_ELASTIC : PCM_CONTACT_NONE;
+		if (u(g) < 0.3)
+		{
+			const int links = 1 + (int)(3 * u(g));
+			for (int k = 0; k < links; k++)
+			{
+				const int j = (int)(u(g) * n) % n;
+				if (j == i)
+					continue;
+				d.states.attached_cells[i].push_back(j);
+				d.states.attached_cells[j].push_back(i);
+			}
+		}
+	}
 	// a few agents leave the box during the first step
 	for (int i = 0; i < n / 100 + 1; i++)
 	{
@@ -230,6 +255,7 @@ void population_steps(int dims, int n, int steps, bool fused)
 		{
 #pragma omp parallel
 			{
+				pcm_dropin::evaluate_interactions(dev);
 				s.position.update_cell_forces(dev);
 				s.position.update_motility(dev);
 				s.position.update_basement_membrane_interactions(dev);

@@ -64,6 +64,7 @@ def rich_population(dims, n, seed, real_bytes, box=150, fill=0.55, spring_cap=3,
     """A small dense population exercising every branch of the mechanics step.
 
     Two cell types; 10% non-movable; 5% dead; pre-existing springs (some duplicated);
+    attached cells with contact functions on half of the agents;
     attachment/detachment, attack, phagocytosis and fusion rates high enough that
     capacity limits and interaction conflicts occur; a few agents drift out of the box.
     """
@@ -122,6 +123,15 @@ def rich_population(dims, n, seed, real_bytes, box=150, fill=0.55, spring_cap=3,
     for a, b in zip(order[1::3], order[2::3]):
         att[a].append(int(b))
         att[b].append(int(a))
+    # a few hubs attached to more cells than a lane group holds (lists stay symmetric, as
+    # cell::attach_cells makes them)
+    for h in order[: max(1, n // 100)]:
+        for x in rng.choice(n, size=min(n, 13), replace=False):
+            if x != h:
+                att[h].append(int(x))
+                att[int(x)].append(int(h))
     st.set_links(abi.LINKS_ATTACHED, att)
+    # half of the agents carry the standard elastic contact function
+    st["contact_function"] = (rng.uniform(size=n) < 0.5).astype(np.uint8)
     params = abi.Params(dt=0.1, virtual_wall=True, springs=True)
     return mesh, st, params

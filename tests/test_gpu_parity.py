@@ -116,7 +116,7 @@ def test_container_add_and_remove():
 
 
 # ------------------------------------------------------------------ per-phase parity
-PHASES = ["update_cell_forces", "update_motility", "update_basement_membrane_interactions",
+PHASES = ["evaluate_interactions", "update_cell_forces", "update_motility", "update_basement_membrane_interactions",
           "update_spring_attachments", "update_positions", "update_cell_cell_interactions",
           "update_cell_container_for_mechanics", "update_mechanics_mesh", "update_cell_neighbors"]
 

@@ -333,6 +333,79 @@ def test_spring_contraction_closed_form():
     assert _close(sim.get_field("velocities").reshape(n, dims), exp, 1e-12)
 
 
+# ------------------------------------------------------------------ contact functions (F4)
+def _contact_case(dims, real_bytes=8):
+    """Agents on a lattice 40 um apart (beyond every interaction distance), two types with
+    asymmetric affinities, symmetric attached lists (cell::attach_cells, cell.cpp:219-223)
+    of about ten cells, half of the agents with the standard elastic contact function, a
+    fifth not movable."""
+    side = {2: 8, 3: 4}[dims]
+    grid = np.stack(np.meshgrid(*[np.arange(side)] * dims, indexing="ij"), -1).reshape(-1, dims)
+    n = len(grid)
+    mesh, st = _state(dims, n, real_bytes, defs=2, box=200)
+    rng = np.random.default_rng(31 + dims)
+    pos = (grid - (side - 1) / 2) * 40.0
+    st["positions"] = pos.ravel()
+    v0 = rng.normal(0, 1, (n, dims))
+    st["velocities"] = v0.ravel()
+    st["attachment_elastic_constant"] = rng.uniform(0.01, 0.2, n)
+    st["cell_adhesion_affinities"] = rng.uniform(0.2, 2.0, 2 * n)
+    st["cell_definition_index"] = rng.integers(0, 2, n)
+    st["is_movable"] = (rng.random(n) < 0.8).astype(np.uint8)
+    st["is_motile"] = 0
+    code = (rng.random(n) < 0.5).astype(np.uint8)
+    st["contact_function"] = code
+    att = [[] for _ in range(n)]
+    for _ in range(5 * n):
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        att[a].append(b)
+        att[b].append(a)
+    st.set_links(abi.LINKS_ATTACHED, att)
+    k = st["attachment_elastic_constant"]
+    aff = st.view("cell_adhesion_affinities")
+    t = st["cell_definition_index"]
+    pull = np.zeros((n, dims))
+    for i in range(n):
+        if code[i] == abi.CONTACT_STANDARD_ELASTIC:
+            for j in att[i]:
+                pull[i] += np.sqrt(k[i] * k[j] * aff[i, t[j]] * aff[j, t[i]]) * (pos[j] - pos[i])
+    return mesh, st, v0, pull
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_contact_function_closed_form(dims):
+    """evaluate_interactions adds sum_j sqrt(k_i k_j aff[i, t_j] aff[j, t_i]) (x_j - x_i) over
+    the attached cells of every agent with the standard elastic contact function, movable
+    or not, and leaves the others (simulator.cpp:65-82, standard_models.cpp:643-654)."""
+    mesh, st, v0, pull = _contact_case(dims)
+    n = st.count
+    sim = _sim(mesh, st)
+    sim.evaluate_interactions(abi.Params())
+    assert _close(sim.get_field("velocities").reshape(n, dims), v0 + pull, 1e-12)
+    assert np.any(pull[st["is_movable"] == 0] != 0)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_contact_function_runs_first_in_the_step(dims):
+    """In a whole step (simulator.cpp:171-192) the pull reaches the AB2 update: movable
+    agents end with previous_velocities = v0 + pull and x += dt (1.5 (v0 + pull) - 0.5 vp);
+    non-movable agents keep v0 + pull in their velocity (no other force acts here)."""
+    mesh, st, v0, pull = _contact_case(dims)
+    n = st.count
+    vp = np.random.default_rng(5).normal(0, 1, (n, dims))
+    st["previous_velocities"] = vp.ravel()
+    sim = _sim(mesh, st)
+    sim.initialize_mesh()
+    sim.step(abi.Params(dt=0.1, virtual_wall=False, springs=False), 1)
+    mov = st["is_movable"] == 1
+    v = v0 + pull
+    x0 = st.view("positions")
+    assert _close(sim.get_field("previous_velocities").reshape(n, dims)[mov], v[mov], 1e-12)
+    assert _close(sim.get_field("velocities").reshape(n, dims)[~mov], v[~mov], 1e-12)
+    exp = np.where(mov[:, None], x0 + 0.1 * (1.5 * v - 0.5 * vp), x0)
+    assert _close(sim.get_field("positions").reshape(n, dims), exp, 1e-12)
+
+
 def test_spring_detach_saturating():
     """detachment_rate * dt >= 1 fires every draw (`<=`, position_solver.cpp:386): every
     spring touching such an agent goes, springs between two zero-rate agents stay."""
