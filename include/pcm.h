@@ -101,7 +101,11 @@ extern "C" {
 	X(INTERNALIZED_SUBSTRATES, PCM_T_REAL, PCM_W_SUBS)            /* agent_data.internalized_substrates */         \
 	X(FRACTION_TRANSFERRED_WHEN_INGESTED, PCM_T_REAL, PCM_W_SUBS) /* agent_data.fraction_transferred_... */        \
 	X(ORIGIN, PCM_T_I32, PCM_W_ONE) /* index of the agent in the last host upload; moves with the agent */          \
-	X(CONTACT_FUNCTION, PCM_T_U8, PCM_W_ONE) /* functions.contact_function as a PCM_CONTACT_* code */
+	X(CONTACT_FUNCTION, PCM_T_U8, PCM_W_ONE) /* functions.contact_function as a PCM_CONTACT_* code */              \
+	X(CHEMOTAXIS_INDEX, PCM_T_I32, PCM_W_ONE)                     /* motilities.chemotaxis_index     */            \
+	X(CHEMOTAXIS_DIRECTION, PCM_T_I32, PCM_W_ONE)                 /* motilities.chemotaxis_direction */            \
+	X(CHEMOTACTIC_SENSITIVITIES, PCM_T_REAL, PCM_W_SUBS)          /* motilities.chemotactic_sensitivities */       \
+	X(MIGRATION_BIAS_DIRECTION_FUNCTION, PCM_T_U8, PCM_W_ONE) /* motilities.update_migration_bias_direction, PCM_BIAS_* */
 
 #define PCM_FIELD_ENUM_(name, type, width) PCM_F_##name,
 enum pcm_field
@@ -120,6 +124,15 @@ enum pcm_field
  * every attached cell; the device knows the one model the reference ships. */
 #define PCM_CONTACT_NONE 0             /* contact_function == nullptr */
 #define PCM_CONTACT_STANDARD_ELASTIC 1 /* standard_elastic_contact_function (standard_models.cpp:643-654) */
+
+/* PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION codes: the direction update that
+ * update_motility_single (position_solver.cpp:132-136) calls after a new random walk, as
+ * builder.cpp:1184-1233 selects it.  The chemotaxis models read the substrate gradients of
+ * pcm_set_gradients. */
+#define PCM_BIAS_NONE 0                              /* no update function                     */
+#define PCM_BIAS_CHEMOTAXIS 1                        /* chemotaxis_function (standard_models.cpp:542-555) */
+#define PCM_BIAS_ADVANCED_CHEMOTAXIS 2               /* advanced_chemotaxis_function<false> (557-583) */
+#define PCM_BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED 3    /* advanced_chemotaxis_function<true>  (557-583) */
 
 /* link lists */
 #define PCM_LINKS_SPRINGS 0  /* states.springs        */
@@ -193,6 +206,13 @@ int pcm_get_deaths_count(pcm_sim* sim, int64_t* deaths);
  * only SoA members follow.  With positions == NULL only the length is returned; otherwise
  * the log is copied (capacity >= length) and restarts. */
 int pcm_removal_log(pcm_sim* sim, int32_t* positions, int64_t capacity, int64_t* length);
+/* The substrate gradients that cell::nearest_gradient reads (paraBioFVM's gradient solver,
+ * run at simulator.cpp:163), for the chemotaxis direction updates.  `diffusion_voxel` is
+ * the microenvironment's voxel shape; the diffusion grid spans the mesh bounding box with
+ * ceil(extent / voxel) voxels per used dimension, x fastest.  `gradients` holds
+ * voxels * substrates * dims reals, index ((voxel * substrates) + substrate) * dims + d.
+ * The device keeps its own copy until the next call. */
+int pcm_set_gradients(pcm_sim* sim, const int32_t diffusion_voxel[3], const void* gradients);
 /* mechanics-step counter that keys the random stream (incremented by
  * pcm_update_cell_forces and by every pcm_step iteration) */
 int pcm_set_step_index(pcm_sim* sim, uint64_t step);

@@ -20,9 +20,14 @@
 // argument-dependent lookup:
 //   auto& pcm_cell_data(Env& e);              // the cell_data SoA (cell_data.h:162-196)
 //   void  pcm_host_remove_at(Env& e, int i);  // the container's remove_at (swap with last)
-// and optionally a third, when cells carry contact functions (cell_functions.h:26):
-//   void  pcm_contact_functions(Env& e, std::vector<uint8_t>& codes); // PCM_CONTACT_* per agent
-// Without it every agent uploads PCM_CONTACT_NONE.
+// and optionally, when cells carry contact functions (cell_functions.h:26) or chemotaxis
+// direction updates (cell_data.h:100):
+//   void  pcm_contact_functions(Env& e, std::vector<uint8_t>& codes);        // PCM_CONTACT_* per agent
+//   void  pcm_bias_direction_functions(Env& e, std::vector<uint8_t>& codes); // PCM_BIAS_* per agent
+//   void  pcm_gradients(Env& e, int32_t diffusion_voxel[3], std::vector<real_t>& gradients);
+// Without them every agent uploads PCM_CONTACT_NONE / PCM_BIAS_NONE.  pcm_gradients fills
+// the layout of pcm_set_gradients; pcm_dropin::gradients_modified(e) after each gradient
+// solve (simulator.cpp:163) makes the next device phase upload them.
 #pragma once
 
 #include <cstdint>
@@ -133,6 +138,9 @@ void for_each_field(CellData& d, F&& f)
 	f(PCM_F_FLAGS, d.flags);
 	f(PCM_F_INTERNALIZED_SUBSTRATES, d.agent_data.internalized_substrates);
 	f(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED, d.agent_data.fraction_transferred_when_ingested);
+	f(PCM_F_CHEMOTAXIS_INDEX, d.motilities.chemotaxis_index);
+	f(PCM_F_CHEMOTAXIS_DIRECTION, d.motilities.chemotaxis_direction);
+	f(PCM_F_CHEMOTACTIC_SENSITIVITIES, d.motilities.chemotactic_sensitivities);
 }
 
 // vector<vector<index_t>> <-> CSR
@@ -166,6 +174,7 @@ struct residency
 	int capacity = 0;
 	bool host_newer = true;    // the device copy must be refreshed before the next phase
 	bool device_newer = false; // the host copy must be refreshed before host code reads it
+	bool gradients_newer = true; // the substrate gradients must be uploaded before the next phase
 	int64_t deaths_seen = 0;
 	uint64_t seed = 0;
 
@@ -218,6 +227,13 @@ pcm_step_params params_of(Env& e)
 
 template <class Env>
 concept has_contact_functions = requires(Env& e, std::vector<uint8_t>& codes) { pcm_contact_functions(e, codes); };
+template <class Env>
+concept has_bias_direction_functions =
+	requires(Env& e, std::vector<uint8_t>& codes) { pcm_bias_direction_functions(e, codes); };
+template <class Env>
+concept has_gradients = requires(Env& e, int32_t* voxel) {
+	pcm_gradients(e, voxel, pcm_cell_data(e).agent_data.positions);
+};
 
 template <class Env>
 void upload(Env& e, residency& r)
@@ -245,6 +261,7 @@ void upload(Env& e, residency& r)
 		check(pcm_create(&c, &r.sim), "pcm_create");
 		r.capacity = c.capacity;
 		r.deaths_seen = 0;
+		r.gradients_newer = true; // a new handle holds no gradients
 	}
 	check(pcm_set_count(r.sim, n), "pcm_set_count");
 	for_each_field(d, [&](int id, auto& v) { check(pcm_set_field(r.sim, id, v.data()), "pcm_set_field"); });
@@ -253,6 +270,12 @@ void upload(Env& e, residency& r)
 		std::vector<uint8_t> codes(std::max(n, 1), PCM_CONTACT_NONE);
 		pcm_contact_functions(e, codes);
 		check(pcm_set_field(r.sim, PCM_F_CONTACT_FUNCTION, codes.data()), "pcm_set_field");
+	}
+	if constexpr (has_bias_direction_functions<Env>)
+	{
+		std::vector<uint8_t> codes(std::max(n, 1), PCM_BIAS_NONE);
+		pcm_bias_direction_functions(e, codes);
+		check(pcm_set_field(r.sim, PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION, codes.data()), "pcm_set_field");
 	}
 	std::vector<int32_t> off, idx;
 	to_csr(d.states.springs, n, off, idx);
@@ -274,7 +297,24 @@ residency& on_device(Env& e)
 	auto& r = state_of(e);
 	if (r.host_newer || !r.sim)
 		upload(e, r);
+	if constexpr (has_gradients<Env>)
+		if (r.gradients_newer)
+		{
+			int32_t voxel[3] = { 0, 0, 0 };
+			std::remove_reference_t<decltype(pcm_cell_data(e).agent_data.positions)> g;
+			pcm_gradients(e, voxel, g);
+			if (!g.empty()) // an environment without a gradient solve supplies none
+				check(pcm_set_gradients(r.sim, voxel, g.data()), "pcm_set_gradients");
+			r.gradients_newer = false;
+		}
 	return r;
+}
+
+// The gradient solver ran (simulator.cpp:163): upload the gradients before the next phase.
+template <class Env>
+void gradients_modified(Env& e)
+{
+	on_one_thread([&] { state_of(e).gradients_newer = true; });
 }
 
 // Bring the host SoA up to date: replay the device's removals on the host container so

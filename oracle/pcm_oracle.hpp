@@ -147,6 +147,10 @@ struct motility_data_t
 	std::vector<std::uint8_t> is_motile;
 	std::vector<R> persistence_time, migration_speed, migration_bias_direction, migration_bias, motility_vector;
 	std::vector<std::uint8_t> restrict_to_2d;
+	std::vector<index_t> chemotaxis_index, chemotaxis_direction; // cell_data.h:96-97
+	std::vector<R> chemotactic_sensitivities;                     // cell_data.h:98
+	// cell_data.h:100 update_migration_bias_direction, as a PCM_BIAS_* code per agent
+	std::vector<std::uint8_t> bias_direction_function;
 };
 
 struct death_data_t
@@ -252,6 +256,10 @@ void for_each_field(cell_data<R>& d, index_t dims, index_t defs, index_t subs, F
 	f(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED, d.agent_data.fraction_transferred_when_ingested, subs);
 	f(PCM_F_ORIGIN, d.origin, 1);
 	f(PCM_F_CONTACT_FUNCTION, d.contact_function, 1);
+	f(PCM_F_CHEMOTAXIS_INDEX, d.motilities.chemotaxis_index, 1);
+	f(PCM_F_CHEMOTAXIS_DIRECTION, d.motilities.chemotaxis_direction, 1);
+	f(PCM_F_CHEMOTACTIC_SENSITIVITIES, d.motilities.chemotactic_sensitivities, subs);
+	f(PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION, d.motilities.bias_direction_function, 1);
 }
 
 // ------------------------------------------------------------------ random service
@@ -323,6 +331,22 @@ struct environment
 	{
 		cartesian_mesh mesh;
 		index_t substrates_count = 1;
+		// the gradient solver's output (simulator.cpp:163): per diffusion voxel, substrate and
+		// dimension, ((voxel * substrates) + substrate) * dims + d
+		std::vector<R> gradients;
+
+		// cell::nearest_gradient(s) (cell.h, paraBioFVM's agent): the gradient of substrate s at
+		// the diffusion voxel that contains the position
+		const R* nearest_gradient(const R* position, index_t s) const
+		{
+			R p3[3] = { 0, 0, 0 };
+			for (index_t k = 0; k < mesh.dims; k++)
+				p3[k] = position[k];
+			index_t c[3];
+			mesh.voxel_position(p3, c);
+			const std::size_t v = (std::size_t)c[0] + mesh.grid_shape[0] * ((std::size_t)c[1] + mesh.grid_shape[1] * c[2]);
+			return gradients.data() + (v * substrates_count + s) * mesh.dims;
+		}
 	} m;
 
 	bool virtual_wall_at_domain_edges = false;
@@ -652,6 +676,55 @@ public:
 	// The reference hands the restrict_to_2d *array pointer* to random_walk(bool, ...)
 	// (position_solver.cpp:131), so in 3D the walk always takes the planar branch
 	// (solver_helper.h:252-256); kept as the reference behaves.
+	// normalize (solver_helper.h:30-45)
+	template <int D>
+	static void normalize(R* v)
+	{
+		R length = 0;
+		for (int d = 0; d < D; d++)
+			length += v[d] * v[d];
+		length = std::sqrt(length);
+		for (int d = 0; d < D; d++)
+			v[d] = length > zero_threshold<R> ? v[d] / length : 0;
+	}
+
+	// The direction updates the reference ships: chemotaxis_function and
+	// advanced_chemotaxis_function<normalize_each_gradient> (standard_models.cpp:542-583).
+	// They draw no random numbers.
+	template <int D>
+	void update_migration_bias_direction(env_t& e, index_t i)
+	{
+		auto& d = e.data;
+		auto& mot = d.motilities;
+		const R* x = d.agent_data.positions.data() + (std::size_t)i * D;
+		R* dir = mot.migration_bias_direction.data() + (std::size_t)i * D;
+		const std::uint8_t code = mot.bias_direction_function[i];
+		if (code == PCM_BIAS_CHEMOTAXIS)
+		{
+			const R* g = e.m.nearest_gradient(x, mot.chemotaxis_index[i]);
+			for (int k = 0; k < D; k++)
+				dir[k] = g[k] * (R)mot.chemotaxis_direction[i];
+		}
+		else
+		{
+			const index_t subs = e.m.substrates_count;
+			for (int k = 0; k < D; k++)
+				dir[k] = 0;
+			for (index_t s = 0; s < subs; s++)
+			{
+				R g[D];
+				const R* src = e.m.nearest_gradient(x, s);
+				for (int k = 0; k < D; k++)
+					g[k] = src[k];
+				if (code == PCM_BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED)
+					normalize<D>(g);
+				for (int k = 0; k < D; k++)
+					dir[k] += g[k] * mot.chemotactic_sensitivities[(std::size_t)i * subs + s];
+			}
+		}
+		normalize<D>(dir);
+	}
+
 	template <int D>
 	void motility_internal(env_t& e)
 	{
@@ -681,6 +754,9 @@ public:
 					if constexpr (D == 3)
 						walk[2] = 0;
 				}
+				// update_migration_bias_direction_f[i](cell) (position_solver.cpp:132-136)
+				if (mot.bias_direction_function[i] != PCM_BIAS_NONE)
+					update_migration_bias_direction<D>(e, i);
 				const R b = mot.migration_bias[i];
 				const R* dir = mot.migration_bias_direction.data() + (std::size_t)i * D;
 				for (int k = 0; k < D; k++)

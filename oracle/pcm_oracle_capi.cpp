@@ -38,6 +38,8 @@ struct oracle_base
 	virtual index_t& deaths() = 0;
 	virtual random_service& rng() = 0;
 	virtual void set_params(const pcm_step_params* p) = 0;
+	virtual int set_gradients(const int32_t* voxel, const void* g) = 0;
+	virtual int ready() = 0;
 	virtual void phase(int which) = 0;
 	virtual void step(int steps) = 0;
 };
@@ -94,6 +96,21 @@ struct oracle_impl final : oracle_base
 
 	int set_field(int f, const void* src) override
 	{
+		const index_t n = env.data.agents_count;
+		const index_t subs = env.m.substrates_count;
+		if (f == PCM_F_CHEMOTAXIS_INDEX)
+			for (index_t i = 0; i < n; i++)
+				if (((const int32_t*)src)[i] < 0 || ((const int32_t*)src)[i] >= std::max<index_t>(subs, 1))
+					return fail(PCM_ERR_ARGUMENT, "chemotaxis_index out of range");
+		if (f == PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION)
+			for (index_t i = 0; i < n; i++)
+				if (((const uint8_t*)src)[i] > PCM_BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED
+					|| (((const uint8_t*)src)[i] != PCM_BIAS_NONE && subs < 1))
+					return fail(PCM_ERR_UNSUPPORTED, "unknown direction update code, or chemotaxis without substrates");
+		if (f == PCM_F_CONTACT_FUNCTION)
+			for (index_t i = 0; i < n; i++)
+				if (((const uint8_t*)src)[i] > PCM_CONTACT_STANDARD_ELASTIC)
+					return fail(PCM_ERR_UNSUPPORTED, "unknown contact function code");
 		return with_field(f, [&](auto& v, index_t w) {
 			using T = typename std::remove_reference_t<decltype(v)>::value_type;
 			std::memcpy(v.data(), src, sizeof(T) * (std::size_t)env.data.agents_count * w);
@@ -120,6 +137,30 @@ struct oracle_impl final : oracle_base
 	std::vector<std::vector<index_t>>& buckets() override { return env.cells_in_mechanics_voxels; }
 	index_t& deaths() override { return env.deaths_count; }
 	random_service& rng() override { return env.rng; }
+
+	int set_gradients(const int32_t* voxel, const void* g) override
+	{
+		const auto& mm = env.mechanics_mesh;
+		for (int k = 0; k < 3; k++)
+			if (voxel[k] < 1)
+				return fail(PCM_ERR_ARGUMENT, "diffusion voxel shape must be positive");
+		env.m.mesh = cartesian_mesh(mm.dims, mm.bounding_box_mins, mm.bounding_box_maxs, voxel);
+		env.m.gradients.resize(env.m.mesh.voxel_count() * env.m.substrates_count * mm.dims);
+		std::memcpy(env.m.gradients.data(), g, sizeof(R) * env.m.gradients.size());
+		return PCM_OK;
+	}
+
+	// chemotaxis reads gradients: refuse to step without them
+	int ready() override
+	{
+		if (!env.m.gradients.empty())
+			return PCM_OK;
+		const auto& code = env.data.motilities.bias_direction_function;
+		for (index_t i = 0; i < env.data.agents_count; i++)
+			if (code[i] != PCM_BIAS_NONE)
+				return fail(PCM_ERR_STATE, "chemotaxis without gradients (pcm_set_gradients)");
+		return PCM_OK;
+	}
 
 	void set_params(const pcm_step_params* p) override
 	{
@@ -381,6 +422,8 @@ int oracle_get_step_index(void* h, uint64_t* s)
 	{                                                                                                                  \
 		if (p)                                                                                                         \
 			cast(h)->set_params(p);                                                                                    \
+		if (int rc = cast(h)->ready())                                                                                 \
+			return rc;                                                                                                 \
 		cast(h)->phase(ID);                                                                                            \
 		return PCM_OK;                                                                                                 \
 	}
@@ -396,9 +439,18 @@ ORACLE_PHASE(update_cell_cell_interactions, PH_INTERACTIONS)
 ORACLE_PHASE(update_cell_container_for_mechanics, PH_CONTAINER)
 ORACLE_PHASE(evaluate_interactions, PH_CONTACT)
 
+int oracle_set_gradients(void* h, const int32_t* voxel, const void* g)
+{
+	if (!voxel || !g)
+		return fail(PCM_ERR_ARGUMENT, "null argument");
+	return cast(h)->set_gradients(voxel, g);
+}
+
 int oracle_step(void* h, const pcm_step_params* p, int32_t steps)
 {
 	cast(h)->set_params(p);
+	if (int rc = cast(h)->ready())
+		return rc;
 	cast(h)->step(steps);
 	return PCM_OK;
 }
@@ -407,6 +459,8 @@ int oracle_step(void* h, const pcm_step_params* p, int32_t steps)
 int oracle_time_steps(void* h, const pcm_step_params* p, int32_t warmup, int32_t steps, double* ms)
 {
 	cast(h)->set_params(p);
+	if (int rc = cast(h)->ready())
+		return rc;
 	cast(h)->step(warmup);
 	const auto t0 = std::chrono::steady_clock::now();
 	cast(h)->step(steps);

@@ -83,9 +83,17 @@ FIELDS = [
     ("fraction_transferred_when_ingested", T_REAL, W_SUBS),
     ("origin", T_I32, W_ONE),
     ("contact_function", T_U8, W_ONE),
+    ("chemotaxis_index", T_I32, W_ONE),
+    ("chemotaxis_direction", T_I32, W_ONE),
+    ("chemotactic_sensitivities", T_REAL, W_SUBS),
+    ("migration_bias_direction_function", T_U8, W_ONE),
 ]
 CONTACT_NONE = 0  # PCM_CONTACT_NONE
 CONTACT_STANDARD_ELASTIC = 1  # PCM_CONTACT_STANDARD_ELASTIC
+BIAS_NONE = 0  # PCM_BIAS_NONE
+BIAS_CHEMOTAXIS = 1  # PCM_BIAS_CHEMOTAXIS
+BIAS_ADVANCED_CHEMOTAXIS = 2  # PCM_BIAS_ADVANCED_CHEMOTAXIS
+BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED = 3  # PCM_BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED
 FIELD_ID = {name: i for i, (name, _, _) in enumerate(FIELDS)}
 
 
@@ -147,6 +155,10 @@ class MeshSpec:
             m.bbox_max[k] = int(self.bbox_max[k])
             m.voxel[k] = int(self.voxel[k])
         return m
+
+    def with_voxel(self, voxel) -> "MeshSpec":
+        """The same bounding box with another voxel shape (the diffusion mesh)."""
+        return MeshSpec(self.dims, self.bbox_min, self.bbox_max, tuple(voxel))
 
     def grid_shape(self):
         g = []

@@ -69,7 +69,11 @@ struct SimCtx
 	R dt;
 	int wall;
 	int springs;
-	int contacts; // some uploaded agent has a contact function
+	int contacts;   // some uploaded agent has a contact function
+	int chemotaxis; // some uploaded agent has a migration bias direction function
+	// substrate gradients on the diffusion grid (pcm_set_gradients), or null
+	const R* grad;
+	int dvox[3], dgrid[3];
 	Scalars* sc;
 
 	// agent fields (include/pcm.h order); fptr/fbytes give the generic view
@@ -94,6 +98,9 @@ struct SimCtx
 	R *internal, *frac_ingest;
 	int* origin;
 	uint8_t* contact; // PCM_CONTACT_* code per agent
+	int *chemo_index, *chemo_dir;
+	R* sensitivities;
+	uint8_t* bias_fn; // PCM_BIAS_* code per agent
 
 	// link lists
 	int *spr_cnt, *spr_idx, *atc_cnt, *atc_idx;
@@ -433,6 +440,70 @@ __device__ __forceinline__ motility_in<D, R> load_motility(const SimCtx<R>& c, i
 	return in;
 }
 
+// normalize (solver_helper.h:30-45)
+template <int D, class R>
+__device__ __forceinline__ void normalize_dir(R* v)
+{
+	R length = 0;
+	for (int k = 0; k < D; k++)
+		length += v[k] * v[k];
+	length = sqrt(length);
+	for (int k = 0; k < D; k++)
+		v[k] = length > zero_threshold<R>() ? v[k] / length : (R)0;
+}
+
+// cell::nearest_gradient (paraBioFVM agent): the gradients of the diffusion voxel that
+// holds position p, for every substrate
+template <int D, class R>
+__device__ __forceinline__ const R* nearest_gradients(const SimCtx<R>& c, const R* p)
+{
+	int co[3] = { 0, 0, 0 };
+	for (int k = 0; k < D; k++)
+	{
+		const R q = (p[k] - (R)c.bmin[k]) / (R)c.dvox[k];
+		int v = (int)q;
+		v = v < 0 ? 0 : (v >= c.dgrid[k] ? c.dgrid[k] - 1 : v);
+		co[k] = v;
+	}
+	const size_t v = (size_t)co[0] + (size_t)c.dgrid[0] * ((size_t)co[1] + (size_t)c.dgrid[1] * co[2]);
+	return c.grad + v * c.S * D;
+}
+
+// The migration bias direction updates the reference ships (standard_models.cpp:542-583):
+// chemotaxis_function and advanced_chemotaxis_function<normalize_each_gradient>.
+template <int D, class R>
+__device__ __forceinline__ void update_bias_direction(const SimCtx<R>& c, int i, int code, R* dir)
+{
+	R p[D];
+	for (int k = 0; k < D; k++)
+		p[k] = c.pos[(size_t)i * D + k];
+	const R* g = nearest_gradients<D, R>(c, p);
+	if (code == PCM_BIAS_CHEMOTAXIS)
+	{
+		const R sign = (R)c.chemo_dir[i];
+		const R* gs = g + (size_t)c.chemo_index[i] * D;
+		for (int k = 0; k < D; k++)
+			dir[k] = gs[k] * sign;
+	}
+	else
+	{
+		for (int k = 0; k < D; k++)
+			dir[k] = 0;
+		for (int s = 0; s < c.S; s++)
+		{
+			R gs[D];
+			for (int k = 0; k < D; k++)
+				gs[k] = g[(size_t)s * D + k];
+			if (code == PCM_BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED)
+				normalize_dir<D, R>(gs);
+			const R w = c.sensitivities[(size_t)i * c.S + s];
+			for (int k = 0; k < D; k++)
+				dir[k] += gs[k] * w;
+		}
+	}
+	normalize_dir<D, R>(dir);
+}
+
 // update_motility_single (position_solver.cpp:115-145) for one agent.  The 3D walk is
 // planar because the reference passes the restrict_to_2d array pointer as the bool.
 template <int D, class R>
@@ -461,9 +532,24 @@ __device__ __forceinline__ void motility_apply(const SimCtx<R>& c, int i, unsign
 			if constexpr (D == 3)
 				walk[2] = 0;
 		}
+		R dir[D];
+		for (int k = 0; k < D; k++)
+			dir[k] = in.dir[k];
+		if (c.chemotaxis)
+		{
+			// update_migration_bias_direction_f[i] (position_solver.cpp:132-136)
+			const int code = c.bias_fn[i];
+			if (code != PCM_BIAS_NONE)
+			{
+				update_bias_direction<D, R>(c, i, code, dir);
+				if (writer)
+					for (int k = 0; k < D; k++)
+						c.bias_dir[(size_t)i * D + k] = dir[k];
+			}
+		}
 		const R b = in.bias;
 		for (int k = 0; k < D; k++)
-			m[k] = (1 - b) * walk[k] + b * in.dir[k];
+			m[k] = (1 - b) * walk[k] + b * dir[k];
 		R length;
 		if constexpr (D == 1)
 			length = fabs(m[0]);

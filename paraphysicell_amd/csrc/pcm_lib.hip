@@ -80,6 +80,7 @@ struct pcm_sim
 	virtual int get_deaths(int64_t* d) = 0;
 	virtual int set_step(uint64_t s) = 0;
 	virtual int get_step(uint64_t* s) = 0;
+	virtual int set_gradients(const int32_t* voxel, const void* g) = 0;
 	virtual int phase(int which, const pcm_step_params* p) = 0;
 	virtual int step(const pcm_step_params* p, int steps) = 0;
 	virtual int synchronize() = 0;
@@ -285,10 +286,12 @@ public:
 			PCM_HIP(hipMemset(h_.atc_cnt, 0, sizeof(int) * n));
 			PCM_HIP(hipMemset(h_.nbr_cnt, 0, sizeof(int) * n));
 			PCM_HIP(hipMemset(h_.contact, 0, n));
+			PCM_HIP(hipMemset(h_.bias_fn, 0, n));
 		}
-		if (h_.contacts)
+		if (h_.contacts || h_.chemotaxis)
 		{
 			h_.contacts = 0;
+			h_.chemotaxis = 0;
 			PCM_TRY(push_ctx());
 		}
 		return PCM_OK;
@@ -317,6 +320,29 @@ public:
 				return fail(PCM_ERR_UNSUPPORTED, "maximum_number_of_attachments above 32 is not supported");
 			if (mx > h_.L)
 				PCM_TRY(grow_springs(mx));
+		}
+		if (f == PCM_F_CHEMOTAXIS_INDEX)
+		{
+			// gradient rows are read at this index
+			for (int i = 0; i < n; i++)
+				if (((const int32_t*)src)[i] < 0 || ((const int32_t*)src)[i] >= std::max(h_.S, 1))
+					return fail(PCM_ERR_ARGUMENT, "chemotaxis_index out of range");
+		}
+		if (f == PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION)
+		{
+			const uint8_t* code = (const uint8_t*)src;
+			int any = 0;
+			for (int i = 0; i < n; i++)
+			{
+				if (code[i] > PCM_BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED || (code[i] != PCM_BIAS_NONE && h_.S < 1))
+					return fail(PCM_ERR_UNSUPPORTED, "unknown direction update code, or chemotaxis without substrates");
+				any |= code[i] != PCM_BIAS_NONE;
+			}
+			if (any != h_.chemotaxis)
+			{
+				h_.chemotaxis = any;
+				PCM_TRY(push_ctx());
+			}
 		}
 		if (f == PCM_F_CONTACT_FUNCTION)
 		{
@@ -553,9 +579,50 @@ public:
 		return PCM_OK;
 	}
 
+	int set_gradients(const int32_t* voxel, const void* g) override
+	{
+		int grid[3];
+		size_t voxels = 1;
+		for (int k = 0; k < 3; k++)
+		{
+			if (voxel[k] < 1)
+				return fail(PCM_ERR_ARGUMENT, "diffusion voxel shape must be positive");
+			const int extent = cfg_.mesh.bbox_max[k] - cfg_.mesh.bbox_min[k];
+			grid[k] = k < h_.dims ? std::max(1, (extent + voxel[k] - 1) / voxel[k]) : 1;
+			voxels *= (size_t)grid[k];
+		}
+		const size_t count = voxels * (size_t)h_.S * (size_t)h_.dims;
+		PCM_HIP(hipStreamSynchronize(stream_));
+		if (count != grad_count_)
+		{
+			release((void*)h_.grad);
+			R* buf = nullptr;
+			PCM_TRY(alloc_t(&buf, std::max<size_t>(count, 1)));
+			h_.grad = buf;
+			grad_count_ = count;
+		}
+		if (count > 0)
+			PCM_HIP(hipMemcpy((void*)h_.grad, g, sizeof(R) * count, hipMemcpyHostToDevice));
+		for (int k = 0; k < 3; k++)
+		{
+			h_.dvox[k] = voxel[k];
+			h_.dgrid[k] = grid[k];
+		}
+		return push_ctx();
+	}
+
+	// chemotaxis reads the gradients: no step or phase runs without them
+	int ready() const
+	{
+		if (h_.chemotaxis && !h_.grad)
+			return fail(PCM_ERR_STATE, "chemotaxis without gradients (pcm_set_gradients)");
+		return PCM_OK;
+	}
+
 	// ---------------------------------------------------------------- phases
 	int phase(int which, const pcm_step_params* p) override
 	{
+		PCM_TRY(ready());
 		if (p)
 			PCM_TRY(set_params(*p));
 		switch (h_.dims)
@@ -573,6 +640,7 @@ public:
 	{
 		if (!p || steps < 0)
 			return fail(PCM_ERR_ARGUMENT, "null params or negative step count");
+		PCM_TRY(ready());
 		PCM_TRY(set_params(*p));
 		PCM_TRY(read_scalars());
 		if (sc_.status)
@@ -644,6 +712,7 @@ private:
 	int grid_n_ = 0; // agent bound of grid sizes (0: n_launch_)
 	int grid_L_ = 0; // spring-slot bound of grid sizes (0: h_.L)
 	uint64_t ctx_version_ = 0; // increments whenever the device context changes
+	size_t grad_count_ = 0;    // reals in h_.grad
 	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
@@ -749,6 +818,10 @@ private:
 		h.frac_ingest = (R*)F(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED);
 		h.origin = (int*)F(PCM_F_ORIGIN);
 		h.contact = (uint8_t*)F(PCM_F_CONTACT_FUNCTION);
+		h.chemo_index = (int*)F(PCM_F_CHEMOTAXIS_INDEX);
+		h.chemo_dir = (int*)F(PCM_F_CHEMOTAXIS_DIRECTION);
+		h.sensitivities = (R*)F(PCM_F_CHEMOTACTIC_SENSITIVITIES);
+		h.bias_fn = (uint8_t*)F(PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION);
 	}
 
 	int alloc_neighbor_storage()
@@ -1576,6 +1649,7 @@ private:
 		for (auto& slot : g.slots)
 			if (slot.sim)
 			{
+				PCM_TRY(slot.sim->ready());
 				PCM_TRY(slot.sim->apply_params(p));
 				grid_n = std::max(grid_n, slot.sim->launch_bound());
 				grid_L = std::max(grid_L, slot.sim->spring_slots());
@@ -1911,6 +1985,14 @@ int pcm_get_deaths_count(pcm_sim* sim, int64_t* deaths)
 	if (!deaths)
 		return fail(PCM_ERR_ARGUMENT, "null output");
 	return sim->get_deaths(deaths);
+}
+
+int pcm_set_gradients(pcm_sim* sim, const int32_t diffusion_voxel[3], const void* gradients)
+{
+	PCM_NEED(sim);
+	if (!diffusion_voxel || !gradients)
+		return fail(PCM_ERR_ARGUMENT, "null diffusion voxel or gradients");
+	return sim->set_gradients(diffusion_voxel, gradients);
 }
 
 int pcm_set_step_index(pcm_sim* sim, uint64_t step)

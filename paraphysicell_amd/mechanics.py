@@ -47,6 +47,7 @@ def _declare(lib, prefix: str):
         "get_voxel_count": [P, ctypes.POINTER(ctypes.c_int64)],
         "get_voxel_buckets": [P, P, P],
         "get_deaths_count": [P, ctypes.POINTER(ctypes.c_int64)],
+        "set_gradients": [P, P, P],
         "set_step_index": [P, ctypes.c_uint64],
         "get_step_index": [P, ctypes.POINTER(ctypes.c_uint64)],
         "update_mechanics_mesh": None,
@@ -146,6 +147,19 @@ class _LibSim:
             if idx.size == 0:
                 idx = np.zeros(1, np.int32)
             self._call("set_neighbors", self._h, off.ctypes.data, idx.ctypes.data)
+        if state.gradients is not None:
+            self.set_gradients(*state.gradients)
+
+    def set_gradients(self, diffusion_voxel, gradients) -> None:
+        """Substrate gradients per diffusion voxel (voxels x substrates x dims, x fastest
+        over voxels), the input of the chemotaxis direction updates."""
+        vox = np.ascontiguousarray(diffusion_voxel, dtype=np.int32).reshape(3)
+        g = np.ascontiguousarray(gradients, dtype=np.float32 if self.real_bytes == 4 else np.float64)
+        voxels = int(np.prod(self.mesh.with_voxel(tuple(int(v) for v in vox)).grid_shape()))
+        need = voxels * self.subs * self.mesh.dims
+        if g.size != need:
+            raise PcmError(abi.PCM_ERR_ARGUMENT, f"gradients: {g.size} values, the diffusion grid needs {need}")
+        self._call("set_gradients", self._h, vox.ctypes.data, g.ctypes.data)
 
     def get_field(self, name: str) -> np.ndarray:
         ftype, wkind = abi.FIELDS[abi.FIELD_ID[name]][1:]

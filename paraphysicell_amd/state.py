@@ -29,6 +29,8 @@ class AgentState:
         self.links = {abi.LINKS_SPRINGS: empty, abi.LINKS_ATTACHED: (empty[0].copy(), empty[1].copy())}
         # states.neighbors as CSR, or None when the state carries no lists
         self.neighbors = None
+        # (diffusion voxel shape, substrate gradients) for the chemotaxis models, or None
+        self.gradients = None
 
     # ------------------------------------------------------------------ access
     @property
@@ -113,6 +115,7 @@ class AgentState:
             "number_of_nuclei": 1,
             "is_movable": 1,
             "maximum_number_of_attachments": 12,
+            "chemotaxis_direction": 1,
         }
         for k, v in vals.items():
             self.arrays[k][:] = v

@@ -30,6 +30,21 @@ void pcm_contact_functions(environment<R>& e, std::vector<uint8_t>& codes)
 	for (index_t i = 0; i < e.data.agents_count; i++)
 		codes[i] = e.data.contact_function[i];
 }
+
+template <class R>
+void pcm_bias_direction_functions(environment<R>& e, std::vector<uint8_t>& codes)
+{
+	for (index_t i = 0; i < e.data.agents_count; i++)
+		codes[i] = e.data.motilities.bias_direction_function[i];
+}
+
+template <class R>
+void pcm_gradients(environment<R>& e, int32_t voxel[3], std::vector<R>& gradients)
+{
+	for (int k = 0; k < 3; k++)
+		voxel[k] = e.m.mesh.voxel_shape[k];
+	gradients = e.m.gradients;
+}
 } // namespace pcm_oracle
 
 using namespace pcm_oracle;
@@ -193,6 +208,23 @@ void fill_population(environment<R>& e, int n, unsigned seed)
 				d.states.attached_cells[j].push_back(i);
 			}
 		}
+	}
+	// chemotaxis models over a random gradient field on a 25 x 20 x 15 um diffusion grid
+	for (int i = 0; i < n; i++)
+	{
+		d.motilities.bias_direction_function[i] = (uint8_t)(4 * u(g)) % 4;
+		d.motilities.chemotaxis_index[i] = 0;
+		d.motilities.chemotaxis_direction[i] = u(g) < 0.5 ? -1 : 1;
+		for (int s = 0; s < e.m.substrates_count; s++)
+			d.motilities.chemotactic_sensitivities[i * e.m.substrates_count + s] = (R)(2 * u(g) - 1);
+	}
+	{
+		const int32_t dvox[3] = { 25, 20, 15 };
+		const auto& mm = e.mechanics_mesh;
+		e.m.mesh = cartesian_mesh(mm.dims, mm.bounding_box_mins, mm.bounding_box_maxs, dvox);
+		e.m.gradients.resize(e.m.mesh.voxel_count() * e.m.substrates_count * D);
+		for (auto& v : e.m.gradients)
+			v = (R)(2 * u(g) - 1);
 	}
 	// a few agents leave the box during the first step
 	for (int i = 0; i < n / 100 + 1; i++)

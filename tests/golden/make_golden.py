@@ -54,6 +54,9 @@ def main():
             out[f"in_field_{name}"] = st.arrays[name]
         for which, label in ((abi.LINKS_SPRINGS, "springs"), (abi.LINKS_ATTACHED, "attached")):
             out[f"in_{label}_off"], out[f"in_{label}_idx"] = st.links[which]
+        if st.gradients is not None:
+            out["in_gradient_voxel"] = np.asarray(st.gradients[0], np.int32)
+            out["in_gradients"] = st.gradients[1]
         meta = {"dims": dims, "real_bytes": real_bytes, "defs": st.defs, "subs": st.subs, "count": n,
                 "seed": SEED, "bbox_min": list(mesh.bbox_min), "bbox_max": list(mesh.bbox_max),
                 "voxel": list(mesh.voxel), "dt": params.dt, "virtual_wall": params.virtual_wall,

@@ -64,7 +64,8 @@ def rich_population(dims, n, seed, real_bytes, box=150, fill=0.55, spring_cap=3,
     """A small dense population exercising every branch of the mechanics step.
 
     Two cell types; 10% non-movable; 5% dead; pre-existing springs (some duplicated);
-    attached cells with contact functions on half of the agents;
+    attached cells with contact functions on half of the agents; chemotaxis models on a
+    random gradient field;
     attachment/detachment, attack, phagocytosis and fusion rates high enough that
     capacity limits and interaction conflicts occur; a few agents drift out of the box.
     """
@@ -133,5 +134,16 @@ def rich_population(dims, n, seed, real_bytes, box=150, fill=0.55, spring_cap=3,
     st.set_links(abi.LINKS_ATTACHED, att)
     # half of the agents carry the standard elastic contact function
     st["contact_function"] = (rng.uniform(size=n) < 0.5).astype(np.uint8)
+    # chemotaxis: every direction update model over a random gradient field on an
+    # anisotropic diffusion grid (a few zero gradients, so normalisation meets length 0)
+    st["migration_bias_direction_function"] = rng.integers(0, 4, n).astype(np.uint8)
+    st["chemotaxis_index"] = rng.integers(0, 2, n)
+    st["chemotaxis_direction"] = rng.choice([-1, 1], n)
+    st["chemotactic_sensitivities"] = rng.uniform(-1, 1, n * 2)
+    dvox = (25, 20, 15)
+    voxels = int(np.prod(mesh.with_voxel(dvox).grid_shape()))
+    grad = rng.normal(0, 1, (voxels, 2, dims))
+    grad[rng.uniform(size=voxels) < 0.05] = 0.0
+    st.gradients = (dvox, grad.astype(st.real).ravel())
     params = abi.Params(dt=0.1, virtual_wall=True, springs=True)
     return mesh, st, params

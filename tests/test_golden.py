@@ -30,6 +30,8 @@ def load_case(path):
         st.arrays[name] = z[f"in_field_{name}"].copy()
     st.links[abi.LINKS_SPRINGS] = (z["in_springs_off"], z["in_springs_idx"])
     st.links[abi.LINKS_ATTACHED] = (z["in_attached_off"], z["in_attached_idx"])
+    if "in_gradients" in z:
+        st.gradients = (tuple(int(v) for v in z["in_gradient_voxel"]), z["in_gradients"])
     params = abi.Params(dt=meta["dt"], virtual_wall=meta["virtual_wall"], springs=meta["springs"])
     return z, meta, mesh, st, params
 

@@ -406,6 +406,90 @@ def test_contact_function_runs_first_in_the_step(dims):
     assert _close(sim.get_field("positions").reshape(n, dims), exp, 1e-12)
 
 
+# ------------------------------------------------------------------ chemotaxis (F3)
+def _chemotaxis_case(dims, code, seed=3):
+    """Motile agents that always redraw (dt / persistence > 1) with full bias, on a
+    diffusion grid of 25 x 20 x 15 um voxels whose gradients identify their voxel and
+    substrate; a few agents sit outside the box, one in a zero-gradient voxel."""
+    n, subs = 96, 2
+    mesh, st = _state(dims, n, subs=subs, box=100)
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-100, 100, (n, dims))
+    pos[:4] = rng.uniform(-130, 130, (4, dims))  # clamped to the edge voxels
+    st["positions"] = pos.ravel()
+    st["is_motile"] = 1
+    st["migration_bias"] = 1.0
+    st["persistence_time"] = 0.01
+    st["migration_speed"] = rng.uniform(0.5, 2.0, n)
+    st["migration_bias_direction"] = rng.normal(size=n * dims)
+    st["velocities"] = 0.0
+    st["migration_bias_direction_function"] = code
+    st["chemotaxis_index"] = rng.integers(0, subs, n)
+    st["chemotaxis_direction"] = rng.choice([-1, 1], n)
+    st["chemotactic_sensitivities"] = rng.uniform(-1, 1, n * subs)
+    dvox = (25, 20, 15)
+    g3 = mesh.with_voxel(dvox).grid_shape()
+    voxels = int(np.prod(g3))
+    grad = rng.normal(0, 1, (voxels, subs, dims))
+    # the voxel of agent 5 has no gradient at all
+    c5 = [int(np.clip(np.floor((pos[5, k] + 100) / dvox[k]), 0, g3[k] - 1)) if k < dims else 0 for k in range(3)]
+    grad[c5[0] + g3[0] * (c5[1] + g3[1] * c5[2])] = 0.0
+    st.gradients = (dvox, grad.ravel())
+    # expected: gradient of the voxel holding each position (x fastest, clamped)
+    co = [np.clip(np.floor((pos[:, k] + 100) / dvox[k]).astype(int), 0, g3[k] - 1) if k < dims
+          else np.zeros(n, int) for k in range(3)]
+    g_at = grad[co[0] + g3[0] * (co[1] + g3[1] * co[2])]  # (n, subs, dims)
+    return mesh, st, g_at
+
+
+def _unit(v):
+    length = np.linalg.norm(v, axis=-1, keepdims=True)
+    return np.where(length > 1e-16, v / np.where(length > 0, length, 1), 0.0)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+@pytest.mark.parametrize("code", [abi.BIAS_CHEMOTAXIS, abi.BIAS_ADVANCED_CHEMOTAXIS,
+                                  abi.BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED])
+def test_chemotaxis_direction_closed_form(dims, code):
+    """After a redraw with bias 1 the bias direction is chemotaxis_function's
+    unit(direction * grad[chemotaxis_index]) or advanced_chemotaxis_function's
+    unit(sum_s sensitivity_s * g_s) with g_s = grad_s or unit(grad_s)
+    (standard_models.cpp:542-583), grad taken at the diffusion voxel that holds the agent;
+    the motility vector is that direction times the speed (position_solver.cpp:132-141)."""
+    mesh, st, g_at = _chemotaxis_case(dims, code)
+    n = st.count
+    sim = _sim(mesh, st)
+    sim.update_motility(abi.Params(dt=0.1))
+    if code == abi.BIAS_CHEMOTAXIS:
+        exp = _unit(g_at[np.arange(n), st["chemotaxis_index"]] * st["chemotaxis_direction"][:, None])
+    else:
+        g = _unit(g_at) if code == abi.BIAS_ADVANCED_CHEMOTAXIS_NORMALIZED else g_at
+        exp = _unit(np.einsum("ns,nsd->nd", st.view("chemotactic_sensitivities"), g))
+    assert _close(sim.get_field("migration_bias_direction").reshape(n, dims), exp, 1e-12)
+    assert np.all(exp[5] == 0)
+    speed = st["migration_speed"][:, None]
+    assert _close(sim.get_field("motility_vector").reshape(n, dims), exp * speed, 1e-12)
+
+
+def test_chemotaxis_only_on_redraw():
+    """Without a redraw (dt / persistence ~ 0) the direction update is not called and the
+    bias direction keeps its value (position_solver.cpp:127-137)."""
+    mesh, st, _ = _chemotaxis_case(3, abi.BIAS_CHEMOTAXIS)
+    st["persistence_time"] = 1e30
+    sim = _sim(mesh, st)
+    sim.update_motility(abi.Params(dt=0.1))
+    assert np.array_equal(sim.get_field("migration_bias_direction"), st["migration_bias_direction"])
+
+
+def test_chemotaxis_needs_gradients():
+    """A direction update without uploaded gradients is refused, not read out of bounds."""
+    mesh, st, _ = _chemotaxis_case(2, abi.BIAS_CHEMOTAXIS)
+    st.gradients = None
+    sim = _sim(mesh, st)
+    with pytest.raises(Exception, match="gradients"):
+        sim.update_motility(abi.Params(dt=0.1))
+
+
 def test_spring_detach_saturating():
     """detachment_rate * dt >= 1 fires every draw (`<=`, position_solver.cpp:386): every
     spring touching such an agent goes, springs between two zero-rate agents stay."""
