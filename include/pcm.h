@@ -105,7 +105,9 @@ extern "C" {
 	X(CHEMOTAXIS_INDEX, PCM_T_I32, PCM_W_ONE)                     /* motilities.chemotaxis_index     */            \
 	X(CHEMOTAXIS_DIRECTION, PCM_T_I32, PCM_W_ONE)                 /* motilities.chemotaxis_direction */            \
 	X(CHEMOTACTIC_SENSITIVITIES, PCM_T_REAL, PCM_W_SUBS)          /* motilities.chemotactic_sensitivities */       \
-	X(MIGRATION_BIAS_DIRECTION_FUNCTION, PCM_T_U8, PCM_W_ONE) /* motilities.update_migration_bias_direction, PCM_BIAS_* */
+	X(MIGRATION_BIAS_DIRECTION_FUNCTION, PCM_T_U8, PCM_W_ONE) /* motilities.update_migration_bias_direction, PCM_BIAS_* */ \
+	X(ORIENTATION, PCM_T_REAL, PCM_W_DIMS)                        /* states.orientation              */            \
+	X(POLARITY, PCM_T_REAL, PCM_W_ONE)                            /* geometries.polarity             */
 
 #define PCM_FIELD_ENUM_(name, type, width) PCM_F_##name,
 enum pcm_field
@@ -241,6 +243,25 @@ int pcm_update_positions(pcm_sim* sim, const pcm_step_params* params);
 int pcm_update_cell_cell_interactions(pcm_sim* sim, const pcm_step_params* params);
 /* containers_solver::update_cell_container_for_mechanics (containers_solver.cpp:140-147) */
 int pcm_update_cell_container_for_mechanics(pcm_sim* sim, const pcm_step_params* params);
+/* containers_solver::update_cell_container_for_phenotype (containers_solver.cpp:149-187) in
+ * the reference's single-thread order: every agent of [0, count) flagged
+ * PCM_FLAG_TO_DIVIDE divides (cell::divide, cell.cpp:108-163) in ascending index order, its
+ * daughter appended at count + rank; then the removal loop visits [0, count) (removals and
+ * the removal log as in pcm_update_cell_container_for_mechanics).  Daughters copy the
+ * parent's phenotype fields (phenotype.cpp:18-30, after the parent's volume halving) and
+ * start with zero velocities, pressure and damage, one nucleus, movable, ORIGIN -1 and a
+ * random ORIENTATION (cell::assign_orientation without set_orientation).  The parent's
+ * INTERNALIZED_SUBSTRATES halve when compute_internalized_substrates != 0 (the
+ * microenvironment option of cell.cpp:122).  release_internalized_substrates and custom
+ * data (paraBioFVM, host members) are the caller's: see pcm_division_log. */
+int pcm_update_cell_container_for_phenotype(pcm_sim* sim, const pcm_step_params* params,
+											int32_t compute_internalized_substrates);
+/* environment::divisions_count increments made on the device */
+int pcm_get_divisions_count(pcm_sim* sim, int64_t* divisions);
+/* The parents of the last pcm_update_cell_container_for_phenotype's divisions, in division
+ * order; division r created the agent at (count before the call) + r.  With parents == NULL
+ * only the length is returned. */
+int pcm_division_log(pcm_sim* sim, int32_t* parents, int64_t capacity, int64_t* length);
 
 /* The mechanics step of simulator::simulate_mechanics (src/simulator.cpp:171-192), run
  * `steps` times without host round trips: contact functions, forces, motility, basement
@@ -308,7 +329,7 @@ int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t 
 	X(MOTILITY) X(MEMBRANE) X(CONTRACT) X(POSITIONS) X(VELOCITY_FUSED) X(SPRING_FIRE) X(SPRING_ERASE)                   \
 	X(SPRING_COMPACT) X(SPRING_COLLECT) X(SPRING_RESOLVE) X(SPRING_APPEND) X(INTERACT_TENTATIVE) X(INTERACT_CHECK)       \
 	X(INTERACT_APPLY) X(INTERACT_SERIAL) X(COMPACT_FLAG) X(COMPACT_PLAN) X(COMPACT_APPLY) X(COMPACT_RESTORE)            \
-	X(COMPACT_DONE) X(FORCE_RECORDS) X(CONTACT)
+	X(COMPACT_DONE) X(FORCE_RECORDS) X(CONTACT) X(DIVIDE_PLAN) X(DIVIDE_APPLY) X(DIVIDE_LINKS)
 #define PCM_KERNEL_ENUM_(name) PCM_K_##name,
 enum pcm_kernel
 {

@@ -20,6 +20,8 @@
 // argument-dependent lookup:
 //   auto& pcm_cell_data(Env& e);              // the cell_data SoA (cell_data.h:162-196)
 //   void  pcm_host_remove_at(Env& e, int i);  // the container's remove_at (swap with last)
+// and, to divide on the device (update_cell_container_for_phenotype), a third:
+//   void  pcm_host_divide(Env& e, int parent); // append the daughter's host object
 // and optionally, when cells carry contact functions (cell_functions.h:26) or chemotaxis
 // direction updates (cell_data.h:100):
 //   void  pcm_contact_functions(Env& e, std::vector<uint8_t>& codes);        // PCM_CONTACT_* per agent
@@ -175,6 +177,7 @@ struct residency
 	bool host_newer = true;    // the device copy must be refreshed before the next phase
 	bool device_newer = false; // the host copy must be refreshed before host code reads it
 	bool gradients_newer = true; // the substrate gradients must be uploaded before the next phase
+	int min_capacity = 0;        // agent capacity the next handle needs at least
 	int64_t deaths_seen = 0;
 	uint64_t seed = 0;
 
@@ -256,7 +259,7 @@ void upload(Env& e, residency& r)
 		c.real_bytes = (int)sizeof(real);
 		c.cell_defs = e.cell_definitions_count;
 		c.substrates = e.m.substrates_count;
-		c.capacity = n < 1024 ? 2048 : 2 * n;
+		c.capacity = std::max(n < 1024 ? 2048 : 2 * n, r.min_capacity);
 		c.seed = r.seed;
 		check(pcm_create(&c, &r.sim), "pcm_create");
 		r.capacity = c.capacity;
@@ -493,6 +496,52 @@ public:
 		run(e, [&](pcm_sim* s) {
 			const auto p = params_of(e);
 			check(pcm_update_cell_container_for_mechanics(s, &p), "pcm_update_cell_container_for_mechanics");
+		});
+	}
+
+	// containers_solver::update_cell_container_for_phenotype (containers_solver.cpp:149-187):
+	// divisions and the removal loop on the device, then the host container appends each
+	// daughter's host object through pcm_host_divide(e, parent) (c.create() and the host
+	// members of cell::copy_from: type, custom data with divide_conserved_quantities,
+	// parameters, functions, phenotype members outside the device fields); the removals
+	// replay at the next sync_to_host.  `cell_solver` is the reference's BioFVM argument:
+	// release_internalized_substrates is not run here.
+	template <class CellSolver>
+	void update_cell_container_for_phenotype(Env& e, CellSolver& cell_solver)
+	{
+		(void)cell_solver;
+		on_one_thread([&] {
+			auto& r0 = state_of(e);
+			if (r0.sim && r0.device_newer && !r0.host_newer)
+				download(e, r0); // earlier removals replay before the daughters are appended
+			int32_t internalized = 0;
+			if constexpr (requires { e.m.compute_internalized_substrates; })
+				internalized = e.m.compute_internalized_substrates ? 1 : 0;
+			const auto p = params_of(e);
+			auto& r = on_device(e);
+			int rc = pcm_update_cell_container_for_phenotype(r.sim, &p, internalized);
+			if (rc == PCM_ERR_CAPACITY)
+			{
+				// divisions at most double the population: a larger handle, then again
+				int32_t n = 0;
+				check(pcm_get_count(r.sim, &n), "pcm_get_count");
+				if (r.device_newer)
+					download(e, r);
+				r.min_capacity = 2 * n + 1024;
+				check(pcm_destroy(r.sim), "pcm_destroy");
+				r.sim = nullptr;
+				upload(e, r);
+				rc = pcm_update_cell_container_for_phenotype(r.sim, &p, internalized);
+			}
+			check(rc, "pcm_update_cell_container_for_phenotype");
+			int64_t len = 0;
+			check(pcm_division_log(r.sim, nullptr, 0, &len), "pcm_division_log");
+			std::vector<int32_t> parents(len + 1);
+			check(pcm_division_log(r.sim, parents.data(), len + 1, &len), "pcm_division_log");
+			for (int64_t k = 0; k < len; k++)
+				pcm_host_divide(e, parents[k]);
+			e.divisions_count += (decltype(e.divisions_count))len;
+			r.device_newer = true;
 		});
 	}
 };

@@ -129,7 +129,7 @@ struct volume_data_t
 template <class R>
 struct geometry_data_t
 {
-	std::vector<R> radius, nuclear_radius, surface_area;
+	std::vector<R> radius, nuclear_radius, surface_area, polarity;
 };
 
 template <class R>
@@ -169,6 +169,7 @@ template <class R>
 struct cell_state_data_t
 {
 	std::vector<std::vector<index_t>> neighbors, springs, attached_cells;
+	std::vector<R> orientation; // cell_data.h:144
 	std::vector<R> simple_pressure;
 	std::vector<index_t> number_of_nuclei;
 	std::vector<R> damage, total_attack_time;
@@ -260,6 +261,8 @@ void for_each_field(cell_data<R>& d, index_t dims, index_t defs, index_t subs, F
 	f(PCM_F_CHEMOTAXIS_DIRECTION, d.motilities.chemotaxis_direction, 1);
 	f(PCM_F_CHEMOTACTIC_SENSITIVITIES, d.motilities.chemotactic_sensitivities, subs);
 	f(PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION, d.motilities.bias_direction_function, 1);
+	f(PCM_F_ORIENTATION, d.states.orientation, dims);
+	f(PCM_F_POLARITY, d.geometries.polarity, 1);
 }
 
 // ------------------------------------------------------------------ random service
@@ -312,12 +315,20 @@ struct random_service
 	template <class R>
 	R uniform(pcm_stream stream, index_t agent, index_t slot, std::uint32_t draw, R lo = 0, R hi = 1)
 	{
+		return uniform_at<R>(step, stream, agent, slot, draw, lo, hi);
+	}
+
+	// The same with an explicit counter in place of the mechanics step.
+	template <class R>
+	R uniform_at(std::uint64_t counter, pcm_stream stream, index_t agent, index_t slot, std::uint32_t draw, R lo = 0,
+				 R hi = 1)
+	{
 		if (mode == rng_mode::mt)
 		{
 			std::uniform_real_distribution<R> distribution(lo, hi);
 			return distribution(generator());
 		}
-		const R u = pcm_uniform01<R>(seed, step, stream, (std::uint32_t)agent, (std::uint32_t)slot, draw);
+		const R u = pcm_uniform01<R>(seed, counter, stream, (std::uint32_t)agent, (std::uint32_t)slot, draw);
 		return pcm_scale_uniform<R>(u, lo, hi);
 	}
 };
@@ -353,6 +364,10 @@ struct environment
 	bool automated_spring_adhesion = true;
 	index_t divisions_count = 0;
 	index_t deaths_count = 0;
+	// calls of update_cell_container_for_phenotype: keys the division draws
+	std::uint64_t division_epoch = 0;
+	// parents of the last phenotype update's divisions, in division order
+	std::vector<index_t> division_log;
 	cartesian_mesh mechanics_mesh;
 	R mechanics_time_step = (R)0.1;
 	index_t cell_definitions_count = 1;
@@ -1142,6 +1157,165 @@ public:
 			if (i == d.agents_count)
 				return;
 		}
+	}
+
+	// position_helper<D>::random_walk(false, walk) (solver_helper.h:82-86, 160-165, 250-267),
+	// keyed by the division epoch, the parent and a slot
+	template <int D>
+	static void free_walk(env_t& e, std::uint64_t epoch, index_t parent, index_t slot, R* walk)
+	{
+		auto u = [&](std::uint32_t draw, R lo, R hi) {
+			return e.rng.template uniform_at<R>(epoch, PCM_STREAM_DIVISION, parent, slot, draw, lo, hi);
+		};
+		if constexpr (D == 1)
+			walk[0] = u(0, 0, 1) < (R)0.5 ? -1 : 1;
+		else if constexpr (D == 2)
+		{
+			const R theta = u(0, 0, 2 * std::numbers::pi_v<R>);
+			walk[0] = std::cos(theta);
+			walk[1] = std::sin(theta);
+		}
+		else
+		{
+			const R theta = u(0, 0, 2 * std::numbers::pi_v<R>);
+			const R z = u(1, -1, 1);
+			const R r = std::sqrt(1 - z * z);
+			walk[0] = std::cos(theta) * r;
+			walk[1] = std::sin(theta) * r;
+			walk[2] = z;
+		}
+	}
+
+	// cell_data::add + the cell constructor (cell_data.cpp:262-283, cell.cpp:33-60): a new
+	// agent at the end with zero velocities, no flag, movable, default state.  Its phenotype
+	// fields are overwritten by copy_from.
+	index_t create(env_t& e)
+	{
+		auto& d = e.data;
+		const index_t k = d.agents_count;
+		const index_t dims = e.mechanics_mesh.dims;
+		for_each_field(d, dims, e.cell_definitions_count, e.m.substrates_count,
+					   [k](int, auto& v, index_t w) { v.resize((std::size_t)(k + 1) * w); });
+		d.agents_count = k + 1;
+		d.states.neighbors.resize(k + 1);
+		d.states.springs.resize(k + 1);
+		d.states.attached_cells.resize(k + 1);
+		d.states.neighbors[k].clear();
+		d.states.springs[k].clear();
+		d.states.attached_cells[k].clear();
+		for (index_t c = 0; c < dims; c++)
+		{
+			d.velocities[k * dims + c] = 0;
+			d.previous_velocities[k * dims + c] = 0;
+			d.states.orientation[k * dims + c] = 0;
+		}
+		d.flags[k] = cell_state_flag::none;
+		d.is_movable[k] = 1;
+		d.states.simple_pressure[k] = 0;
+		d.states.number_of_nuclei[k] = 1;
+		d.states.damage[k] = 0;
+		d.states.total_attack_time[k] = 0;
+		d.origin[k] = -1; // not part of any upload
+		return k;
+	}
+
+	// The fields cell::copy_from hands from the parent to the daughter: the phenotype
+	// (phenotype.cpp:18-30 over data_storage_phenotype.cpp:388-527 and Death::copy,
+	// implicit_storage_phenotype.cpp:473-480), the cell definition index and the cell
+	// functions (cell.cpp:92-106).  Positions, velocities, state, flags and movability are
+	// the daughter's own.
+	static bool copied_on_division(int f)
+	{
+		switch (f)
+		{
+			case PCM_F_POSITIONS:
+			case PCM_F_VELOCITIES:
+			case PCM_F_PREVIOUS_VELOCITIES:
+			case PCM_F_SIMPLE_PRESSURE:
+			case PCM_F_NUMBER_OF_NUCLEI:
+			case PCM_F_DAMAGE:
+			case PCM_F_TOTAL_ATTACK_TIME:
+			case PCM_F_IS_MOVABLE:
+			case PCM_F_FLAGS:
+			case PCM_F_ORIGIN:
+			case PCM_F_ORIENTATION:
+				return false;
+			default:
+				return true;
+		}
+	}
+
+	// cell::divide (cell.cpp:108-163) of `parent` into the fresh agent `daughter`
+	template <int D>
+	void divide(env_t& e, index_t parent, index_t daughter, std::uint64_t epoch, bool compute_internalized)
+	{
+		auto& d = e.data;
+		const index_t subs = e.m.substrates_count;
+		strip_links(parent, d.states.springs);
+		strip_links(parent, d.states.attached_cells);
+		d.states.springs[parent].clear();
+		d.states.attached_cells[parent].clear();
+		if (compute_internalized) // molecular_t::divide (data_storage_phenotype.cpp:552-558)
+			for (index_t s = 0; s < subs; s++)
+				d.agent_data.internalized_substrates[parent * subs + s] *= (R)0.5;
+		d.states.damage[parent] = 0;
+		d.states.total_attack_time[parent] = 0;
+
+		R walk[D];
+		free_walk<D>(e, epoch, parent, 0, walk);
+		const R* o = d.states.orientation.data() + parent * D;
+		R tmp = 0;
+		for (int k = 0; k < D; k++)
+			tmp += walk[k] * o[k];
+		for (int k = 0; k < D; k++)
+			walk[k] = (walk[k] - tmp * o[k] * d.geometries.polarity[parent]) * d.geometries.radius[parent];
+
+		// volume_t::divide = multiply_by_factor(0.5) (data_storage_phenotype.cpp:530-550)
+		for (auto* v : { &d.agent_data.volumes, &d.volumes.solid, &d.volumes.fluid, &d.volumes.nuclear,
+						 &d.volumes.nuclear_fluid, &d.volumes.nuclear_solid, &d.volumes.cytoplasmic,
+						 &d.volumes.cytoplasmic_fluid, &d.volumes.cytoplasmic_solid, &d.volumes.target_solid_cytoplasmic,
+						 &d.volumes.target_solid_nuclear })
+			(*v)[parent] *= (R)0.5;
+		update_geometry(parent, d.geometries.radius.data(), d.geometries.nuclear_radius.data(),
+						d.geometries.surface_area.data(), d.agent_data.volumes.data(), d.volumes.nuclear.data());
+
+		for_each_field(d, D, e.cell_definitions_count, subs, [&](int f, auto& v, index_t w) {
+			if (copied_on_division(f))
+				for (index_t c = 0; c < w; c++)
+					v[daughter * w + c] = v[parent * w + c];
+		});
+		free_walk<D>(e, epoch, parent, 1, d.states.orientation.data() + daughter * D); // assign_orientation
+
+		R* x = d.agent_data.positions.data();
+		for (int k = 0; k < D; k++)
+		{
+			x[daughter * D + k] = x[parent * D + k] + walk[k];
+			x[parent * D + k] -= walk[k] * (R)0.5;
+		}
+	}
+
+	// containers_solver::update_cell_container_for_phenotype (containers_solver.cpp:149-187),
+	// single-threaded: divisions in ascending parent order, then the removal loop over the
+	// agents that existed before.  release_internalized_substrates (paraBioFVM) is not part
+	// of the restatement.
+	void update_cell_container_for_phenotype(env_t& e, bool compute_internalized)
+	{
+		auto& d = e.data;
+		const index_t n = d.agents_count;
+		const std::uint64_t epoch = e.division_epoch++;
+		e.division_log.clear();
+		for (index_t i = 0; i < n; i++)
+		{
+			if (d.flags[i] != cell_state_flag::to_divide)
+				continue;
+			d.flags[i] = cell_state_flag::none;
+			const index_t k = create(e);
+			e.divisions_count++;
+			e.division_log.push_back(i);
+			dispatch(e, [&]<int D>() { divide<D>(e, i, k, epoch, compute_internalized); });
+		}
+		for (index_t i = 0; i < n; i++)
+			remove_single(e, i);
 	}
 
 	// containers_solver::update_cell_container_for_mechanics (containers_solver.cpp:125-147)

@@ -40,6 +40,9 @@ struct oracle_base
 	virtual void set_params(const pcm_step_params* p) = 0;
 	virtual int set_gradients(const int32_t* voxel, const void* g) = 0;
 	virtual int ready() = 0;
+	virtual void phenotype(bool compute_internalized) = 0;
+	virtual index_t divisions() const = 0;
+	virtual const std::vector<index_t>& division_log() const = 0;
 	virtual void phase(int which) = 0;
 	virtual void step(int steps) = 0;
 };
@@ -149,6 +152,13 @@ struct oracle_impl final : oracle_base
 		std::memcpy(env.m.gradients.data(), g, sizeof(R) * env.m.gradients.size());
 		return PCM_OK;
 	}
+
+	void phenotype(bool compute_internalized) override
+	{
+		s.update_cell_container_for_phenotype(env, compute_internalized);
+	}
+	index_t divisions() const override { return env.divisions_count; }
+	const std::vector<index_t>& division_log() const override { return env.division_log; }
 
 	// chemotaxis reads gradients: refuse to step without them
 	int ready() override
@@ -438,6 +448,34 @@ ORACLE_PHASE(update_positions, PH_POSITIONS)
 ORACLE_PHASE(update_cell_cell_interactions, PH_INTERACTIONS)
 ORACLE_PHASE(update_cell_container_for_mechanics, PH_CONTAINER)
 ORACLE_PHASE(evaluate_interactions, PH_CONTACT)
+
+int oracle_update_cell_container_for_phenotype(void* h, const pcm_step_params* p, int32_t compute_internalized)
+{
+	if (p)
+		cast(h)->set_params(p);
+	cast(h)->phenotype(compute_internalized != 0);
+	return PCM_OK;
+}
+
+int oracle_get_divisions_count(void* h, int64_t* d)
+{
+	*d = cast(h)->divisions();
+	return PCM_OK;
+}
+
+int oracle_division_log(void* h, int32_t* parents, int64_t capacity, int64_t* length)
+{
+	const auto& log = cast(h)->division_log();
+	*length = (int64_t)log.size();
+	if (parents)
+	{
+		if (capacity < (int64_t)log.size())
+			return fail(PCM_ERR_CAPACITY, "division log longer than the buffer");
+		for (std::size_t k = 0; k < log.size(); k++)
+			parents[k] = log[k];
+	}
+	return PCM_OK;
+}
 
 int oracle_set_gradients(void* h, const int32_t* voxel, const void* g)
 {

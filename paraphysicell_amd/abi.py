@@ -87,6 +87,8 @@ FIELDS = [
     ("chemotaxis_direction", T_I32, W_ONE),
     ("chemotactic_sensitivities", T_REAL, W_SUBS),
     ("migration_bias_direction_function", T_U8, W_ONE),
+    ("orientation", T_REAL, W_DIMS),
+    ("polarity", T_REAL, W_ONE),
 ]
 CONTACT_NONE = 0  # PCM_CONTACT_NONE
 CONTACT_STANDARD_ELASTIC = 1  # PCM_CONTACT_STANDARD_ELASTIC

@@ -36,17 +36,18 @@ struct Scalars
 	unsigned long long step; // mechanics step index (keys the random stream)
 	unsigned long long claim; // voxel claims of the mesh update: agents << 32 | voxels (zero between updates)
 	int n_binned;          // agents binned by the last mesh update (for the sparse clear)
-	int pad0;
+	int div_count;         // divisions planned by the last phenotype-cadence update
 	int cand_count;        // interaction candidate slots collected this step
 	int conflict;          // interaction speculation found a cross-agent dependency
 	int bad_count;         // agents the container update removes this step
 	int max_neighbors;     // largest neighbor count seen by the last neighbor build
 	int moves;             // agents moved by the container update
 	long long deaths;      // environment::deaths_count increments
+	long long divisions;   // environment::divisions_count increments
 	int n_occ;             // occupied mechanics voxels of the last mesh update
 	int order_valid;       // bucket[0, n) is a permutation of the current agents
 	int log_len;           // entries in the removal log (may exceed log_cap: log lost)
-	int pad;
+	int compact_limit;     // > 0: the removal loop visits agents below it only
 	// diagnostics of the last step (pcm_get_stats): attachment events, resolver rounds,
 	// resolver chunks, interaction candidates, speculation conflict, removals, occupied voxels
 	int stat[8];
@@ -70,6 +71,8 @@ struct SimCtx
 	int wall;
 	int springs;
 	int contacts;   // some uploaded agent has a contact function
+	unsigned long long div_epoch; // phenotype-update call: keys the division draws
+	int div_internal;             // divisions halve internalized substrates
 	int chemotaxis; // some uploaded agent has a migration bias direction function
 	// substrate gradients on the diffusion grid (pcm_set_gradients), or null
 	const R* grad;
@@ -101,6 +104,8 @@ struct SimCtx
 	int *chemo_index, *chemo_dir;
 	R* sensitivities;
 	uint8_t* bias_fn; // PCM_BIAS_* code per agent
+	R *orientation, *polarity;
+	int* div_list; // dividing agents of the last phenotype update, ascending
 
 	// link lists
 	int *spr_cnt, *spr_idx, *atc_cnt, *atc_idx;

@@ -1655,6 +1655,60 @@ __device__ inline void block_sort_ints(int* a, int count)
 		}
 }
 
+// The removal loop of update_cell_container_for_phenotype (containers_solver.cpp:182-186)
+// visits only i < limit: a bad agent at or above it is removed only when, as the last
+// agent, it is moved into a visited hole; otherwise it stays.  `list` holds the bad agents
+// ascending.  Single thread.
+template <class R>
+__device__ void compact_plan_limited(const SimCtx<R>& c, const int* list, int b, int limit)
+{
+	const int n0 = c.sc->n;
+	int log = c.sc->log_len;
+	int top = n0, tb = b - 1, q = 0, moves = 0, removed = 0;
+	long long deaths = 0;
+	for (int h = 0; h < b; h++)
+	{
+		const int hole = list[h];
+		if (hole >= limit || hole >= top)
+			break;
+		int victim = hole;
+		for (;;)
+		{
+			c.remap[victim] = -1 - q++;
+			deaths += c.flags[victim] == PCM_FLAG_TO_REMOVE;
+			removed++;
+			if (log < c.log_cap)
+				c.rm_log[log] = hole;
+			log++;
+			const int t = --top;
+			if (t == hole)
+				break; // the hole held the last agent
+			while (tb >= 0 && list[tb] > t)
+				tb--;
+			if (tb >= 0 && list[tb] == t)
+			{
+				victim = t; // the last agent is bad too: it moves into the hole and goes
+				tb--;
+				continue;
+			}
+			c.move_src[moves] = t;
+			c.move_dst[moves] = hole;
+			c.remap[t] = hole;
+			moves++;
+			break;
+		}
+		if (top == hole)
+			break; // the count fell to the visited index: the loop ends
+	}
+	c.sc->log_len = log;
+	c.sc->moves = moves;
+	c.sc->stat[5] = removed;
+	c.sc->stat[6] = c.sc->n_occ;
+	c.sc->n = top;
+	c.sc->deaths += deaths;
+	c.sc->order_valid = 0;
+}
+
 template <class R>
 __device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
 {
@@ -1682,6 +1736,12 @@ __device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
 	if (threadIdx.x != 0)
 		return;
 	const int n0 = c.sc->n;
+	const int limit = c.sc->compact_limit;
+	if (limit != 0)
+	{
+		compact_plan_limited<R>(c, list, b, limit < 0 ? 0 : limit);
+		return;
+	}
 	const int n1 = n0 - b;
 	long long deaths = 0;
 	for (int t = 0; t < b; t++)
@@ -1898,6 +1958,199 @@ __global__ void k_compact_done(const SimCtx<R>* __restrict__ ctxs)
 	{
 		c.sc->bad_count = 0;
 		c.sc->moves = 0;
+		c.sc->compact_limit = 0;
+	}
+}
+
+// ------------------------------------------------------------------ phenotype-cadence update
+// update_cell_container_for_phenotype (containers_solver.cpp:149-187) in the reference's
+// single-thread order: divisions in ascending parent order (cell::divide, cell.cpp:108-163),
+// then the removal loop over the agents that existed before (compact_plan_block with
+// compact_limit).
+
+// the agents flagged to_divide, ascending, and their count; one block of 1024
+template <class R>
+__global__ void __launch_bounds__(1024) k_divide_plan(const SimCtx<R>* __restrict__ ctxs)
+{
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	__shared__ int wave_total[16];
+	__shared__ int base;
+	if (c.sc->status)
+		return;
+	const int n = c.sc->n;
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (threadIdx.x == 0)
+		base = 0;
+	__syncthreads();
+	for (int start = 0; start < n; start += 1024)
+	{
+		const int i = start + threadIdx.x;
+		const bool div = i < n && c.flags[i] == PCM_FLAG_TO_DIVIDE;
+		const unsigned long long m = __ballot(div);
+		if (lane == 0)
+			wave_total[w] = __popcll(m);
+		__syncthreads();
+		int off = base;
+		for (int k = 0; k < w; k++)
+			off += wave_total[k];
+		if (div)
+			c.div_list[off + __popcll(m & ((1ull << lane) - 1ull))] = i;
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			int t = 0;
+			for (int k = 0; k < 16; k++)
+				t += wave_total[k];
+			base += t;
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		c.sc->div_count = base;
+}
+
+// position_helper<D>::random_walk(false, walk) (solver_helper.h:82-86, 160-165, 250-267),
+// keyed by the phenotype-update call, the parent and a slot
+template <int D, class R>
+__device__ __forceinline__ void division_walk(const SimCtx<R>& c, int parent, uint32_t slot, R* walk)
+{
+	const pcm_u32x4 blk = pcm_block(c.seed, c.div_epoch, PCM_STREAM_DIVISION, (uint32_t)parent, slot, 0u);
+	if constexpr (D == 1)
+		walk[0] = pcm_draw<R>::from(blk, 0) < (R)0.5 ? (R)-1 : (R)1;
+	else
+	{
+		const R two_pi = (R)2 * (R)3.14159265358979323846;
+		const R theta = pcm_scale_uniform<R>(pcm_draw<R>::from(blk, 0), (R)0, two_pi);
+		if constexpr (D == 2)
+		{
+			walk[0] = cos(theta);
+			walk[1] = sin(theta);
+		}
+		else
+		{
+			const R z = pcm_scale_uniform<R>(pcm_draw<R>::from(blk, 1), (R)-1, (R)1);
+			const R r = sqrt(1 - z * z);
+			walk[0] = cos(theta) * r;
+			walk[1] = sin(theta) * r;
+			walk[2] = z;
+		}
+	}
+}
+
+// the fields cell::copy_from hands to the daughter: phenotype (phenotype.cpp:18-30), cell
+// definition index and cell functions (cell.cpp:92-106)
+__device__ __forceinline__ bool copied_on_division(int f)
+{
+	return !(f == PCM_F_POSITIONS || f == PCM_F_VELOCITIES || f == PCM_F_PREVIOUS_VELOCITIES
+			 || f == PCM_F_SIMPLE_PRESSURE || f == PCM_F_NUMBER_OF_NUCLEI || f == PCM_F_DAMAGE
+			 || f == PCM_F_TOTAL_ATTACK_TIME || f == PCM_F_IS_MOVABLE || f == PCM_F_FLAGS || f == PCM_F_ORIGIN
+			 || f == PCM_F_ORIENTATION);
+}
+
+// cell::divide of division r: parent div_list[r], daughter at n + r
+template <class R, int D>
+__global__ void k_divide_apply(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int r = gid();
+	if (r >= c.sc->div_count)
+		return;
+	const int p = c.div_list[r];
+	const int d = c.sc->n + r;
+	c.flags[p] = PCM_FLAG_NONE;
+	// the parent leaves its partners' lists in division order (cell.cpp:111-112)
+	c.remap[p] = -1 - r;
+	c.spr_cnt[p] = 0;
+	c.atc_cnt[p] = 0;
+	if (c.div_internal)
+		for (int s = 0; s < c.S; s++)
+			c.internal[(size_t)p * c.S + s] *= (R)0.5;
+	c.damage[p] = 0;
+	c.attack_time[p] = 0;
+
+	R walk[D];
+	division_walk<D, R>(c, p, 0u, walk);
+	R tmp = 0;
+	for (int k = 0; k < D; k++)
+		tmp += walk[k] * c.orientation[(size_t)p * D + k];
+	const R pol = c.polarity[p], rad = c.radius[p];
+	for (int k = 0; k < D; k++)
+		walk[k] = (walk[k] - tmp * c.orientation[(size_t)p * D + k] * pol) * rad;
+
+	// volume_t::divide (data_storage_phenotype.cpp:530-550), geometry_t::update
+	R* halves[11] = { c.volume, c.solid, c.fluid, c.nuclear, c.nuc_fluid, c.nuc_solid,
+					  c.cyto, c.cyto_fluid, c.cyto_solid, c.target_cyto, c.target_nuc };
+	for (int k = 0; k < 11; k++)
+		halves[k][p] *= (R)0.5;
+	update_geometry<R>(c, p);
+
+	for (int f = 0; f < PCM_FIELD_COUNT; f++)
+	{
+		if (!copied_on_division(f))
+			continue;
+		const int nb = c.fbytes[f];
+		const uint8_t* src = (const uint8_t*)c.fptr[f] + (size_t)p * nb;
+		uint8_t* dst = (uint8_t*)c.fptr[f] + (size_t)d * nb;
+		if ((nb & 3) == 0)
+			for (int k = 0; k < nb / 4; k++)
+				((uint32_t*)dst)[k] = ((const uint32_t*)src)[k];
+		else
+			for (int k = 0; k < nb; k++)
+				dst[k] = src[k];
+	}
+	for (int k = 0; k < D; k++)
+	{
+		c.vel[(size_t)d * D + k] = 0;
+		c.pvel[(size_t)d * D + k] = 0;
+	}
+	c.pressure[d] = 0;
+	c.nuclei[d] = 1;
+	c.damage[d] = 0;
+	c.attack_time[d] = 0;
+	c.movable[d] = 1;
+	c.flags[d] = PCM_FLAG_NONE;
+	c.origin[d] = -1;
+	c.spr_cnt[d] = 0;
+	c.atc_cnt[d] = 0;
+	c.nbr_cnt[d] = 0;
+	division_walk<D, R>(c, p, 1u, c.orientation + (size_t)d * D); // cell::assign_orientation
+	for (int k = 0; k < D; k++)
+	{
+		const R x = c.pos[(size_t)p * D + k];
+		c.pos[(size_t)d * D + k] = x + walk[k];
+		c.pos[(size_t)p * D + k] = x - walk[k] * (R)0.5;
+	}
+}
+
+// partners of dividing agents drop them, in division order (remove_attached, swap with back)
+template <class R>
+__global__ void k_divide_links(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int i = gid();
+	if (i >= c.sc->n || c.sc->div_count == 0 || c.remap[i] < 0)
+		return;
+	edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
+	edit_links(c.atc_idx + (size_t)i * c.LA, c.atc_cnt + i, c.remap);
+}
+
+// restore the parents' remap, count the daughters in, and bound the removal loop that
+// follows to the agents that existed before
+template <class R>
+__global__ void k_divide_done(const SimCtx<R>* __restrict__ ctxs)
+{
+	PCM_PROLOGUE(c)
+	const int r = gid();
+	const int count = c.sc->div_count;
+	if (r < count)
+		c.remap[c.div_list[r]] = c.div_list[r];
+	if (r == 0)
+	{
+		const int n = c.sc->n;
+		c.sc->compact_limit = n > 0 ? n : -1;
+		c.sc->n = n + count;
+		c.sc->divisions += count;
+		c.sc->order_valid = 0;
 	}
 }
 

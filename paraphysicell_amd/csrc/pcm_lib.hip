@@ -78,6 +78,9 @@ struct pcm_sim
 	virtual int get_buckets(int32_t* counts, int32_t* agents) = 0;
 	virtual long long voxel_count() const = 0;
 	virtual int get_deaths(int64_t* d) = 0;
+	virtual int get_divisions(int64_t* d) = 0;
+	virtual int division_log(int32_t* parents, int64_t capacity, int64_t* length) = 0;
+	virtual int phenotype_update(const pcm_step_params* p, int compute_internalized) = 0;
 	virtual int set_step(uint64_t s) = 0;
 	virtual int get_step(uint64_t* s) = 0;
 	virtual int set_gradients(const int32_t* voxel, const void* g) = 0;
@@ -241,6 +244,7 @@ public:
 		PCM_TRY(alloc_t(&h.remap, cap));
 		PCM_TRY(alloc_t(&h.move_src, cap));
 		PCM_TRY(alloc_t(&h.move_dst, cap));
+		PCM_TRY(alloc_t(&h.div_list, cap));
 		h.log_cap = (int)std::min<long long>(0x7ffffff0LL, 4LL * cap + 64);
 		PCM_TRY(alloc_t(&h.rm_log, (size_t)h.log_cap));
 		(void)dims;
@@ -564,6 +568,73 @@ public:
 		return PCM_OK;
 	}
 
+	int get_divisions(int64_t* d) override
+	{
+		PCM_TRY(read_scalars());
+		*d = sc_.divisions;
+		return PCM_OK;
+	}
+
+	int division_log(int32_t* parents, int64_t capacity, int64_t* length) override
+	{
+		*length = div_log_len_;
+		if (!parents || div_log_len_ == 0)
+			return PCM_OK;
+		if (capacity < div_log_len_)
+			return fail(PCM_ERR_CAPACITY, "division log longer than the buffer");
+		PCM_HIP(hipStreamSynchronize(stream_));
+		PCM_HIP(hipMemcpy(parents, h_.div_list, sizeof(int32_t) * div_log_len_, hipMemcpyDeviceToHost));
+		return PCM_OK;
+	}
+
+	// update_cell_container_for_phenotype (containers_solver.cpp:149-187): plan the
+	// divisions, check the capacity before anything changes, divide, then the removal loop
+	// over the agents that existed before
+	int phenotype_update(const pcm_step_params* p, int compute_internalized) override
+	{
+		if (p)
+			PCM_TRY(set_params(*p));
+		PCM_TRY(read_scalars());
+		if (sc_.status)
+			return fail(PCM_ERR_STATE, "device status word is set");
+		h_.div_epoch = division_epoch_;
+		h_.div_internal = compute_internalized != 0;
+		PCM_TRY(push_ctx());
+		PCM_TRY(launch(PCM_K_DIVIDE_PLAN, k_divide_plan<R>, 1, 1024));
+		PCM_TRY(read_scalars());
+		const long long total = (long long)sc_.n + sc_.div_count;
+		if (total > h_.cap)
+			return fail(PCM_ERR_CAPACITY, "divisions would exceed the handle's agent capacity");
+		division_epoch_++;
+		div_log_len_ = sc_.div_count;
+		n_launch_ = (int)total;
+		mesh_fresh_ = false;
+		records_fresh_ = false;
+		const int blocks = std::max(1, (sc_.div_count + BLOCK - 1) / BLOCK);
+		switch (h_.dims)
+		{
+			case 1:
+				PCM_TRY(launch(PCM_K_DIVIDE_APPLY, k_divide_apply<R, 1>, blocks, BLOCK));
+				break;
+			case 2:
+				PCM_TRY(launch(PCM_K_DIVIDE_APPLY, k_divide_apply<R, 2>, blocks, BLOCK));
+				break;
+			default:
+				PCM_TRY(launch(PCM_K_DIVIDE_APPLY, k_divide_apply<R, 3>, blocks, BLOCK));
+		}
+		PCM_TRY(per_agent(PCM_K_DIVIDE_LINKS, k_divide_links<R>));
+		PCM_TRY(launch(PCM_K_COMPACT_DONE, k_divide_done<R>, blocks, BLOCK));
+		switch (h_.dims)
+		{
+			case 1:
+				return container_kernels<1>();
+			case 2:
+				return container_kernels<2>();
+			default:
+				return container_kernels<3>();
+		}
+	}
+
 	int set_step(uint64_t s) override
 	{
 		PCM_TRY(read_scalars());
@@ -713,6 +784,8 @@ private:
 	int grid_L_ = 0; // spring-slot bound of grid sizes (0: h_.L)
 	uint64_t ctx_version_ = 0; // increments whenever the device context changes
 	size_t grad_count_ = 0;    // reals in h_.grad
+	uint64_t division_epoch_ = 0; // phenotype-update calls so far
+	int div_log_len_ = 0;         // divisions of the last phenotype update
 	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
@@ -822,6 +895,8 @@ private:
 		h.chemo_dir = (int*)F(PCM_F_CHEMOTAXIS_DIRECTION);
 		h.sensitivities = (R*)F(PCM_F_CHEMOTACTIC_SENSITIVITIES);
 		h.bias_fn = (uint8_t*)F(PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION);
+		h.orientation = (R*)F(PCM_F_ORIENTATION);
+		h.polarity = (R*)F(PCM_F_POLARITY);
 	}
 
 	int alloc_neighbor_storage()
@@ -1977,6 +2052,31 @@ int pcm_get_voxel_buckets(pcm_sim* sim, int32_t* counts, int32_t* agents)
 	if (!counts || !agents)
 		return fail(PCM_ERR_ARGUMENT, "null output");
 	return sim->get_buckets(counts, agents);
+}
+
+int pcm_update_cell_container_for_phenotype(pcm_sim* sim, const pcm_step_params* params,
+											int32_t compute_internalized_substrates)
+{
+	PCM_NEED(sim);
+	if (!params)
+		return fail(PCM_ERR_ARGUMENT, "null params");
+	return sim->phenotype_update(params, compute_internalized_substrates);
+}
+
+int pcm_get_divisions_count(pcm_sim* sim, int64_t* divisions)
+{
+	PCM_NEED(sim);
+	if (!divisions)
+		return fail(PCM_ERR_ARGUMENT, "null output");
+	return sim->get_divisions(divisions);
+}
+
+int pcm_division_log(pcm_sim* sim, int32_t* parents, int64_t capacity, int64_t* length)
+{
+	PCM_NEED(sim);
+	if (!length)
+		return fail(PCM_ERR_ARGUMENT, "null length");
+	return sim->division_log(parents, capacity, length);
 }
 
 int pcm_get_deaths_count(pcm_sim* sim, int64_t* deaths)

@@ -11,6 +11,8 @@
 //   spring detachment          position_solver.cpp:386                         -> DETACH
 //   spring attachment          position_solver.cpp:435                         -> ATTACH
 //   cell-cell interactions     interactions_solver.cpp:167                     -> INTERACT
+//   cell::divide, assign_orientation  cell.cpp:134-139,172-184                 -> DIVISION
+//     (keyed by the phenotype-update call in place of the mechanics step)
 //
 // Header-only; compiles as host C++ (g++) and as HIP device code (hipcc).
 #pragma once
@@ -29,6 +31,7 @@ enum pcm_stream : uint32_t
 	PCM_STREAM_DETACH = 2,
 	PCM_STREAM_ATTACH = 3,
 	PCM_STREAM_INTERACT = 4,
+	PCM_STREAM_DIVISION = 5,
 };
 
 struct pcm_u32x4

@@ -60,6 +60,9 @@ def _declare(lib, prefix: str):
         "update_positions": [P, ctypes.POINTER(abi.StepParams)],
         "update_cell_cell_interactions": [P, ctypes.POINTER(abi.StepParams)],
         "update_cell_container_for_mechanics": [P, ctypes.POINTER(abi.StepParams)],
+        "update_cell_container_for_phenotype": [P, ctypes.POINTER(abi.StepParams), ctypes.c_int32],
+        "get_divisions_count": [P, ctypes.POINTER(ctypes.c_int64)],
+        "division_log": [P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
         "step": [P, ctypes.POINTER(abi.StepParams), ctypes.c_int32],
     }
     for name, args in sig.items():
@@ -210,6 +213,20 @@ class _LibSim:
         return d.value
 
     @property
+    def divisions_count(self) -> int:
+        d = ctypes.c_int64()
+        self._call("get_divisions_count", self._h, ctypes.byref(d))
+        return d.value
+
+    def division_log(self) -> np.ndarray:
+        """Parents of the last phenotype update's divisions, in division order."""
+        length = ctypes.c_int64()
+        self._call("division_log", self._h, None, 0, ctypes.byref(length))
+        out = np.zeros(max(length.value, 1), np.int32)
+        self._call("division_log", self._h, out.ctypes.data, length.value, ctypes.byref(length))
+        return out[: length.value]
+
+    @property
     def step_index(self) -> int:
         s = ctypes.c_uint64()
         self._call("get_step_index", self._h, ctypes.byref(s))
@@ -257,6 +274,13 @@ class _LibSim:
 
     def update_cell_container_for_mechanics(self, params=None):
         self._phase("update_cell_container_for_mechanics", params)
+
+    def update_cell_container_for_phenotype(self, params=None, compute_internalized_substrates=False):
+        """Divisions of agents flagged to divide, then the removal loop
+        (src/solver/host/containers_solver.cpp:149-187)."""
+        p = (params or abi.Params()).to_c()
+        self._call("update_cell_container_for_phenotype", self._h, ctypes.byref(p),
+                   1 if compute_internalized_substrates else 0)
 
     def initialize_mesh(self):
         """simulator::initialize tail (src/simulator.cpp:47-48): mesh, then neighbors."""
@@ -462,7 +486,8 @@ class Solver:
         self.position = _Facade(sim, ["update_cell_forces", "update_cell_neighbors", "update_motility",
                                       "update_basement_membrane_interactions", "update_spring_attachments",
                                       "update_positions"])
-        self.containers = _Facade(sim, ["update_mechanics_mesh", "update_cell_container_for_mechanics"])
+        self.containers = _Facade(sim, ["update_mechanics_mesh", "update_cell_container_for_mechanics",
+                                        "update_cell_container_for_phenotype"])
         self.interactions = _Facade(sim, ["update_cell_cell_interactions"])
         # the free function evaluate_interactions(environment&) of src/simulator.cpp:65-82
         self.evaluate_interactions = sim.evaluate_interactions

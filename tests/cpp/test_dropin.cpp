@@ -31,6 +31,14 @@ void pcm_contact_functions(environment<R>& e, std::vector<uint8_t>& codes)
 		codes[i] = e.data.contact_function[i];
 }
 
+// c.create() for the stand-in: one more agent; the device fields arrive with the download
+template <class R>
+void pcm_host_divide(environment<R>& e, int parent)
+{
+	(void)parent;
+	e.resize(e.data.agents_count + 1);
+}
+
 template <class R>
 void pcm_bias_direction_functions(environment<R>& e, std::vector<uint8_t>& codes)
 {
@@ -348,6 +356,83 @@ void population_steps(int dims, int n, int steps, bool fused)
 	pcm_dropin::release(dev);
 }
 
+// update_cell_container_for_phenotype through the drop-in against the oracle's, then steps
+template <class R>
+void phenotype_update(int dims, int n)
+{
+	const double tol = sizeof(R) == 4 ? 1e-5 : 1e-12;
+	pcm_mesh_desc md {};
+	md.dims = dims;
+	for (int k = 0; k < 3; k++)
+	{
+		md.bbox_min[k] = -150;
+		md.bbox_max[k] = 150;
+		md.voxel[k] = 30;
+	}
+	environment<R> ref(md, 2, 1), dev(md, 2, 1);
+	for (auto* e : { &ref, &dev })
+	{
+		fill_population(*e, n, 77 + dims);
+		for (int i = 0; i < n; i += 20)
+			e->data.flags[i] = cell_state_flag::to_divide;
+		for (int i = 0; i < n; i++)
+		{
+			e->data.geometries.polarity[i] = (R)(i % 3 == 0);
+			for (int k = 0; k < dims; k++)
+				e->data.states.orientation[i * dims + k] = (R)(k == i % dims);
+		}
+		e->mechanics_time_step = (R)0.1;
+		e->virtual_wall_at_domain_edges = true;
+		e->automated_spring_adhesion = true;
+	}
+	ref.rng.mode = rng_mode::keyed;
+	ref.rng.seed = 5;
+	pcm_dropin::set_seed(dev, 5);
+
+	solver<R> oracle;
+	oracle.initialize(ref);
+	oracle.update_mechanics_mesh(ref);
+	oracle.update_cell_neighbors(ref);
+	pcm_dropin::solver<environment<R>> s;
+	s.initialize(dev);
+	s.containers.update_mechanics_mesh(dev);
+	s.position.update_cell_neighbors(dev);
+
+	oracle.update_cell_container_for_phenotype(ref, false);
+	int cell_solver_stub = 0;
+#pragma omp parallel
+	s.containers.update_cell_container_for_phenotype(dev, cell_solver_stub);
+	for (int step = 0; step < 3; step++)
+	{
+		pcm_dropin::sync_to_host(dev);
+		auto& a = dev.data;
+		auto& b = ref.data;
+		EXPECT(a.agents_count == b.agents_count, "phenotype dims %d step %d: count %d vs %d", dims, step,
+			   a.agents_count, b.agents_count);
+		EXPECT(dev.divisions_count == ref.divisions_count, "phenotype dims %d: divisions %d vs %d", dims,
+			   dev.divisions_count, ref.divisions_count);
+		if (a.agents_count != b.agents_count)
+			break;
+		const size_t nn = (size_t)b.agents_count;
+		EXPECT(rel_err(a.agent_data.positions, b.agent_data.positions, nn * dims) <= tol,
+			   "phenotype dims %d step %d positions", dims, step);
+		EXPECT(rel_err(a.agent_data.volumes, b.agent_data.volumes, nn) <= tol, "phenotype dims %d step %d volumes",
+			   dims, step);
+		EXPECT(rel_err(a.states.orientation, b.states.orientation, nn * dims) <= tol,
+			   "phenotype dims %d step %d orientation", dims, step);
+		bool lists_equal = true;
+		for (size_t i = 0; i < nn; i++)
+			lists_equal = lists_equal && a.states.springs[i] == b.states.springs[i]
+						  && a.states.attached_cells[i] == b.states.attached_cells[i];
+		EXPECT(lists_equal, "phenotype dims %d step %d: link lists differ", dims, step);
+		oracle.step(ref);
+		s.simulate_mechanics(dev, 1);
+	}
+	std::printf("phenotype dims %d %s: %d agents, %d divisions\n", dims, sizeof(R) == 4 ? "FP32" : "FP64",
+				dev.data.agents_count, dev.divisions_count);
+	pcm_dropin::release(dev);
+}
+
 int main()
 {
 	for (int dims = 1; dims <= 3; dims++)
@@ -360,6 +445,11 @@ int main()
 		population_steps<float>(dims, dims == 3 ? 1500 : 700, 4, false);
 		population_steps<float>(dims, dims == 3 ? 1500 : 700, 4, true);
 		population_steps<double>(dims, dims == 3 ? 1500 : 700, 3, false);
+	}
+	for (int dims = 1; dims <= 3; dims++)
+	{
+		phenotype_update<float>(dims, dims == 3 ? 1200 : 500);
+		phenotype_update<double>(dims, dims == 3 ? 1200 : 500);
 	}
 	std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
 	return failures ? 1 : 0;

@@ -207,3 +207,59 @@ def test_deterministic():
         runs.append(d)
     parity.assert_same_state(runs[0], runs[1], tol=0.0, what="rerun")
     parity.assert_same_neighbors(runs[0], runs[1], "rerun")
+
+
+def _flag_divisions(st, seed):
+    """5% of the agents divide, 2% are flagged for removal; random unit orientations and
+    polarity 0 or 1."""
+    n, dims = st.count, st.dims
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    flags = np.zeros(n, np.uint8)
+    flags[perm[: n // 20]] = 2  # PCM_FLAG_TO_DIVIDE
+    flags[perm[n // 20: n // 20 + n // 50]] = 1  # PCM_FLAG_TO_REMOVE
+    st["flags"] = flags
+    o = rng.normal(size=(n, dims))
+    st["orientation"] = (o / np.linalg.norm(o, axis=1, keepdims=True)).ravel()
+    st["polarity"] = rng.integers(0, 2, n).astype(float)
+
+
+@pytest.mark.parametrize("real_bytes", [4, 8])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_phenotype_container_update(dims, real_bytes):
+    """pcm_update_cell_container_for_phenotype against the oracle: divisions (daughters at
+    the box edge included), the removal loop over the old agents, the division log and
+    counters; then fused steps from the new population."""
+    n = {1: 200, 2: 800, 3: 1500}[dims]
+    mesh, st, p = parity.rich_population(dims, n, 60 + dims, real_bytes, rates="busy")
+    _flag_divisions(st, 70 + dims)
+    dev, ora = parity.make_pair(mesh, st, seed=4, capacity=2 * n)
+    for s in (dev, ora):
+        s.initialize_mesh()
+        s.update_cell_container_for_phenotype(p, compute_internalized_substrates=real_bytes == 8)
+    parity.assert_same_state(dev, ora, what="phenotype update")
+    assert np.array_equal(dev.division_log(), ora.division_log())
+    assert dev.divisions_count == ora.divisions_count == n // 20
+    assert dev.deaths_count == ora.deaths_count
+    for step in range(2):
+        dev.step(p, 1)
+        ora.step(p, 1)
+        parity.assert_same_state(dev, ora, what=f"step {step} after the phenotype update")
+        parity.assert_same_neighbors(dev, ora, f"step {step} after the phenotype update")
+        dev.upload(ora.download())
+        dev.step_index = ora.step_index
+        dev.initialize_mesh()
+
+
+def test_phenotype_update_capacity_is_checked_first():
+    """Divisions beyond the handle's capacity fail before any agent changes."""
+    mesh, st, p = parity.rich_population(3, 400, 81, 4, rates="rare")
+    _flag_divisions(st, 82)
+    dev = device(mesh, st, seed=1, capacity=410)
+    dev.initialize_mesh()
+    before = dev.download()
+    with pytest.raises(Exception, match="capacity"):
+        dev.update_cell_container_for_phenotype(p)
+    after = dev.download()
+    for name, _, _ in abi.FIELDS:
+        assert np.array_equal(before.arrays[name], after.arrays[name]), name

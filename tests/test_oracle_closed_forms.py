@@ -490,6 +490,174 @@ def test_chemotaxis_needs_gradients():
         sim.update_motility(abi.Params(dt=0.1))
 
 
+# ------------------------------------------------------------------ division (F1)
+VOLUME_FIELDS = ["volume", "solid", "fluid", "nuclear", "nuclear_fluid", "nuclear_solid", "cytoplasmic",
+                 "cytoplasmic_fluid", "cytoplasmic_solid", "target_solid_cytoplasmic", "target_solid_nuclear"]
+OWN_FIELDS = {"positions", "velocities", "previous_velocities", "simple_pressure", "number_of_nuclei", "damage",
+              "total_attack_time", "is_movable", "flags", "origin", "orientation"}
+
+
+def _division_case(dims, seed=12, n=80):
+    """Agents 40 um apart with random phenotypes, orientations and polarity 0 or 1;
+    every fourth agent flagged to divide; symmetric springs and attached cells among all."""
+    side = {1: n, 2: 10, 3: 5}[dims]
+    grid = np.stack(np.meshgrid(*[np.arange(side)] * dims, indexing="ij"), -1).reshape(-1, dims)[:n]
+    mesh, st = _state(dims, n, defs=2, subs=2, box=2000 if dims == 1 else 400, seed=seed)
+    rng = np.random.default_rng(seed)
+    st["positions"] = ((grid - (side - 1) / 2) * 40.0).ravel()
+    st["velocities"] = rng.normal(size=n * dims)
+    st["previous_velocities"] = rng.normal(size=n * dims)
+    for f in VOLUME_FIELDS:
+        st[f] = rng.uniform(500, 3000, n)
+    st["radius"] = rng.uniform(6, 11, n)
+    st["damage"] = rng.uniform(0, 5, n)
+    st["total_attack_time"] = rng.uniform(0, 5, n)
+    st["internalized_substrates"] = rng.uniform(0, 10, 2 * n)
+    st["cell_definition_index"] = rng.integers(0, 2, n)
+    st["is_movable"] = (rng.random(n) < 0.7).astype(np.uint8)
+    st["migration_speed"] = rng.uniform(0.1, 3, n)
+    st["attack_rates"] = rng.uniform(0, 1, 2 * n)
+    o = rng.normal(size=(n, dims))
+    st["orientation"] = (o / np.linalg.norm(o, axis=1, keepdims=True)).ravel()
+    st["polarity"] = rng.integers(0, 2, n).astype(float)
+    st["number_of_nuclei"] = 2
+    st["simple_pressure"] = 3.0
+    flags = np.zeros(n, np.uint8)
+    flags[::4] = 2  # PCM_FLAG_TO_DIVIDE
+    st["flags"] = flags
+    for which in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED):
+        lists = [[] for _ in range(n)]
+        for _ in range(2 * n):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            lists[a].append(b)
+            lists[b].append(a)
+        st.set_links(which, lists)
+    return mesh, st
+
+
+def _swap_with_back_removals(lists, gone_in_order):
+    """remove_attached (containers_solver.cpp:58-69) for each agent in turn, brute force."""
+    lists = [list(l) for l in lists]
+    for g in gone_in_order:
+        for partner in list(lists[g]):
+            l = lists[partner]
+            k = l.index(g)
+            l[k] = l[-1]
+            l.pop()
+        lists[g] = []
+    return lists
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_division_closed_form(dims):
+    """cell::divide (cell.cpp:108-163) through update_cell_container_for_phenotype
+    (containers_solver.cpp:149-187): daughters appended in ascending parent order; the
+    parent's eleven volumes halve, its geometry follows the halved volume, internalized
+    substrates halve, damage and attack time reset; the daughter carries the parent's
+    phenotype and its own zero velocities, pressure and damage, one nucleus, movable, a unit
+    orientation; it sits at x + w and the parent at x - w/2 with |w| = radius (polarity 0)
+    or w perpendicular to the orientation (polarity 1); both leave every link list."""
+    mesh, st = _division_case(dims)
+    n = st.count
+    sim = _sim(mesh, st)
+    sim.update_cell_container_for_phenotype(abi.Params(), compute_internalized_substrates=True)
+    parents = np.flatnonzero(st["flags"] == 2)
+    m = len(parents)
+    assert sim.count == n + m and sim.divisions_count == m
+    assert np.array_equal(sim.division_log(), parents)
+    out = {name: sim.get_field(name) for name, _, _ in abi.FIELDS}
+    view = lambda name, a: a.reshape(n + m, -1)  # noqa: E731
+    for f in VOLUME_FIELDS:
+        assert np.array_equal(out[f][parents], st[f][parents] * 0.5), f
+        assert np.array_equal(out[f][n:], st[f][parents] * 0.5), f
+    sphere = np.pi * 4.0 / 3.0
+    assert _close(out["radius"][parents], np.cbrt(st["volume"][parents] * 0.5 / sphere), 1e-12)
+    assert _close(out["nuclear_radius"][parents], np.cbrt(st["nuclear"][parents] * 0.5 / sphere), 1e-12)
+    assert _close(out["surface_area"][parents], 4 * np.pi * out["radius"][parents] ** 2, 1e-12)
+    ins = view("internalized_substrates", out["internalized_substrates"])
+    assert np.array_equal(ins[parents], st.view("internalized_substrates")[parents] * 0.5)
+    for f in ("damage", "total_attack_time", "simple_pressure"):
+        assert np.all(out[f][n:] == 0), f
+    assert np.all(out["damage"][parents] == 0) and np.all(out["total_attack_time"][parents] == 0)
+    assert np.all(out["number_of_nuclei"][n:] == 1) and np.all(out["is_movable"][n:] == 1)
+    assert np.all(out["flags"][: n + m] == 0) and np.all(out["origin"][n:] == -1)
+    assert np.all(view("velocities", out["velocities"])[n:] == 0)
+    assert np.all(view("previous_velocities", out["previous_velocities"])[n:] == 0)
+    for name, _, _ in abi.FIELDS:
+        if name not in OWN_FIELDS:  # the daughter copies the divided parent
+            a = view(name, out[name])
+            assert np.array_equal(a[n:], a[parents]), name
+    o_d = view("orientation", out["orientation"])[n:]
+    if dims > 1:  # a fresh draw (cell::assign_orientation), not the parent's
+        assert _close(np.linalg.norm(o_d, axis=1), np.ones(m), 1e-12)
+        assert not np.any(np.all(o_d == st.view("orientation")[parents], axis=1))
+    else:
+        assert np.all(np.abs(o_d) == 1)
+    x0 = st.view("positions")[parents]
+    w = view("positions", out["positions"])[n:] - x0
+    assert _close(view("positions", out["positions"])[parents], x0 - 0.5 * w, 1e-12)
+    r0 = st["radius"][parents]
+    o = st.view("orientation")[parents]
+    pol = st["polarity"][parents]
+    if dims > 1:
+        free = pol == 0
+        assert _close(np.linalg.norm(w[free], axis=1), r0[free], 1e-12)
+        assert np.all(np.abs(np.sum(w[~free] * o[~free], axis=1)) <= 1e-9 * r0[~free])
+    else:
+        # 1D: walk and orientation are +-1, so w = +-radius (polarity 0) or 0 (polarity 1)
+        assert _close(np.abs(w[:, 0]), np.where(pol == 1, 0.0, r0), 1e-12)
+    for which in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED):
+        want = _swap_with_back_removals(st.get_links(which), parents) + [[] for _ in range(m)]
+        assert _lists(*sim.links(which)) == want
+
+
+def _loop_removals(pos, flags, bmin, bmax, limit):
+    """The removal loop of update_cell_container_for_phenotype (containers_solver.cpp:
+    182-186, remove_single 81-123) over i < limit, brute force on (position, flag) rows."""
+    rows = [(tuple(p), f) for p, f in zip(pos, flags)]
+    bad = lambda r: r[1] == 1 or any(x < lo or x > hi for x, lo, hi in zip(r[0], bmin, bmax))  # noqa: E731
+    for i in range(limit):
+        while i < len(rows) and bad(rows[i]):
+            rows[i] = rows[-1]
+            rows.pop()
+    return rows
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_division_removal_loop_visits_old_agents_only(dims):
+    """Daughters placed outside the box stay unless a removal below the old count pulls them
+    in as the last agent; then each bad one pulled in goes too, until a good agent fills
+    the hole.  Parents sit near the upper corner, so many daughters leave the box."""
+    n = 40
+    mesh, st = _state(dims, n, box=100)
+    rng = np.random.default_rng(7 + dims)
+    pos = rng.uniform(-60, 60, (n, dims))
+    pos[30:] = 95.0  # the last ten agents divide at the upper corner: parents move at most
+    # radius/2 (stay inside), daughters a radius (often leave)
+    st["positions"] = pos.ravel()
+    flags = np.zeros(n, np.uint8)
+    flags[30:] = 2
+    st["flags"] = flags
+    p = abi.Params()
+    a = _sim(mesh, st)
+    a.update_cell_container_for_phenotype(p)
+    after = a.download()
+    x = after.view("positions")
+    out_of_box = np.any((x < -100) | (x > 100), axis=1)
+    assert a.count == n + 10 and out_of_box[n:].sum() >= 3  # nothing visited below n was bad
+    # the same divisions with agents 3 and 17 flagged for removal
+    flags2 = flags.copy()
+    flags2[[3, 17]] = 1
+    st["flags"] = flags2
+    b = _sim(mesh, st)
+    b.update_cell_container_for_phenotype(p)
+    post_division_flags = np.r_[np.where(flags2 == 1, 1, 0), np.zeros(10, int)]  # dividers reset to none
+    rows = _loop_removals(x, post_division_flags, (-100,) * dims, (100,) * dims, n)
+    assert b.count == len(rows)
+    assert np.array_equal(b.get_field("positions").reshape(-1, dims), np.array([r[0] for r in rows]))
+    assert b.deaths_count == 2
+
+
 def test_spring_detach_saturating():
     """detachment_rate * dt >= 1 fires every draw (`<=`, position_solver.cpp:386): every
     spring touching such an agent goes, springs between two zero-rate agents stay."""
