@@ -252,8 +252,11 @@ int pcm_update_cell_container_for_mechanics(pcm_sim* sim, const pcm_step_params*
  * start with zero velocities, pressure and damage, one nucleus, movable, ORIGIN -1 and a
  * random ORIENTATION (cell::assign_orientation without set_orientation).  The parent's
  * INTERNALIZED_SUBSTRATES halve when compute_internalized_substrates != 0 (the
- * microenvironment option of cell.cpp:122).  release_internalized_substrates and custom
- * data (paraBioFVM, host members) are the caller's: see pcm_division_log. */
+ * microenvironment option of cell.cpp:120).  release_internalized_substrates and custom
+ * data (paraBioFVM, host members) are the caller's: see pcm_division_log.  As in
+ * simulate_phenotype (simulator.cpp:208-216), pcm_update_mechanics_mesh and
+ * pcm_update_cell_neighbors follow before the next mechanics step: the neighbor lists
+ * still name the agents of before. */
 int pcm_update_cell_container_for_phenotype(pcm_sim* sim, const pcm_step_params* params,
 											int32_t compute_internalized_substrates);
 /* environment::divisions_count increments made on the device */

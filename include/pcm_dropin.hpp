@@ -143,6 +143,8 @@ void for_each_field(CellData& d, F&& f)
 	f(PCM_F_CHEMOTAXIS_INDEX, d.motilities.chemotaxis_index);
 	f(PCM_F_CHEMOTAXIS_DIRECTION, d.motilities.chemotaxis_direction);
 	f(PCM_F_CHEMOTACTIC_SENSITIVITIES, d.motilities.chemotactic_sensitivities);
+	f(PCM_F_ORIENTATION, d.states.orientation);
+	f(PCM_F_POLARITY, d.geometries.polarity);
 }
 
 // vector<vector<index_t>> <-> CSR

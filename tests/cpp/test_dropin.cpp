@@ -400,8 +400,14 @@ void phenotype_update(int dims, int n)
 
 	oracle.update_cell_container_for_phenotype(ref, false);
 	int cell_solver_stub = 0;
+	oracle.update_mechanics_mesh(ref); // simulate_phenotype's housekeeping (simulator.cpp:208-216)
+	oracle.update_cell_neighbors(ref);
 #pragma omp parallel
-	s.containers.update_cell_container_for_phenotype(dev, cell_solver_stub);
+	{
+		s.containers.update_cell_container_for_phenotype(dev, cell_solver_stub);
+		s.containers.update_mechanics_mesh(dev);
+		s.position.update_cell_neighbors(dev);
+	}
 	for (int step = 0; step < 3; step++)
 	{
 		pcm_dropin::sync_to_host(dev);

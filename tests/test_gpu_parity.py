@@ -229,7 +229,8 @@ def _flag_divisions(st, seed):
 def test_phenotype_container_update(dims, real_bytes):
     """pcm_update_cell_container_for_phenotype against the oracle: divisions (daughters at
     the box edge included), the removal loop over the old agents, the division log and
-    counters; then fused steps from the new population."""
+    counters; then the rest of simulate_phenotype's housekeeping (mesh, neighbors,
+    simulator.cpp:208-216) and fused steps from the new population."""
     n = {1: 200, 2: 800, 3: 1500}[dims]
     mesh, st, p = parity.rich_population(dims, n, 60 + dims, real_bytes, rates="busy")
     _flag_divisions(st, 70 + dims)
@@ -241,6 +242,14 @@ def test_phenotype_container_update(dims, real_bytes):
     assert np.array_equal(dev.division_log(), ora.division_log())
     assert dev.divisions_count == ora.divisions_count == n // 20
     assert dev.deaths_count == ora.deaths_count
+    for s in (dev, ora):
+        s.initialize_mesh()
+    parity.assert_same_buckets(dev, ora, "mesh after the phenotype update")
+    parity.assert_same_neighbors(dev, ora, "neighbors after the phenotype update")
+    # the steps start from the oracle's state: a daughter sits 1.5 radii from its parent,
+    # where the repulsion turns a last-bit difference of the walk's cos/sin into more
+    dev.upload(ora.download())
+    dev.initialize_mesh()
     for step in range(2):
         dev.step(p, 1)
         ora.step(p, 1)
