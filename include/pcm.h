@@ -202,6 +202,14 @@ int pcm_get_voxel_count(pcm_sim* sim, int64_t* voxel_count);
 int pcm_get_voxel_buckets(pcm_sim* sim, int32_t* counts, int32_t* agents);
 /* environment::deaths_count increments made on the device since creation */
 int pcm_get_deaths_count(pcm_sim* sim, int64_t* deaths);
+/* Output packing, for the per-cell matrix of the MultiCellDS writer
+ * (PhysiCell_MultiCellDS.cpp:1735-1790 writes each cell as a column of doubles): for every
+ * agent, fields[0..nfields) in order, each as widths[k] doubles — the field's own elements,
+ * zero-padded when widths[k] is larger (fill(v, dims) of the writer pads 2D vectors to 3),
+ * widths[k] == 0 meaning the field's own width.  `out` receives count * row doubles,
+ * agent-major, row = the sum of the widths; the packing runs on the device, one copy comes
+ * back.  At most 256 columns. */
+int pcm_pack_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, const int32_t* widths, double* out);
 /* The removals made on the device since the last call, as the sequence of positions the
  * reference's container.remove_at(i) received (containers_solver.cpp:81-123): replaying
  * remove_at on a host container in this order reproduces the device's agent order, so host-
@@ -332,7 +340,7 @@ int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t 
 	X(MOTILITY) X(MEMBRANE) X(CONTRACT) X(POSITIONS) X(VELOCITY_FUSED) X(SPRING_FIRE) X(SPRING_ERASE)                   \
 	X(SPRING_COMPACT) X(SPRING_COLLECT) X(SPRING_RESOLVE) X(SPRING_APPEND) X(INTERACT_TENTATIVE) X(INTERACT_CHECK)       \
 	X(INTERACT_APPLY) X(INTERACT_SERIAL) X(COMPACT_FLAG) X(COMPACT_PLAN) X(COMPACT_APPLY) X(COMPACT_RESTORE)            \
-	X(COMPACT_DONE) X(FORCE_RECORDS) X(CONTACT) X(DIVIDE_PLAN) X(DIVIDE_APPLY) X(DIVIDE_LINKS)
+	X(COMPACT_DONE) X(FORCE_RECORDS) X(CONTACT) X(DIVIDE_PLAN) X(DIVIDE_APPLY) X(DIVIDE_LINKS) X(PACK)
 #define PCM_KERNEL_ENUM_(name) PCM_K_##name,
 enum pcm_kernel
 {

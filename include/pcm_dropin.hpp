@@ -34,6 +34,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -180,6 +181,7 @@ struct residency
 	bool device_newer = false; // the host copy must be refreshed before host code reads it
 	bool gradients_newer = true; // the substrate gradients must be uploaded before the next phase
 	int min_capacity = 0;        // agent capacity the next handle needs at least
+	unsigned stale = 0;          // sync_parts the host copy lacks
 	int64_t deaths_seen = 0;
 	uint64_t seed = 0;
 
@@ -189,6 +191,23 @@ struct residency
 			pcm_destroy(sim);
 	}
 };
+
+// What sync_to_host brings back.  sync_fields: the removals replayed on the host container,
+// every device field, springs, attached cells and deaths_count — what phenotype code
+// reads.  The neighbor lists and the voxel buckets (read by signal_behavior.cpp:645-661,
+// 757-765, 870, 893-895, cell.cpp:261-267 and the output writers) come only when asked for.
+enum sync_parts : unsigned
+{
+	sync_fields = 1,
+	sync_neighbors = 2,
+	sync_buckets = 4,
+	sync_all = 7
+};
+
+template <class Env>
+void download(Env& e, residency& r, unsigned parts = sync_all);
+template <class Env>
+void download_fields(Env& e, residency& r);
 
 inline std::map<const void*, std::unique_ptr<residency>>& registry()
 {
@@ -294,12 +313,20 @@ void upload(Env& e, residency& r)
 	std::vector<int32_t> discard(log_len + 1);
 	check(pcm_removal_log(r.sim, discard.data(), log_len + 1, &log_len), "pcm_removal_log");
 	r.host_newer = false;
+	r.stale = 0;
 }
 
 template <class Env>
 residency& on_device(Env& e)
 {
 	auto& r = state_of(e);
+	if (r.host_newer && r.sim && r.stale)
+	{
+		// the upload resends the lists: fetch the parts the host never synced
+		if (r.stale & sync_fields)
+			throw std::runtime_error("pcm_dropin: host_modified without a prior sync_to_host of the fields");
+		download(e, r, r.stale);
+	}
 	if (r.host_newer || !r.sim)
 		upload(e, r);
 	if constexpr (has_gradients<Env>)
@@ -322,10 +349,50 @@ void gradients_modified(Env& e)
 	on_one_thread([&] { state_of(e).gradients_newer = true; });
 }
 
-// Bring the host SoA up to date: replay the device's removals on the host container so
-// host-only members follow, then copy every device field, the lists and the buckets.
+// Bring the requested parts of the host copy up to date.  Lists and buckets follow the
+// device's agent order, so they imply the field part (and its removal replay).
 template <class Env>
-void download(Env& e, residency& r)
+void download(Env& e, residency& r, unsigned parts)
+{
+	auto& d = pcm_cell_data(e);
+	parts &= r.stale;
+	if (parts & (sync_neighbors | sync_buckets))
+		parts |= r.stale & sync_fields;
+	const int32_t n_host = (int32_t)d.agents_count;
+	if (parts & sync_fields)
+		download_fields(e, r);
+	const int32_t n = (int32_t)d.agents_count;
+	(void)n_host;
+	if (parts & sync_neighbors)
+	{
+		int64_t total = 0;
+		check(pcm_get_neighbors_size(r.sim, &total), "pcm_get_neighbors_size");
+		std::vector<int32_t> off(n + 1), idx(total + 1);
+		check(pcm_get_neighbors(r.sim, off.data(), idx.data()), "pcm_get_neighbors");
+		from_csr(d.states.neighbors, n, off, idx);
+	}
+	if (parts & sync_buckets)
+	{
+		int64_t voxels = 0;
+		check(pcm_get_voxel_count(r.sim, &voxels), "pcm_get_voxel_count");
+		std::vector<int32_t> counts(voxels), agents(std::max<int64_t>(r.capacity, 1));
+		check(pcm_get_voxel_buckets(r.sim, counts.data(), agents.data()), "pcm_get_voxel_buckets");
+		int64_t t = 0;
+		for (int64_t v = 0; v < voxels; v++)
+		{
+			auto& b = e.cells_in_mechanics_voxels[v];
+			b.assign(agents.begin() + t, agents.begin() + t + counts[v]);
+			t += counts[v];
+		}
+	}
+	r.stale &= ~parts;
+	r.device_newer = r.stale != 0;
+}
+
+// The field part: replay the device's removals on the host container so host-only members
+// follow, then copy every device field, springs, attached cells and the deaths count.
+template <class Env>
+void download_fields(Env& e, residency& r)
 {
 	auto& d = pcm_cell_data(e);
 	int64_t log_len = 0;
@@ -348,40 +415,21 @@ void download(Env& e, residency& r)
 	};
 	fetch(PCM_LINKS_SPRINGS, d.states.springs);
 	fetch(PCM_LINKS_ATTACHED, d.states.attached_cells);
-	{
-		int64_t total = 0;
-		check(pcm_get_neighbors_size(r.sim, &total), "pcm_get_neighbors_size");
-		std::vector<int32_t> off(n + 1), idx(total + 1);
-		check(pcm_get_neighbors(r.sim, off.data(), idx.data()), "pcm_get_neighbors");
-		from_csr(d.states.neighbors, n, off, idx);
-	}
-	{
-		int64_t voxels = 0;
-		check(pcm_get_voxel_count(r.sim, &voxels), "pcm_get_voxel_count");
-		std::vector<int32_t> counts(voxels), agents(std::max<int64_t>(r.capacity, 1));
-		check(pcm_get_voxel_buckets(r.sim, counts.data(), agents.data()), "pcm_get_voxel_buckets");
-		int64_t t = 0;
-		for (int64_t v = 0; v < voxels; v++)
-		{
-			auto& b = e.cells_in_mechanics_voxels[v];
-			b.assign(agents.begin() + t, agents.begin() + t + counts[v]);
-			t += counts[v];
-		}
-	}
 	int64_t deaths = 0;
 	check(pcm_get_deaths_count(r.sim, &deaths), "pcm_get_deaths_count");
 	e.deaths_count += (decltype(e.deaths_count))(deaths - r.deaths_seen);
 	r.deaths_seen = deaths;
-	r.device_newer = false;
 }
 
+// sync_to_host(e) brings everything back; sync_to_host(e, sync_fields) leaves the neighbor
+// lists and buckets on the device until a later call asks for them.
 template <class Env>
-void sync_to_host(Env& e)
+void sync_to_host(Env& e, unsigned parts = sync_all)
 {
 	on_one_thread([&] {
 		auto& r = state_of(e);
 		if (r.sim && r.device_newer)
-			download(e, r);
+			download(e, r, parts);
 	});
 }
 
@@ -405,7 +453,112 @@ void run(Env& e, Call&& call)
 		auto& r = on_device(e);
 		call(r.sim);
 		r.device_newer = true;
+		r.stale = sync_all;
 	});
+}
+
+// ---------------------------------------------------------------- output formats
+// Optional: the agent id the writers print (agent::id), by index; without it the index.
+//   long long pcm_agent_id(Env& e, int i);
+template <class Env>
+concept has_agent_ids = requires(Env& e) { pcm_agent_id(e, 0); };
+
+template <class Env>
+long long agent_id(Env& e, int i)
+{
+	if constexpr (has_agent_ids<Env>)
+		return (long long)pcm_agent_id(e, i);
+	else
+		return i;
+}
+
+// write_neighbor_graph / write_attached_cells_graph (PhysiCell_MultiCellDS.cpp:2051-2137)
+// straight from the device's lists, without building the host's vector-of-vectors: one
+// line "id: id,id,..." per agent, lines separated by '\n', none after the last.
+template <class Env>
+void write_graph(const std::string& filename, Env& e, int which)
+{
+	on_one_thread([&] {
+		auto& r = state_of(e);
+		if (!r.sim)
+			throw std::runtime_error("pcm_dropin: no device state to write");
+		if (r.device_newer)
+			download(e, r, sync_fields); // the host's agent order, for the ids
+		int32_t n = 0;
+		check(pcm_get_count(r.sim, &n), "pcm_get_count");
+		int64_t total = 0;
+		std::vector<int32_t> off(n + 1);
+		if (which == PCM_LINKS_ATTACHED)
+			check(pcm_get_links_size(r.sim, which, &total), "pcm_get_links_size");
+		else
+			check(pcm_get_neighbors_size(r.sim, &total), "pcm_get_neighbors_size");
+		std::vector<int32_t> idx(total + 1);
+		if (which == PCM_LINKS_ATTACHED)
+			check(pcm_get_links(r.sim, which, off.data(), idx.data()), "pcm_get_links");
+		else
+			check(pcm_get_neighbors(r.sim, off.data(), idx.data()), "pcm_get_neighbors");
+		std::string text;
+		for (int32_t i = 0; i < n; i++)
+		{
+			text += std::to_string(agent_id(e, i));
+			text += ": ";
+			for (int32_t k = off[i]; k < off[i + 1]; k++)
+			{
+				text += std::to_string(agent_id(e, idx[k]));
+				if (k != off[i + 1] - 1)
+					text += ',';
+			}
+			if (i != n - 1)
+				text += '\n';
+		}
+		std::ofstream of(filename, std::ios::out | std::ios::binary);
+		of << text;
+		if (!of)
+			throw std::runtime_error("pcm_dropin: cannot write " + filename);
+	});
+}
+
+template <class Env>
+void write_neighbor_graph(const std::string& filename, Env& e)
+{
+	write_graph(filename, e, -1);
+}
+
+template <class Env>
+void write_attached_cells_graph(const std::string& filename, Env& e)
+{
+	write_graph(filename, e, PCM_LINKS_ATTACHED);
+}
+
+// Columns of the MultiCellDS per-cell matrix that live on the device
+// (PhysiCell_MultiCellDS.cpp:1735-1790), packed there: see pcm_pack_fields.  Returns
+// count * sum(widths) doubles, agent-major.
+template <class Env>
+std::vector<double> pack_fields(Env& e, const std::vector<int32_t>& fields, const std::vector<int32_t>& widths)
+{
+	std::vector<double> out;
+	on_one_thread([&] {
+		auto& r = on_device(e);
+		int32_t n = 0;
+		check(pcm_get_count(r.sim, &n), "pcm_get_count");
+		int64_t row = 0;
+		for (std::size_t k = 0; k < fields.size(); k++)
+		{
+			int32_t type = 0, kind = 0;
+			const char* name = nullptr;
+			check(pcm_field_info(fields[k], &type, &kind, &name), "pcm_field_info");
+			const int own = kind == PCM_W_DIMS ? e.m.mesh.dims
+							: kind == PCM_W_DEFS ? e.cell_definitions_count
+							: kind == PCM_W_SUBS ? e.m.substrates_count
+												 : 1;
+			row += widths[k] ? widths[k] : own;
+		}
+		out.assign((std::size_t)n * row, 0.0);
+		if (n > 0)
+			check(pcm_pack_fields(r.sim, (int32_t)fields.size(), fields.data(), widths.data(), out.data()),
+				  "pcm_pack_fields");
+	});
+	return out;
 }
 
 // ---------------------------------------------------------------- reference classes
@@ -515,7 +668,7 @@ public:
 		on_one_thread([&] {
 			auto& r0 = state_of(e);
 			if (r0.sim && r0.device_newer && !r0.host_newer)
-				download(e, r0); // earlier removals replay before the daughters are appended
+				download(e, r0, sync_fields); // earlier removals replay before the daughters are appended
 			int32_t internalized = 0;
 			if constexpr (requires { e.m.compute_internalized_substrates; })
 				internalized = e.m.compute_internalized_substrates ? 1 : 0;
@@ -544,6 +697,7 @@ public:
 				pcm_host_divide(e, parents[k]);
 			e.divisions_count += (decltype(e.divisions_count))len;
 			r.device_newer = true;
+			r.stale = sync_all;
 		});
 	}
 };

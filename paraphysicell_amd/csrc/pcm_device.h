@@ -53,6 +53,17 @@ struct Scalars
 	int stat[8];
 };
 
+// element type (PCM_T_*) of a field id
+__host__ __device__ constexpr int pcm_field_type(int f)
+{
+	constexpr int types[PCM_FIELD_COUNT] = {
+#define PCM_FIELD_TYPE_(name, type, width) type,
+		PCM_FIELD_LIST(PCM_FIELD_TYPE_)
+#undef PCM_FIELD_TYPE_
+	};
+	return types[f];
+}
+
 template <class R>
 struct SimCtx
 {

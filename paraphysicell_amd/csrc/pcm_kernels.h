@@ -1962,6 +1962,40 @@ __global__ void k_compact_done(const SimCtx<R>* __restrict__ ctxs)
 	}
 }
 
+// ------------------------------------------------------------------ output packing
+// One output double per thread: element (agent, column) of the agent-major matrix; column k
+// reads component comp[k] of field fld[k] (a component beyond the field's width is 0).
+template <class R>
+__global__ void k_pack(const SimCtx<R>* __restrict__ ctxs, const int* __restrict__ fld, const int* __restrict__ comp,
+					   int row, double* __restrict__ out)
+{
+	PCM_PROLOGUE(c)
+	const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long total = (long long)c.sc->n * row;
+	if (e >= total)
+		return;
+	const int i = (int)(e / row), k = (int)(e % row);
+	const int f = fld[k], m = comp[k];
+	const int nb = c.fbytes[f];
+	const uint8_t* base = (const uint8_t*)c.fptr[f] + (size_t)i * nb;
+	double v = 0;
+	switch (pcm_field_type(f))
+	{
+		case PCM_T_REAL:
+			if (m < nb / (int)sizeof(R))
+				v = (double)((const R*)base)[m];
+			break;
+		case PCM_T_I32:
+			if (m < nb / 4)
+				v = (double)((const int*)base)[m];
+			break;
+		default:
+			if (m < nb)
+				v = (double)base[m];
+	}
+	out[e] = v;
+}
+
 // ------------------------------------------------------------------ phenotype-cadence update
 // update_cell_container_for_phenotype (containers_solver.cpp:149-187) in the reference's
 // single-thread order: divisions in ascending parent order (cell::divide, cell.cpp:108-163),

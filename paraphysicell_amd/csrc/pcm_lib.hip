@@ -79,6 +79,7 @@ struct pcm_sim
 	virtual long long voxel_count() const = 0;
 	virtual int get_deaths(int64_t* d) = 0;
 	virtual int get_divisions(int64_t* d) = 0;
+	virtual int pack_fields(int nfields, const int32_t* fields, const int32_t* widths, double* out) = 0;
 	virtual int division_log(int32_t* parents, int64_t capacity, int64_t* length) = 0;
 	virtual int phenotype_update(const pcm_step_params* p, int compute_internalized) = 0;
 	virtual int set_step(uint64_t s) = 0;
@@ -568,6 +569,51 @@ public:
 		return PCM_OK;
 	}
 
+	// the per-cell output matrix, packed on the device (PhysiCell_MultiCellDS.cpp:1735-1790)
+	int pack_fields(int nfields, const int32_t* fields, const int32_t* widths, double* out) override
+	{
+		std::vector<int> fld, comp;
+		for (int k = 0; k < nfields; k++)
+		{
+			if (fields[k] < 0 || fields[k] >= PCM_FIELD_COUNT || widths[k] < 0)
+				return fail(PCM_ERR_ARGUMENT, "bad field id or width");
+			const int own = width(k_fields[fields[k]].width_kind);
+			const int w = widths[k] == 0 ? own : widths[k];
+			for (int m = 0; m < w; m++)
+			{
+				fld.push_back(fields[k]);
+				comp.push_back(m);
+			}
+		}
+		const int row = (int)fld.size();
+		if (row == 0 || row > 256)
+			return fail(PCM_ERR_ARGUMENT, "between 1 and 256 output columns");
+		PCM_TRY(read_scalars());
+		const long long total = (long long)sc_.n * row;
+		if (total == 0)
+			return PCM_OK;
+		if (!pack_cols_)
+			PCM_TRY(alloc_t(&pack_cols_, 512));
+		if ((size_t)total > pack_cap_)
+		{
+			release(pack_out_);
+			pack_out_ = nullptr;
+			PCM_TRY(alloc_t(&pack_out_, (size_t)total));
+			pack_cap_ = (size_t)total;
+		}
+		std::vector<int> cols(512, 0);
+		std::copy(fld.begin(), fld.end(), cols.begin());
+		std::copy(comp.begin(), comp.end(), cols.begin() + 256);
+		PCM_HIP(hipMemcpy(pack_cols_, cols.data(), sizeof(int) * 512, hipMemcpyHostToDevice));
+		const int blocks = (int)((total + BLOCK - 1) / BLOCK);
+		hipLaunchKernelGGL(k_pack<R>, dim3(blocks), dim3(BLOCK), 0, stream_, (const SimCtx<R>*)d_,
+						   (const int*)pack_cols_, (const int*)(pack_cols_ + 256), row, pack_out_);
+		PCM_HIP(hipGetLastError());
+		PCM_HIP(hipStreamSynchronize(stream_));
+		PCM_HIP(hipMemcpy(out, pack_out_, sizeof(double) * (size_t)total, hipMemcpyDeviceToHost));
+		return PCM_OK;
+	}
+
 	int get_divisions(int64_t* d) override
 	{
 		PCM_TRY(read_scalars());
@@ -785,6 +831,9 @@ private:
 	uint64_t ctx_version_ = 0; // increments whenever the device context changes
 	size_t grad_count_ = 0;    // reals in h_.grad
 	uint64_t division_epoch_ = 0; // phenotype-update calls so far
+	int* pack_cols_ = nullptr;     // column table of pcm_pack_fields: 256 field ids, 256 components
+	double* pack_out_ = nullptr;   // its output matrix
+	size_t pack_cap_ = 0;
 	int div_log_len_ = 0;         // divisions of the last phenotype update
 	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
@@ -2052,6 +2101,14 @@ int pcm_get_voxel_buckets(pcm_sim* sim, int32_t* counts, int32_t* agents)
 	if (!counts || !agents)
 		return fail(PCM_ERR_ARGUMENT, "null output");
 	return sim->get_buckets(counts, agents);
+}
+
+int pcm_pack_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, const int32_t* widths, double* out)
+{
+	PCM_NEED(sim);
+	if (nfields < 1 || !fields || !widths || !out)
+		return fail(PCM_ERR_ARGUMENT, "null argument or no fields");
+	return sim->pack_fields(nfields, fields, widths, out);
 }
 
 int pcm_update_cell_container_for_phenotype(pcm_sim* sim, const pcm_step_params* params,

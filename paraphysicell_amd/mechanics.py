@@ -357,6 +357,25 @@ class DeviceSim(_LibSim):
     def synchronize(self):
         self._call("synchronize", self._h)
 
+    def pack_fields(self, names, widths=None) -> np.ndarray:
+        """The named fields of every agent as one (count, columns) float64 matrix packed on
+        the device (pcm_pack_fields); widths pad vectors (0: the field's own width)."""
+        widths = [0] * len(names) if widths is None else list(widths)
+        fields = np.array([abi.FIELD_ID[n] for n in names], np.int32)
+        w = np.array(widths, np.int32)
+        row = 0
+        for name, wk in zip(names, widths):
+            own = abi.field_width(abi.FIELDS[abi.FIELD_ID[name]][2], self.mesh.dims, self.defs, self.subs)
+            row += wk or own
+        n = self.count
+        out = np.zeros(max(n * row, 1), np.float64)
+        fn = self._fn("pack_fields")
+        fn.argtypes = [_PTR, ctypes.c_int32, _PTR, _PTR, _PTR]
+        fn.restype = ctypes.c_int
+        if n > 0:
+            self._call("pack_fields", self._h, len(fields), fields.ctypes.data, w.ctypes.data, out.ctypes.data)
+        return out[: n * row].reshape(n, row)
+
     # ------------------------------------------------------------------ launch shape
     @property
     def tuning(self) -> tuple:

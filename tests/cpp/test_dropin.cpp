@@ -5,7 +5,9 @@
 // Exit status 0 when every case passes.  Run by tests/test_dropin.py on a GPU box.
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <random>
+#include <sstream>
 
 #include "../../include/pcm_dropin.hpp"
 #include "../../oracle/pcm_oracle.hpp"
@@ -439,6 +441,121 @@ void phenotype_update(int dims, int n)
 	pcm_dropin::release(dev);
 }
 
+// write_neighbor_graph's text (PhysiCell_MultiCellDS.cpp:2051-2093) from host lists, with
+// the agent index as id: a stringstream drained into the output after every agent
+static std::string graph_text(const std::vector<std::vector<index_t>>& lists, int n)
+{
+	std::stringstream buffer;
+	std::ostringstream of;
+	for (int i = 0; i < n; i++)
+	{
+		buffer << i << ": ";
+		const int size = (int)lists[i].size();
+		for (int j = 0; j < size; j++)
+		{
+			buffer << lists[i][j];
+			if (j != size - 1)
+				buffer << ",";
+		}
+		if (i != n - 1)
+			buffer << std::endl;
+		of << buffer.rdbuf();
+	}
+	return of.str();
+}
+
+static std::string read_file(const std::string& path)
+{
+	std::ifstream f(path, std::ios::binary);
+	std::stringstream ss;
+	ss << f.rdbuf();
+	return ss.str();
+}
+
+// lazy host formats: partial sync, graph dumps from the device lists, packed output columns
+template <class R>
+void formats(int dims, int n)
+{
+	pcm_mesh_desc md {};
+	md.dims = dims;
+	for (int k = 0; k < 3; k++)
+	{
+		md.bbox_min[k] = -150;
+		md.bbox_max[k] = 150;
+		md.voxel[k] = 30;
+	}
+	environment<R> ref(md, 2, 1), dev(md, 2, 1);
+	fill_population(ref, n, 91 + dims);
+	fill_population(dev, n, 91 + dims);
+	for (auto* e : { &ref, &dev })
+	{
+		e->mechanics_time_step = (R)0.1;
+		e->virtual_wall_at_domain_edges = true;
+		e->automated_spring_adhesion = true;
+	}
+	ref.rng.mode = rng_mode::keyed;
+	ref.rng.seed = 3;
+	pcm_dropin::set_seed(dev, 3);
+	solver<R> oracle;
+	oracle.initialize(ref);
+	oracle.update_mechanics_mesh(ref);
+	oracle.update_cell_neighbors(ref);
+	pcm_dropin::solver<environment<R>> s;
+	s.initialize(dev);
+	s.containers.update_mechanics_mesh(dev);
+	s.position.update_cell_neighbors(dev);
+	for (int step = 0; step < 2; step++)
+	{
+		oracle.step(ref);
+		s.simulate_mechanics(dev, 1);
+	}
+	// the field part only: counts and fields current, neighbor lists still the old ones
+	pcm_dropin::sync_to_host(dev, pcm_dropin::sync_fields);
+	EXPECT(dev.data.agents_count == ref.data.agents_count, "formats dims %d: count", dims);
+	EXPECT(dev.data.states.springs == ref.data.states.springs, "formats dims %d: springs after sync_fields", dims);
+	EXPECT((pcm_dropin::state_of(dev).stale & (pcm_dropin::sync_neighbors | pcm_dropin::sync_buckets)) != 0,
+		   "formats dims %d: neighbor lists or buckets synced too early", dims);
+	const int nn = dev.data.agents_count;
+
+	const std::string dir = "/tmp/pcm_dropin_formats_" + std::to_string(dims);
+	pcm_dropin::write_neighbor_graph(dir + "_neighbors.txt", dev);
+	pcm_dropin::write_attached_cells_graph(dir + "_attached.txt", dev);
+	EXPECT(read_file(dir + "_neighbors.txt") == graph_text(ref.data.states.neighbors, nn),
+		   "formats dims %d: neighbor graph text", dims);
+	EXPECT(read_file(dir + "_attached.txt") == graph_text(ref.data.states.attached_cells, nn),
+		   "formats dims %d: attached graph text", dims);
+
+	const std::vector<int32_t> fields = { PCM_F_POSITIONS, PCM_F_VOLUME, PCM_F_ORIENTATION, PCM_F_VELOCITIES,
+										  PCM_F_SIMPLE_PRESSURE, PCM_F_NUMBER_OF_NUCLEI, PCM_F_CELL_DEFINITION_INDEX,
+										  PCM_F_DEAD };
+	const std::vector<int32_t> widths = { 3, 0, 3, 3, 0, 0, 0, 0 };
+	const auto packed = pcm_dropin::pack_fields(dev, fields, widths);
+	const int row = 3 + 1 + 3 + 3 + 1 + 1 + 1 + 1;
+	bool same = packed.size() == (size_t)nn * row;
+	const auto& d = dev.data;
+	for (int i = 0; same && i < nn; i++)
+	{
+		const double* p = packed.data() + (size_t)i * row;
+		for (int k = 0; k < 3; k++)
+		{
+			same = same && p[k] == (k < dims ? (double)d.agent_data.positions[i * dims + k] : 0.0);
+			same = same && p[4 + k] == (k < dims ? (double)d.states.orientation[i * dims + k] : 0.0);
+			same = same && p[7 + k] == (k < dims ? (double)d.velocities[i * dims + k] : 0.0);
+		}
+		same = same && p[3] == (double)d.agent_data.volumes[i] && p[10] == (double)d.states.simple_pressure[i]
+			   && p[11] == (double)d.states.number_of_nuclei[i] && p[12] == (double)d.cell_definition_indices[i]
+			   && p[13] == (double)d.deaths.dead[i];
+	}
+	EXPECT(same, "formats dims %d: packed output columns", dims);
+	// everything: the neighbor lists arrive
+	pcm_dropin::sync_to_host(dev);
+	EXPECT(dev.data.states.neighbors == ref.data.states.neighbors, "formats dims %d: neighbors after full sync", dims);
+	EXPECT(dev.cells_in_mechanics_voxels == ref.cells_in_mechanics_voxels, "formats dims %d: buckets", dims);
+	EXPECT(pcm_dropin::state_of(dev).stale == 0, "formats dims %d: parts left stale", dims);
+	std::printf("formats dims %d %s: %d agents\n", dims, sizeof(R) == 4 ? "FP32" : "FP64", nn);
+	pcm_dropin::release(dev);
+}
+
 int main()
 {
 	for (int dims = 1; dims <= 3; dims++)
@@ -452,6 +569,8 @@ int main()
 		population_steps<float>(dims, dims == 3 ? 1500 : 700, 4, true);
 		population_steps<double>(dims, dims == 3 ? 1500 : 700, 3, false);
 	}
+	for (int dims = 2; dims <= 3; dims++)
+		formats<float>(dims, dims == 3 ? 1200 : 600);
 	for (int dims = 1; dims <= 3; dims++)
 	{
 		phenotype_update<float>(dims, dims == 3 ? 1200 : 500);

@@ -272,3 +272,27 @@ def test_phenotype_update_capacity_is_checked_first():
     after = dev.download()
     for name, _, _ in abi.FIELDS:
         assert np.array_equal(before.arrays[name], after.arrays[name]), name
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_pack_fields_is_the_field_matrix(dims):
+    """pcm_pack_fields gathers the MultiCellDS writer's device columns
+    (PhysiCell_MultiCellDS.cpp:1735-1790) exactly: values as doubles, vectors padded to 3."""
+    mesh, st, p = parity.rich_population(dims, 700, 90 + dims, 4, rates="busy")
+    st["orientation"] = np.random.default_rng(5).normal(size=st.count * dims)
+    dev = device(mesh, st, seed=2)
+    dev.initialize_mesh()
+    dev.step(p, 2)
+    names = ["positions", "volume", "nuclear", "cytoplasmic", "fluid_fraction", "orientation", "polarity",
+             "velocities", "simple_pressure", "number_of_nuclei", "damage", "total_attack_time",
+             "cell_adhesion_affinities", "dead", "flags", "internalized_substrates"]
+    widths = [3, 0, 0, 0, 0, 3, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0]
+    got = dev.pack_fields(names, widths)
+    n = dev.count
+    cols = []
+    for name, w in zip(names, widths):
+        a = dev.get_field(name).reshape(n, -1).astype(np.float64)
+        if w > a.shape[1]:
+            a = np.hstack([a, np.zeros((n, w - a.shape[1]))])
+        cols.append(a)
+    assert np.array_equal(got, np.hstack(cols))
