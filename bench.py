@@ -286,6 +286,7 @@ def main():
             "real_t": "float32",
             "mean_neighbors": round(k_mean, 2),
             "mean_springs": round(s_mean, 2),
+            "last_step_counters": dev.stats(),
             "launch_shape": {"velocity_lanes": tuned[0], "event_lanes": tuned[1], "neighbor_waves": tuned[2],
                              "picked_by": "autotuner on warm-up steps"},
         },

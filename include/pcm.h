@@ -340,7 +340,8 @@ int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t 
 	X(MOTILITY) X(MEMBRANE) X(CONTRACT) X(POSITIONS) X(VELOCITY_FUSED) X(SPRING_FIRE) X(SPRING_ERASE)                   \
 	X(SPRING_COMPACT) X(SPRING_COLLECT) X(SPRING_RESOLVE) X(SPRING_APPEND) X(INTERACT_TENTATIVE) X(INTERACT_CHECK)       \
 	X(INTERACT_APPLY) X(INTERACT_SERIAL) X(COMPACT_FLAG) X(COMPACT_PLAN) X(COMPACT_APPLY) X(COMPACT_RESTORE)            \
-	X(COMPACT_DONE) X(FORCE_RECORDS) X(CONTACT) X(DIVIDE_PLAN) X(DIVIDE_APPLY) X(DIVIDE_LINKS) X(PACK)
+	X(COMPACT_DONE) X(FORCE_RECORDS) X(CONTACT) X(DIVIDE_PLAN) X(DIVIDE_APPLY) X(DIVIDE_LINKS) X(PACK)             \
+	X(SPRING_RESOLVE_FIRST)
 #define PCM_KERNEL_ENUM_(name) PCM_K_##name,
 enum pcm_kernel
 {
@@ -349,8 +350,8 @@ enum pcm_kernel
 #undef PCM_KERNEL_ENUM_
 int pcm_kernel_name(int32_t kernel, const char** name);
 /* Work counters of the most recent step, 8 entries: spring attachment events, resolution
- * rounds, resolution chunks, interaction candidates, interaction replay used (0/1), agents
- * removed, occupied mechanics voxels, reserved. */
+ * rounds, attaching agents, interaction candidates, interaction replay used (0/1), agents
+ * removed, occupied mechanics voxels, reserved (0). */
 int pcm_get_stats(pcm_sim* sim, int32_t* stats);
 int pcm_profile_begin(pcm_sim* sim, uint64_t kernel_mask);
 int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches);

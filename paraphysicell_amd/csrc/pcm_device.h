@@ -48,6 +48,8 @@ struct Scalars
 	int order_valid;       // bucket[0, n) is a permutation of the current agents
 	int log_len;           // entries in the removal log (may exceed log_cap: log lost)
 	int compact_limit;     // > 0: the removal loop visits agents below it only
+	int res_open;          // actors the first resolve round left with undecided events
+	int pad1;
 	// diagnostics of the last step (pcm_get_stats): attachment events, resolver rounds,
 	// resolver chunks, interaction candidates, speculation conflict, removals, occupied voxels
 	int stat[8];
@@ -136,7 +138,9 @@ struct SimCtx
 	int* in_cnt;             // inbox fill (zero outside the spring phase)
 	int* att_list;           // actors with attachment events, a fixed segment per collect block
 	int* blk_cnt;            // actors listed in each collect block's segment
+	int* blk_ev;             // attachment events of each segment (counted by the first resolve round)
 	int* att_c;              // the actors of all segments, compacted by the resolver
+	int* att_c2;             // the resolver's next-round actors (beyond its LDS lists)
 	// interaction scratch
 	int* cand;               // candidate (actor, slot) pairs
 	int *n_rem_t, *n_atk_t;  // removal / attack events targeting an agent

@@ -755,6 +755,8 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const int i = gid();
+	if (i == 0)
+		c.sc->res_open = 0; // the resolver's open-actor list starts empty every update
 	if (i >= c.sc->n)
 		return;
 	const int cnt = c.spr_cnt[i];
@@ -927,9 +929,9 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 }
 
 // One actor's undecided attachment events, in its event order, against the states decided
-// so far (0 undecided, 1 success, 2 failure).  Undecided events left count into `unknown`.
+// so far (0 undecided, 1 success, 2 failure).  Returns whether an event stays undecided.
 template <class R>
-__device__ __forceinline__ void resolve_actor(const SimCtx<R>& c, volatile uint8_t* st, int a, int na, int& unknown)
+__device__ __forceinline__ bool resolve_actor(const SimCtx<R>& c, volatile uint8_t* st, int a, int na)
 {
 	const int c0a = c.spr_cnt[a], maxa = c.max_att[a];
 	int in_lo = 0, in_hi = 0;
@@ -941,6 +943,7 @@ __device__ __forceinline__ void resolve_actor(const SimCtx<R>& c, volatile uint8
 		in_lo += sx == 1;
 		in_hi += sx != 2;
 	}
+	bool open = false;
 	int own_lo = 0, own_hi = 0;
 	for (int k = 0; k < na; k++)
 	{
@@ -970,11 +973,12 @@ __device__ __forceinline__ void resolve_actor(const SimCtx<R>& c, volatile uint8
 			if (sk != 0)
 				st[(size_t)a * c.K + k] = (uint8_t)sk;
 			else
-				atomicAdd(&unknown, 1);
+				open = true;
 		}
 		own_lo += sk == 1;
 		own_hi += sk != 2;
 	}
+	return open;
 }
 
 // Attachment resolution in the reference's order: event (i, j) succeeds iff, when the
@@ -982,69 +986,89 @@ __device__ __forceinline__ void resolve_actor(const SimCtx<R>& c, volatile uint8
 // springs than their maximum.  The events before it at endpoint i are all of i's inbox
 // (their actors are smaller) and i's earlier own events; at endpoint j they are j's inbox
 // entries with a smaller actor.  Each endpoint's count is bounded below by the successes
-// and above by the successes plus the undecided among those events; rounds repeat until
-// every event is decided (the earliest undecided event is always decidable).
+// and above by the successes plus the undecided among those events; rounds repeat over
+// the actors that still hold undecided events until none is left (the earliest undecided
+// event is always decidable).  Decisions only ever turn undecided into final, so a round
+// may read states of the same round or older ones and stay correct.
+//
+// First round: every actor at once, one block per collect block's actor segment; actors
+// left with undecided events go to att_c for the single-block rounds that follow.
 template <class R, int G>
+__global__ void __launch_bounds__(64) k_spring_resolve_first(const SimCtx<R>* __restrict__ ctxs)
+{
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	if (c.sc->status)
+		return;
+	constexpr int APB = BLOCK / G;
+	const int b = blockIdx.x;
+	const int n = c.sc->n;
+	if (b == 0 && threadIdx.x == 0)
+		c.sc->stat[7] = (n + APB - 1) / APB; // collect segments, for the event and actor counts
+	if (b * APB >= n)
+		return;
+	const int m = c.blk_cnt[b];
+	volatile uint8_t* st = c.own_st;
+	int events = 0;
+	for (int k = threadIdx.x; k < m; k += 64)
+	{
+		const int a = c.att_list[b * APB + k];
+		const int na = c.own_cnt[a];
+		events += na;
+		if (resolve_actor<R>(c, st, a, na))
+			c.att_c[atomicAdd(&c.sc->res_open, 1)] = a;
+	}
+	for (int o = 32; o > 0; o >>= 1)
+		events += __shfl_xor(events, o);
+	if (threadIdx.x == 0)
+		c.blk_ev[b] = events;
+}
+
+// Later rounds, one block, over the actors the first round left open.
+template <class R>
 __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs)
 {
 	const SimCtx<R>& c = ctxs[blockIdx.y];
-	__shared__ int s_unknown, s_events;
+	__shared__ int s_next;
 	if (c.sc->status)
 		return;
-	// compact the collect blocks' actor segments into att_c (the collect kernel ran with G
-	// lanes per agent): thread t takes a contiguous range of segments
-	constexpr int APB = BLOCK / G;
-	const int nb = (c.sc->n + APB - 1) / APB;
-	const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
-	const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
-	int mine = 0;
-	for (int b = b0; b < b1; b++)
-		mine += c.blk_cnt[b];
-	int A;
-	int at = block_exclusive_scan(mine, A);
-	for (int b = b0; b < b1; b++)
-	{
-		const int m = c.blk_cnt[b];
-		for (int k = 0; k < m; k++)
-			c.att_c[at++] = c.att_list[b * APB + k];
-	}
-	const long long E = (long long)A * c.K; // bound on the events
+	int count = c.sc->res_open;
+	int* cur = c.att_c;
+	int* nxt = c.att_c2;
 	if (threadIdx.x == 0)
-		s_events = 0;
+		s_next = 0;
 	__syncthreads();
 	volatile uint8_t* st = c.own_st;
-	int rounds = 0;
-	for (;; rounds++)
+	int rounds = 1;
+	while (count > 0)
 	{
-		if (rounds > E + 1)
+		if (rounds > (long long)count * c.K + 1)
 		{
 			if (threadIdx.x == 0)
 				atomicOr(&c.sc->status, ST_CAPACITY);
 			return;
 		}
-		if (threadIdx.x == 0)
-			s_unknown = 0;
-		__syncthreads();
-		for (int t = threadIdx.x; t < A; t += blockDim.x)
+		for (int t = threadIdx.x; t < count; t += blockDim.x)
 		{
-			const int a = c.att_c[t];
-			const int na = c.own_cnt[a];
-			if (rounds == 0)
-				atomicAdd(&s_events, na);
-			resolve_actor<R>(c, st, a, na, s_unknown);
+			const int a = cur[t];
+			if (resolve_actor<R>(c, st, a, c.own_cnt[a]))
+				nxt[atomicAdd(&s_next, 1)] = a;
 		}
 		__threadfence();
 		__syncthreads();
-		const int u = s_unknown;
+		count = s_next;
+		rounds++;
 		__syncthreads();
-		if (u == 0)
-			break;
+		if (threadIdx.x == 0)
+			s_next = 0;
+		int* t = cur;
+		cur = nxt;
+		nxt = t;
+		__syncthreads();
 	}
 	if (threadIdx.x == 0)
 	{
-		c.sc->stat[0] = s_events;
-		c.sc->stat[1] = rounds + 1;
-		c.sc->stat[2] = A;
+		c.sc->stat[1] = rounds;
+		c.sc->res_open = 0;
 	}
 }
 

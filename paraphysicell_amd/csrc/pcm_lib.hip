@@ -231,7 +231,9 @@ public:
 		// one segment of BLOCK / lanes actors per collect block (lanes <= MAX_EVENT_LANES)
 		PCM_TRY(alloc_t(&h.att_list, (size_t)cap + BLOCK));
 		PCM_TRY(alloc_t(&h.blk_cnt, (size_t)cap / (BLOCK / MAX_EVENT_LANES) + 2));
+		PCM_TRY(alloc_t(&h.blk_ev, (size_t)cap / (BLOCK / MAX_EVENT_LANES) + 2));
 		PCM_TRY(alloc_t(&h.att_c, cap));
+		PCM_TRY(alloc_t(&h.att_c2, cap));
 		PCM_TRY(alloc_t(&h.fire, cap));
 		PCM_TRY(alloc_t(&h.erase, cap));
 		PCM_TRY(alloc_t(&h.n_rem_t, cap));
@@ -803,6 +805,20 @@ public:
 		PCM_TRY(read_scalars());
 		for (int k = 0; k < 8; k++)
 			out[k] = sc_.stat[k];
+		// attachment events and actors: summed over the collect segments of the last step
+		const int segments = sc_.stat[7];
+		out[0] = out[2] = out[7] = 0;
+		if (segments > 0)
+		{
+			std::vector<int> cnt(segments), ev(segments);
+			PCM_HIP(hipMemcpy(cnt.data(), h_.blk_cnt, sizeof(int) * segments, hipMemcpyDeviceToHost));
+			PCM_HIP(hipMemcpy(ev.data(), h_.blk_ev, sizeof(int) * segments, hipMemcpyDeviceToHost));
+			for (int b = 0; b < segments; b++)
+			{
+				out[0] += ev[b];
+				out[2] += cnt[b];
+			}
+		}
 		return PCM_OK;
 	}
 
@@ -1238,20 +1254,23 @@ private:
 		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R>));
 		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
 					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK));
+		// the first resolve round runs one block per collect block (segments of BLOCK / G)
+		const int segments = std::max(1, (int)(((long long)gn() * tune_.event_lanes + BLOCK - 1) / BLOCK));
 		switch (tune_.event_lanes)
 		{
 			case 4:
 				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 4>, 4));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R, 4>, 1, RESOLVE_THREADS));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 4>, segments, 64));
 				break;
 			case 16:
 				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 16>, 16));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R, 16>, 1, RESOLVE_THREADS));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 16>, segments, 64));
 				break;
 			default:
 				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 8>, 8));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R, 8>, 1, RESOLVE_THREADS));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 8>, segments, 64));
 		}
+		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
 		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
 	}
 
@@ -1420,7 +1439,8 @@ public:
 	{
 		static const pcm_tuning candidates[3] = { { 4, 4, 1 }, { 8, 8, 2 }, { 16, 16, 4 } };
 		const uint64_t mask = (1ull << PCM_K_VELOCITY_FUSED) | (1ull << PCM_K_SPRING_COLLECT)
-							  | (1ull << PCM_K_SPRING_RESOLVE) | (1ull << PCM_K_INTERACT_TENTATIVE)
+							  | (1ull << PCM_K_SPRING_RESOLVE) | (1ull << PCM_K_SPRING_RESOLVE_FIRST)
+							  | (1ull << PCM_K_INTERACT_TENTATIVE)
 							  | (1ull << PCM_K_NEIGHBORS);
 		PCM_TRY(run(1)); // warm-up with the current shape
 		double vel[3] = { 1e300, 1e300, 1e300 }, ev[3] = { 1e300, 1e300, 1e300 }, nb[3] = { 1e300, 1e300, 1e300 };
@@ -1436,8 +1456,8 @@ public:
 			PCM_TRY(profile_end(ms.data(), cnt.data()));
 			auto avg = [&](int tag) { return cnt[tag] ? ms[tag] / (double)cnt[tag] : 0.0; };
 			vel[k] = std::min(vel[k], avg(PCM_K_VELOCITY_FUSED));
-			ev[k] = std::min(ev[k], avg(PCM_K_SPRING_COLLECT) + avg(PCM_K_SPRING_RESOLVE)
-										+ avg(PCM_K_INTERACT_TENTATIVE));
+			ev[k] = std::min(ev[k], avg(PCM_K_SPRING_COLLECT) + avg(PCM_K_SPRING_RESOLVE_FIRST)
+										+ avg(PCM_K_SPRING_RESOLVE) + avg(PCM_K_INTERACT_TENTATIVE));
 			nb[k] = std::min(nb[k], avg(PCM_K_NEIGHBORS));
 		}
 		auto best = [](const double* t) {

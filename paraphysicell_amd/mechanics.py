@@ -395,7 +395,7 @@ class DeviceSim(_LibSim):
         self._call("autotune", self._h, ctypes.byref(p), int(steps_per_candidate), ctypes.byref(t))
         return t.as_tuple()
 
-    STAT_NAMES = ("attach_events", "resolve_rounds", "resolve_chunks", "interaction_candidates",
+    STAT_NAMES = ("attach_events", "resolve_rounds", "attaching_agents", "interaction_candidates",
                   "interaction_replay", "removed", "occupied_voxels", "reserved")
 
     def stats(self) -> dict:
