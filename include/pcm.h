@@ -354,6 +354,10 @@ int pcm_kernel_name(int32_t kernel, const char** name);
  * removed, occupied mechanics voxels, reserved (0). */
 int pcm_get_stats(pcm_sim* sim, int32_t* stats);
 int pcm_profile_begin(pcm_sim* sim, uint64_t kernel_mask);
+/* Time one launch in `every` of each selected kernel (default 1): the event pair around a
+ * launch costs the stream a few microseconds, so a sampled profile disturbs the throughput
+ * it is taken from less.  pcm_profile_end reports the timed launches only. */
+int pcm_profile_every(pcm_sim* sim, int32_t every);
 int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus

@@ -66,6 +66,17 @@ __host__ __device__ constexpr int pcm_field_type(int f)
 	return types[f];
 }
 
+// Pointers into device memory.  In device code they carry the global address space, so
+// accesses compile to global_* instructions; generic (flat) ones would also count against
+// the LDS counter and make every LDS or cross-lane wait drain outstanding memory traffic.
+#if defined(__HIP_DEVICE_COMPILE__)
+	#define PCM_GLOBAL __attribute__((address_space(1)))
+#else
+	#define PCM_GLOBAL
+#endif
+template <class T>
+using gptr = T PCM_GLOBAL*;
+
 template <class R>
 struct SimCtx
 {
@@ -88,69 +99,68 @@ struct SimCtx
 	int div_internal;             // divisions halve internalized substrates
 	int chemotaxis; // some uploaded agent has a migration bias direction function
 	// substrate gradients on the diffusion grid (pcm_set_gradients), or null
-	const R* grad;
+	gptr<const R> grad;
 	int dvox[3], dgrid[3];
-	Scalars* sc;
+	gptr<Scalars> sc;
 
 	// agent fields (include/pcm.h order); fptr/fbytes give the generic view
-	void* fptr[PCM_FIELD_COUNT];
+	gptr<void> fptr[PCM_FIELD_COUNT];
 	int fbytes[PCM_FIELD_COUNT]; // bytes per agent
-	R *pos, *vel, *pvel, *radius, *nuc_radius, *surface, *volume, *fluid, *solid, *nuclear, *nuc_fluid, *nuc_solid,
-		*cyto, *cyto_fluid, *cyto_solid, *fluid_frac, *c2n, *target_cyto, *target_nuc;
-	R *adhesion, *repulsion, *bm_repulsion, *rel_adh, *affinity;
-	int* max_att;
-	R *elastic, *att_rate, *det_rate;
-	uint8_t* motile;
-	R *persistence, *speed, *bias_dir, *bias, *motility;
-	uint8_t* restrict2d;
-	uint8_t* dead;
-	R *dead_phago, *live_phago, *damage_rate, *attack, *immuno, *fusion;
-	R* pressure;
-	int* nuclei;
-	R *damage, *attack_time;
-	int* type;
-	uint8_t* movable;
-	uint8_t* flags;
-	R *internal, *frac_ingest;
-	int* origin;
-	uint8_t* contact; // PCM_CONTACT_* code per agent
-	int *chemo_index, *chemo_dir;
-	R* sensitivities;
-	uint8_t* bias_fn; // PCM_BIAS_* code per agent
-	R *orientation, *polarity;
-	int* div_list; // dividing agents of the last phenotype update, ascending
+	gptr<R> pos, vel, pvel, radius, nuc_radius, surface, volume, fluid, solid, nuclear, nuc_fluid, nuc_solid, cyto, cyto_fluid, cyto_solid, fluid_frac, c2n, target_cyto, target_nuc;
+	gptr<R> adhesion, repulsion, bm_repulsion, rel_adh, affinity;
+	gptr<int> max_att;
+	gptr<R> elastic, att_rate, det_rate;
+	gptr<uint8_t> motile;
+	gptr<R> persistence, speed, bias_dir, bias, motility;
+	gptr<uint8_t> restrict2d;
+	gptr<uint8_t> dead;
+	gptr<R> dead_phago, live_phago, damage_rate, attack, immuno, fusion;
+	gptr<R> pressure;
+	gptr<int> nuclei;
+	gptr<R> damage, attack_time;
+	gptr<int> type;
+	gptr<uint8_t> movable;
+	gptr<uint8_t> flags;
+	gptr<R> internal, frac_ingest;
+	gptr<int> origin;
+	gptr<uint8_t> contact; // PCM_CONTACT_* code per agent
+	gptr<int> chemo_index, chemo_dir;
+	gptr<R> sensitivities;
+	gptr<uint8_t> bias_fn; // PCM_BIAS_* code per agent
+	gptr<R> orientation, polarity;
+	gptr<int> div_list; // dividing agents of the last phenotype update, ascending
 
 	// link lists
-	int *spr_cnt, *spr_idx, *atc_cnt, *atc_idx;
+	gptr<int> spr_cnt, spr_idx, atc_cnt, atc_idx;
 	// neighbor lists
-	int *nbr_cnt, *nbr_idx;
+	gptr<int> nbr_cnt, nbr_idx;
 	// mechanics voxel grid
-	int *vcnt, *vstart, *vox_of, *vrank, *btmp, *bucket, *occ;
-	R* sorted; // 4 reals per bucket slot: x, y, z, rel*radius
-	R* frec;   // 8 reals per agent: x, y, z, radius, repulsion, adhesion, rel, type|movable<<30 bits
+	gptr<int> vcnt, vstart, vox_of, vrank, btmp, bucket, occ;
+	gptr<R> sorted; // 4 reals per bucket slot: x, y, z, rel*radius
+	gptr<R> frec;   // 8 reals per agent: x, y, z, radius, repulsion, adhesion, rel, type|movable<<30 bits
 
 	// spring scratch
-	uint32_t *fire, *erase;  // per agent slot masks (L <= 32)
-	int* own_tgt;            // per actor, K slots: partners of its attachment events, in slot order
-	uint8_t* own_st;         // per actor, K slots: 0 undecided, 1 success, 2 failure
-	int* own_cnt;            // attachment events of an actor this step
-	int* inbox;              // per agent, K (actor, k) pairs: events that target it
-	int* in_cnt;             // inbox fill (zero outside the spring phase)
-	int* att_list;           // actors with attachment events, a fixed segment per collect block
-	int* blk_cnt;            // actors listed in each collect block's segment
-	int* blk_ev;             // attachment events of each segment (counted by the first resolve round)
-	int* att_c;              // the actors of all segments, compacted by the resolver
-	int* att_c2;             // the resolver's next-round actors (beyond its LDS lists)
+	gptr<uint32_t> fire, erase;  // per agent slot masks (L <= 32)
+	gptr<int> own_tgt;            // per actor, K slots: partners of its attachment events, in slot order
+	gptr<uint8_t> own_st;         // per actor, K slots: 0 undecided, 1 success, 2 failure
+	gptr<int> own_cnt;            // attachment events of an actor this step
+	gptr<int> inbox;              // per agent, K (actor, k) pairs: events that target it
+	gptr<int> in_cnt;             // inbox fill (zero outside the spring phase)
+	gptr<int> att_list;           // actors with attachment events, a fixed segment per collect block
+	gptr<int> blk_cnt;            // actors listed in each collect block's segment
+	gptr<int> blk_ev;             // attachment events of each segment (counted by the first resolve round)
+	gptr<int> att_c;              // the actors of all segments, compacted by the resolver
+	gptr<int> att_c2;             // the resolver's next-round actors (beyond its LDS lists)
 	// interaction scratch
-	int* cand;               // candidate (actor, slot) pairs
-	int *n_rem_t, *n_atk_t;  // removal / attack events targeting an agent
-	int* rem_cnt;            // tentative removal events of an actor (<= 4), -1 = none recorded
-	int* rem_ev;             // 4 per actor: slot | kind << 24
-	int* atk_tgt;            // attacked agent of an actor, -1 = none
-	uint8_t* acted;          // actor reached a lock at least once
+	gptr<int> cand;               // candidate (actor, slot) pairs
+	gptr<int> n_rem_t, n_atk_t;  // removal / attack events targeting an agent
+	gptr<int> rem_cnt;            // tentative removal events of an actor (<= 4), -1 = none recorded
+	gptr<int> rem_ev;             // 4 per actor: slot | kind << 24
+	gptr<int> atk_tgt;            // attacked agent of an actor, -1 = none
+	gptr<uint8_t> acted;          // actor reached a lock at least once
 	// compaction scratch
-	int *bad, *remap, *seq, *move_src, *move_dst;
-	int* rm_log; // positions passed to remove_at, in the reference's order, since the last read
+	gptr<int> bad, remap, seq, move_src, move_dst;
+	gptr<int> rm_log; // positions passed to remove_at, in the reference's order, since the last read
 	int log_cap;
 	// compaction copy plan: one entry per 4-byte word (or byte) of an agent's fields
 	int mv_n;

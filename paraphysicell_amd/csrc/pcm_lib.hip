@@ -91,6 +91,7 @@ struct pcm_sim
 	virtual int removal_log(int32_t* positions, int64_t capacity, int64_t* length) = 0;
 	virtual int stats(int32_t* out) = 0;
 	virtual int profile_begin(uint64_t mask) = 0;
+	virtual int profile_every(int every) = 0;
 	virtual int profile_end(double* total_ms, int64_t* launches) = 0;
 	virtual int set_tuning(const pcm_tuning* t) = 0;
 	virtual int get_tuning(pcm_tuning* t) = 0;
@@ -717,7 +718,7 @@ public:
 			release((void*)h_.grad);
 			R* buf = nullptr;
 			PCM_TRY(alloc_t(&buf, std::max<size_t>(count, 1)));
-			h_.grad = buf;
+			h_.grad = (gptr<const R>)buf;
 			grad_count_ = count;
 		}
 		if (count > 0)
@@ -872,11 +873,15 @@ private:
 		}
 	}
 
-	int alloc(void** p, size_t bytes)
+	// zeroed device memory into any pointer type (global-address-space pointers included)
+	template <class P>
+	int alloc(P* p, size_t bytes)
 	{
-		PCM_HIP(hipMalloc(p, std::max<size_t>(bytes, 16)));
-		PCM_HIP(hipMemset(*p, 0, std::max<size_t>(bytes, 16)));
-		allocations_.push_back(*p);
+		void* v = nullptr;
+		PCM_HIP(hipMalloc(&v, std::max<size_t>(bytes, 16)));
+		PCM_HIP(hipMemset(v, 0, std::max<size_t>(bytes, 16)));
+		allocations_.push_back(v);
+		*p = (P)v;
 		return PCM_OK;
 	}
 
@@ -903,65 +908,65 @@ private:
 	{
 		auto& h = h_;
 		auto F = [&](int f) { return h.fptr[f]; };
-		h.pos = (R*)F(PCM_F_POSITIONS);
-		h.vel = (R*)F(PCM_F_VELOCITIES);
-		h.pvel = (R*)F(PCM_F_PREVIOUS_VELOCITIES);
-		h.radius = (R*)F(PCM_F_RADIUS);
-		h.nuc_radius = (R*)F(PCM_F_NUCLEAR_RADIUS);
-		h.surface = (R*)F(PCM_F_SURFACE_AREA);
-		h.volume = (R*)F(PCM_F_VOLUME);
-		h.fluid = (R*)F(PCM_F_FLUID);
-		h.solid = (R*)F(PCM_F_SOLID);
-		h.nuclear = (R*)F(PCM_F_NUCLEAR);
-		h.nuc_fluid = (R*)F(PCM_F_NUCLEAR_FLUID);
-		h.nuc_solid = (R*)F(PCM_F_NUCLEAR_SOLID);
-		h.cyto = (R*)F(PCM_F_CYTOPLASMIC);
-		h.cyto_fluid = (R*)F(PCM_F_CYTOPLASMIC_FLUID);
-		h.cyto_solid = (R*)F(PCM_F_CYTOPLASMIC_SOLID);
-		h.fluid_frac = (R*)F(PCM_F_FLUID_FRACTION);
-		h.c2n = (R*)F(PCM_F_CYTOPLASMIC_TO_NUCLEAR_RATIO);
-		h.target_cyto = (R*)F(PCM_F_TARGET_SOLID_CYTOPLASMIC);
-		h.target_nuc = (R*)F(PCM_F_TARGET_SOLID_NUCLEAR);
-		h.adhesion = (R*)F(PCM_F_CELL_CELL_ADHESION_STRENGTH);
-		h.repulsion = (R*)F(PCM_F_CELL_CELL_REPULSION_STRENGTH);
-		h.bm_repulsion = (R*)F(PCM_F_CELL_BM_REPULSION_STRENGTH);
-		h.rel_adh = (R*)F(PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE);
-		h.affinity = (R*)F(PCM_F_CELL_ADHESION_AFFINITIES);
-		h.max_att = (int*)F(PCM_F_MAXIMUM_NUMBER_OF_ATTACHMENTS);
-		h.elastic = (R*)F(PCM_F_ATTACHMENT_ELASTIC_CONSTANT);
-		h.att_rate = (R*)F(PCM_F_ATTACHMENT_RATE);
-		h.det_rate = (R*)F(PCM_F_DETACHMENT_RATE);
-		h.motile = (uint8_t*)F(PCM_F_IS_MOTILE);
-		h.persistence = (R*)F(PCM_F_PERSISTENCE_TIME);
-		h.speed = (R*)F(PCM_F_MIGRATION_SPEED);
-		h.bias_dir = (R*)F(PCM_F_MIGRATION_BIAS_DIRECTION);
-		h.bias = (R*)F(PCM_F_MIGRATION_BIAS);
-		h.motility = (R*)F(PCM_F_MOTILITY_VECTOR);
-		h.restrict2d = (uint8_t*)F(PCM_F_RESTRICT_TO_2D);
-		h.dead = (uint8_t*)F(PCM_F_DEAD);
-		h.dead_phago = (R*)F(PCM_F_DEAD_PHAGOCYTOSIS_RATE);
-		h.live_phago = (R*)F(PCM_F_LIVE_PHAGOCYTOSIS_RATES);
-		h.damage_rate = (R*)F(PCM_F_DAMAGE_RATE);
-		h.attack = (R*)F(PCM_F_ATTACK_RATES);
-		h.immuno = (R*)F(PCM_F_IMMUNOGENICITIES);
-		h.fusion = (R*)F(PCM_F_FUSION_RATES);
-		h.pressure = (R*)F(PCM_F_SIMPLE_PRESSURE);
-		h.nuclei = (int*)F(PCM_F_NUMBER_OF_NUCLEI);
-		h.damage = (R*)F(PCM_F_DAMAGE);
-		h.attack_time = (R*)F(PCM_F_TOTAL_ATTACK_TIME);
-		h.type = (int*)F(PCM_F_CELL_DEFINITION_INDEX);
-		h.movable = (uint8_t*)F(PCM_F_IS_MOVABLE);
-		h.flags = (uint8_t*)F(PCM_F_FLAGS);
-		h.internal = (R*)F(PCM_F_INTERNALIZED_SUBSTRATES);
-		h.frac_ingest = (R*)F(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED);
-		h.origin = (int*)F(PCM_F_ORIGIN);
-		h.contact = (uint8_t*)F(PCM_F_CONTACT_FUNCTION);
-		h.chemo_index = (int*)F(PCM_F_CHEMOTAXIS_INDEX);
-		h.chemo_dir = (int*)F(PCM_F_CHEMOTAXIS_DIRECTION);
-		h.sensitivities = (R*)F(PCM_F_CHEMOTACTIC_SENSITIVITIES);
-		h.bias_fn = (uint8_t*)F(PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION);
-		h.orientation = (R*)F(PCM_F_ORIENTATION);
-		h.polarity = (R*)F(PCM_F_POLARITY);
+		h.pos = (gptr<R>)F(PCM_F_POSITIONS);
+		h.vel = (gptr<R>)F(PCM_F_VELOCITIES);
+		h.pvel = (gptr<R>)F(PCM_F_PREVIOUS_VELOCITIES);
+		h.radius = (gptr<R>)F(PCM_F_RADIUS);
+		h.nuc_radius = (gptr<R>)F(PCM_F_NUCLEAR_RADIUS);
+		h.surface = (gptr<R>)F(PCM_F_SURFACE_AREA);
+		h.volume = (gptr<R>)F(PCM_F_VOLUME);
+		h.fluid = (gptr<R>)F(PCM_F_FLUID);
+		h.solid = (gptr<R>)F(PCM_F_SOLID);
+		h.nuclear = (gptr<R>)F(PCM_F_NUCLEAR);
+		h.nuc_fluid = (gptr<R>)F(PCM_F_NUCLEAR_FLUID);
+		h.nuc_solid = (gptr<R>)F(PCM_F_NUCLEAR_SOLID);
+		h.cyto = (gptr<R>)F(PCM_F_CYTOPLASMIC);
+		h.cyto_fluid = (gptr<R>)F(PCM_F_CYTOPLASMIC_FLUID);
+		h.cyto_solid = (gptr<R>)F(PCM_F_CYTOPLASMIC_SOLID);
+		h.fluid_frac = (gptr<R>)F(PCM_F_FLUID_FRACTION);
+		h.c2n = (gptr<R>)F(PCM_F_CYTOPLASMIC_TO_NUCLEAR_RATIO);
+		h.target_cyto = (gptr<R>)F(PCM_F_TARGET_SOLID_CYTOPLASMIC);
+		h.target_nuc = (gptr<R>)F(PCM_F_TARGET_SOLID_NUCLEAR);
+		h.adhesion = (gptr<R>)F(PCM_F_CELL_CELL_ADHESION_STRENGTH);
+		h.repulsion = (gptr<R>)F(PCM_F_CELL_CELL_REPULSION_STRENGTH);
+		h.bm_repulsion = (gptr<R>)F(PCM_F_CELL_BM_REPULSION_STRENGTH);
+		h.rel_adh = (gptr<R>)F(PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE);
+		h.affinity = (gptr<R>)F(PCM_F_CELL_ADHESION_AFFINITIES);
+		h.max_att = (gptr<int>)F(PCM_F_MAXIMUM_NUMBER_OF_ATTACHMENTS);
+		h.elastic = (gptr<R>)F(PCM_F_ATTACHMENT_ELASTIC_CONSTANT);
+		h.att_rate = (gptr<R>)F(PCM_F_ATTACHMENT_RATE);
+		h.det_rate = (gptr<R>)F(PCM_F_DETACHMENT_RATE);
+		h.motile = (gptr<uint8_t>)F(PCM_F_IS_MOTILE);
+		h.persistence = (gptr<R>)F(PCM_F_PERSISTENCE_TIME);
+		h.speed = (gptr<R>)F(PCM_F_MIGRATION_SPEED);
+		h.bias_dir = (gptr<R>)F(PCM_F_MIGRATION_BIAS_DIRECTION);
+		h.bias = (gptr<R>)F(PCM_F_MIGRATION_BIAS);
+		h.motility = (gptr<R>)F(PCM_F_MOTILITY_VECTOR);
+		h.restrict2d = (gptr<uint8_t>)F(PCM_F_RESTRICT_TO_2D);
+		h.dead = (gptr<uint8_t>)F(PCM_F_DEAD);
+		h.dead_phago = (gptr<R>)F(PCM_F_DEAD_PHAGOCYTOSIS_RATE);
+		h.live_phago = (gptr<R>)F(PCM_F_LIVE_PHAGOCYTOSIS_RATES);
+		h.damage_rate = (gptr<R>)F(PCM_F_DAMAGE_RATE);
+		h.attack = (gptr<R>)F(PCM_F_ATTACK_RATES);
+		h.immuno = (gptr<R>)F(PCM_F_IMMUNOGENICITIES);
+		h.fusion = (gptr<R>)F(PCM_F_FUSION_RATES);
+		h.pressure = (gptr<R>)F(PCM_F_SIMPLE_PRESSURE);
+		h.nuclei = (gptr<int>)F(PCM_F_NUMBER_OF_NUCLEI);
+		h.damage = (gptr<R>)F(PCM_F_DAMAGE);
+		h.attack_time = (gptr<R>)F(PCM_F_TOTAL_ATTACK_TIME);
+		h.type = (gptr<int>)F(PCM_F_CELL_DEFINITION_INDEX);
+		h.movable = (gptr<uint8_t>)F(PCM_F_IS_MOVABLE);
+		h.flags = (gptr<uint8_t>)F(PCM_F_FLAGS);
+		h.internal = (gptr<R>)F(PCM_F_INTERNALIZED_SUBSTRATES);
+		h.frac_ingest = (gptr<R>)F(PCM_F_FRACTION_TRANSFERRED_WHEN_INGESTED);
+		h.origin = (gptr<int>)F(PCM_F_ORIGIN);
+		h.contact = (gptr<uint8_t>)F(PCM_F_CONTACT_FUNCTION);
+		h.chemo_index = (gptr<int>)F(PCM_F_CHEMOTAXIS_INDEX);
+		h.chemo_dir = (gptr<int>)F(PCM_F_CHEMOTAXIS_DIRECTION);
+		h.sensitivities = (gptr<R>)F(PCM_F_CHEMOTACTIC_SENSITIVITIES);
+		h.bias_fn = (gptr<uint8_t>)F(PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION);
+		h.orientation = (gptr<R>)F(PCM_F_ORIENTATION);
+		h.polarity = (gptr<R>)F(PCM_F_POLARITY);
 	}
 
 	int alloc_neighbor_storage()
@@ -990,17 +995,18 @@ private:
 		return push_ctx();
 	}
 
-	int grow_rows(int** idx, int oldW, int newW)
+	template <class P>
+	int grow_rows(P* idx, int oldW, int newW)
 	{
 		const size_t cap = std::max(1, h_.cap);
 		std::vector<int> rows(cap * oldW), grown(cap * newW, -1);
-		PCM_HIP(hipMemcpy(rows.data(), *idx, sizeof(int) * cap * oldW, hipMemcpyDeviceToHost));
+		PCM_HIP(hipMemcpy(rows.data(), (void*)*idx, sizeof(int) * cap * oldW, hipMemcpyDeviceToHost));
 		for (size_t i = 0; i < cap; i++)
 			for (int s = 0; s < oldW; s++)
 				grown[i * newW + s] = rows[i * oldW + s];
-		release(*idx);
-		PCM_TRY(alloc_t(idx, cap * newW));
-		PCM_HIP(hipMemcpy(*idx, grown.data(), sizeof(int) * cap * newW, hipMemcpyHostToDevice));
+		release((void*)*idx);
+		PCM_TRY(alloc(idx, sizeof(int) * cap * newW));
+		PCM_HIP(hipMemcpy((void*)*idx, grown.data(), sizeof(int) * cap * newW, hipMemcpyHostToDevice));
 		return PCM_OK;
 	}
 
@@ -1072,7 +1078,7 @@ private:
 	template <class Kern>
 	int launch(int tag, Kern kernel, int blocks, int threads)
 	{
-		const bool timed = (prof_mask_ >> tag) & 1ull;
+		const bool timed = ((prof_mask_ >> tag) & 1ull) && prof_seen_[tag]++ % prof_every_ == 0;
 		hipEvent_t ev[2] = { nullptr, nullptr };
 		if (timed)
 		{
@@ -1109,6 +1115,8 @@ private:
 		hipEvent_t start, stop;
 	};
 	uint64_t prof_mask_ = 0;
+	int prof_every_ = 1;                    // time one launch in this many of each kernel
+	long long prof_seen_[PCM_KERNEL_COUNT] = {}; // launches of each kernel while profiling
 	std::vector<prof_record> prof_records_;
 	std::vector<hipEvent_t> event_pool_;
 
@@ -1138,6 +1146,16 @@ public:
 		}
 		prof_records_.clear();
 		prof_mask_ = mask;
+		for (auto& k : prof_seen_)
+			k = 0;
+		return PCM_OK;
+	}
+
+	int profile_every(int every) override
+	{
+		if (every < 1)
+			return fail(PCM_ERR_ARGUMENT, "profile sampling stride must be >= 1");
+		prof_every_ = every;
 		return PCM_OK;
 	}
 
@@ -2253,6 +2271,12 @@ int pcm_kernel_name(int32_t kernel, const char** name)
 		return fail(PCM_ERR_ARGUMENT, "bad kernel id");
 	*name = names[kernel];
 	return PCM_OK;
+}
+
+int pcm_profile_every(pcm_sim* sim, int32_t every)
+{
+	PCM_NEED(sim);
+	return sim->profile_every(every);
 }
 
 int pcm_profile_begin(pcm_sim* sim, uint64_t kernel_mask)

@@ -357,6 +357,18 @@ class DeviceSim(_LibSim):
     def synchronize(self):
         self._call("synchronize", self._h)
 
+    def removal_log(self) -> np.ndarray:
+        """Positions passed to the reference's remove_at since the last read, in order
+        (pcm_removal_log); reading restarts the log."""
+        fn = self._fn("removal_log")
+        fn.argtypes = [_PTR, _PTR, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        fn.restype = ctypes.c_int
+        length = ctypes.c_int64()
+        self._call("removal_log", self._h, None, 0, ctypes.byref(length))
+        out = np.zeros(max(length.value, 1), np.int32)
+        self._call("removal_log", self._h, out.ctypes.data, length.value, ctypes.byref(length))
+        return out[: length.value]
+
     def pack_fields(self, names, widths=None) -> np.ndarray:
         """The named fields of every agent as one (count, columns) float64 matrix packed on
         the device (pcm_pack_fields); widths pad vectors (0: the field's own width)."""
@@ -412,6 +424,13 @@ class DeviceSim(_LibSim):
         for k in kernels:
             mask |= 1 << names.index(k)
         self._call("profile_begin", self._h, ctypes.c_uint64(mask))
+
+    def profile_every(self, every: int) -> None:
+        """Time one launch in `every` of each profiled kernel."""
+        fn = self._fn("profile_every")
+        fn.argtypes = [_PTR, ctypes.c_int32]
+        fn.restype = ctypes.c_int
+        self._call("profile_every", self._h, int(every))
 
     def profile_end(self) -> dict:
         """{kernel name: (total ms, launches)} for the kernels timed since profile_begin."""

@@ -296,3 +296,18 @@ def test_pack_fields_is_the_field_matrix(dims):
             a = np.hstack([a, np.zeros((n, w - a.shape[1]))])
         cols.append(a)
     assert np.array_equal(got, np.hstack(cols))
+
+
+def test_removal_log_replays_the_compaction():
+    """Replaying pcm_removal_log's positions as swap-with-last removals on the uploaded
+    order gives the device's final order (ORIGIN)."""
+    mesh, st, p = parity.rich_population(3, 1200, 95, 4, rates="busy")
+    dev = device(mesh, st, seed=6)
+    dev.initialize_mesh()
+    dev.removal_log()
+    dev.step(p, 3)
+    order = list(range(st.count))
+    for i in dev.removal_log():
+        order[i] = order[-1]
+        order.pop()
+    assert np.array_equal(np.array(order, np.int32), dev.get_field("origin"))
