@@ -334,7 +334,9 @@ int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t 
  * simulator_measurer.h:5-33), at kernel granularity: while a profile is open, every launch
  * of a kernel whose bit is set in `kernel_mask` is bracketed by HIP events on the handle's
  * stream; pcm_profile_end waits for the work and returns, per kernel id, the summed event
- * time in milliseconds and the launch count (arrays of PCM_KERNEL_COUNT entries). */
+ * time in milliseconds and the launch count (arrays of PCM_KERNEL_COUNT entries).
+ * INTERACT_CHECK is no longer launched (the interaction pass detects its own conflicts);
+ * the id is kept so that the other ids do not change. */
 #define PCM_KERNEL_LIST(X)                                                                                             \
 	X(BEGIN_STEP) X(BIN_CLEAR) X(BIN_COUNT) X(BIN_CLAIM) X(BIN_SCATTER) X(BIN_ORDER) X(NEIGHBORS) X(FORCES)              \
 	X(MOTILITY) X(MEMBRANE) X(CONTRACT) X(POSITIONS) X(VELOCITY_FUSED) X(SPRING_FIRE) X(SPRING_ERASE)                   \

@@ -153,11 +153,10 @@ struct SimCtx
 	gptr<int> att_c2;             // the resolver's next-round actors (beyond its LDS lists)
 	// interaction scratch
 	gptr<int> cand;               // candidate (actor, slot) pairs
-	gptr<int> n_rem_t, n_atk_t;  // removal / attack events targeting an agent
+	gptr<unsigned long long> ev_t; // interaction events of an agent (EVW_* fields), raised atomically
 	gptr<int> rem_cnt;            // tentative removal events of an actor (<= 4), -1 = none recorded
 	gptr<int> rem_ev;             // 4 per actor: slot | kind << 24
 	gptr<int> atk_tgt;            // attacked agent of an actor, -1 = none
-	gptr<uint8_t> acted;          // actor reached a lock at least once
 	// compaction scratch
 	gptr<int> bad, remap, seq, move_src, move_dst;
 	gptr<int> rm_log; // positions passed to remove_at, in the reference's order, since the last read

@@ -750,7 +750,9 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 //          entries of p in a's list);
 //   a > p: erased iff s fired, or s is among the first f_p entries of p in a's list
 //          (f_p = fired entries of a in p's list).
-template <class R>
+// STARTS_STEP: the kernel opens a mechanics step, whose counter is one above the stored
+// one until k_spring_erase stores it (k_begin_step's work, without its launch)
+template <class R, bool STARTS_STEP>
 __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
@@ -761,7 +763,7 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
 		return;
 	const int cnt = c.spr_cnt[i];
 	const R rate = c.det_rate[i] * c.dt;
-	const unsigned long long step = c.sc->step;
+	const unsigned long long step = c.sc->step + (STARTS_STEP ? 1 : 0);
 	uint32_t mask = 0;
 	for (int s = 0; s < cnt; s++)
 	{
@@ -773,11 +775,13 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
 	c.erase[i] = 0;
 }
 
-template <class R>
+template <class R, bool STARTS_STEP>
 __global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
 	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (STARTS_STEP && t == 0)
+		c.sc->step++; // no thread of this kernel reads the counter
 	const int i = (int)(t / c.L), s = (int)(t % c.L);
 	const int n = c.sc->n;
 	if (i >= n)
@@ -1206,14 +1210,23 @@ __device__ inline int block_chunk_end(const int* rec, int stride, int count, int
 // neighbor, records the slots where any branch can fire (candidates) and plays the
 // reference's branch logic against the flags as they stand before the phase.  The
 // outcome equals the reference's order-dependent result unless agents interact through
-// more than one event (an agent removed and also acting, attacked or removed twice);
-// k_interact_check detects that, and then k_interact_serial replays all candidates in
-// the reference's order on one thread instead of k_interact_apply.
+// more than one event (an agent removed and also acting, attacked or removed twice).
+// Every event is an atomic read-modify-write of one word per agent (removals targeting
+// it, attacks targeting it, whether it acted): of two events on one agent the later one
+// sees the earlier, so the pass itself raises the conflict flag, and then
+// k_interact_serial replays all candidates in the reference's order on one thread
+// instead of k_interact_apply.
 enum : int
 {
 	EV_INGEST = 0,
 	EV_FUSE = 2
 };
+constexpr unsigned long long EVW_REMOVE = 1ull;                 // bits 0-31: removals targeting the agent
+constexpr unsigned long long EVW_REMOVE_MASK = 0xffffffffull;
+constexpr unsigned long long EVW_ATTACK = 1ull << 32;           // bits 32-62: attacks targeting the agent
+constexpr unsigned long long EVW_ACTED = 1ull << 63;            // the agent acted
+__device__ __forceinline__ int evw_removals(unsigned long long w) { return (int)(w & EVW_REMOVE_MASK); }
+__device__ __forceinline__ int evw_attacks(unsigned long long w) { return (int)((w >> 32) & 0x7fffffffull); }
 
 // AB2: first apply the agent's position update (update_positions precedes the interactions
 // and this pass reads no positions).  G lanes share an agent: lane l loads and draws for
@@ -1334,7 +1347,9 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 					if (lane == 0)
 					{
 						c.atk_tgt[i] = jj;
-						atomicAdd(&c.n_atk_t[jj], 1);
+						// an attack conflicts with a removal of the same target
+						if (atomicAdd(&c.ev_t[jj], EVW_ATTACK) & EVW_REMOVE_MASK)
+							c.sc->conflict = 1;
 					}
 					attacked_once = true;
 					continue;
@@ -1348,7 +1363,9 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 				}
 				if (lane == 0)
 				{
-					atomicAdd(&c.n_rem_t[jj], 1);
+					// a removal conflicts with any other event on its target
+					if (atomicAdd(&c.ev_t[jj], EVW_REMOVE) != 0)
+						c.sc->conflict = 1;
 					if (n_rem < 4)
 						c.rem_ev[4 * i + n_rem] = ss | ((kind == 2 ? EV_FUSE : EV_INGEST) << 24);
 					else
@@ -1361,20 +1378,10 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 	if (lane == 0)
 	{
 		c.rem_cnt[i] = n_rem;
-		c.acted[i] = acted;
+		// an actor conflicts with a removal targeting it
+		if (acted && (atomicOr(&c.ev_t[i], EVW_ACTED) & EVW_REMOVE_MASK))
+			c.sc->conflict = 1;
 	}
-}
-
-template <class R>
-__global__ void k_interact_check(const SimCtx<R>* __restrict__ ctxs)
-{
-	PCM_PROLOGUE(c)
-	const int i = gid();
-	if (i >= c.sc->n)
-		return;
-	const int r = c.n_rem_t[i];
-	if (r >= 1 && (r >= 2 || c.n_atk_t[i] >= 1 || c.acted[i]))
-		c.sc->conflict = 1;
 }
 
 // out of the box of the microenvironment mesh (containers_solver.cpp:89-96)
@@ -1415,7 +1422,8 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 		else
 			ingest<R>(c, i, j);
 	}
-	const int na = c.n_atk_t[i];
+	const unsigned long long w = c.ev_t[i];
+	const int na = evw_attacks(w);
 	if (na > 0)
 	{
 		// attackers, ascending: the reference applies attacks in actor order
@@ -1439,9 +1447,8 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 		}
 	}
 	if (FLAG)
-		removed = c.n_rem_t[i] > 0 || c.flags[i] == PCM_FLAG_TO_REMOVE || out_of_bounds<R, D>(c, i);
-	c.n_rem_t[i] = 0;
-	c.n_atk_t[i] = 0;
+		removed = evw_removals(w) > 0 || c.flags[i] == PCM_FLAG_TO_REMOVE || out_of_bounds<R, D>(c, i);
+	c.ev_t[i] = 0;
 	}
 	if (FLAG)
 	{
@@ -1579,13 +1586,13 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCt
 			__syncthreads();
 			lo = hi;
 		}
-		// clear the per-target counters the tentative pass raised
+		// clear the event words the tentative pass raised (on targets and on actors)
 		for (int e = threadIdx.x; e < C; e += blockDim.x)
 		{
 			const int i = c.cand[2 * e];
 			const int j = c.nbr_idx[(size_t)i * c.K + c.cand[2 * e + 1]];
-			c.n_rem_t[j] = 0;
-			c.n_atk_t[j] = 0;
+			c.ev_t[j] = 0;
+			c.ev_t[i] = 0;
 		}
 	}
 	__syncthreads();

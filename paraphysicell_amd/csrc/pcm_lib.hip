@@ -237,12 +237,10 @@ public:
 		PCM_TRY(alloc_t(&h.att_c2, cap));
 		PCM_TRY(alloc_t(&h.fire, cap));
 		PCM_TRY(alloc_t(&h.erase, cap));
-		PCM_TRY(alloc_t(&h.n_rem_t, cap));
-		PCM_TRY(alloc_t(&h.n_atk_t, cap));
+		PCM_TRY(alloc_t(&h.ev_t, cap));
 		PCM_TRY(alloc_t(&h.rem_cnt, cap));
 		PCM_TRY(alloc_t(&h.rem_ev, (size_t)cap * 4));
 		PCM_TRY(alloc_t(&h.atk_tgt, cap));
-		PCM_TRY(alloc_t(&h.acted, cap));
 		h.bad_cap = next_pow2(cap);
 		PCM_TRY(alloc_t(&h.bad, (size_t)h.bad_cap));
 		PCM_TRY(alloc_t(&h.remap, cap));
@@ -1267,10 +1265,13 @@ private:
 		}
 	}
 
+	// STARTS_STEP: these kernels open a mechanics step, so they also advance the step
+	// counter (the detachment draws use the new value; the erase kernel stores it)
+	template <bool STARTS_STEP>
 	int spring_update_kernels()
 	{
-		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R>));
-		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
+		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R, STARTS_STEP>));
+		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R, STARTS_STEP>,
 					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK));
 		// the first resolve round runs one block per collect block (segments of BLOCK / G)
 		const int segments = std::max(1, (int)(((long long)gn() * tune_.event_lanes + BLOCK - 1) / BLOCK));
@@ -1296,7 +1297,6 @@ private:
 	int interaction_kernels()
 	{
 		PCM_TRY((launch_tentative<D, false>()));
-		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, false>));
 		return launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, false>, 1, RESOLVE_THREADS);
 	}
@@ -1349,7 +1349,7 @@ private:
 			case PH_SPRINGS:
 				if (!h_.springs)
 					return PCM_OK;
-				PCM_TRY(spring_update_kernels());
+				PCM_TRY(spring_update_kernels<false>());
 				return per_agent(PCM_K_CONTRACT, k_contract<R, D>);
 			case PH_POSITIONS:
 				mesh_fresh_ = false;
@@ -1370,9 +1370,10 @@ private:
 	template <int D>
 	int step_d(bool records_fresh, bool fold_clear)
 	{
-		PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 		if (h_.springs)
-			PCM_TRY(spring_update_kernels());
+			PCM_TRY(spring_update_kernels<true>());
+		else
+			PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 		if (!records_fresh)
 			PCM_TRY(per_agent(PCM_K_FORCE_RECORDS, k_force_records<R, D>));
 		// fold_clear: the previous mesh update's voxel counts are cleared by the velocity
@@ -1382,7 +1383,6 @@ private:
 		else
 			PCM_TRY((launch_velocity<D, false>()));
 		PCM_TRY((launch_tentative<D, true>()));
-		PCM_TRY(per_agent(PCM_K_INTERACT_CHECK, k_interact_check<R>));
 		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, true>));
 		PCM_TRY(launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, true>, 1, RESOLVE_THREADS));
 		PCM_TRY(compact_apply_kernel());
