@@ -188,7 +188,49 @@ __global__ void k_force_records(const SimCtx<R>* __restrict__ ctxs)
 template <class R>
 constexpr int nbr_tile()
 {
-	return sizeof(R) == 4 ? 448 : 320; // candidates staged per wavefront
+	return sizeof(R) == 4 ? 448 : 320; // candidates staged per wavefront (a multiple of 64)
+}
+
+// a coordinate whose square overflows: no staged padding record is within reach
+template <class R>
+constexpr R far_coordinate()
+{
+	return sizeof(R) == 4 ? (R)1e30 : (R)1e300;
+}
+
+// pair_within for a staged candidate record (position, rel_adh * radius) against the
+// agent's: the same rounding, with within_reach's decision written so that the common
+// outcomes need no branch (the square root runs only for lanes within the rounding band)
+template <class R>
+__device__ __forceinline__ bool within_reach_lanes(R s, R reach)
+{
+	const R e = sizeof(R) == 4 ? (R)1e-5 : (R)1e-12;
+	const R r2 = reach * reach;
+	const bool normal = r2 >= (sizeof(R) == 4 ? (R)1.2e-38 : (R)2.3e-308);
+	const bool beyond = s > r2 * ((R)1 + e);
+	const bool inside = s < r2 * ((R)1 - e);
+	bool in = !beyond;
+	if (!normal || !(beyond || inside))
+		in = sqrt(s) <= reach;
+	return in;
+}
+
+template <int D, class R>
+__device__ __forceinline__ bool candidate_within(const R* pi, const R* q)
+{
+	if constexpr (D == 1)
+		return fabs(pi[0] - q[0]) <= pi[3] + q[3];
+	else
+	{
+		const R d0 = pi[0] - q[0], d1 = pi[1] - q[1];
+		R s = d0 * d0 + d1 * d1;
+		if constexpr (D == 3)
+		{
+			const R d2 = pi[2] - q[2];
+			s = s + d2 * d2;
+		}
+		return within_reach_lanes<R>(s, pi[3] + q[3]);
+	}
 }
 
 // One agent's list straight from HBM.  DIRECT: the buckets may predate the current
@@ -301,8 +343,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))
 				s_pre[27] = C;
 		}
 		__syncthreads();
-		const int C = s_pre[27];
-		const int m = s_pre[14] - s_pre[13]; // the centre voxel is stencil entry 13
+		const int C = __builtin_amdgcn_readfirstlane(s_pre[27]);
+		const int m = __builtin_amdgcn_readfirstlane(s_pre[14] - s_pre[13]); // the centre voxel is stencil entry 13
 		if (C <= TILE)
 		{
 			// stage the stencil in stencil order; a candidate's stencil entry is the number
@@ -324,36 +366,35 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))
 				for (int k = 0; k < 4; k++)
 					s_rec[t * 4 + k] = rec[k];
 			}
+			// the rest of the last 64-candidate tile: far-away records, so that the test
+			// below runs on every lane with no bounds branch (their lanes are masked too)
+			for (int t = C + tid; t < ((C + 63) & ~63); t += 64 * W)
+			{
+				for (int k = 0; k < 3; k++)
+					s_rec[t * 4 + k] = far_coordinate<R>();
+				s_rec[t * 4 + 3] = 0;
+			}
 			__syncthreads();
 			const int own0 = __builtin_amdgcn_readlane(lane_pre, 13);
 			for (int a = wave; a < m; a += W)
 			{
-				const int i = s_idx[own0 + a];
+				const int self_t = own0 + a;
+				const int i = __builtin_amdgcn_readfirstlane(s_idx[self_t]);
 				R pi[4];
 				for (int k = 0; k < 4; k++)
-					pi[k] = s_rec[(own0 + a) * 4 + k];
+					pi[k] = s_rec[self_t * 4 + k];
 				int* out = c.nbr_idx + (size_t)i * c.K;
 				int base = 0;
 				for (int t0 = 0; t0 < C; t0 += 64)
 				{
 					const int t = t0 + lane;
-					bool pass = false;
-					int j = -1;
-					if (t < C)
-					{
-						j = s_idx[t];
-						if (j != i)
-						{
-							const R* q = s_rec + t * 4;
-							pass = pair_within<D, R>(pi, q, pi[3] + q[3]);
-						}
-					}
+					const bool pass = candidate_within<D, R>(pi, s_rec + t * 4) && t != self_t && t < C;
 					const unsigned long long mask = __ballot(pass);
 					if (pass)
 					{
 						const int slot = base + __popcll(mask & below);
 						if (slot < c.K)
-							out[slot] = j;
+							out[slot] = s_idx[t];
 					}
 					base += __popcll(mask);
 				}
@@ -565,6 +606,73 @@ __device__ __forceinline__ R pair_force_rec(const force_rec<R>& a, const force_r
 	return term;
 }
 
+// Accepted attachments appended to agent a's spring list (the append step of
+// update_spring_attachments_internal, position_solver.cpp:430-449): incoming partners in
+// ascending actor order, then the agent's own accepted events in event order.  The G
+// lanes of a group run the same loops; entry e is stored by lane e % G, which is the lane
+// that reads it back in the contraction loop, and lane 0 stores the counters.  Returns the
+// list length.
+template <class R, int G>
+__device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, int lane)
+{
+	const int nin = min(c.in_cnt[a], c.K);
+	const int na = c.own_cnt[a];
+	int cnt = c.spr_cnt[a];
+	if (nin + na == 0)
+		return cnt;
+	int* out = c.spr_idx + (size_t)a * c.L;
+	bool overflow = false;
+	int last = -1;
+	for (int found = 0; found < nin; found++)
+	{
+		int best = 0x7fffffff, best_k = 0;
+		for (int q = 0; q < nin; q++)
+		{
+			const int x = c.inbox[2 * ((size_t)a * c.K + q)];
+			if (x > last && x < best)
+			{
+				best = x;
+				best_k = c.inbox[2 * ((size_t)a * c.K + q) + 1];
+			}
+		}
+		last = best;
+		if (c.own_st[(size_t)best * c.K + best_k] == 1)
+		{
+			if (cnt < c.L)
+			{
+				if (cnt % G == lane)
+					out[cnt] = best;
+			}
+			else
+				overflow = true;
+			cnt++;
+		}
+	}
+	for (int k = 0; k < na; k++)
+		if (c.own_st[(size_t)a * c.K + k] == 1)
+		{
+			const int j = c.own_tgt[(size_t)a * c.K + k];
+			if (cnt < c.L)
+			{
+				if (cnt % G == lane)
+					out[cnt] = j;
+			}
+			else
+				overflow = true;
+			cnt++;
+		}
+	cnt = min(cnt, c.L);
+	if (lane == 0)
+	{
+		if (overflow)
+			atomicOr(&c.sc->status, ST_LIST_OVERFLOW);
+		c.spr_cnt[a] = cnt;
+		c.in_cnt[a] = 0;
+		c.own_cnt[a] = 0;
+	}
+	return cnt;
+}
+
 // Forces, motility, basement membrane and spring contraction of one agent in the
 // reference's summation order (simulator.cpp:173-176), written to the velocity; neighbor
 // and spring-partner data come from the force records.  G lanes share an agent: lane s of
@@ -688,10 +796,11 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 		motility_apply<D, R>(c, i, c.sc->step, load_motility<D, R>(c, i), v, lane == 0);
 	if (mov && c.wall)
 		membrane_apply<D, R>(c, me.r, c.bm_repulsion[i], p, v);
+	// the step's attachments join the list first (no separate append launch in a step)
+	const int ns = c.springs ? spring_append_group<R, G>(c, i, lane) : 0;
 	if (mov && c.springs)
 	{
 		// spring_contract_function (position_solver.cpp:337-370)
-		const int ns = c.spr_cnt[i];
 		const R ki = c.elastic[i];
 		const int* sp = c.spr_idx + (size_t)i * c.L;
 		for (int base = 0; base < ns; base += B * G)

@@ -1265,13 +1265,14 @@ private:
 		}
 	}
 
-	// STARTS_STEP: these kernels open a mechanics step, so they also advance the step
-	// counter (the detachment draws use the new value; the erase kernel stores it)
-	template <bool STARTS_STEP>
+	// IN_STEP: the kernels open a fused mechanics step, so they also advance the step
+	// counter (the detachment draws use the new value; the erase kernel stores it), and the
+	// step's velocity kernel appends the accepted attachments instead of k_spring_append
+	template <bool IN_STEP>
 	int spring_update_kernels()
 	{
-		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R, STARTS_STEP>));
-		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R, STARTS_STEP>,
+		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R, IN_STEP>));
+		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R, IN_STEP>,
 					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK));
 		// the first resolve round runs one block per collect block (segments of BLOCK / G)
 		const int segments = std::max(1, (int)(((long long)gn() * tune_.event_lanes + BLOCK - 1) / BLOCK));
@@ -1290,7 +1291,7 @@ private:
 				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 8>, segments, 64));
 		}
 		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
-		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
+		return IN_STEP ? PCM_OK : per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
 	}
 
 	template <int D>
