@@ -299,8 +299,17 @@ __device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
 	c.nbr_cnt[i] = cnt;
 }
 
+// the path for stencils larger than the LDS tile, out of line so that its registers do
+// not count against the tiled path; with 7 wavefronts per SIMD (72 VGPRs) the tiled path
+// keeps its loop-invariant lane values in registers instead of scratch
+template <class R, int D>
+__device__ __noinline__ void neighbors_one_agent_large_stencil(const SimCtx<R>& c, int i)
+{
+	neighbors_one_agent<R, D, false>(c, i);
+}
+
 template <class R, int D, int W>
-__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))) k_neighbors(
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 7 : 6))) k_neighbors(
 	const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
@@ -415,7 +424,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(8))
 			// stencil larger than the LDS tile: one lane per agent, straight from HBM
 			const int s0 = c.vstart[v];
 			for (int a = tid; a < m; a += 64 * W)
-				neighbors_one_agent<R, D, false>(c, c.bucket[s0 + a]);
+				neighbors_one_agent_large_stencil<R, D>(c, c.bucket[s0 + a]);
 		}
 		__syncthreads();
 	}
