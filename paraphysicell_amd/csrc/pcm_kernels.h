@@ -356,16 +356,19 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 		const int m = __builtin_amdgcn_readfirstlane(s_pre[14] - s_pre[13]); // the centre voxel is stencil entry 13
 		if (C <= TILE)
 		{
-			// stage the stencil in stencil order; a candidate's stencil entry is the number
-			// of entry starts at or below its position (lane k of every wavefront holds the
-			// start of entry k; read back with readlane, no LDS round trip per entry)
+			// stage the stencil in stencil order.  A candidate's stencil entry is the last
+			// entry starting at or below its position: first its row of three entries (the
+			// number of row starts at or below it; lane k of every wavefront holds the start
+			// of entry k, read back with readlane), then the entry within the row
 			const int lane_pre = s_pre[lane < 28 ? lane : 27];
 			for (int t = tid; t < C; t += 64 * W)
 			{
-				int sv = 0;
+				int row = 0;
 #pragma unroll
-				for (int k = 1; k < 27; k++)
-					sv += __builtin_amdgcn_readlane(lane_pre, k) <= t;
+				for (int r = 1; r < 9; r++)
+					row += __builtin_amdgcn_readlane(lane_pre, 3 * r) <= t;
+				const int e0 = 3 * row;
+				const int sv = e0 + (s_pre[e0 + 1] <= t) + (s_pre[e0 + 2] <= t);
 				const int g = s_vs[sv] + t - s_pre[sv];
 				const int j = c.bucket[g];
 				R rec[4];
