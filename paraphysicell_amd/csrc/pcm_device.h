@@ -45,7 +45,6 @@ struct Scalars
 	long long deaths;      // environment::deaths_count increments
 	long long divisions;   // environment::divisions_count increments
 	int n_occ;             // occupied mechanics voxels of the last mesh update
-	int order_valid;       // bucket[0, n) is a permutation of the current agents
 	int log_len;           // entries in the removal log (may exceed log_cap: log lost)
 	int compact_limit;     // > 0: the removal loop visits agents below it only
 	int res_open;          // actors the first resolve round left with undecided events
@@ -167,14 +166,6 @@ struct SimCtx
 	uint16_t mv_unit[192];
 	uint8_t mv_bytes[192];
 };
-
-// index of the agent a thread handles: voxel (bucket) order when the last mesh update
-// still describes the agents, so neighboring lanes gather neighboring agents' data
-template <class R>
-__device__ __forceinline__ int agent_at(const SimCtx<R>& c, int g)
-{
-	return c.sc->order_valid ? c.bucket[g] : g;
-}
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 

@@ -59,10 +59,7 @@ __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs)
 	if (i < c.sc->n_binned)
 		c.vcnt[c.vox_of[i]] = 0;
 	if (i == 0)
-	{
 		c.sc->n_occ = 0;
-		c.sc->order_valid = 0;
-	}
 }
 
 // RESTORE: also return the container update's remap entries to the identity
@@ -161,8 +158,6 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 		out[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
 	out[3] = c.rel_adh[i] * c.radius[i];
 	write_force_record<R, D>(c, i);
-	if (i == 0)
-		c.sc->order_valid = 1;
 }
 
 // force records of the current agent state, when the mesh update's copy is stale
@@ -482,7 +477,7 @@ __global__ void k_forces(const SimCtx<R>* __restrict__ ctxs)
 	const int g = gid();
 	if (g >= c.sc->n)
 		return;
-	const int i = agent_at<R>(c, g);
+	const int i = g;
 	R pi[D], v[D];
 	for (int k = 0; k < D; k++)
 	{
@@ -706,7 +701,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	const int lane = t & (G - 1);
 	if (g >= c.sc->n)
 		return;
-	const int i = agent_at<R>(c, g);
+	const int i = g;
 	// the agent's record, velocity, count and first list slots are requested together;
 	// the motility, membrane and spring inputs are loaded where they are used, which keeps
 	// the register count (and so the occupancy) of the neighbor loop low
@@ -995,7 +990,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 	int i = -1;
 	if (g < c.sc->n)
 	{
-		i = agent_at<R>(c, g);
+		i = g;
 		if (lane == 0)
 			spring_compact_agent<R>(c, i);
 		const int cnt = visible_neighbors<R>(c, i);
@@ -1364,7 +1359,7 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 	const unsigned long long group_mask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
 	if (g >= c.sc->n)
 		return;
-	const int i = agent_at<R>(c, g);
+	const int i = g;
 	if (AB2 && lane < D && c.movable[i])
 	{
 		const R factor = (R)((double)c.dt * 1.5);
@@ -1858,7 +1853,6 @@ __device__ void compact_plan_limited(const SimCtx<R>& c, const int* list, int b,
 	c.sc->stat[6] = c.sc->n_occ;
 	c.sc->n = top;
 	c.sc->deaths += deaths;
-	c.sc->order_valid = 0;
 }
 
 template <class R>
@@ -1958,7 +1952,6 @@ __device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
 	c.sc->stat[6] = c.sc->n_occ;
 	c.sc->n = n1;
 	c.sc->deaths += deaths;
-	c.sc->order_valid = 0;
 }
 
 template <class R>
@@ -2336,7 +2329,6 @@ __global__ void k_divide_done(const SimCtx<R>* __restrict__ ctxs)
 		c.sc->compact_limit = n > 0 ? n : -1;
 		c.sc->n = n + count;
 		c.sc->divisions += count;
-		c.sc->order_valid = 0;
 	}
 }
 

@@ -275,7 +275,6 @@ public:
 		s.conflict = 0;
 		s.bad_count = 0;
 		s.moves = 0;
-		s.order_valid = 0;
 		s.claim = 0;
 		s.log_len = 0;
 		PCM_HIP(hipMemcpy(h_.sc, &s, sizeof(s), hipMemcpyHostToDevice));
