@@ -618,13 +618,11 @@ __device__ __forceinline__ R pair_force_rec(const force_rec<R>& a, const force_r
 // ascending actor order, then the agent's own accepted events in event order.  The G
 // lanes of a group run the same loops; entry e is stored by lane e % G, which is the lane
 // that reads it back in the contraction loop, and lane 0 stores the counters.  Returns the
-// list length.
+// list length.  in_cnt, na and cnt are the agent's in_cnt, own_cnt and spr_cnt.
 template <class R, int G>
-__device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, int lane)
+__device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, int lane, int in_cnt, int na, int cnt)
 {
-	const int nin = min(c.in_cnt[a], c.K);
-	const int na = c.own_cnt[a];
-	int cnt = c.spr_cnt[a];
+	const int nin = min(in_cnt, c.K);
 	if (nin + na == 0)
 		return cnt;
 	int* out = c.spr_idx + (size_t)a * c.L;
@@ -799,16 +797,32 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	}
 	if (lane == 0)
 		c.pressure[i] = pressure;
-	if (c.motile[i] != 0)
-		motility_apply<D, R>(c, i, c.sc->step, load_motility<D, R>(c, i), v, lane == 0);
+	// the per-agent inputs of motility, membrane, spring append and contraction are
+	// requested together and awaited once (the empty asm needs their values), instead of
+	// one round trip per stage
+	const motility_in<D, R> mot = load_motility<D, R>(c, i);
+	const R bm_rep = c.wall ? c.bm_repulsion[i] : (R)0;
+	int in_cnt = 0, own_cnt = 0, ns = 0;
+	R ki = 0;
+	if (c.springs)
+	{
+		in_cnt = c.in_cnt[i];
+		own_cnt = c.own_cnt[i];
+		ns = c.spr_cnt[i];
+		ki = c.elastic[i];
+	}
+	asm volatile("" ::"v"(mot.speed), "v"(mot.persistence), "v"(mot.bias), "v"(bm_rep), "v"(in_cnt), "v"(own_cnt),
+				 "v"(ns), "v"(ki));
+	if (mot.motile)
+		motility_apply<D, R>(c, i, c.sc->step, mot, v, lane == 0);
 	if (mov && c.wall)
-		membrane_apply<D, R>(c, me.r, c.bm_repulsion[i], p, v);
+		membrane_apply<D, R>(c, me.r, bm_rep, p, v);
 	// the step's attachments join the list first (no separate append launch in a step)
-	const int ns = c.springs ? spring_append_group<R, G>(c, i, lane) : 0;
+	if (c.springs)
+		ns = spring_append_group<R, G>(c, i, lane, in_cnt, own_cnt, ns);
 	if (mov && c.springs)
 	{
 		// spring_contract_function (position_solver.cpp:337-370)
-		const R ki = c.elastic[i];
 		const int* sp = c.spr_idx + (size_t)i * c.L;
 		for (int base = 0; base < ns; base += B * G)
 		{
@@ -1376,14 +1390,12 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 	bool acted = false;
 	// every branch compares a draw u in [0,1) with `<` against rate * dt, so an actor whose
 	// rates are all <= 0 never acts and its draws need not be made
-	bool any_rate = false;
-	if (c.dead[i] != 1)
-	{
-		any_rate = c.dead_phago[i] > 0;
-		for (int k = 0; k < c.T; k++)
-			any_rate = any_rate || c.live_phago[(size_t)i * c.T + k] > 0 || c.attack[(size_t)i * c.T + k] > 0
-					   || c.fusion[(size_t)i * c.T + k] > 0;
-	}
+	// (bitwise ors, not short-circuits: the rates are requested together, one round trip)
+	bool any_rate = c.dead_phago[i] > 0;
+	for (int k = 0; k < c.T; k++)
+		any_rate |= (c.live_phago[(size_t)i * c.T + k] > 0) | (c.attack[(size_t)i * c.T + k] > 0)
+					| (c.fusion[(size_t)i * c.T + k] > 0);
+	any_rate &= c.dead[i] != 1;
 	if (any_rate)
 	{
 		const int cnt = visible_neighbors<R>(c, i);
@@ -1410,7 +1422,7 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 				const R p_attack = c.attack[(size_t)i * c.T + tj] * c.immuno[(size_t)j * c.T + ti] * dt;
 				const R p_fuse = c.fusion[(size_t)i * c.T + tj] * dt;
 				// no branch can fire when its threshold is <= 0: u < threshold is false
-				if (dead_j ? p_dead > 0 : (p_live > 0 || p_attack > 0 || p_fuse > 0))
+				if (dead_j ? p_dead > 0 : ((p_live > 0) | (p_attack > 0) | (p_fuse > 0)))
 				{
 					const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
 					const bool lt_live = u < p_live, lt_attack = u < p_attack, lt_fuse = u < p_fuse;
@@ -1500,17 +1512,14 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 	}
 }
 
-// out of the box of the microenvironment mesh (containers_solver.cpp:89-96)
+// out of the box of the microenvironment mesh (containers_solver.cpp:89-96), with no
+// early exit so that the coordinates are requested together
 template <class R, int D>
-__device__ __forceinline__ bool out_of_bounds(const SimCtx<R>& c, int i)
+__device__ __forceinline__ bool outside_box(const SimCtx<R>& c, const R* p)
 {
 	bool out = false;
 	for (int k = 0; k < D; k++)
-	{
-		const R p = c.pos[(size_t)i * D + k];
-		if (p < (R)c.bmin[k] || p > (R)c.bmax[k])
-			out = true;
-	}
+		out |= (p[k] < (R)c.bmin[k]) | (p[k] > (R)c.bmax[k]);
 	return out;
 }
 
@@ -1529,6 +1538,18 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 	if (live_thread)
 	{
 	const int nr = c.rem_cnt[i];
+	const unsigned long long w = c.ev_t[i];
+	// the removal test's inputs are requested with the counters, in one round trip; only
+	// the actor's own fusion moves it, so then its position is read again after its events
+	// (another actor's event on it makes it a removal target, removed whatever its flag)
+	R p[D];
+	uint8_t fl = 0;
+	if (FLAG)
+	{
+		fl = c.flags[i];
+		for (int k = 0; k < D; k++)
+			p[k] = c.pos[(size_t)i * D + k];
+	}
 	for (int k = 0; k < nr; k++)
 	{
 		const int e = c.rem_ev[4 * i + k];
@@ -1538,7 +1559,6 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 		else
 			ingest<R>(c, i, j);
 	}
-	const unsigned long long w = c.ev_t[i];
 	const int na = evw_attacks(w);
 	if (na > 0)
 	{
@@ -1563,8 +1583,14 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 		}
 	}
 	if (FLAG)
-		removed = evw_removals(w) > 0 || c.flags[i] == PCM_FLAG_TO_REMOVE || out_of_bounds<R, D>(c, i);
-	c.ev_t[i] = 0;
+	{
+		if (nr > 0)
+			for (int k = 0; k < D; k++)
+				p[k] = c.pos[(size_t)i * D + k];
+		removed = (evw_removals(w) > 0) | (fl == PCM_FLAG_TO_REMOVE) | outside_box<R, D>(c, p);
+	}
+	if (w != 0) // written by few agents: the others' lines stay clean
+		c.ev_t[i] = 0;
 	}
 	if (FLAG)
 	{
