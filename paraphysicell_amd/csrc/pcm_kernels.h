@@ -1997,13 +1997,15 @@ __device__ inline void edit_links(int* list, int* cnt_ptr, const int* remap)
 	int cnt = *cnt_ptr;
 	if (cnt == 0)
 		return;
+	// no early exit: eight entries and their remap values are requested at a time, so an
+	// untouched list (the common case) costs two round trips per eight entries
 	bool touched = false;
+#pragma unroll 8
 	for (int s = 0; s < cnt; s++)
-		if (remap[list[s]] != list[s])
-		{
-			touched = true;
-			break;
-		}
+	{
+		const int e = list[s];
+		touched |= remap[e] != e;
+	}
 	if (!touched)
 		return;
 	for (;;)
@@ -2062,34 +2064,55 @@ __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs)
 		for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
 		{
 			const int src = c.move_src[k], dst = c.move_dst[k];
-			for (int u = lane; u < c.mv_n; u += 64)
+			// the copy plan's words (at most MOVE_UNITS per lane): every lane reads its plan
+			// entries and source words first, then stores, so the words cost one round trip
+			// instead of one per 64 (the stores could alias the plan for the compiler)
+			constexpr int MOVE_UNITS = (int)(sizeof(c.mv_field) / sizeof(c.mv_field[0]) + 63) / 64;
+			uint8_t* dst_at[MOVE_UNITS];
+			uint32_t word[MOVE_UNITS];
+			int bytes[MOVE_UNITS];
+#pragma unroll
+			for (int r = 0; r < MOVE_UNITS; r++)
 			{
-				const int f = c.mv_field[u];
-				const int nb = c.fbytes[f];
-				if (c.mv_bytes[u] == 4)
+				const int u = lane + 64 * r;
+				bytes[r] = 0;
+				if (u < c.mv_n)
 				{
-					const uint32_t* sp = (const uint32_t*)((const uint8_t*)c.fptr[f] + (size_t)src * nb);
-					uint32_t* dp = (uint32_t*)((uint8_t*)c.fptr[f] + (size_t)dst * nb);
-					dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
-				}
-				else
-				{
-					const uint8_t* sp = (const uint8_t*)c.fptr[f] + (size_t)src * nb;
-					uint8_t* dp = (uint8_t*)c.fptr[f] + (size_t)dst * nb;
-					dp[c.mv_unit[u]] = sp[c.mv_unit[u]];
+					const int f = c.mv_field[u];
+					const size_t nb = (size_t)c.fbytes[f];
+					bytes[r] = c.mv_bytes[u];
+					const size_t off = (size_t)c.mv_unit[u] * bytes[r];
+					const uint8_t* base = (const uint8_t*)c.fptr[f];
+					dst_at[r] = (uint8_t*)base + (size_t)dst * nb + off;
+					word[r] = bytes[r] == 4 ? *(const uint32_t*)(base + (size_t)src * nb + off)
+											: (uint32_t)base[(size_t)src * nb + off];
 				}
 			}
+#pragma unroll
+			for (int r = 0; r < MOVE_UNITS; r++)
+			{
+				if (bytes[r] == 4)
+					*(uint32_t*)dst_at[r] = word[r];
+				else if (bytes[r] == 1)
+					*dst_at[r] = (uint8_t)word[r];
+			}
+			// list rows: lanes over the entries (a serial copy costs one round trip per entry)
 			for (int t = lane; t < c.K; t += 64)
 				c.nbr_idx[(size_t)dst * c.K + t] = c.nbr_idx[(size_t)src * c.K + t];
+			for (int t = lane; t < c.L; t += 64)
+				c.spr_idx[(size_t)dst * c.L + t] = c.spr_idx[(size_t)src * c.L + t];
+			for (int t = lane; t < c.LA; t += 64)
+				c.atc_idx[(size_t)dst * c.LA + t] = c.atc_idx[(size_t)src * c.LA + t];
 			if (lane == 0)
 			{
 				c.nbr_cnt[dst] = c.nbr_cnt[src];
-				for (int t = 0; t < c.L; t++)
-					c.spr_idx[(size_t)dst * c.L + t] = c.spr_idx[(size_t)src * c.L + t];
 				c.spr_cnt[dst] = c.spr_cnt[src];
-				for (int t = 0; t < c.LA; t++)
-					c.atc_idx[(size_t)dst * c.LA + t] = c.atc_idx[(size_t)src * c.LA + t];
 				c.atc_cnt[dst] = c.atc_cnt[src];
+			}
+			// the lane-0 edits below read rows the other lanes wrote
+			__threadfence_block();
+			if (lane == 0)
+			{
 				edit_links(c.spr_idx + (size_t)dst * c.L, c.spr_cnt + dst, c.remap);
 				edit_links(c.atc_idx + (size_t)dst * c.LA, c.atc_cnt + dst, c.remap);
 			}
