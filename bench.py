@@ -99,6 +99,57 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
 
 
+def pcie_leg(dev, params, steps: int):
+    """The drop-in's full host round trip per step (pcm_dropin.hpp upload / sync_to_host):
+    count, every field, both link lists and the neighbor lists uploaded from host memory, one
+    mechanics step, then every field, both link lists and the neighbor lists downloaded (as
+    when host code reads and writes the whole cell_data between steps).  Host buffers are
+    ordinary pageable numpy arrays, like the reference's std::vectors."""
+    import numpy as np
+    from paraphysicell_amd import abi
+
+    host = dev.download()
+    bufs = [(abi.FIELD_ID[name], np.ascontiguousarray(host.arrays[name])) for name, _, _ in abi.FIELDS]
+    lists = {w: host.links[w] for w in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED)}
+    nbrs = host.neighbors
+    field_bytes = sum(b.nbytes for _, b in bufs)
+
+    def i32(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        return a if a.size else np.zeros(1, np.int32)
+
+    n = n0 = dev.count
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev._call("set_count", dev._h, n)
+        for fid, b in bufs:
+            dev._call("set_field", dev._h, fid, b.ctypes.data)
+        for w, (off, idx) in lists.items():
+            o, e = i32(off), i32(idx)  # referenced until the call returns
+            dev._call("set_links", dev._h, w, o.ctypes.data, e.ctypes.data)
+        o, e = i32(nbrs[0]), i32(nbrs[1])
+        dev._call("set_neighbors", dev._h, o.ctypes.data, e.ctypes.data)
+        dev.step(params, 1)
+        n = dev.count
+        for fid, b in bufs:
+            dev._call("get_field", dev._h, fid, b.ctypes.data)
+        lists = {w: dev.links(w) for w in lists}
+        nbrs = dev.neighbors()
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+    agent_steps = 0.5 * (n0 + n) * steps
+    return {
+        "workload": f"the 88.7k-agent step with the drop-in's full host round trip per step ({steps} steps): count, "
+                    f"{len(bufs)} fields ({field_bytes / 1e6:.1f} MB), link and neighbor lists up; one step; all down",
+        "value": round(agent_steps / elapsed, 1),
+        "unit": "agent-steps/s",
+        "ms_per_step": round(elapsed * 1000.0 / steps, 3),
+        "field_transfer_gbs": round(2 * field_bytes * steps / elapsed / 1e9, 2),
+        "note": "PCIe-inclusive rate; not the metric (the timed step keeps the population resident in HBM)",
+    }
+
+
 def ensemble_leg(steps: int):
     """64 replicates (seeds 0-63, 500-20000 agents log-uniform, 2D even / 3D odd seeds, the
     900-agent 2D workload's density and coefficients), `steps` steps each: one batch with
@@ -201,6 +252,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-replicate ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=600)
     ap.add_argument("--no-serving", action="store_true", help="skip the served-requests leg")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the host round-trip leg")
     ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 launches): the reference point that "
@@ -312,6 +364,8 @@ def main():
                                 for k, v in sorted(warm_prof.items(), key=lambda kv: -kv[1][0])[:8]},
     }
 
+    if not args.no_pcie:
+        result["pcie_inclusive"] = pcie_leg(dev, params, 20)
     if not args.no_cpu:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         threads = min(threads, 16)

@@ -186,6 +186,9 @@ int pcm_version(void);
  * [0, count); upload fields and lists after it. */
 int pcm_set_count(pcm_sim* sim, int32_t count);
 int pcm_get_count(pcm_sim* sim, int32_t* count);
+/* One field, `count * width` elements, where count is the current agent count
+ * (pcm_get_count: below the last pcm_set_count once the container update has removed
+ * agents).  Setting reads, and getting writes, exactly that many elements. */
 int pcm_set_field(pcm_sim* sim, int32_t field, const void* host);
 int pcm_get_field(pcm_sim* sim, int32_t field, void* host);
 /* springs / attached_cells as CSR (offsets has count+1 entries) */

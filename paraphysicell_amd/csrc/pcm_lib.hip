@@ -313,8 +313,9 @@ public:
 	{
 		if (f < 0 || f >= PCM_FIELD_COUNT || !src)
 			return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer");
-		PCM_HIP(hipStreamSynchronize(stream_));
-		const int n = n_launch_;
+		// the current count: removals since the last upload leave it below n_launch_
+		PCM_TRY(read_scalars());
+		const int n = sc_.n;
 		if (f == PCM_F_MAXIMUM_NUMBER_OF_ATTACHMENTS && n > 0)
 		{
 			const int* m = (const int*)src;
@@ -387,8 +388,8 @@ public:
 			return fail(PCM_ERR_ARGUMENT, "bad link list id");
 		if (!off)
 			return fail(PCM_ERR_ARGUMENT, "null offsets");
-		PCM_HIP(hipStreamSynchronize(stream_));
-		const int n = n_launch_;
+		PCM_TRY(read_scalars());
+		const int n = sc_.n;
 		int longest = 0;
 		for (int i = 0; i < n; i++)
 		{
@@ -462,28 +463,35 @@ public:
 		PCM_TRY(read_scalars());
 		const int n = sc_.n;
 		const int K = h_.K;
-		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * K);
+		std::vector<int> cnt(std::max(n, 1));
 		std::vector<uint8_t> mov(std::max(n, 1));
 		if (n > 0)
 		{
 			PCM_HIP(hipMemcpy(cnt.data(), h_.nbr_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
-			PCM_HIP(hipMemcpy(rows.data(), h_.nbr_idx, sizeof(int) * (size_t)n * K, hipMemcpyDeviceToHost));
 			PCM_HIP(hipMemcpy(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
 		}
 		int64_t t = 0;
+		for (int i = 0; i < n; i++)
+			t += mov[i] ? cnt[i] : 0;
+		*total = t;
+		if (!off && !idx)
+			return PCM_OK; // the size query needs no rows
+		std::unique_ptr<int[]> rows(new int[(size_t)std::max(n, 1) * K]);
+		if (n > 0)
+			PCM_HIP(hipMemcpy(rows.get(), h_.nbr_idx, sizeof(int) * (size_t)n * K, hipMemcpyDeviceToHost));
+		t = 0;
 		for (int i = 0; i < n; i++)
 		{
 			if (off)
 				off[i] = (int32_t)t;
 			if (!mov[i])
 				continue;
-			for (int s = 0; s < cnt[i]; s++, t++)
-				if (idx)
-					idx[t] = rows[(size_t)i * K + s];
+			if (idx)
+				std::copy(rows.get() + (size_t)i * K, rows.get() + (size_t)i * K + cnt[i], idx + t);
+			t += cnt[i];
 		}
 		if (off)
 			off[n] = (int32_t)t;
-		*total = t;
 		return PCM_OK;
 	}
 
@@ -494,47 +502,53 @@ public:
 	{
 		if (!off || !idx)
 			return fail(PCM_ERR_ARGUMENT, "null neighbor lists");
-		PCM_HIP(hipStreamSynchronize(stream_));
-		const int n = n_launch_;
+		PCM_TRY(read_scalars());
+		const int n = sc_.n;
 		std::vector<uint8_t> mov(std::max(n, 1));
 		if (n > 0)
 			PCM_HIP(hipMemcpy(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
-		std::vector<std::vector<int>> lists(std::max(n, 1));
+		// two passes over the flat lists: count (a movable agent's own entries, a
+		// non-movable agent's listers), then fill the fixed-stride rows
+		std::vector<int> cnt(std::max(n, 1), 0);
 		for (int i = 0; i < n; i++)
 		{
 			if (off[i + 1] < off[i])
 				return fail(PCM_ERR_ARGUMENT, "offsets must be non-decreasing");
 			if (!mov[i])
 				continue;
+			cnt[i] += off[i + 1] - off[i];
 			for (int s = off[i]; s < off[i + 1]; s++)
 			{
 				const int j = idx[s];
 				if (j < 0 || j >= n || j == i)
 					return fail(PCM_ERR_ARGUMENT, "neighbor entry is not another agent's index");
-				lists[i].push_back(j);
+				if (!mov[j])
+					cnt[j]++;
 			}
 		}
+		int longest = 0;
 		for (int i = 0; i < n; i++)
-			if (mov[i])
-				for (int s = off[i]; s < off[i + 1]; s++)
-					if (!mov[idx[s]])
-						lists[idx[s]].push_back(i);
-		size_t longest = 0;
-		for (int i = 0; i < n; i++)
-			longest = std::max(longest, lists[i].size());
-		if ((int)longest > h_.K)
-			PCM_TRY(grow_neighbors((int)longest));
+			longest = std::max(longest, cnt[i]);
+		if (longest > h_.K)
+			PCM_TRY(grow_neighbors(longest));
 		const int K = h_.K;
-		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * K, -1);
-		for (int i = 0; i < n; i++)
+		std::unique_ptr<int[]> rows(new int[(size_t)std::max(n, 1) * K]);
+		std::vector<int> fill(std::max(n, 1), 0);
+		for (int i = 0; i < n; i++) // ascending: a non-movable agent's listers in order
 		{
-			cnt[i] = (int)lists[i].size();
-			std::copy(lists[i].begin(), lists[i].end(), rows.begin() + (size_t)i * K);
+			if (!mov[i])
+				continue;
+			std::copy(idx + off[i], idx + off[i + 1], rows.get() + (size_t)i * K);
+			for (int s = off[i]; s < off[i + 1]; s++)
+				if (!mov[idx[s]])
+					rows[(size_t)idx[s] * K + fill[idx[s]]++] = i;
 		}
+		for (int i = 0; i < n; i++)
+			std::fill(rows.get() + (size_t)i * K + cnt[i], rows.get() + (size_t)(i + 1) * K, -1);
 		if (n > 0)
 		{
 			PCM_HIP(hipMemcpy(h_.nbr_cnt, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-			PCM_HIP(hipMemcpy(h_.nbr_idx, rows.data(), sizeof(int) * (size_t)n * K, hipMemcpyHostToDevice));
+			PCM_HIP(hipMemcpy(h_.nbr_idx, rows.get(), sizeof(int) * (size_t)n * K, hipMemcpyHostToDevice));
 		}
 		return PCM_OK;
 	}

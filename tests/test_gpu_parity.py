@@ -311,3 +311,36 @@ def test_removal_log_replays_the_compaction():
         order[i] = order[-1]
         order.pop()
     assert np.array_equal(np.array(order, np.int32), dev.get_field("origin"))
+
+
+def test_uploads_after_removals_read_count_rows():
+    """After the container update removed agents, pcm_set_field, pcm_set_links and
+    pcm_set_neighbors read exactly the current count of rows: a guard tail of invalid
+    values after them is never read, and the round trip is exact."""
+    mesh, st, p = parity.rich_population(3, 1200, 95, 4, rates="busy")
+    dev = device(mesh, st, seed=6)
+    dev.initialize_mesh()
+    dev.step(p, 3)
+    n = dev.count
+    assert n < st.count
+    for name, bad in (("maximum_number_of_attachments", 1000), ("contact_function", 200)):
+        a = dev.get_field(name)
+        guard = np.full(st.count, bad, dtype=a.dtype)
+        guard[:n] = a
+        dev._call("set_field", dev._h, abi.FIELD_ID[name], guard.ctypes.data)
+        assert np.array_equal(dev.get_field(name), a)
+    for which in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED):
+        off, idx = dev.links(which)
+        guard = np.zeros(st.count + 1, np.int32)  # decreasing offsets after the count
+        guard[: n + 1] = off
+        entries = np.append(idx, 0).astype(np.int32)  # referenced until the call returns
+        dev._call("set_links", dev._h, which, guard.ctypes.data, entries.ctypes.data)
+        off2, idx2 = dev.links(which)
+        assert np.array_equal(off2, off) and np.array_equal(idx2, idx)
+    off, idx = dev.neighbors()
+    guard = np.zeros(st.count + 1, np.int32)
+    guard[: n + 1] = off
+    entries = np.append(idx, 0).astype(np.int32)
+    dev._call("set_neighbors", dev._h, guard.ctypes.data, entries.ctypes.data)
+    off2, idx2 = dev.neighbors()
+    assert np.array_equal(off2, off) and np.array_equal(idx2, idx)
