@@ -146,18 +146,23 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 	const int i = gid();
 	if (i >= c.sc->n)
 		return;
+	// the agent's own data first: its loads overlap the rank chain below
+	write_force_record<R, D>(c, i);
+	R rec[4];
+	for (int k = 0; k < 3; k++)
+		rec[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
+	rec[3] = c.rel_adh[i] * c.radius[i];
 	const int v = c.vox_of[i];
 	const int s = c.vstart[v], m = c.vcnt[v];
 	int rank = 0;
+#pragma unroll 8
 	for (int k = s; k < s + m; k++)
 		rank += c.btmp[k] > i;
 	const int slot = s + rank;
 	c.bucket[slot] = i;
 	R* out = c.sorted + (size_t)slot * 4;
-	for (int k = 0; k < 3; k++)
-		out[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
-	out[3] = c.rel_adh[i] * c.radius[i];
-	write_force_record<R, D>(c, i);
+	for (int k = 0; k < 4; k++)
+		out[k] = rec[k];
 }
 
 // force records of the current agent state, when the mesh update's copy is stale
@@ -916,14 +921,14 @@ __global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs)
 	const int n = c.sc->n;
 	if (i >= n)
 		return;
-	const int cnt = c.spr_cnt[i];
-	if (s >= cnt)
-		return;
+	// count, slot and fire mask in one round trip (the slot is inside the row's storage
+	// and is used only below the count)
 	const int* li = c.spr_idx + (size_t)i * c.L;
+	const int cnt = c.spr_cnt[i];
 	const int p = li[s];
-	if (p < 0 || p >= n)
-		return;
 	const uint32_t fi = c.fire[i];
+	if (s >= cnt || p < 0 || p >= n)
+		return;
 	if ((fi >> s) & 1u)
 	{
 		atomicOr(&c.erase[i], 1u << s);
