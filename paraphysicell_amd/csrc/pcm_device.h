@@ -220,6 +220,37 @@ __device__ __forceinline__ int wave_reserve(int* counter, int count)
 	return base + pre;
 }
 
+// An agent's own row of a per-cell-type table (affinities, rates), read by the type of a
+// partner: with at most TYPE_ROW_REGS types the row is loaded once into registers, so
+// the lookup costs no load after the partner's type arrives.
+constexpr int TYPE_ROW_REGS = 4;
+template <class R>
+struct type_row
+{
+	R v0, v1, v2, v3; // named, not an array: a selected array element would go to scratch
+	gptr<const R> row;
+	int T;
+	__device__ __forceinline__ R at(int t) const
+	{
+		if (T > TYPE_ROW_REGS)
+			return row[t];
+		return t == 0 ? v0 : (t == 1 ? v1 : (t == 2 ? v2 : v3));
+	}
+};
+
+template <class R>
+__device__ __forceinline__ type_row<R> load_type_row(gptr<const R> row, int T)
+{
+	type_row<R> r;
+	r.row = row;
+	r.T = T;
+	r.v0 = T > 0 ? row[0] : (R)0;
+	r.v1 = T > 1 ? row[1] : (R)0;
+	r.v2 = T > 2 ? row[2] : (R)0;
+	r.v3 = T > 3 ? row[3] : (R)0;
+	return r;
+}
+
 // -------------------------------------------------------------------------- force records
 __device__ __forceinline__ float bits_to_real(int b, float) { return __int_as_float(b); }
 __device__ __forceinline__ double bits_to_real(int b, double) { return __longlong_as_double((long long)b); }

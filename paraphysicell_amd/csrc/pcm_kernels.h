@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 					v[k] += __shfl(f[k], src, G);
 		}
 	}
-	const R* aff_row = c.affinity + (size_t)i * c.T;
+	const type_row<R> aff_row = load_type_row<R>(c.affinity + (size_t)i * c.T, c.T);
 	const R coefficient = pressure_coefficient<R>();
 	R pressure = 0;
 	for (int base = 0; base < cnt; base += B * G)
@@ -779,7 +779,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 			theirs[u] = 0;
 			if (base + u * G + lane < cnt)
 			{
-				const R term = pair_force_rec<D, R>(me, other[u], aff_row[other[u].type], aff_ba[u], mov, f[u]);
+				const R term = pair_force_rec<D, R>(me, other[u], aff_row.at(other[u].type), aff_ba[u], mov, f[u]);
 				const R tp = term * coefficient;
 				mine[u] = mov ? tp : (R)0;
 				theirs[u] = other[u].movable ? tp : (R)0;
@@ -853,7 +853,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 					f[u][k] = 0;
 				if (base + u * G + lane < ns)
 				{
-					const R kij = ki * kj[u] * aff_row[other[u].type] * aff_ba[u];
+					const R kij = ki * kj[u] * aff_row.at(other[u].type) * aff_ba[u];
 					const R adhesion = (R)sqrt((double)kij);
 					for (int k = 0; k < D; k++)
 						f[u][k] = adhesion * (other[u].p[k] - p[k]);
@@ -1016,7 +1016,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
 		const int ti = c.type[i];
 		const R rate_i = c.att_rate[i] * c.dt;
-		const R* aff_i = c.affinity + (size_t)i * c.T;
+		const type_row<R> aff_i = load_type_row<R>(c.affinity + (size_t)i * c.T, c.T);
 		const unsigned long long step = c.sc->step;
 		for (int base = 0; base < cnt; base += G)
 		{
@@ -1028,7 +1028,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 				j = nb[s];
 				if (j > i)
 				{
-					const R p_l = rate_i * aff_i[c.type[j]];
+					const R p_l = rate_i * aff_i.at(c.type[j]);
 					const R p_r = c.att_rate[j] * c.dt * c.affinity[(size_t)j * c.T + ti];
 					const pcm_u32x4 b = pcm_block(c.seed, step, PCM_STREAM_ATTACH, (uint32_t)i, (uint32_t)s, 0u);
 					event = pcm_draw<R>::from(b, 0) <= p_l || pcm_draw<R>::from(b, 1) <= p_r;
