@@ -755,12 +755,11 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	for (int base = 0; base < cnt; base += B * G)
 	{
 		R f[B][D], mine[B], theirs[B];
-		if (base > 0)
-		{
+		// the next iteration's slots are requested now, ahead of this iteration's gathers
+		int jn_next[B];
 #pragma unroll
-			for (int u = 0; u < B; u++)
-				jn[u] = base + u * G + lane < cnt ? nb[base + u * G + lane] : -1;
-		}
+		for (int u = 0; u < B; u++)
+			jn_next[u] = base + B * G + u * G + lane < cnt ? nb[base + B * G + u * G + lane] : -1;
 		force_rec<R> other[B];
 		R aff_ba[B];
 #pragma unroll
@@ -799,6 +798,9 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 				pressure += __shfl(theirs[u], src, G);
 			}
 		}
+#pragma unroll
+		for (int u = 0; u < B; u++)
+			jn[u] = jn_next[u];
 	}
 	if (lane == 0)
 		c.pressure[i] = pressure;
@@ -1018,14 +1020,15 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 		const R rate_i = c.att_rate[i] * c.dt;
 		const type_row<R> aff_i = load_type_row<R>(c.affinity + (size_t)i * c.T, c.T);
 		const unsigned long long step = c.sc->step;
+		int j_next = lane < cnt ? nb[lane] : -1; // each slot is requested one iteration ahead
 		for (int base = 0; base < cnt; base += G)
 		{
 			const int s = base + lane;
 			bool event = false;
-			int j = -1;
+			int j = j_next;
+			j_next = s + G < cnt ? nb[s + G] : -1;
 			if (s < cnt)
 			{
-				j = nb[s];
 				if (j > i)
 				{
 					const R p_l = rate_i * aff_i.at(c.type[j]);
@@ -1411,16 +1414,17 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 		const bool self_removed = c.flags[i] == PCM_FLAG_TO_REMOVE;
 		const R p_dead = c.dead_phago[i] * dt;
 		bool attacked_once = false, phagocytosed_once = false, fused_once = false, stop = false;
+		int j_next = lane < cnt ? nb[lane] : -1; // each slot is requested one iteration ahead
 		for (int base = 0; base < cnt && !stop; base += G)
 		{
 			const int s = base + lane;
-			int j = -1;
+			int j = j_next;
+			j_next = s + G < cnt ? nb[s + G] : -1;
 			// bit 0 candidate, 1 dead target, 2 u < p_live, 3 u < p_attack, 4 u < p_fuse,
 			// 5 target flagged for removal
 			int bits = 0;
 			if (s < cnt)
 			{
-				j = nb[s];
 				const int tj = c.type[j];
 				const bool dead_j = c.dead[j] == 1;
 				const R p_live = c.live_phago[(size_t)i * c.T + tj] * dt;
