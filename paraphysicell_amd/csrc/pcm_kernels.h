@@ -101,6 +101,9 @@ __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __
 		c.sc->bad_count = 0;
 		c.sc->moves = 0;
 	}
+	// blocks past the agents skip the block-wide scans (block 0 always claims, for n = 0)
+	if (blockIdx.x > 0 && (int)blockIdx.x * CLAIM_THREADS >= n)
+		return;
 	int amount = 0, v = -1;
 	if (i < n && c.vrank[i] == 0)
 	{
@@ -1002,6 +1005,10 @@ template <class R, int G>
 __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
+	// a block past the agents (a batch's grid covers its largest member) owns no segment
+	// and skips the block-wide scan below
+	if ((int)blockIdx.x * (BLOCK / G) >= c.sc->n)
+		return;
 	const int t = gid();
 	const int g = t / G;
 	const int lane = t & (G - 1);
@@ -1542,6 +1549,8 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 	if (c.sc->conflict)
 		return;
 	const int i = gid();
+	if ((i & ~63) >= c.sc->n)
+		return; // the whole wavefront is past the agents: nothing to reserve
 	const bool live_thread = i < c.sc->n;
 	bool removed = false;
 	if (live_thread)
