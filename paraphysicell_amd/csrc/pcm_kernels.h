@@ -1580,18 +1580,22 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
 	const int na = evw_attacks(w);
 	if (na > 0)
 	{
-		// attackers, ascending: the reference applies attacks in actor order
+		// attackers, ascending: the reference applies attacks in actor order.  Each pass
+		// is branch-free and unrolled, so eight list entries and their targets are
+		// requested per round trip (a serial scan made the attacked agents' wavefronts the
+		// kernel's tail)
 		const int cnt = c.nbr_cnt[i];
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
 		int last = -1;
 		for (int found = 0; found < na; found++)
 		{
 			int best = 0x7fffffff;
+#pragma unroll 8
 			for (int s = 0; s < cnt; s++)
 			{
 				const int k = nb[s];
-				if (k > last && k < best && c.atk_tgt[k] == i)
-					best = k;
+				const bool hit = (k > last) & (k < best) & (c.atk_tgt[k] == i);
+				best = hit ? k : best;
 			}
 			if (best == 0x7fffffff)
 				break;
