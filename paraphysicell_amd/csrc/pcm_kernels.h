@@ -2033,15 +2033,16 @@ __device__ inline void edit_links(int* list, int* cnt_ptr, const int* remap)
 	for (;;)
 	{
 		// next removal op touching this list: smallest sequence number among removed entries
+		// (branch-free and unrolled: eight entries and their remap values per round trip)
 		int best_seq = 0x7fffffff, best_id = -1;
+#pragma unroll 8
 		for (int s = 0; s < cnt; s++)
 		{
-			const int r = remap[list[s]];
-			if (r < 0 && -1 - r < best_seq)
-			{
-				best_seq = -1 - r;
-				best_id = list[s];
-			}
+			const int e = list[s];
+			const int r = remap[e];
+			const bool better = (r < 0) & (-1 - r < best_seq);
+			best_seq = better ? -1 - r : best_seq;
+			best_id = better ? e : best_id;
 		}
 		if (best_id < 0)
 			break;
@@ -2061,8 +2062,21 @@ __device__ inline void edit_links(int* list, int* cnt_ptr, const int* remap)
 			cnt--;
 		}
 	}
-	for (int s = 0; s < cnt; s++)
-		list[s] = remap[list[s]];
+	// renames: eight entries and their new indices are read before the eight stores
+	for (int s0 = 0; s0 < cnt; s0 += 8)
+	{
+		int v[8];
+#pragma unroll
+		for (int u = 0; u < 8; u++)
+			v[u] = s0 + u < cnt ? list[s0 + u] : 0;
+#pragma unroll
+		for (int u = 0; u < 8; u++)
+			v[u] = s0 + u < cnt ? remap[v[u]] : 0;
+#pragma unroll
+		for (int u = 0; u < 8; u++)
+			if (s0 + u < cnt)
+				list[s0 + u] = v[u];
+	}
 	*cnt_ptr = cnt;
 }
 
