@@ -207,13 +207,15 @@ constexpr R far_coordinate()
 template <class R>
 __device__ __forceinline__ bool within_reach_lanes(R s, R reach)
 {
+	// each comparison is used in one polarity only (no negated twins); for a NaN s or r2
+	// the result is false, as within_reach's is
 	const R e = sizeof(R) == 4 ? (R)1e-5 : (R)1e-12;
 	const R r2 = reach * reach;
-	const bool normal = r2 >= (sizeof(R) == 4 ? (R)1.2e-38 : (R)2.3e-308);
-	const bool beyond = s > r2 * ((R)1 + e);
-	const bool inside = s < r2 * ((R)1 - e);
-	bool in = !beyond;
-	if (!normal || !(beyond || inside))
+	const bool small = r2 < (sizeof(R) == 4 ? (R)1.2e-38 : (R)2.3e-308); // zero or subnormal threshold
+	const bool below_hi = s <= r2 * ((R)1 + e);
+	const bool above_lo = s >= r2 * ((R)1 - e);
+	bool in = below_hi;
+	if (small || (below_hi && above_lo))
 		in = sqrt(s) <= reach;
 	return in;
 }
@@ -407,7 +409,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 					const unsigned long long mask = __ballot(pass);
 					if (pass)
 					{
-						const int slot = base + __popcll(mask & below);
+						// rank among the passing lanes below, plus base
+						const int slot = (int)__builtin_amdgcn_mbcnt_hi(
+							(unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)base));
 						if (slot < c.K)
 							out[slot] = s_idx[t];
 					}
