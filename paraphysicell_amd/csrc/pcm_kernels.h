@@ -838,12 +838,21 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 	{
 		// spring_contract_function (position_solver.cpp:337-370)
 		const int* sp = c.spr_idx + (size_t)i * c.L;
+		// each round's slots are requested one round ahead (after the append above: an
+		// appended entry is read back by the lane that stored it)
+		int js_next[B];
+#pragma unroll
+		for (int u = 0; u < B; u++)
+			js_next[u] = u * G + lane < ns ? sp[u * G + lane] : -1;
 		for (int base = 0; base < ns; base += B * G)
 		{
 			int js[B];
 #pragma unroll
 			for (int u = 0; u < B; u++)
-				js[u] = base + u * G + lane < ns ? sp[base + u * G + lane] : -1;
+			{
+				js[u] = js_next[u];
+				js_next[u] = base + B * G + u * G + lane < ns ? sp[base + B * G + u * G + lane] : -1;
+			}
 			R f[B][D];
 			force_rec<R> other[B];
 			R kj[B], aff_ba[B];
