@@ -1800,12 +1800,18 @@ private:
 		return PCM_OK;
 	}
 
-	static int round_up_pow2(int x)
+	// a group's grid bound: its largest member rounded up to four significant bits, at most
+	// 1/8 above it (a power of two wasted up to half of the blocks); eight bounds per octave
+	// keep graph recaptures rare as members join and leave
+	static int round_up_grid(int x)
 	{
-		int p = 1;
-		while (p < x)
-			p <<= 1;
-		return p;
+		if (x <= 8)
+			return std::max(x, 1);
+		int s = 0;
+		while ((x >> s) >= 16)
+			s++;
+		const long long step = 1ll << s;
+		return (int)(((x + step - 1) / step) * step);
 	}
 
 	int launch_group(group& g, const pcm_step_params& p, int steps)
@@ -1831,7 +1837,7 @@ private:
 				grid_L = std::max(grid_L, slot.sim->spring_slots());
 				records_fresh = records_fresh && slot.sim->records_fresh();
 			}
-		grid_n = round_up_pow2(grid_n);
+		grid_n = round_up_grid(grid_n);
 		bool fold_clear = true;
 		for (auto& slot : g.slots)
 			if (slot.sim)
