@@ -1402,7 +1402,21 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 	if (g >= c.sc->n)
 		return;
 	const int i = g;
-	if (AB2 && lane < D && c.movable[i])
+	// the actor's inputs are requested first: the compiler cannot move loads past the
+	// position update's stores, so they would cost a second round trip after it.
+	// Every branch compares a draw u in [0,1) with `<` against rate * dt, so an actor whose
+	// rates are all <= 0 never acts and its draws need not be made (bitwise ors, not
+	// short-circuits: the rates are requested together)
+	bool any_rate = c.dead_phago[i] > 0;
+	for (int k = 0; k < c.T; k++)
+		any_rate |= (c.live_phago[(size_t)i * c.T + k] > 0) | (c.attack[(size_t)i * c.T + k] > 0)
+					| (c.fusion[(size_t)i * c.T + k] > 0);
+	any_rate &= c.dead[i] != 1;
+	const bool movable = c.movable[i];
+	const int listed = c.nbr_cnt[i];
+	const int ti = c.type[i];
+	const bool self_removed = c.flags[i] == PCM_FLAG_TO_REMOVE;
+	if (AB2 && lane < D && movable)
 	{
 		const R factor = (R)((double)c.dt * 1.5);
 		const R previous_factor = (R)((double)c.dt * -0.5);
@@ -1416,22 +1430,12 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 		c.atk_tgt[i] = -1;
 	int n_rem = 0;
 	bool acted = false;
-	// every branch compares a draw u in [0,1) with `<` against rate * dt, so an actor whose
-	// rates are all <= 0 never acts and its draws need not be made
-	// (bitwise ors, not short-circuits: the rates are requested together, one round trip)
-	bool any_rate = c.dead_phago[i] > 0;
-	for (int k = 0; k < c.T; k++)
-		any_rate |= (c.live_phago[(size_t)i * c.T + k] > 0) | (c.attack[(size_t)i * c.T + k] > 0)
-					| (c.fusion[(size_t)i * c.T + k] > 0);
-	any_rate &= c.dead[i] != 1;
 	if (any_rate)
 	{
-		const int cnt = visible_neighbors<R>(c, i);
+		const int cnt = movable ? listed : 0; // visible_neighbors
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
-		const int ti = c.type[i];
 		const R dt = c.dt;
 		const unsigned long long step = c.sc->step;
-		const bool self_removed = c.flags[i] == PCM_FLAG_TO_REMOVE;
 		const R p_dead = c.dead_phago[i] * dt;
 		bool attacked_once = false, phagocytosed_once = false, fused_once = false, stop = false;
 		int j_next = lane < cnt ? nb[lane] : -1; // each slot is requested one iteration ahead
