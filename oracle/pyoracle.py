@@ -1,4 +1,4 @@
-"""ORACLE (test infrastructure only): Python driver of liboracle.so.
+"""ORACLE (test infrastructure only): Python wrapper of liboracle.so.
 
 Same interface as paraphysicell_amd.mechanics.DeviceSim, so a test can run the device
 library and the CPU restatement side by side on identical host arrays.  Only tests/,

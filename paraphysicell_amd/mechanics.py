@@ -74,7 +74,7 @@ def _declare(lib, prefix: str):
 
 
 class _LibSim:
-    """Common driver over a C library exposing the pcm.h entry points under ``prefix``."""
+    """Common wrapper over a C library exposing the pcm.h entry points under ``prefix``."""
 
     prefix = "pcm_"
     _mesh_only_phases = ("update_mechanics_mesh", "update_cell_neighbors")
