@@ -23,10 +23,31 @@ __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadId
 
 
 
+// Each wavefront reads the status word and returns on its own.  That is only safe where the
+// word cannot change during the launch, or where the block's wavefronts share nothing
+// (no LDS, no barrier): a kernel that can raise the status itself while its blocks are
+// still being dispatched uses PCM_PROLOGUE_BLOCK instead.
 #define PCM_PROLOGUE(c)                                                                                                \
 	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
 	if (c.sc->status)                                                                                                  \
 		return;
+
+// Block-uniform exit, for kernels whose wavefronts cooperate through LDS or barriers and
+// whose own blocks can raise the status word.  With a per-wavefront test, another block's
+// raise landing between two wavefronts' reads lets one wavefront leave while the other
+// stays: the one that stays then reads LDS the other was to fill (stencil offsets, scan
+// partials) and indexes global memory with it.  Thread 0 reads the word once and every
+// wavefront decides on that one value.
+#define PCM_PROLOGUE_BLOCK(c)                                                                                          \
+	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
+	{                                                                                                                  \
+		__shared__ int s_status_;                                                                                      \
+		if (threadIdx.x == 0)                                                                                          \
+			s_status_ = c.sc->status;                                                                                  \
+		__syncthreads();                                                                                               \
+		if (s_status_)                                                                                                 \
+			return;                                                                                                    \
+	}
 
 // ------------------------------------------------------------------ step bookkeeping
 template <class R>
@@ -317,7 +338,7 @@ template <class R, int D, int W>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 7 : 6))) k_neighbors(
 	const SimCtx<R>* __restrict__ ctxs)
 {
-	PCM_PROLOGUE(c)
+	PCM_PROLOGUE_BLOCK(c)
 	constexpr int TILE = nbr_tile<R>();
 	__shared__ R s_rec[TILE * 4];
 	__shared__ int s_idx[TILE];
@@ -1017,7 +1038,7 @@ __device__ __forceinline__ void spring_compact_agent(const SimCtx<R>& c, int i)
 template <class R, int G>
 __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
 {
-	PCM_PROLOGUE(c)
+	PCM_PROLOGUE_BLOCK(c)
 	// a block past the agents (a batch's grid covers its largest member) owns no segment
 	// and skips the block-wide scan below
 	if ((int)blockIdx.x * (BLOCK / G) >= c.sc->n)
