@@ -49,6 +49,17 @@ __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadId
 			return;                                                                                                    \
 	}
 
+// The neighbor builds ignore the one bit they raise themselves (neighbor overflow), so their
+// exit decision cannot change during their launch and a per-wavefront test is uniform over
+// each block.  Every block then finishes its lists (rows hold their first K entries) and
+// max_neighbors is the longest list of the whole population, which the host's regrowth
+// needs.  After an overflow the rest of the step's kernels return, so a neighbor build of a
+// later step in the same call recomputes the same lists from the same mesh, in bounds.
+#define PCM_PROLOGUE_NEIGHBORS(c)                                                                                      \
+	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
+	if (c.sc->status & ~ST_NEIGHBOR_OVERFLOW)                                                                          \
+		return;
+
 // ------------------------------------------------------------------ step bookkeeping
 template <class R>
 __global__ void k_begin_step(const SimCtx<R>* __restrict__ ctxs)
@@ -163,7 +174,9 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs)
 	c.btmp[c.vstart[c.vox_of[i]] + c.vrank[i]] = i;
 }
 
-template <class R, int D>
+// RECORDS: also write the agents' force records for the step's gather kernels (a mesh
+// update for the neighbor lists alone has no force-record buffer)
+template <class R, int D, bool RECORDS = true>
 __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 {
 	PCM_PROLOGUE(c)
@@ -171,7 +184,8 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 	if (i >= c.sc->n)
 		return;
 	// the agent's own data first: its loads overlap the rank chain below
-	write_force_record<R, D>(c, i);
+	if (RECORDS)
+		write_force_record<R, D>(c, i);
 	R rec[4];
 	for (int k = 0; k < 3; k++)
 		rec[k] = k < D ? c.pos[(size_t)i * D + k] : (R)0;
@@ -338,7 +352,7 @@ template <class R, int D, int W>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 7 : 6))) k_neighbors(
 	const SimCtx<R>* __restrict__ ctxs)
 {
-	PCM_PROLOGUE_BLOCK(c)
+	PCM_PROLOGUE_NEIGHBORS(c)
 	constexpr int TILE = nbr_tile<R>();
 	__shared__ R s_rec[TILE * 4];
 	__shared__ int s_idx[TILE];
@@ -464,7 +478,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 template <class R, int D>
 __global__ void k_neighbors_direct(const SimCtx<R>* __restrict__ ctxs)
 {
-	PCM_PROLOGUE(c)
+	PCM_PROLOGUE_NEIGHBORS(c)
 	const int i = gid();
 	if (i >= c.sc->n)
 		return;
