@@ -365,6 +365,33 @@ int pcm_profile_begin(pcm_sim* sim, uint64_t kernel_mask);
 int pcm_profile_every(pcm_sim* sim, int32_t every);
 int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches);
 
+/* ---------------------------------------------------------------- allocation-free neighbor build
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  It is
+ * pcm_update_mechanics_mesh followed by pcm_update_cell_neighbors (containers_solver.cpp:15-56,
+ * position_solver.cpp:262-286) on caller-owned device arrays, with no handle: the mesh
+ * scratch (voxel counts and starts, buckets, the voxel-sorted copy) lives in one fixed
+ * library area of PCM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  The call
+ * allocates nothing, copies nothing between host and device and does not wait: it enqueues
+ * kernels on `stream` (a hipStream_t; NULL is the null stream) and returns.
+ *   positions     count * mesh->dims reals (real_bytes wide), device memory;
+ *   radius, relative_maximum_adhesion_distance   count reals each, device memory;
+ *   is_movable    count bytes, device memory, or NULL (every agent movable);
+ *   neighbor_counts[count], neighbor_indices[count * max_neighbors]   device outputs: row i
+ *                 holds agent i's list in the reference's order; non-movable agents get an
+ *                 empty list, as in pcm_get_neighbors;
+ *   needed[1]     device output: 0 when every list fit in max_neighbors, else the longest
+ *                 list, and rows then hold their first max_neighbors entries.
+ * The call needs at most 8 KiB + 8 * voxels + (20 + 4 * real_bytes) * count bytes of the
+ * area (about 22 MB for 88,700 3D agents on a 134^3 grid); a larger call fails with PCM_ERR_CAPACITY
+ * before enqueueing anything.  Every call on one GPU shares that GPU's area and leaves its
+ * contents behind, so calls that overlap on one GPU (two streams, two threads) race. */
+#define PCM_NOALLOC_SCRATCH_BYTES (96u << 20)
+int pcm_neighbors_noalloc(const pcm_mesh_desc* mesh, int32_t real_bytes, int32_t count, const void* positions,
+						  const void* radius, const void* relative_maximum_adhesion_distance, const uint8_t* is_movable,
+						  int32_t max_neighbors, int32_t* neighbor_counts, int32_t* neighbor_indices, int32_t* needed,
+						  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

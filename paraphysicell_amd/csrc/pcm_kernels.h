@@ -485,6 +485,49 @@ __global__ void k_neighbors_direct(const SimCtx<R>* __restrict__ ctxs)
 	neighbors_one_agent<R, D, true>(c, i);
 }
 
+// ------------------------------------------------------------------ neighbor build without allocation
+// The mesh and neighbor kernels above, run on caller-owned agent arrays with the mesh
+// scratch in a fixed device area.  The context arrives as a kernel argument and is stored
+// into that area for the kernels that follow; the step scalars restart and the voxel counts
+// of the whole grid are cleared (the area keeps whatever the previous call left).
+template <class R>
+__global__ void k_noalloc_setup(const SimCtx<R> h, SimCtx<R>* __restrict__ dst, int* __restrict__ needed)
+{
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (blockIdx.x == 0)
+	{
+		constexpr int words = (int)(sizeof(SimCtx<R>) / sizeof(int));
+		static_assert(sizeof(SimCtx<R>) % sizeof(int) == 0, "context copied in 4-byte words");
+		const int* src = (const int*)&h;
+		int* out = (int*)dst;
+		for (int w = threadIdx.x; w < words; w += blockDim.x)
+			out[w] = src[w];
+		if (threadIdx.x == 0)
+		{
+			Scalars s = {};
+			s.n = h.cap;
+			*h.sc = s;
+			*needed = 0;
+		}
+	}
+	for (long long v = t; v < h.vcount; v += (long long)gridDim.x * blockDim.x)
+		h.vcnt[v] = 0;
+}
+
+// Lists of non-movable agents are reported empty (update_cell_neighbors_single lists movable
+// agents only, position_solver.cpp:190-191); `needed` receives the longest list when some
+// list did not fit.
+template <class R>
+__global__ void k_noalloc_finish(const SimCtx<R>* __restrict__ ctxs, int* __restrict__ needed)
+{
+	const SimCtx<R>& c = ctxs[0];
+	const int i = gid();
+	if (i == 0)
+		*needed = (c.sc->status & ST_NEIGHBOR_OVERFLOW) ? c.sc->max_neighbors : 0;
+	if (i < c.sc->n && c.movable && !c.movable[i])
+		c.nbr_cnt[i] = 0;
+}
+
 // number of neighbors of i the reference would list (empty for non-movable agents)
 template <class R>
 __device__ __forceinline__ int visible_neighbors(const SimCtx<R>& c, int i)

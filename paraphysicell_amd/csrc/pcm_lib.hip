@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1913,6 +1914,141 @@ private:
 
 pcm_sim* checked(pcm_sim* s) { return s; }
 
+// ---------------------------------------------------------------------------- allocation-free neighbor build
+// The mesh scratch of pcm_neighbors_noalloc: one fixed area per GPU, held until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device memory: the device
+// address comes from hipGetSymbolAddress, taken once per device with that device current,
+// and only that address is ever handed to kernels.
+static __device__ __align__(256) unsigned char scratch[PCM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_of_current_device(unsigned char** out)
+{
+	int dev = -1;
+	PCM_HIP(hipGetDevice(&dev));
+	if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+		return fail(PCM_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_ptr[dev])
+	{
+		void* p = nullptr;
+		PCM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		if (!p)
+			return fail(PCM_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch area");
+		g_scratch_ptr[dev] = p;
+	}
+	*out = (unsigned char*)g_scratch_ptr[dev];
+	return PCM_OK;
+}
+
+// bytes of the scratch area a call needs (the layout of noalloc_build, each part aligned)
+constexpr size_t noalloc_align(size_t b) { return (b + 255) & ~(size_t)255; }
+size_t noalloc_bytes(long long voxels, long long count, int real_bytes)
+{
+	const size_t n = (size_t)std::max<long long>(count, 1);
+	return noalloc_align(sizeof(SimCtx<double>)) + noalloc_align(sizeof(Scalars)) + 2 * noalloc_align(4 * (size_t)voxels) +
+		   5 * noalloc_align(4 * n) + noalloc_align(4 * (size_t)real_bytes * n);
+}
+
+template <class R, int D>
+int noalloc_build(const SimCtx<R>& proto, unsigned char* area, int* needed, hipStream_t stream)
+{
+	SimCtx<R> h = proto;
+	const size_t n = (size_t)std::max(h.cap, 1);
+	unsigned char* p = area;
+	auto take = [&p](size_t bytes) {
+		unsigned char* q = p;
+		p += noalloc_align(bytes);
+		return q;
+	};
+	SimCtx<R>* d_ctx = (SimCtx<R>*)take(sizeof(SimCtx<double>));
+	h.sc = (gptr<Scalars>)take(sizeof(Scalars));
+	h.vcnt = (gptr<int>)take(4 * (size_t)h.vcount);
+	h.vstart = (gptr<int>)take(4 * (size_t)h.vcount);
+	h.vox_of = (gptr<int>)take(4 * n);
+	h.vrank = (gptr<int>)take(4 * n);
+	h.btmp = (gptr<int>)take(4 * n);
+	h.bucket = (gptr<int>)take(4 * n);
+	h.occ = (gptr<int>)take(4 * n);
+	h.sorted = (gptr<R>)take(4 * sizeof(R) * n);
+	if ((size_t)(p - area) > PCM_NOALLOC_SCRATCH_BYTES)
+		return fail(PCM_ERR_CAPACITY, "scratch layout exceeds the area"); // checked by the caller first
+
+	const int agents = h.cap;
+	const int blocks = std::max(1, (agents + BLOCK - 1) / BLOCK);
+	const int clear_blocks = (int)std::min<long long>(std::max<long long>(1, (h.vcount + BLOCK - 1) / BLOCK), 8192);
+	hipLaunchKernelGGL(k_noalloc_setup<R>, dim3(clear_blocks), dim3(BLOCK), 0, stream, h, d_ctx, needed);
+	PCM_HIP(hipGetLastError());
+	const SimCtx<R>* ctx = d_ctx;
+	hipLaunchKernelGGL((k_bin_count<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx);
+	hipLaunchKernelGGL((k_bin_claim<R, false>), dim3(std::max(1, (agents + CLAIM_THREADS - 1) / CLAIM_THREADS)),
+					   dim3(CLAIM_THREADS), 0, stream, ctx);
+	hipLaunchKernelGGL(k_bin_scatter<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx);
+	hipLaunchKernelGGL((k_bin_order<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx);
+	hipLaunchKernelGGL((k_neighbors<R, D, 2>), dim3(std::max(1, std::min(agents, 16384))), dim3(128), 0, stream, ctx);
+	hipLaunchKernelGGL(k_noalloc_finish<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx, needed);
+	PCM_HIP(hipGetLastError());
+	return PCM_OK;
+}
+
+template <class R>
+int noalloc_dispatch(const SimCtx<R>& h, unsigned char* area, int* needed, hipStream_t stream)
+{
+	switch (h.dims)
+	{
+		case 1:
+			return noalloc_build<R, 1>(h, area, needed, stream);
+		case 2:
+			return noalloc_build<R, 2>(h, area, needed, stream);
+		default:
+			return noalloc_build<R, 3>(h, area, needed, stream);
+	}
+}
+
+template <class R>
+int noalloc_entry(const pcm_mesh_desc& mesh, int count, const void* positions, const void* radius, const void* rel_adh,
+				  const uint8_t* movable, int K, int32_t* counts, int32_t* indices, int32_t* needed, hipStream_t stream)
+{
+	SimCtx<R> h;
+	std::memset(&h, 0, sizeof(h));
+	h.dims = mesh.dims;
+	for (int k = 0; k < 3; k++)
+	{
+		h.bmin[k] = mesh.bbox_min[k];
+		h.bmax[k] = mesh.bbox_max[k];
+		h.vox[k] = mesh.voxel[k];
+		if (k < h.dims)
+		{
+			if (h.vox[k] <= 0 || h.bmax[k] < h.bmin[k])
+				return fail(PCM_ERR_ARGUMENT, "bad mesh box or voxel shape");
+			h.grid[k] = std::max(1, (h.bmax[k] - h.bmin[k] + h.vox[k] - 1) / h.vox[k]);
+		}
+		else
+			h.grid[k] = 1;
+	}
+	h.vcount = (long long)h.grid[0] * h.grid[1] * h.grid[2];
+	const size_t need = noalloc_bytes(h.vcount, count, (int)sizeof(R));
+	if (need > PCM_NOALLOC_SCRATCH_BYTES)
+		return fail(PCM_ERR_CAPACITY, "pcm_neighbors_noalloc: " + std::to_string(h.vcount) + " voxels and " +
+										  std::to_string(count) + " agents need " + std::to_string(need) +
+										  " bytes of scratch, above PCM_NOALLOC_SCRATCH_BYTES (" +
+										  std::to_string((size_t)PCM_NOALLOC_SCRATCH_BYTES) + ")");
+	h.cap = count;
+	h.K = K;
+	h.pos = (gptr<R>)(R*)positions;
+	h.radius = (gptr<R>)(R*)radius;
+	h.rel_adh = (gptr<R>)(R*)rel_adh;
+	h.movable = (gptr<uint8_t>)(uint8_t*)movable;
+	h.nbr_cnt = (gptr<int>)counts;
+	h.nbr_idx = (gptr<int>)indices;
+	unsigned char* area = nullptr;
+	PCM_TRY(scratch_of_current_device(&area));
+	return noalloc_dispatch<R>(h, area, needed, stream);
+}
+
 } // namespace
 
 // ============================================================================ C-ABI
@@ -2309,6 +2445,30 @@ int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches)
 {
 	PCM_NEED(sim);
 	return sim->profile_end(total_ms, launches);
+}
+
+int pcm_neighbors_noalloc(const pcm_mesh_desc* mesh, int32_t real_bytes, int32_t count, const void* positions,
+						  const void* radius, const void* relative_maximum_adhesion_distance, const uint8_t* is_movable,
+						  int32_t max_neighbors, int32_t* neighbor_counts, int32_t* neighbor_indices, int32_t* needed,
+						  void* stream)
+{
+	if (!mesh || mesh->dims < 1 || mesh->dims > 3)
+		return fail(PCM_ERR_ARGUMENT, "pcm_neighbors_noalloc: mesh dims must be 1, 2 or 3");
+	if (real_bytes != 4 && real_bytes != 8)
+		return fail(PCM_ERR_ARGUMENT, "pcm_neighbors_noalloc: real_bytes must be 4 or 8");
+	if (count < 0 || max_neighbors < 1)
+		return fail(PCM_ERR_ARGUMENT, "pcm_neighbors_noalloc: negative count or max_neighbors below 1");
+	if (!needed || (count > 0 && (!positions || !radius || !relative_maximum_adhesion_distance || !neighbor_counts ||
+								  !neighbor_indices)))
+		return fail(PCM_ERR_ARGUMENT, "pcm_neighbors_noalloc: null device buffer");
+	if ((long long)count * max_neighbors > 0x7fffffffLL)
+		return fail(PCM_ERR_ARGUMENT, "pcm_neighbors_noalloc: count * max_neighbors exceeds 2^31 entries");
+	hipStream_t s = (hipStream_t)stream;
+	if (real_bytes == 4)
+		return noalloc_entry<float>(*mesh, count, positions, radius, relative_maximum_adhesion_distance, is_movable,
+									max_neighbors, neighbor_counts, neighbor_indices, needed, s);
+	return noalloc_entry<double>(*mesh, count, positions, radius, relative_maximum_adhesion_distance, is_movable,
+								 max_neighbors, neighbor_counts, neighbor_indices, needed, s);
 }
 
 } // extern "C"

@@ -532,3 +532,34 @@ class Solver:
 
     def initialize(self):
         self.sim.initialize_mesh()
+
+
+def neighbors_noalloc(mesh: abi.MeshSpec, real_bytes: int, count: int, positions, radius, rel_adh, movable,
+                      max_neighbors: int, counts, indices, needed, stream=None, lib=None) -> None:
+    """pcm_neighbors_noalloc: the mechanics mesh and neighbor lists of `count` agents whose
+    fields already sit in device memory, with no device allocation and no host round trip.
+
+    Buffers are device addresses (ints) or objects with ``data_ptr()`` (torch tensors):
+    positions (count * dims reals), radius and rel_adh (count reals), movable (count bytes
+    or None); outputs counts (count int32), indices (count * max_neighbors int32) and
+    needed (one int32: 0, or the longest list when some list did not fit).  The kernels run
+    on `stream` (a raw hipStream_t address, or an object with ``cuda_stream``); the call
+    returns without waiting for them."""
+    lib = lib or load_library()
+    fn = lib.pcm_neighbors_noalloc
+    fn.argtypes = [ctypes.POINTER(abi.MeshDesc), ctypes.c_int32, ctypes.c_int32, _PTR, _PTR, _PTR, _PTR,
+                   ctypes.c_int32, _PTR, _PTR, _PTR, _PTR]
+    fn.restype = ctypes.c_int
+
+    def addr(b):
+        if b is None:
+            return None
+        return b.data_ptr() if hasattr(b, "data_ptr") else int(b)
+
+    s = getattr(stream, "cuda_stream", stream)
+    desc = mesh.to_c()
+    rc = fn(ctypes.byref(desc), real_bytes, count, addr(positions), addr(radius), addr(rel_adh), addr(movable),
+            max_neighbors, addr(counts), addr(indices), addr(needed), s)
+    if rc != abi.PCM_OK:
+        msg = lib.pcm_last_error()
+        raise PcmError(rc, msg.decode() if msg else "")

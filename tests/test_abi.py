@@ -92,3 +92,29 @@ def test_tuning_argument_errors(lib):
     assert lib.pcm_set_tuning(None, ctypes.byref(t)) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_autotune(None, None, 1, None) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_batch_autotune(None, None, 1) == abi.PCM_ERR_ARGUMENT
+
+
+def test_noalloc_refuses_bad_sizes_before_touching_the_device(lib):
+    """The allocation-free neighbor build checks its arguments and its scratch need on the
+    host, before any device call, so these errors come back on a machine without a GPU too."""
+    fake = 4096  # never dereferenced: every call below fails its host-side checks first
+    small = abi.MeshSpec(3, (0, 0, 0), (90, 90, 90))
+    with pytest.raises(mechanics.PcmError) as e:
+        mechanics.neighbors_noalloc(small, 2, 10, fake, fake, fake, None, 8, fake, fake, fake, lib=lib)
+    assert e.value.code == abi.PCM_ERR_ARGUMENT
+    with pytest.raises(mechanics.PcmError) as e:
+        mechanics.neighbors_noalloc(small, 4, 10, None, fake, fake, None, 8, fake, fake, fake, lib=lib)
+    assert e.value.code == abi.PCM_ERR_ARGUMENT
+    with pytest.raises(mechanics.PcmError) as e:
+        mechanics.neighbors_noalloc(small, 4, 10, fake, fake, fake, None, 0, fake, fake, fake, lib=lib)
+    assert e.value.code == abi.PCM_ERR_ARGUMENT
+    # 334^3 = 37M voxels need 298 MB of voxel counts and starts: beyond the fixed area
+    huge = abi.MeshSpec(3, (-5000,) * 3, (5000,) * 3)
+    with pytest.raises(mechanics.PcmError) as e:
+        mechanics.neighbors_noalloc(huge, 4, 10, fake, fake, fake, None, 8, fake, fake, fake, lib=lib)
+    assert e.value.code == abi.PCM_ERR_CAPACITY
+    assert "PCM_NOALLOC_SCRATCH_BYTES" in str(e.value)
+    # so do 5M agents on a small grid
+    with pytest.raises(mechanics.PcmError) as e:
+        mechanics.neighbors_noalloc(small, 8, 5_000_000, fake, fake, fake, None, 1, fake, fake, fake, lib=lib)
+    assert e.value.code == abi.PCM_ERR_CAPACITY
