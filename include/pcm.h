@@ -319,8 +319,14 @@ int pcm_batch_destroy(pcm_batch* batch);
 int pcm_batch_admit(pcm_batch* batch, pcm_sim* sim);
 int pcm_batch_retire(pcm_batch* batch, pcm_sim* sim);
 int pcm_batch_size(pcm_batch* batch, int32_t* sims);
-/* every member advances `steps` mechanics steps under the same parameters */
+/* Every member advances `steps` mechanics steps.  With params != NULL all members step under
+ * those parameters (and keep them); with params == NULL each member steps under its own:
+ * those of its last pcm_set_step_params, pcm_step or phase call. */
 int pcm_batch_step(pcm_batch* batch, const pcm_step_params* params, int32_t steps);
+/* The environment scalars a simulation steps under when its batch is stepped with
+ * params == NULL (environment::mechanics_time_step, virtual_wall_at_domain_edges,
+ * automated_spring_adhesion: each simulation of the reference owns its environment). */
+int pcm_set_step_params(pcm_sim* sim, const pcm_step_params* params);
 int pcm_batch_synchronize(pcm_batch* batch);
 /* enable != 0: each group's step is captured once as a hipGraph over all member slots and
  * replayed per step (joining and leaving rewrite slots without a new capture).  A capture

@@ -493,6 +493,14 @@ int oracle_step(void* h, const pcm_step_params* p, int32_t steps)
 	return PCM_OK;
 }
 
+int oracle_set_step_params(void* h, const pcm_step_params* p)
+{
+	if (!p)
+		return fail(PCM_ERR_ARGUMENT, "null params");
+	cast(h)->set_params(p);
+	return PCM_OK;
+}
+
 // Wall time of `steps` mechanics steps (after `warmup` untimed ones), in milliseconds.
 int oracle_time_steps(void* h, const pcm_step_params* p, int32_t warmup, int32_t steps, double* ms)
 {

@@ -992,7 +992,9 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
 		c.sc->res_open = 0; // the resolver's open-actor list starts empty every update
 	if (i >= c.sc->n)
 		return;
-	const int cnt = c.spr_cnt[i];
+	// a batch launches the spring kernels when any member has springs on; a member with them
+	// off fires nothing and draws nothing (its erase pass then only stores the step counter)
+	const int cnt = c.springs ? c.spr_cnt[i] : 0;
 	const R rate = c.det_rate[i] * c.dt;
 	const unsigned long long step = c.sc->step + (STARTS_STEP ? 1 : 0);
 	uint32_t mask = 0;
@@ -1112,7 +1114,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 		i = g;
 		if (lane == 0)
 			spring_compact_agent<R>(c, i);
-		const int cnt = visible_neighbors<R>(c, i);
+		const int cnt = c.springs ? visible_neighbors<R>(c, i) : 0; // springs off: no attachment draws
 		const int* nb = c.nbr_idx + (size_t)i * c.K;
 		const int ti = c.type[i];
 		const R rate_i = c.att_rate[i] * c.dt;

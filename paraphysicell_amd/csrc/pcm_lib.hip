@@ -84,6 +84,7 @@ struct pcm_sim
 	virtual int division_log(int32_t* parents, int64_t capacity, int64_t* length) = 0;
 	virtual int phenotype_update(const pcm_step_params* p, int compute_internalized) = 0;
 	virtual int set_step(uint64_t s) = 0;
+	virtual int set_step_params(const pcm_step_params& p) = 0;
 	virtual int get_step(uint64_t* s) = 0;
 	virtual int set_gradients(const int32_t* voxel, const void* g) = 0;
 	virtual int phase(int which, const pcm_step_params* p) = 0;
@@ -695,6 +696,8 @@ public:
 				return container_kernels<3>();
 		}
 	}
+
+	int set_step_params(const pcm_step_params& p) override { return set_params(p); }
 
 	int set_step(uint64_t s) override
 	{
@@ -1660,13 +1663,15 @@ public:
 		return PCM_OK;
 	}
 
+	// p == nullptr: every member steps under its own parameters (its last pcm_step,
+	// phase call or pcm_set_step_params); each kernel reads them from its slot's context
 	int step(const pcm_step_params* p, int steps) override
 	{
-		if (!p || steps < 0)
-			return fail(PCM_ERR_ARGUMENT, "null params or negative step count");
+		if (steps < 0)
+			return fail(PCM_ERR_ARGUMENT, "negative step count");
 		// launch every group, then collect: the groups' streams run concurrently
 		for (auto& g : groups_)
-			PCM_TRY(launch_group(g, *p, steps));
+			PCM_TRY(launch_group(g, p, steps));
 		for (auto& g : groups_)
 			PCM_TRY(collect_group(g, steps));
 		return PCM_OK;
@@ -1685,8 +1690,8 @@ public:
 	// tune each group's launch shape on its real steps (graphs are off while tuning)
 	int autotune(const pcm_step_params* p, int steps) override
 	{
-		if (!p || steps < 1)
-			return fail(PCM_ERR_ARGUMENT, "null params or no steps per candidate");
+		if (steps < 1)
+			return fail(PCM_ERR_ARGUMENT, "no steps per candidate");
 		const bool graphs = graphs_;
 		graphs_ = false;
 		int rc = PCM_OK;
@@ -1700,7 +1705,7 @@ public:
 			rc = g.placeholder->autotune_with(
 				steps,
 				[&](int k) {
-					PCM_TRY(launch_group(g, *p, k));
+					PCM_TRY(launch_group(g, p, k));
 					return collect_group(g, k);
 				},
 				nullptr);
@@ -1815,7 +1820,7 @@ private:
 		return (int)(((x + step - 1) / step) * step);
 	}
 
-	int launch_group(group& g, const pcm_step_params& p, int steps)
+	int launch_group(group& g, const pcm_step_params* p, int steps)
 	{
 		g.active = false;
 		int members = 0;
@@ -1828,12 +1833,23 @@ private:
 		PCM_TRY(reserve(g, m));
 		int grid_n = 1, grid_L = 1;
 		bool records_fresh = true;
-		PCM_TRY(g.placeholder->apply_params(p));
+		// the leader launches the spring kernels when any member has springs on (members with
+		// them off skip the work inside those kernels)
+		int any_springs = 0;
 		for (auto& slot : g.slots)
 			if (slot.sim)
 			{
 				PCM_TRY(slot.sim->ready());
-				PCM_TRY(slot.sim->apply_params(p));
+				if (p)
+					PCM_TRY(slot.sim->apply_params(*p));
+				any_springs |= slot.sim->host_ctx().springs;
+			}
+		pcm_step_params lp = p ? *p : pcm_step_params { 0.1, 0, 0 };
+		lp.automated_spring_adhesion = any_springs;
+		PCM_TRY(g.placeholder->apply_params(lp));
+		for (auto& slot : g.slots)
+			if (slot.sim)
+			{
 				grid_n = std::max(grid_n, slot.sim->launch_bound());
 				grid_L = std::max(grid_L, slot.sim->spring_slots());
 				records_fresh = records_fresh && slot.sim->records_fresh();
@@ -2149,6 +2165,13 @@ int pcm_batch_size(pcm_batch* batch, int32_t* sims)
 	if (!batch || !sims)
 		return fail(PCM_ERR_ARGUMENT, "null argument");
 	return batch->size(sims);
+}
+
+int pcm_set_step_params(pcm_sim* sim, const pcm_step_params* params)
+{
+	if (!sim || !params)
+		return fail(PCM_ERR_ARGUMENT, "null handle or params");
+	return sim->set_step_params(*params);
 }
 
 int pcm_batch_step(pcm_batch* batch, const pcm_step_params* params, int32_t steps)

@@ -64,6 +64,7 @@ def _declare(lib, prefix: str):
         "get_divisions_count": [P, ctypes.POINTER(ctypes.c_int64)],
         "division_log": [P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
         "step": [P, ctypes.POINTER(abi.StepParams), ctypes.c_int32],
+        "set_step_params": [P, ctypes.POINTER(abi.StepParams)],
     }
     for name, args in sig.items():
         fn = getattr(lib, prefix + name)
@@ -291,6 +292,11 @@ class _LibSim:
         p = (params or abi.Params()).to_c()
         self._call("step", self._h, ctypes.byref(p), int(steps))
 
+    def set_step_params(self, params) -> None:
+        """The parameters this simulation steps under in a batch stepped with params None."""
+        p = params.to_c()
+        self._call("set_step_params", self._h, ctypes.byref(p))
+
     def run_phases(self, params=None):
         """The mechanics step as the sequence of per-phase calls (src/simulator.cpp:171-192)."""
         for name in ("evaluate_interactions", "update_cell_forces", "update_motility", "update_basement_membrane_interactions",
@@ -471,7 +477,12 @@ class Batch:
         return n.value
 
     def step(self, params=None, steps: int = 1) -> None:
-        p = (params or abi.Params()).to_c()
+        """Advance every member `steps` steps: under `params`, or with params None each
+        member under its own (DeviceSim.set_step_params or its last step / phase call)."""
+        if params is None:
+            self._check(self._lib.pcm_batch_step(self._h, None, int(steps)))
+            return
+        p = params.to_c()
         self._check(self._lib.pcm_batch_step(self._h, ctypes.byref(p), int(steps)))
 
     def synchronize(self) -> None:
@@ -482,8 +493,8 @@ class Batch:
 
     def autotune(self, params=None, steps_per_candidate: int = 5) -> None:
         """Pick each dimensionality group's launch shape on the batch's own steps."""
-        p = (params or abi.Params()).to_c()
-        self._check(self._lib.pcm_batch_autotune(self._h, ctypes.byref(p), int(steps_per_candidate)))
+        p = None if params is None else ctypes.byref(params.to_c())
+        self._check(self._lib.pcm_batch_autotune(self._h, p, int(steps_per_candidate)))
 
     def close(self) -> None:
         if self._h:

@@ -64,12 +64,14 @@ def _start(req: Request) -> DeviceSim:
     sim = DeviceSim(req.mesh, st.real_bytes, st.defs, st.subs, capacity=st.count, seed=req.seed)
     sim.upload(st)
     sim.initialize_mesh()
+    sim.set_step_params(req.params)  # the request's own dt, wall and springs in the batch
     return sim
 
 
 def _result(sim: DeviceSim) -> dict:
-    return {"count": sim.count, "positions": sim.get_field("positions"), "velocities": sim.get_field("velocities"),
-            "deaths": sim.deaths_count}
+    out = {name: sim.get_field(name) for name in ("positions", "velocities", "simple_pressure", "damage")}
+    out.update(count=sim.count, deaths=sim.deaths_count, springs=sim.links(abi.LINKS_SPRINGS))
+    return out
 
 
 class Server:
@@ -111,8 +113,9 @@ class Server:
                 continue
             k = min(self.chunk, min(q.remaining for q in live))
             before = sum(q.sim.count for q in live)
-            self.batch.step(live[0].params, k)
-            self.agent_steps += before * k
+            self.batch.step(None, k)  # every member under its own request's parameters
+            after = sum(q.sim.count for q in live)
+            self.agent_steps += 0.5 * (before + after) * k
             for q in list(live):
                 q.remaining -= k
                 if q.remaining == 0:
@@ -137,8 +140,9 @@ def run_alone(requests: list[Request]) -> tuple[float, int]:
     agent_steps = 0
     for q in requests:
         sim = _start(q)
-        agent_steps += sim.count * q.steps
+        before = sim.count
         sim.step(q.params, q.steps)
+        agent_steps += 0.5 * (before + sim.count) * q.steps
         q.result = _result(sim)
         sim.close()
     return time.perf_counter() - t0, agent_steps
@@ -155,8 +159,6 @@ def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, 
     """Advance every (mesh, state, seed) member `steps` steps; returns (seconds, agent-steps,
     sims).  The timed region covers the steps only: states are resident and the launch
     shape tuned (batch groups, or each simulation alone) before it starts."""
-    import torch
-
     sims = []
     for mesh, st, seed in members:
         sim = DeviceSim(mesh, st.real_bytes, st.defs, st.subs, capacity=st.count, seed=seed)
@@ -179,7 +181,6 @@ def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, 
                 s.autotune(params, 2)
             s.step(params, 1)
             s.synchronize()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     if batched:
         batch.step(params, steps)
@@ -189,7 +190,6 @@ def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, 
             s.step(params, steps)
         for s in sims:
             s.synchronize()
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     agent_steps = sum(s.count for s in sims) * 0.5 * steps + agents * 0.5 * steps
     if batch is not None:

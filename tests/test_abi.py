@@ -82,7 +82,8 @@ def test_batch_argument_errors(lib):
     assert lib.pcm_batch_size(b, ctypes.byref(n)) == abi.PCM_OK and n.value == 0
     assert lib.pcm_batch_admit(b, None) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_batch_retire(b, None) == abi.PCM_ERR_ARGUMENT
-    assert lib.pcm_batch_step(b, None, 1) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_step(b, None, -1) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_step(b, None, 1) == abi.PCM_OK  # no members: nothing to do
     assert lib.pcm_batch_step(None, None, 1) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_batch_destroy(b) == abi.PCM_OK
 
