@@ -65,24 +65,65 @@ def measured_traffic(tag: str):
     return None
 
 
-def time_cpu_baseline(seconds: float, threads: int, n_agents: int):
-    """Oracle in OpenMP mode (reference RNG) on the host: bounded sample of cfg2 steps."""
-    from oracle.pyoracle import OracleSim
-    from paraphysicell_amd import synth
+def host_cpu_info() -> dict:
+    """The host's CPU counts: the CPUs this process may run on, the OS count, and physical
+    cores and sockets as lscpu reports them."""
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        import subprocess
 
-    mesh, st, params = synth.config2(4, n=n_agents)
-    ora = OracleSim(mesh, 4, st.defs, st.subs, capacity=st.count, seed=2, threads=threads, rng="mt")
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        pairs = {tuple(line.split(",")[:2]) for line in out.splitlines() if line and not line.startswith("#")}
+        info["physical_cores"] = len(pairs)
+        info["sockets"] = len({p[1] for p in pairs})
+        model = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for line in model.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except (OSError, ValueError, Exception):  # lscpu missing or unparsable: counts above stand
+        pass
+    return info
+
+
+def cpu_step_times(make, threads: int, warmup: int, steps: int, seconds: float):
+    """Oracle in OpenMP mode (the reference's mt19937 draws) with `threads` threads: `warmup`
+    untimed steps, then up to `steps` steps timed one by one (stopping early after `seconds`).
+    Returns (median per-step ms, mean agents over the timed steps, timed steps)."""
+    import numpy as np
+
+    from oracle.pyoracle import OracleSim
+
+    mesh, st, params = make()
+    ora = OracleSim(mesh, st.real_bytes, st.defs, st.subs, capacity=st.count, seed=2, threads=threads, rng="mt")
     ora.upload(st)
     ora.initialize_mesh()
-    ora.time_steps(params, 2, 0)  # warm-up: 2 steps
-    steps, ms, agent_steps = 0, 0.0, 0.0
-    while ms < seconds * 1000.0 and steps < 600:
+    ora.time_steps(params, warmup, 0)
+    per_step, agents, total = [], [], 0.0
+    while len(per_step) < steps and total < seconds * 1000.0:
         n0 = ora.count
-        chunk = 5
-        ms += ora.time_steps(params, 0, chunk)
-        agent_steps += 0.5 * (n0 + ora.count) * chunk
-        steps += chunk
-    return agent_steps / (ms / 1000.0), steps, ms
+        ms = ora.time_steps(params, 0, 1)
+        per_step.append(ms)
+        agents.append(0.5 * (n0 + ora.count))
+        total += ms
+    ora.close()
+    return float(np.median(per_step)), float(np.mean(agents)), len(per_step)
+
+
+def cpu_baseline_leg(make, threads: int, steps: int, seconds: float, one_thread_steps: int):
+    """BASELINE protocol on the host: 10 warm-up steps, then the median per-step time over
+    `steps` steps with `threads` threads, and a 1-thread figure over fewer steps."""
+    med, agents, done = cpu_step_times(make, threads, 10, steps, seconds)
+    med1, agents1, done1 = cpu_step_times(make, 1, 10, one_thread_steps, seconds)
+    return {
+        "value": round(agents / (med / 1000.0), 1),
+        "unit": "agent-steps/s",
+        "cores": threads,
+        "median_step_ms": round(med, 3),
+        "timed_steps": done,
+        "one_thread_value": round(agents1 / (med1 / 1000.0), 1),
+        "one_thread_median_step_ms": round(med1, 3),
+        "one_thread_timed_steps": done1,
+    }
 
 
 def copy_calibration(mib: int) -> None:
@@ -240,15 +281,74 @@ def serving_leg(count: int, rate: float, steps: int):
     }
 
 
+ALG_BYTES = {  # algorithmic bytes per agent-step of the whole step (see kernel_bytes for one kernel)
+    "disc_900_2d_f32": 120.0,
+    "ball_14k_3d_f32": 148.0,
+    "ball_14k_3d_f64": 272.0,
+    "ball_88k_3d_f32": 252.0,
+}
+
+
+def rocprof_average_us(kernel_prefix: str):
+    """Average duration of the kernel whose name starts with `kernel_prefix` in the committed
+    rocprofv3 --stats summary of the bench command, or None."""
+    path = os.path.join(ROOT, "profiles", "mechanics_step_88k_3d_f32_kernel_stats.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cols) >= 6 and cols[0].strip("`").startswith(kernel_prefix):
+                    return float(cols[2])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
+def config_leg(name: str, make, steps: int, warmup: int):
+    """One workload on its own handle: `warmup` untimed steps, then `steps` timed steps;
+    throughput and the whole-step roofline fraction at its algorithmic bytes per agent-step."""
+    from paraphysicell_amd.mechanics import DeviceSim
+
+    mesh, st, params = make()
+    dev = DeviceSim(mesh, st.real_bytes, st.defs, st.subs, capacity=st.count, seed=2)
+    dev.upload(st)
+    dev.initialize_mesh()
+    dev.step(params, warmup)
+    dev.synchronize()
+    n0 = dev.count
+    t0 = time.perf_counter()
+    dev.step(params, steps)
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+    n1 = dev.count
+    dev.close()
+    value = 0.5 * (n0 + n1) * steps / elapsed
+    b = ALG_BYTES[name]
+    return {
+        "agents": st.count,
+        "dims": mesh.dims,
+        "real_t": "float32" if st.real_bytes == 4 else "float64",
+        "steps": steps,
+        "value": round(value, 1),
+        "unit": "agent-steps/s",
+        "ms_per_step": round(elapsed * 1000.0 / steps, 4),
+        "alg_bytes_per_agent_step": b,
+        "step_roofline_frac": round(value * b / (HBM_PEAK_GBS * 1e9), 5),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600, help="timed mechanics steps (600 = 60 simulated minutes)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--agents", type=int, default=88700)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS or all host cores")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length per thread count")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: OMP_NUM_THREADS if set (the host's CPU share for this GPU), else every CPU this "
+                         "process may run on")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 900-agent 2D and 14.3k-agent 3D legs")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-replicate ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=600)
     ap.add_argument("--no-serving", action="store_true", help="skip the served-requests leg")
@@ -269,10 +369,9 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     from paraphysicell_amd import synth
-    from paraphysicell_amd.mechanics import DeviceSim
+    from paraphysicell_amd.mechanics import DeviceSim, kernel_names
 
     mesh, st, params = synth.config2(4, n=args.agents)
     dev = DeviceSim(mesh, 4, st.defs, st.subs, capacity=st.count, seed=2)
@@ -280,26 +379,26 @@ def main():
     dev.initialize_mesh()
 
     # warm-up steps: the autotuner picks the launch shape on real steps (1 + 6 * 2 of
-    # them), then the kernel profile of the last few steps picks the dominant kernel
-    tuned = dev.autotune(params, 2) if args.warmup >= 13 else dev.tuning
-    dev.step(params, max(args.warmup - (13 if args.warmup >= 13 else 0) - 5, 0))
-    from paraphysicell_amd.mechanics import kernel_names
-
+    # them) when there are enough of them, then the kernel profile of the last few steps
+    # picks the dominant kernel
+    autotuned = args.warmup >= 13
+    tuned = dev.autotune(params, 2) if autotuned else dev.tuning
+    dev.step(params, max(args.warmup - (13 if autotuned else 0) - 5, 0))
     names = kernel_names()
     dev.profile_begin(names)
     dev.step(params, min(5, args.warmup))
     warm_prof = dev.profile_end()
-    dominant = max(warm_prof, key=lambda k: warm_prof[k][0] / max(warm_prof[k][1], 1) * warm_prof[k][1])
+    dominant = max(warm_prof, key=lambda k: warm_prof[k][0])
 
     n_before = dev.count
-    torch.cuda.synchronize()
-    # the dominant kernel is timed with HIP events over the timed region, one launch in ten:
-    # each event pair stalls the stream for a few microseconds
-    dev.profile_every(10)
+    dev.synchronize()
+    # the dominant kernel is timed with HIP events on the library's stream over the timed
+    # region, every second launch: each event pair stalls the stream for a few microseconds
+    dev.profile_every(2)
     dev.profile_begin([dominant])
     t0 = time.perf_counter()
     dev.step(params, args.steps)
-    torch.cuda.synchronize()
+    dev.synchronize()
     t1 = time.perf_counter()
     prof = dev.profile_end()
     n_after = dev.count
@@ -316,8 +415,12 @@ def main():
     ms_total, launches = prof[dominant]
     avg_ms = ms_total / launches
     bytes_per_agent = kernel_bytes(dominant, 3, 4, st.defs, k_mean, s_mean)
-    bytes_per_launch = bytes_per_agent * 0.5 * (n_before + n_after)
+    agents_mean = 0.5 * (n_before + n_after)
+    bytes_per_launch = bytes_per_agent * agents_mean
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_agent else None
+    prefix = {"NEIGHBORS": "k_neighbors<float, 3, %d>" % tuned[2]}.get(dominant)
+    rp_avg = rocprof_average_us(prefix) if prefix else None
+    step_bytes = ALG_BYTES["ball_88k_3d_f32"]
 
     result = {
         "metric": METRIC,
@@ -343,7 +446,8 @@ def main():
             "mean_springs": round(s_mean, 2),
             "last_step_counters": dev.stats(),
             "launch_shape": {"velocity_lanes": tuned[0], "event_lanes": tuned[1], "neighbor_waves": tuned[2],
-                             "picked_by": "autotuner on warm-up steps"},
+                             "picked_by": "autotuner on warm-up steps" if autotuned
+                             else "library default (fewer than 13 warm-up steps: no tuning)"},
         },
         "roofline": {
             "kernel": dominant,
@@ -356,9 +460,17 @@ def main():
             "traffic_source": "profiles/mechanics_step_88k_3d_f32_hbm_traffic.md (rocprofv3 FETCH_SIZE + "
                               "WRITE_SIZE per launch, copy-calibrated)",
             "bytes_per_agent": round(bytes_per_agent, 1),
+            "bytes_per_launch": round(bytes_per_launch, 1),
+            "bytes_formula": "kernel_bytes() in bench.py at the end-of-run mean neighbors and springs per agent",
             "avg_launch_us": round(avg_ms * 1000.0, 2),
             "timed_launches": launches,
-            "timing": "HIP events on the library stream around one launch in 10 over the timed region",
+            "timing": "HIP events on the library stream around every second launch of the timed region",
+            "rocprof_avg_launch_us": rp_avg,
+            "rocprof_source": "profiles/mechanics_step_88k_3d_f32_kernel_stats.md (same command under rocprofv3)",
+            "frac_at_rocprof_avg": round(bytes_per_launch / (rp_avg * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+            if rp_avg else None,
+            "step_alg_bytes_per_agent": step_bytes,
+            "step_frac": round(value * step_bytes / (HBM_PEAK_GBS * 1e9), 5),
         },
         "kernel_share_warmup": {k: round(v[0] / max(sum(x[0] for x in warm_prof.values()), 1e-9), 3)
                                 for k, v in sorted(warm_prof.items(), key=lambda kv: -kv[1][0])[:8]},
@@ -366,23 +478,41 @@ def main():
 
     if not args.no_pcie:
         result["pcie_inclusive"] = pcie_leg(dev, params, 20)
-    if not args.no_cpu:
-        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        threads = min(threads, 16)
-        cpu_value, cpu_steps, cpu_ms = time_cpu_baseline(args.cpu_seconds, threads, args.agents)
-        result["cpu_baseline"] = {
-            "value": round(cpu_value, 1),
-            "unit": "agent-steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{cpu_steps} mechanics steps of the same workload (after 2 warm-up steps), OpenMP "
-                      f"oracle with the reference's mt19937 draws, {cpu_ms / 1000.0:.1f} s",
-            "gpu_over_cpu": round(value / cpu_value, 1),
+    dev.close()
+    if not args.no_configs:
+        result["configs"] = {
+            "disc_900_2d_f32": config_leg("disc_900_2d_f32", lambda: synth.config0(4), 600, 10),
+            "ball_14k_3d_f32": config_leg("ball_14k_3d_f32", lambda: synth.config1(4), 600, 10),
+            "ball_14k_3d_f64": config_leg("ball_14k_3d_f64", lambda: synth.config1(8), 600, 10),
         }
     if not args.no_ensemble:
         result["ensemble"] = ensemble_leg(args.ensemble_steps)
     if not args.no_serving:
         result["serving"] = serving_leg(args.serving_requests, 20.0, 100)
+    if not args.no_cpu:
+        # the host's CPU share for this GPU (OMP_NUM_THREADS, set by the GPU host's
+        # environment), else every CPU this process may run on; OpenMP places one thread
+        # per core, as BASELINE's protocol asks, before the oracle's runtime starts
+        env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        info = host_cpu_info()
+        threads = args.cpu_threads or env_threads or info["affinity_cpus"]
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        os.environ.setdefault("OMP_PLACES", "cores")
+        cpu = cpu_baseline_leg(lambda: synth.config2(4, n=args.agents), threads, 100, args.cpu_seconds, 20)
+        cpu.update({
+            "kind": "port",
+            "sample": f"the same workload from its initial state: 10 warm-up steps, then the median of "
+                      f"{cpu['timed_steps']} single-step timings with {threads} threads and of "
+                      f"{cpu['one_thread_timed_steps']} with 1 thread; the OpenMP oracle with the reference's "
+                      "mt19937 draws, built -O3 -fopenmp",
+            "host": info,
+            "threads_source": "--cpu-threads" if args.cpu_threads else (
+                "OMP_NUM_THREADS (the host's CPU share for one GPU)" if env_threads else "sched_getaffinity"),
+            "gpu_over_cpu": round(value / cpu["value"], 1),
+            "gpu_over_one_thread": round(value / cpu["one_thread_value"], 1),
+            "target_30x_met": value / cpu["value"] >= 30.0,
+        })
+        result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
     del np
 
