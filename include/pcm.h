@@ -126,6 +126,8 @@ enum pcm_field
  * every attached cell; the device knows the one model the reference ships. */
 #define PCM_CONTACT_NONE 0             /* contact_function == nullptr */
 #define PCM_CONTACT_STANDARD_ELASTIC 1 /* standard_elastic_contact_function (standard_models.cpp:643-654) */
+#define PCM_CONTACT_OTHER 255          /* any other function: pcm_set_field refuses it (PCM_ERR_UNSUPPORTED),
+										  so such a model keeps its host solver */
 
 /* PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION codes: the direction update that
  * update_motility_single (position_solver.cpp:132-136) calls after a new random walk, as
@@ -230,6 +232,17 @@ int pcm_set_gradients(pcm_sim* sim, const int32_t diffusion_voxel[3], const void
  * pcm_update_cell_forces and by every pcm_step iteration) */
 int pcm_set_step_index(pcm_sim* sim, uint64_t step);
 int pcm_get_step_index(pcm_sim* sim, uint64_t* step);
+
+/* ---------------------------------------------------------------- mechanics-mesh lookups (host) */
+/* common_solver::get_mesh_position (src/solver/host/common_solver.cpp:18-38): the mechanics
+ * voxel coordinates of a position of mesh->dims reals (real_bytes wide, host memory), by
+ * the convention the device bins with: (int)((p - bbox_min) / voxel) in real_t, clamped to
+ * the grid; coordinates at and beyond dims are 0.  Needs no GPU. */
+int pcm_get_mesh_position(const pcm_mesh_desc* mesh, int32_t real_bytes, const void* position, int32_t voxel[3]);
+/* common_solver::get_mesh_index (common_solver.cpp:10-16): the linear index
+ * x + gx * (y + gy * z) of voxel coordinates, the index of cells_in_mechanics_voxels and of
+ * pcm_get_voxel_buckets.  Needs no GPU. */
+int pcm_get_mesh_index(const pcm_mesh_desc* mesh, const int32_t voxel[3], int32_t* index);
 
 /* ---------------------------------------------------------------- reference phases */
 /* containers_solver::update_mechanics_mesh (src/solver/host/containers_solver.cpp:15-56) */

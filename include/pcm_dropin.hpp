@@ -27,17 +27,25 @@
 //   void  pcm_contact_functions(Env& e, std::vector<uint8_t>& codes);        // PCM_CONTACT_* per agent
 //   void  pcm_bias_direction_functions(Env& e, std::vector<uint8_t>& codes); // PCM_BIAS_* per agent
 //   void  pcm_gradients(Env& e, int32_t diffusion_voxel[3], std::vector<real_t>& gradients);
-// Without them every agent uploads PCM_CONTACT_NONE / PCM_BIAS_NONE.  pcm_gradients fills
+// Without them every agent uploads PCM_CONTACT_NONE / PCM_BIAS_NONE.  A contact function
+// other than standard_elastic_contact_function is reported as PCM_CONTACT_OTHER, which the
+// device refuses (the upload throws), so such a model keeps the host solver.
+//
+// The solver's free functions that host code calls on host data keep their names here:
+// remove_attached, update_geometry and common_solver::get_mesh_index / get_mesh_position.  pcm_gradients fills
 // the layout of pcm_set_gradients; pcm_dropin::gradients_modified(e) after each gradient
 // solve (simulator.cpp:163) makes the next device phase upload them.
 #pragma once
 
+#include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <numbers>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -733,6 +741,89 @@ public:
 			const auto p = params_of(e);
 			check(pcm_step(s, &p, steps), "pcm_step");
 		});
+	}
+};
+
+// ---------------------------------------------------------------- free functions of the solver
+// Host code outside the solver calls these on host data (the agent data must be on the host:
+// sync_to_host(e) before, host_modified(e) after a write).
+
+// remove_attached (containers_solver.h:29; containers_solver.cpp:58-69), called by
+// cell::divide and cell removal (cell.cpp:111-112, 233): every partner listed in
+// attached[to_remove] loses one entry naming to_remove, replaced by its list's last entry.
+// The lists must be symmetric (as cell::attach_cells makes them); a partner whose list does
+// not name to_remove is left unchanged.
+template <class Index>
+void remove_attached(Index to_remove, std::vector<Index>* __restrict__ attached)
+{
+	for (const Index partner : attached[to_remove])
+	{
+		std::vector<Index>& list = attached[partner];
+		for (std::size_t k = 0; k < list.size(); k++)
+			if (list[k] == to_remove)
+			{
+				list[k] = list.back();
+				list.pop_back();
+				break;
+			}
+	}
+}
+
+// update_geometry (interactions_solver.h:17-20; interactions_solver.cpp:89-96), called by
+// the phenotype's volume update (data_storage_phenotype.cpp:113-118): sphere radii of the
+// total and nuclear volumes and the surface area, computed in double (the reference's
+// literals are double) and stored in real_t, as the device's ingestion and fusion do.
+template <class Index, class Real>
+void update_geometry(Index i, Real* __restrict__ radius, Real* __restrict__ nuclear_radius,
+					 Real* __restrict__ surface_area, const Real* __restrict__ total_volume,
+					 const Real* __restrict__ nuclear_volume)
+{
+	const double sphere = (double)std::numbers::pi_v<Real> * 4.0 / 3.0;
+	radius[i] = (Real)std::cbrt((double)total_volume[i] / sphere);
+	nuclear_radius[i] = (Real)std::cbrt((double)nuclear_volume[i] / sphere);
+	surface_area[i] = (Real)(4.0 * (double)std::numbers::pi_v<Real> * (double)radius[i] * (double)radius[i]);
+}
+
+// common_solver's public lookups (common_solver.h:15-19), used by cell.cpp:261-267 to find an
+// agent's bucket in cells_in_mechanics_voxels.  `Mesh` is the mechanics mesh type (dims,
+// bounding_box_mins/maxs, voxel_shape); the voxel of a position follows the device's
+// binning (pcm_get_mesh_position), so host lookups agree with the device-built buckets.
+struct common_solver
+{
+	template <class Mesh>
+	static pcm_mesh_desc mesh_desc(const Mesh& mesh)
+	{
+		pcm_mesh_desc d {};
+		d.dims = (int32_t)mesh.dims;
+		for (int k = 0; k < 3; k++)
+		{
+			d.bbox_min[k] = (int32_t)mesh.bounding_box_mins[k];
+			d.bbox_max[k] = (int32_t)mesh.bounding_box_maxs[k];
+			d.voxel[k] = (int32_t)mesh.voxel_shape[k];
+		}
+		return d;
+	}
+
+	// common_solver::get_mesh_index (common_solver.cpp:10-16): x + gx * (y + gy * z)
+	template <class Point, class Mesh>
+	static int32_t get_mesh_index(const Point& position, const Mesh& mesh)
+	{
+		const pcm_mesh_desc d = mesh_desc(mesh);
+		const int32_t c[3] = { (int32_t)position[0], (int32_t)position[1], (int32_t)position[2] };
+		int32_t index = 0;
+		check(pcm_get_mesh_index(&d, c, &index), "pcm_get_mesh_index");
+		return index;
+	}
+
+	// common_solver::get_mesh_position (common_solver.cpp:18-38): the voxel coordinates of a
+	// position of mesh.dims reals; coordinates at and beyond dims are 0
+	template <class Real, class Mesh>
+	static std::array<int32_t, 3> get_mesh_position(const Real* position, const Mesh& mesh)
+	{
+		const pcm_mesh_desc d = mesh_desc(mesh);
+		std::array<int32_t, 3> c {};
+		check(pcm_get_mesh_position(&d, (int32_t)sizeof(Real), position, c.data()), "pcm_get_mesh_position");
+		return c;
 	}
 };
 

@@ -336,6 +336,13 @@ public:
 				if (((const int32_t*)src)[i] < 0 || ((const int32_t*)src)[i] >= std::max(h_.S, 1))
 					return fail(PCM_ERR_ARGUMENT, "chemotaxis_index out of range");
 		}
+		if (f == PCM_F_CELL_DEFINITION_INDEX)
+		{
+			// the per-type rows (affinities, rates, immunogenicities) are read at this index
+			for (int i = 0; i < n; i++)
+				if (((const int32_t*)src)[i] < 0 || ((const int32_t*)src)[i] >= std::max(h_.T, 1))
+					return fail(PCM_ERR_ARGUMENT, "cell_definition_index outside [0, cell_defs)");
+		}
 		if (f == PCM_F_MIGRATION_BIAS_DIRECTION_FUNCTION)
 		{
 			const uint8_t* code = (const uint8_t*)src;
@@ -1930,6 +1937,36 @@ private:
 
 pcm_sim* checked(pcm_sim* s) { return s; }
 
+// the mechanics grid of a mesh description, as pcm_create sizes it (ceil of extent / voxel)
+int grid_of(const pcm_mesh_desc& m, int grid[3])
+{
+	long long cells = 1;
+	for (int k = 0; k < 3; k++)
+	{
+		if (k < m.dims)
+		{
+			if (m.voxel[k] <= 0 || m.bbox_max[k] < m.bbox_min[k])
+				return fail(PCM_ERR_ARGUMENT, "bad mesh box or voxel shape");
+			grid[k] = std::max(1, (m.bbox_max[k] - m.bbox_min[k] + m.voxel[k] - 1) / m.voxel[k]);
+		}
+		else
+			grid[k] = 1;
+		cells *= grid[k];
+	}
+	if (cells > 0x7fffffffLL)
+		return fail(PCM_ERR_CAPACITY, "mechanics voxel grid larger than 2^31 voxels");
+	return PCM_OK;
+}
+
+// voxel_index of pcm_device.h for one coordinate, on the host: the same real_t arithmetic
+template <class R>
+int host_voxel_coord(R p, const pcm_mesh_desc& m, const int grid[3], int k)
+{
+	const R q = (p - (R)m.bbox_min[k]) / (R)m.voxel[k];
+	const int v = (int)q;
+	return v < 0 ? 0 : (v >= grid[k] ? grid[k] - 1 : v);
+}
+
 // ---------------------------------------------------------------------------- allocation-free neighbor build
 // The mesh scratch of pcm_neighbors_noalloc: one fixed area per GPU, held until the process
 // ends.  In host code `scratch` names a host-side stand-in, not device memory: the device
@@ -2468,6 +2505,33 @@ int pcm_profile_end(pcm_sim* sim, double* total_ms, int64_t* launches)
 {
 	PCM_NEED(sim);
 	return sim->profile_end(total_ms, launches);
+}
+
+int pcm_get_mesh_position(const pcm_mesh_desc* mesh, int32_t real_bytes, const void* position, int32_t voxel[3])
+{
+	if (!mesh || !position || !voxel || mesh->dims < 1 || mesh->dims > 3 || (real_bytes != 4 && real_bytes != 8))
+		return fail(PCM_ERR_ARGUMENT, "pcm_get_mesh_position: null pointer, dims or real_bytes out of range");
+	int grid[3];
+	PCM_TRY(grid_of(*mesh, grid));
+	for (int k = 0; k < 3; k++)
+		voxel[k] = k < mesh->dims
+					   ? (real_bytes == 4 ? host_voxel_coord<float>(((const float*)position)[k], *mesh, grid, k)
+										  : host_voxel_coord<double>(((const double*)position)[k], *mesh, grid, k))
+					   : 0;
+	return PCM_OK;
+}
+
+int pcm_get_mesh_index(const pcm_mesh_desc* mesh, const int32_t voxel[3], int32_t* index)
+{
+	if (!mesh || !voxel || !index || mesh->dims < 1 || mesh->dims > 3)
+		return fail(PCM_ERR_ARGUMENT, "pcm_get_mesh_index: null pointer or dims out of range");
+	int grid[3];
+	PCM_TRY(grid_of(*mesh, grid));
+	for (int k = 0; k < 3; k++)
+		if (voxel[k] < 0 || voxel[k] >= grid[k])
+			return fail(PCM_ERR_ARGUMENT, "pcm_get_mesh_index: voxel coordinates outside the grid");
+	*index = voxel[0] + grid[0] * (voxel[1] + grid[1] * voxel[2]);
+	return PCM_OK;
 }
 
 int pcm_neighbors_noalloc(const pcm_mesh_desc* mesh, int32_t real_bytes, int32_t count, const void* positions,

@@ -556,8 +556,158 @@ void formats(int dims, int n)
 	pcm_dropin::release(dev);
 }
 
+// The solver's free functions through the drop-in, on the host lists and fields that host
+// code hands them: remove_attached as cell::divide and cell removal call it
+// (cell.cpp:111-112, 233) against the oracle's restatement, then device steps from the
+// edited state; update_geometry against the oracle's; and the bucket lookup of
+// cell.cpp:261-267 (get_mesh_position + get_mesh_index) into the device-built buckets.
+template <class R>
+void free_functions(int dims, int n)
+{
+	const double tol = sizeof(R) == 4 ? 1e-5 : 1e-12;
+	pcm_mesh_desc md {};
+	md.dims = dims;
+	for (int k = 0; k < 3; k++)
+	{
+		md.bbox_min[k] = -150;
+		md.bbox_max[k] = 150;
+		md.voxel[k] = 30;
+	}
+	environment<R> ref(md, 2, 1), dev(md, 2, 1);
+	fill_population(ref, n, 7 + dims);
+	fill_population(dev, n, 7 + dims);
+	for (auto* e : { &ref, &dev })
+	{
+		e->mechanics_time_step = (R)0.1;
+		e->virtual_wall_at_domain_edges = true;
+		e->automated_spring_adhesion = true;
+	}
+	ref.rng.mode = rng_mode::keyed;
+	ref.rng.seed = 5;
+	pcm_dropin::set_seed(dev, 5);
+	solver<R> oracle;
+	oracle.initialize(ref);
+	oracle.update_mechanics_mesh(ref);
+	oracle.update_cell_neighbors(ref);
+	pcm_dropin::solver<environment<R>> s;
+	s.initialize(dev);
+	s.containers.update_mechanics_mesh(dev);
+	s.position.update_cell_neighbors(dev);
+	// a few steps, so that springs form
+	for (int step = 0; step < 3; step++)
+	{
+		oracle.step(ref);
+		s.simulate_mechanics(dev, 1);
+	}
+	pcm_dropin::sync_to_host(dev);
+	auto& a = dev.data;
+	auto& b = ref.data;
+	EXPECT(a.agents_count == b.agents_count, "free functions dims %d: count", dims);
+	if (a.agents_count != b.agents_count)
+	{
+		pcm_dropin::release(dev);
+		return;
+	}
+	const int nn = a.agents_count;
+
+	// the bucket lookup of cell.cpp:261-267, on the device's buckets and against the oracle's
+	bool found_all = true, same_voxel = true;
+	for (int i = 0; i < nn; i++)
+	{
+		const R* p = a.agent_data.positions.data() + (size_t)i * dims;
+		const auto c = pcm_dropin::common_solver::get_mesh_position(p, dev.mechanics_mesh);
+		const int32_t v = pcm_dropin::common_solver::get_mesh_index(c, dev.mechanics_mesh);
+		index_t co[3];
+		solver<R>::mesh_position(p, dev.mechanics_mesh, co);
+		same_voxel = same_voxel && co[0] == c[0] && co[1] == c[1] && co[2] == c[2]
+					 && solver<R>::mesh_index(co, dev.mechanics_mesh) == v;
+		const auto& bucket = dev.cells_in_mechanics_voxels[v];
+		found_all = found_all && std::find(bucket.begin(), bucket.end(), i) != bucket.end();
+	}
+	EXPECT(same_voxel, "dims %d: get_mesh_position / get_mesh_index differ from the oracle's", dims);
+	EXPECT(found_all, "dims %d: an agent is missing from the bucket its position looks up", dims);
+
+	// remove_attached on both lists of every fifth agent with springs, as cell::divide detaches
+	// a cell; the oracle's restatement edits the oracle's copy
+	int removed = 0;
+	for (int i = 0; i < nn; i += 5)
+	{
+		if (b.states.springs[i].empty())
+			continue;
+		pcm_dropin::remove_attached((index_t)i, a.states.springs.data());
+		pcm_dropin::remove_attached((index_t)i, a.states.attached_cells.data());
+		a.states.springs[i].clear();
+		a.states.attached_cells[i].clear();
+		solver<R>::strip_links((index_t)i, b.states.springs);
+		solver<R>::strip_links((index_t)i, b.states.attached_cells);
+		b.states.springs[i].clear();
+		b.states.attached_cells[i].clear();
+		removed++;
+	}
+	EXPECT(removed > 0, "dims %d: no agent with springs", dims);
+	bool lists_equal = true;
+	for (int i = 0; i < nn; i++)
+		lists_equal = lists_equal && a.states.springs[i] == b.states.springs[i]
+					  && a.states.attached_cells[i] == b.states.attached_cells[i];
+	EXPECT(lists_equal, "dims %d: remove_attached differs from the oracle's", dims);
+
+	// update_geometry on new volumes (as the phenotype's volume update calls it)
+	std::mt19937 g(11 + dims);
+	std::uniform_real_distribution<double> u(500.0, 5000.0);
+	bool geometry_equal = true;
+	for (int i = 0; i < nn; i++)
+	{
+		const R total = (R)u(g), nuclear = (R)(0.2 * u(g));
+		a.agent_data.volumes[i] = b.agent_data.volumes[i] = total;
+		a.volumes.nuclear[i] = b.volumes.nuclear[i] = nuclear;
+		pcm_dropin::update_geometry((index_t)i, a.geometries.radius.data(), a.geometries.nuclear_radius.data(),
+									a.geometries.surface_area.data(), a.agent_data.volumes.data(), a.volumes.nuclear.data());
+		pcm_oracle::update_geometry((index_t)i, b.geometries.radius.data(), b.geometries.nuclear_radius.data(),
+									b.geometries.surface_area.data(), b.agent_data.volumes.data(), b.volumes.nuclear.data());
+		geometry_equal = geometry_equal && a.geometries.radius[i] == b.geometries.radius[i]
+						 && a.geometries.nuclear_radius[i] == b.geometries.nuclear_radius[i]
+						 && a.geometries.surface_area[i] == b.geometries.surface_area[i];
+	}
+	EXPECT(geometry_equal, "dims %d: update_geometry differs from the oracle's", dims);
+
+	// continue on the device from the oracle's edited state: one step each, compared
+	std::vector<const void*> src(PCM_FIELD_COUNT, nullptr);
+	std::vector<size_t> bytes(PCM_FIELD_COUNT, 0);
+	for_each_field(b, dims, ref.cell_definitions_count, ref.m.substrates_count, [&](int id, auto& v, index_t) {
+		src[id] = v.data();
+		bytes[id] = v.size() * sizeof(v[0]);
+	});
+	for_each_field(a, dims, ref.cell_definitions_count, ref.m.substrates_count, [&](int id, auto& v, index_t) {
+		std::memcpy((void*)v.data(), src[id], std::min(bytes[id], v.size() * sizeof(v[0])));
+	});
+	pcm_dropin::host_modified(dev);
+	oracle.step(ref);
+	s.simulate_mechanics(dev, 1);
+	pcm_dropin::sync_to_host(dev);
+	EXPECT(a.agents_count == b.agents_count, "dims %d: count after the edit", dims);
+	if (a.agents_count == b.agents_count)
+	{
+		const size_t m = (size_t)b.agents_count;
+		EXPECT(rel_err(a.agent_data.positions, b.agent_data.positions, m * dims) <= tol,
+			   "dims %d: positions after the edit", dims);
+		EXPECT(rel_err(a.geometries.radius, b.geometries.radius, m) <= tol, "dims %d: radii after the edit", dims);
+		bool after = true;
+		for (size_t i = 0; i < m; i++)
+			after = after && a.states.springs[i] == b.states.springs[i] && a.states.neighbors[i] == b.states.neighbors[i];
+		EXPECT(after, "dims %d: lists after the edit", dims);
+	}
+	std::printf("free functions dims %d %s: %d agents, %d detached\n", dims, sizeof(R) == 4 ? "FP32" : "FP64", nn,
+				removed);
+	pcm_dropin::release(dev);
+}
+
 int main()
 {
+	for (int dims = 1; dims <= 3; dims++)
+	{
+		free_functions<float>(dims, dims == 3 ? 1000 : 400);
+		free_functions<double>(dims, dims == 3 ? 1000 : 400);
+	}
 	for (int dims = 1; dims <= 3; dims++)
 	{
 		velocity_simple<float>(dims);

@@ -119,3 +119,57 @@ def test_noalloc_refuses_bad_sizes_before_touching_the_device(lib):
     with pytest.raises(mechanics.PcmError) as e:
         mechanics.neighbors_noalloc(small, 8, 5_000_000, fake, fake, fake, None, 1, fake, fake, fake, lib=lib)
     assert e.value.code == abi.PCM_ERR_CAPACITY
+
+
+def test_mesh_lookups_follow_the_device_binning(lib):
+    """pcm_get_mesh_position / pcm_get_mesh_index (common_solver::get_mesh_position /
+    get_mesh_index) need no GPU: voxel coordinates by (int)((p - min) / voxel) in real_t,
+    clamped to the grid (ceil of extent / voxel), linearised x fastest."""
+    import numpy as np
+
+    lib.pcm_get_mesh_position.argtypes = [ctypes.POINTER(abi.MeshDesc), ctypes.c_int32, ctypes.c_void_p,
+                                          ctypes.POINTER(ctypes.c_int32 * 3)]
+    lib.pcm_get_mesh_index.argtypes = [ctypes.POINTER(abi.MeshDesc), ctypes.POINTER(ctypes.c_int32 * 3),
+                                       ctypes.POINTER(ctypes.c_int32)]
+    rng = np.random.default_rng(3)
+    for dims in (1, 2, 3):
+        mesh = abi.MeshSpec(dims, (-100, -70, -40), (101, 80, 45), (30, 20, 15))
+        desc = mesh.to_c()
+        g = mesh.grid_shape()
+        for real in (np.float32, np.float64):
+            pts = rng.uniform(-130, 130, size=(200, dims)).astype(real)
+            pts[:3] = np.array([-100, -70, -40][:dims], real)  # the box minimum
+            pts[3:6] = np.array([101, 80, 45][:dims], real)  # the box maximum: clamped into the last voxel
+            for p in pts:
+                out = (ctypes.c_int32 * 3)()
+                a = np.ascontiguousarray(p)
+                assert lib.pcm_get_mesh_position(ctypes.byref(desc), a.itemsize, a.ctypes.data, ctypes.byref(out)) == 0
+                want = [0, 0, 0]
+                for k in range(dims):
+                    q = (p[k] - real(mesh.bbox_min[k])) / real(mesh.voxel[k])
+                    want[k] = int(min(max(int(q), 0), g[k] - 1))
+                assert list(out) == want, (dims, real, p)
+                idx = ctypes.c_int32()
+                assert lib.pcm_get_mesh_index(ctypes.byref(desc), ctypes.byref(out), ctypes.byref(idx)) == 0
+                assert idx.value == want[0] + g[0] * (want[1] + g[1] * want[2])
+        bad = (ctypes.c_int32 * 3)(g[0], 0, 0)
+        assert lib.pcm_get_mesh_index(ctypes.byref(desc), ctypes.byref(bad), ctypes.byref(ctypes.c_int32())) \
+            == abi.PCM_ERR_ARGUMENT
+
+
+@pytest.mark.gpu
+def test_cell_type_outside_the_definitions_is_refused():
+    """A cell_definition_index outside [0, cell_defs) would index the per-type rows out of
+    bounds on the device: pcm_set_field refuses it."""
+    import numpy as np
+
+    sim = mechanics.DeviceSim(abi.MeshSpec(3, (0, 0, 0), (90, 90, 90)), 4, 2, 1, capacity=8)
+    sim._call("set_count", sim._h, 4)
+    for bad in ([0, 1, 2, 0], [0, -1, 0, 0]):
+        a = np.array(bad, np.int32)
+        with pytest.raises(mechanics.PcmError) as e:
+            sim._call("set_field", sim._h, abi.FIELD_ID["cell_definition_index"], a.ctypes.data)
+        assert e.value.code == abi.PCM_ERR_ARGUMENT
+    ok = np.array([0, 1, 1, 0], np.int32)
+    sim._call("set_field", sim._h, abi.FIELD_ID["cell_definition_index"], ok.ctypes.data)
+    sim.close()
