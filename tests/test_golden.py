@@ -17,7 +17,7 @@ import pytest
 from paraphysicell_amd import abi
 from paraphysicell_amd.state import AgentState
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "rich_*.npz")))
 TOL = {4: 1e-5, 8: 1e-12}
 
 
