@@ -193,6 +193,10 @@ int pcm_get_count(pcm_sim* sim, int32_t* count);
  * agents).  Setting reads, and getting writes, exactly that many elements. */
 int pcm_set_field(pcm_sim* sim, int32_t field, const void* host);
 int pcm_get_field(pcm_sim* sim, int32_t field, void* host);
+/* pcm_set_field for `nfields` fields in one call (fields[k] read from hosts[k]): every
+ * field is checked before any is copied, the copies are queued together and the call waits
+ * once, so uploading a whole cell_data costs one round trip instead of one per field. */
+int pcm_set_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, const void* const* hosts);
 /* springs / attached_cells as CSR (offsets has count+1 entries) */
 int pcm_set_links(pcm_sim* sim, int32_t which, const int32_t* offsets, const int32_t* indices);
 int pcm_get_links_size(pcm_sim* sim, int32_t which, int64_t* total);
@@ -347,6 +351,10 @@ int pcm_batch_synchronize(pcm_batch* batch);
  * synchronous device calls (pcm_create, uploads, reads) meanwhile, since those invalidate
  * it: disable graphs for steps that overlap such work. */
 int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable);
+/* enable != 0 (the default): each kernel of a group's step is one launch over exactly the
+ * blocks its members need, back to back (a block map gives each block its member), instead
+ * of one block row per member sized for the largest; results are the same either way. */
+int pcm_batch_flatten(pcm_batch* batch, int32_t enable);
 /* pcm_autotune for every dimensionality group of the batch, on the batch's own steps (one
  * shape per group, used by its segmented launches); members advance 1 + 6 * steps. */
 int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t steps_per_candidate);

@@ -23,12 +23,35 @@ __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadId
 
 
 
+// Launch geometry.  A plain launch gives each simulation one grid row: slot blockIdx.y, its
+// blocks along x.  A batch's flattened launch (bmap set) has a single row holding the
+// blocks of all members back to back, so no block is spent on a member smaller than the
+// largest: entry b of bmap is block b's {member slot, block index within the member, the
+// member's block count, 0}, and {-1, ...} pads the grid (such a block returns at once).
+// A block entry is the same for every thread of the block, so the exit is block-uniform.
+#define PCM_GEOMETRY                                                                                                   \
+	int slot_ = (int)blockIdx.y, bid_ = (int)blockIdx.x, nblk_ = (int)gridDim.x;                                      \
+	if (bmap)                                                                                                          \
+	{                                                                                                                  \
+		const int4 e_ = bmap[blockIdx.x];                                                                              \
+		if (e_.x < 0)                                                                                                  \
+			return;                                                                                                    \
+		slot_ = e_.x;                                                                                                  \
+		bid_ = e_.y;                                                                                                   \
+		nblk_ = e_.z;                                                                                                  \
+	}                                                                                                                  \
+	(void)nblk_;
+
+// global thread index within the block's simulation
+#define GID (bid_ * (int)blockDim.x + (int)threadIdx.x)
+
 // Each wavefront reads the status word and returns on its own.  That is only safe where the
 // word cannot change during the launch, or where the block's wavefronts share nothing
 // (no LDS, no barrier): a kernel that can raise the status itself while its blocks are
 // still being dispatched uses PCM_PROLOGUE_BLOCK instead.
 #define PCM_PROLOGUE(c)                                                                                                \
-	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
+	PCM_GEOMETRY                                                                                                       \
+	const SimCtx<R>& c = ctxs[slot_];                                                                                 \
 	if (c.sc->status)                                                                                                  \
 		return;
 
@@ -39,7 +62,8 @@ __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadId
 // partials) and indexes global memory with it.  Thread 0 reads the word once and every
 // wavefront decides on that one value.
 #define PCM_PROLOGUE_BLOCK(c)                                                                                          \
-	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
+	PCM_GEOMETRY                                                                                                       \
+	const SimCtx<R>& c = ctxs[slot_];                                                                                 \
 	{                                                                                                                  \
 		__shared__ int s_status_;                                                                                      \
 		if (threadIdx.x == 0)                                                                                          \
@@ -56,16 +80,17 @@ __device__ __forceinline__ int gid() { return blockIdx.x * blockDim.x + threadId
 // needs.  After an overflow the rest of the step's kernels return, so a neighbor build of a
 // later step in the same call recomputes the same lists from the same mesh, in bounds.
 #define PCM_PROLOGUE_NEIGHBORS(c)                                                                                      \
-	const SimCtx<R>& c = ctxs[blockIdx.y];                                                                            \
+	PCM_GEOMETRY                                                                                                       \
+	const SimCtx<R>& c = ctxs[slot_];                                                                                 \
 	if (c.sc->status & ~ST_NEIGHBOR_OVERFLOW)                                                                          \
 		return;
 
 // ------------------------------------------------------------------ step bookkeeping
 template <class R>
-__global__ void k_begin_step(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_begin_step(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	if (gid() == 0)
+	if (GID == 0)
 		c.sc->step++;
 }
 
@@ -84,10 +109,10 @@ __global__ void k_gather_scalars(const SimCtx<R>* __restrict__ ctxs, Scalars* __
 // unordered scatter -> rank by "number of bucket members with a larger index".
 
 template <class R>
-__global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i < c.sc->n_binned)
 		c.vcnt[c.vox_of[i]] = 0;
 	if (i == 0)
@@ -96,10 +121,10 @@ __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs)
 
 // RESTORE: also return the container update's remap entries to the identity
 template <class R, int D, bool RESTORE>
-__global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	const int n = c.sc->n;
 	if (RESTORE)
 	{
@@ -123,10 +148,10 @@ __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs)
 // cache line serialise across the XCDs, so there are as few of them as possible.
 // RESET: also zero the container update's counters (their readers have finished)
 template <class R, bool RESET>
-__global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	const int n = c.sc->n;
 	if (RESET && i == 0)
 	{
@@ -134,7 +159,7 @@ __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __
 		c.sc->moves = 0;
 	}
 	// blocks past the agents skip the block-wide scans (block 0 always claims, for n = 0)
-	if (blockIdx.x > 0 && (int)blockIdx.x * CLAIM_THREADS >= n)
+	if (bid_ > 0 && (int)bid_ * CLAIM_THREADS >= n)
 		return;
 	int amount = 0, v = -1;
 	if (i < n && c.vrank[i] == 0)
@@ -159,10 +184,10 @@ __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __
 }
 
 template <class R>
-__global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n)
 		return;
 	if (i == 0)
@@ -177,10 +202,10 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs)
 // RECORDS: also write the agents' force records for the step's gather kernels (a mesh
 // update for the neighbor lists alone has no force-record buffer)
 template <class R, int D, bool RECORDS = true>
-__global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n)
 		return;
 	// the agent's own data first: its loads overlap the rank chain below
@@ -205,10 +230,10 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs)
 
 // force records of the current agent state, when the mesh update's copy is stale
 template <class R, int D>
-__global__ void k_force_records(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_force_records(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i < c.sc->n)
 		write_force_record<R, D>(c, i);
 }
@@ -350,7 +375,7 @@ __device__ __noinline__ void neighbors_one_agent_large_stencil(const SimCtx<R>& 
 
 template <class R, int D, int W>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 7 : 6))) k_neighbors(
-	const SimCtx<R>* __restrict__ ctxs)
+	const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE_NEIGHBORS(c)
 	constexpr int TILE = nbr_tile<R>();
@@ -363,7 +388,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 	const int wave = tid >> 6;
 	const int n_occ = c.sc->n_occ;
 	const unsigned long long below = (1ull << lane) - 1ull;
-	for (int w = blockIdx.x; w < n_occ; w += gridDim.x)
+	for (int w = bid_; w < n_occ; w += nblk_)
 	{
 		const int v = c.occ[w];
 		int co[3];
@@ -476,10 +501,10 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 }
 
 template <class R, int D>
-__global__ void k_neighbors_direct(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_neighbors_direct(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE_NEIGHBORS(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n)
 		return;
 	neighbors_one_agent<R, D, true>(c, i);
@@ -561,10 +586,10 @@ __device__ __forceinline__ R forces_agent(const SimCtx<R>& c, int i, const R* pi
 }
 
 template <class R, int D>
-__global__ void k_forces(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_forces(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int g = gid();
+	const int g = GID;
 	if (g >= c.sc->n)
 		return;
 	const int i = g;
@@ -583,10 +608,10 @@ __global__ void k_forces(const SimCtx<R>* __restrict__ ctxs)
 // evaluate_interactions (simulator.cpp:65-82): agents whose contact function is the standard
 // elastic one; there is no movable gating, as in the reference
 template <class R, int D>
-__global__ void k_contact(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_contact(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n || c.contact[i] != PCM_CONTACT_STANDARD_ELASTIC)
 		return;
 	R v[D];
@@ -598,10 +623,10 @@ __global__ void k_contact(const SimCtx<R>* __restrict__ ctxs)
 }
 
 template <class R, int D>
-__global__ void k_motility(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_motility(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n || c.motile[i] == 0)
 		return;
 	R v[D];
@@ -613,10 +638,10 @@ __global__ void k_motility(const SimCtx<R>* __restrict__ ctxs)
 }
 
 template <class R, int D>
-__global__ void k_membrane(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_membrane(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n || !c.movable[i])
 		return;
 	R p[D], v[D];
@@ -631,10 +656,10 @@ __global__ void k_membrane(const SimCtx<R>* __restrict__ ctxs)
 }
 
 template <class R, int D>
-__global__ void k_contract(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_contract(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n || !c.movable[i])
 		return;
 	R p[D], v[D];
@@ -650,10 +675,10 @@ __global__ void k_contract(const SimCtx<R>* __restrict__ ctxs)
 
 // update_positions_internal (position_solver.cpp:481-504): Adams-Bashforth 2
 template <class R, int D>
-__global__ void k_positions(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_positions(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n || !c.movable[i])
 		return;
 	const R factor = (R)((double)c.dt * 1.5);
@@ -778,11 +803,11 @@ __device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, in
 // CLEAR: also clear the voxel counts of the last mesh update (nothing reads them until
 // this step's mesh update)
 template <class R, int D, bool CLEAR, int G>
-__global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	constexpr int B = VELOCITY_ROUNDS;
-	const int t = gid();
+	const int t = GID;
 	if (CLEAR && t < c.sc->n_binned)
 		c.vcnt[c.vox_of[t]] = 0;
 	const int g = t / G;
@@ -984,10 +1009,10 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 // STARTS_STEP: the kernel opens a mechanics step, whose counter is one above the stored
 // one until k_spring_erase stores it (k_begin_step's work, without its launch)
 template <class R, bool STARTS_STEP>
-__global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i == 0)
 		c.sc->res_open = 0; // the resolver's open-actor list starts empty every update
 	if (i >= c.sc->n)
@@ -1009,10 +1034,10 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs)
 }
 
 template <class R, bool STARTS_STEP>
-__global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long t = (long long)bid_ * blockDim.x + threadIdx.x;
 	if (STARTS_STEP && t == 0)
 		c.sc->step++; // no thread of this kernel reads the counter
 	const int i = (int)(t / c.L), s = (int)(t % c.L);
@@ -1095,14 +1120,14 @@ __device__ __forceinline__ void spring_compact_agent(const SimCtx<R>& c, int i)
 // an unordered inbox; actors with events are listed for the resolver.  G lanes share an
 // agent: lane l draws for slots l, l+G, ...; a ballot ranks the events in slot order.
 template <class R, int G>
-__global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE_BLOCK(c)
 	// a block past the agents (a batch's grid covers its largest member) owns no segment
 	// and skips the block-wide scan below
-	if ((int)blockIdx.x * (BLOCK / G) >= c.sc->n)
+	if ((int)bid_ * (BLOCK / G) >= c.sc->n)
 		return;
-	const int t = gid();
+	const int t = GID;
 	const int g = t / G;
 	const int lane = t & (G - 1);
 	const int group_shift = (int)(threadIdx.x & 63) & ~(G - 1);
@@ -1164,10 +1189,10 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 	int total;
 	const int rank = block_exclusive_scan(lead ? 1 : 0, total);
 	if (lead)
-		c.att_list[blockIdx.x * APB + rank] = i;
+		c.att_list[bid_ * APB + rank] = i;
 	// only blocks that cover agents own a segment (a batch's grid covers its largest member)
-	if (threadIdx.x == 0 && (int)blockIdx.x * APB < c.sc->n)
-		c.blk_cnt[blockIdx.x] = total;
+	if (threadIdx.x == 0 && (int)bid_ * APB < c.sc->n)
+		c.blk_cnt[bid_] = total;
 }
 
 // One actor's undecided attachment events, in its event order, against the states decided
@@ -1236,13 +1261,11 @@ __device__ __forceinline__ bool resolve_actor(const SimCtx<R>& c, volatile uint8
 // First round: every actor at once, one block per collect block's actor segment; actors
 // left with undecided events go to att_c for the single-block rounds that follow.
 template <class R, int G>
-__global__ void __launch_bounds__(64) k_spring_resolve_first(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(64) k_spring_resolve_first(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
-	const SimCtx<R>& c = ctxs[blockIdx.y];
-	if (c.sc->status)
-		return;
+	PCM_PROLOGUE(c)
 	constexpr int APB = BLOCK / G;
-	const int b = blockIdx.x;
+	const int b = bid_;
 	const int n = c.sc->n;
 	if (b == 0 && threadIdx.x == 0)
 		c.sc->stat[7] = (n + APB - 1) / APB; // collect segments, for the event and actor counts
@@ -1267,8 +1290,9 @@ __global__ void __launch_bounds__(64) k_spring_resolve_first(const SimCtx<R>* __
 
 // Later rounds, one block, over the actors the first round left open.
 template <class R>
-__global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
+	(void)bmap; // one block per simulation: never flattened
 	const SimCtx<R>& c = ctxs[blockIdx.y];
 	__shared__ int s_next;
 	if (c.sc->status)
@@ -1318,10 +1342,10 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 // pushes them: events from smaller actors first (by actor), then the agent's own events in
 // neighbor order.
 template <class R>
-__global__ void k_spring_append(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_spring_append(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int a = gid();
+	const int a = GID;
 	if (a >= c.sc->n)
 		return;
 	const int nin = min(c.in_cnt[a], c.K);
@@ -1471,10 +1495,10 @@ __device__ __forceinline__ int evw_attacks(unsigned long long w) { return (int)(
 // slots l, l+G, ...; the rare successful draws are then walked in slot order by the whole
 // group (the per-actor state is group-uniform), reading the slot data through shuffles.
 template <class R, int D, bool AB2, int G>
-__global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int t = gid();
+	const int t = GID;
 	const int g = t / G;
 	const int lane = t & (G - 1);
 	const int group_shift = (int)(threadIdx.x & 63) & ~(G - 1);
@@ -1640,12 +1664,12 @@ __device__ __forceinline__ bool outside_box(const SimCtx<R>& c, const R* p)
 // it, and append it to the removal list (the speculation is exact here, so an agent with
 // a removal event targeting it is removed)
 template <class R, int D, bool FLAG>
-__global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	if (c.sc->conflict)
 		return;
-	const int i = gid();
+	const int i = GID;
 	if ((i & ~63) >= c.sc->n)
 		return; // the whole wavefront is past the agents: nothing to reserve
 	const bool live_thread = i < c.sc->n;
@@ -1727,8 +1751,9 @@ __device__ void compact_plan_block(const SimCtx<R>& c, int* sb);
 // PLAN: then build the removal list when the replay ran (the apply pass built it otherwise)
 // and plan the container update in the same block.
 template <class R, int D, bool PLAN>
-__global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
+	(void)bmap; // one block per simulation: never flattened
 	const SimCtx<R>& c = ctxs[blockIdx.y];
 	__shared__ unsigned long long key[CAND_CHUNK];
 	__shared__ int val[CAND_CHUNK];
@@ -1898,10 +1923,10 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCt
 // descending order.  Link lists of survivors are edited as the reference's sequence of
 // remove_attached (swap with back) and rename_attached calls would edit them.
 template <class R, int D>
-__global__ void k_compact_flag(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_compact_flag(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n)
 		return;
 	bool bad = c.flags[i] == PCM_FLAG_TO_REMOVE;
@@ -2100,8 +2125,9 @@ __device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
 }
 
 template <class R>
-__global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
+	(void)bmap; // one block per simulation: never flattened
 	const SimCtx<R>& c = ctxs[blockIdx.y];
 	__shared__ int sb[BAD_CHUNK];
 	if (c.sc->status)
@@ -2184,17 +2210,17 @@ __device__ inline void edit_links(int* list, int* cnt_ptr, const int* remap)
 constexpr int MOVE_BLOCKS = 16;
 
 template <class R>
-__global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	if (c.sc->bad_count == 0)
 		return;
-	if (blockIdx.x < MOVE_BLOCKS)
+	if (bid_ < MOVE_BLOCKS)
 	{
 		const int lane = lane_id();
 		const int waves = MOVE_BLOCKS * (blockDim.x >> 6);
 		const int moves = c.sc->moves;
-		for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
+		for (int k = bid_ * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
 		{
 			const int src = c.move_src[k], dst = c.move_dst[k];
 			// the copy plan's words (at most MOVE_UNITS per lane): every lane reads its plan
@@ -2252,7 +2278,7 @@ __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs)
 		}
 		return;
 	}
-	const int i = (blockIdx.x - MOVE_BLOCKS) * blockDim.x + threadIdx.x;
+	const int i = (bid_ - MOVE_BLOCKS) * blockDim.x + threadIdx.x;
 	if (i >= c.sc->n || c.remap[i] < 0) // holes receive their moved agent's edited lists
 		return;
 	edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
@@ -2260,10 +2286,10 @@ __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs)
 }
 
 template <class R>
-__global__ void k_compact_restore(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_compact_restore(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int k = gid();
+	const int k = GID;
 	const int b = c.sc->bad_count;
 	if (k < b)
 	{
@@ -2278,10 +2304,10 @@ __global__ void k_compact_restore(const SimCtx<R>* __restrict__ ctxs)
 }
 
 template <class R>
-__global__ void k_compact_done(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_compact_done(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	if (gid() == 0)
+	if (GID == 0)
 	{
 		c.sc->bad_count = 0;
 		c.sc->moves = 0;
@@ -2293,7 +2319,7 @@ __global__ void k_compact_done(const SimCtx<R>* __restrict__ ctxs)
 // One output double per thread: element (agent, column) of the agent-major matrix; column k
 // reads component comp[k] of field fld[k] (a component beyond the field's width is 0).
 template <class R>
-__global__ void k_pack(const SimCtx<R>* __restrict__ ctxs, const int* __restrict__ fld, const int* __restrict__ comp,
+__global__ void k_pack(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap, const int* __restrict__ fld, const int* __restrict__ comp,
 					   int row, double* __restrict__ out)
 {
 	PCM_PROLOGUE(c)
@@ -2331,8 +2357,9 @@ __global__ void k_pack(const SimCtx<R>* __restrict__ ctxs, const int* __restrict
 
 // the agents flagged to_divide, ascending, and their count; one block of 1024
 template <class R>
-__global__ void __launch_bounds__(1024) k_divide_plan(const SimCtx<R>* __restrict__ ctxs)
+__global__ void __launch_bounds__(1024) k_divide_plan(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
+	(void)bmap; // one block per simulation: never flattened
 	const SimCtx<R>& c = ctxs[blockIdx.y];
 	__shared__ int wave_total[16];
 	__shared__ int base;
@@ -2410,10 +2437,10 @@ __device__ __forceinline__ bool copied_on_division(int f)
 
 // cell::divide of division r: parent div_list[r], daughter at n + r
 template <class R, int D>
-__global__ void k_divide_apply(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_divide_apply(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int r = gid();
+	const int r = GID;
 	if (r >= c.sc->div_count)
 		return;
 	const int p = c.div_list[r];
@@ -2485,10 +2512,10 @@ __global__ void k_divide_apply(const SimCtx<R>* __restrict__ ctxs)
 
 // partners of dividing agents drop them, in division order (remove_attached, swap with back)
 template <class R>
-__global__ void k_divide_links(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_divide_links(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int i = gid();
+	const int i = GID;
 	if (i >= c.sc->n || c.sc->div_count == 0 || c.remap[i] < 0)
 		return;
 	edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
@@ -2498,10 +2525,10 @@ __global__ void k_divide_links(const SimCtx<R>* __restrict__ ctxs)
 // restore the parents' remap, count the daughters in, and bound the removal loop that
 // follows to the agents that existed before
 template <class R>
-__global__ void k_divide_done(const SimCtx<R>* __restrict__ ctxs)
+__global__ void k_divide_done(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
-	const int r = gid();
+	const int r = GID;
 	const int count = c.sc->div_count;
 	if (r < count)
 		c.remap[c.div_list[r]] = c.div_list[r];

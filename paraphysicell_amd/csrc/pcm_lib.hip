@@ -9,8 +9,10 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -62,6 +64,107 @@ int next_pow2(long long x)
 	return p;
 }
 
+// ---------------------------------------------------------------------------- device memory pool
+// Device memory of destroyed simulations is kept for later handles on the same device
+// instead of going back to the runtime: a serving loop creates and destroys a handle per
+// request, and hipMalloc/hipFree per request would cost milliseconds each, hipFree waiting
+// for the whole device (the other simulations' queued steps included).  A cached block
+// serves any request from half its size up to its size.  Blocks return to the runtime only
+// when the cache exceeds POOL_CACHE_LIMIT or a hipMalloc fails for lack of memory.  A block
+// is given back only once the stream that used it is idle (every caller synchronizes its
+// stream first), so its next owner cannot race its last one.
+constexpr size_t POOL_CACHE_LIMIT = 64ull << 30;
+constexpr int POOL_DEVICES = 64;
+
+class device_pool
+{
+public:
+	int take(size_t bytes, void** out, size_t* got, int* device)
+	{
+		int dev = 0;
+		PCM_HIP(hipGetDevice(&dev));
+		if (dev < 0 || dev >= POOL_DEVICES)
+			return fail(PCM_ERR_UNSUPPORTED, "device ordinal beyond the memory pool's table");
+		const size_t want = (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+		{
+			std::lock_guard<std::mutex> lock(mu_);
+			auto& m = free_[dev];
+			auto it = m.lower_bound(want);
+			if (it != m.end() && it->first / 2 <= want)
+			{
+				*out = it->second;
+				*got = it->first;
+				*device = dev;
+				cached_ -= it->first;
+				m.erase(it);
+				return PCM_OK;
+			}
+		}
+		void* p = nullptr;
+		hipError_t e = hipMalloc(&p, want);
+		if (e == hipErrorOutOfMemory)
+		{
+			(void)hipGetLastError();
+			trim(dev);
+			e = hipMalloc(&p, want);
+		}
+		if (e != hipSuccess)
+			return fail(PCM_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+		*out = p;
+		*got = want;
+		*device = dev;
+		return PCM_OK;
+	}
+
+	void give(void* p, size_t bytes, int dev)
+	{
+		std::lock_guard<std::mutex> lock(mu_);
+		free_[dev].emplace(bytes, p);
+		cached_ += bytes;
+		while (cached_ > POOL_CACHE_LIMIT)
+		{
+			// the largest cached block of any device goes back to the runtime
+			int big = -1;
+			for (int d = 0; d < POOL_DEVICES; d++)
+				if (!free_[d].empty() && (big < 0 || free_[d].rbegin()->first > free_[big].rbegin()->first))
+					big = d;
+			auto it = std::prev(free_[big].end());
+			(void)hipFree(it->second);
+			cached_ -= it->first;
+			free_[big].erase(it);
+		}
+	}
+
+private:
+	void trim(int dev)
+	{
+		std::lock_guard<std::mutex> lock(mu_);
+		for (auto& [bytes, p] : free_[dev])
+		{
+			(void)hipFree(p);
+			cached_ -= bytes;
+		}
+		free_[dev].clear();
+	}
+
+	std::mutex mu_;
+	std::multimap<size_t, void*> free_[POOL_DEVICES];
+	size_t cached_ = 0;
+};
+
+// A stream capture and calls on other threads: the runtime invalidates a batch's graph
+// capture when another thread allocates, copies synchronously or waits on a stream during
+// it (a serving loop sets requests up on worker threads while it steps the batch).  Every
+// simulation-level entry point holds this lock shared; a batch holds it exclusively only
+// between the begin and the end of a capture.
+std::shared_mutex g_capture_mu;
+
+device_pool& pool()
+{
+	static device_pool* p = new device_pool(); // never destroyed: blocks outlive static destructors
+	return *p;
+}
+
 } // namespace
 
 struct pcm_sim
@@ -71,6 +174,7 @@ struct pcm_sim
 	virtual int set_count(int n) = 0;
 	virtual int get_count(int* n) = 0;
 	virtual int set_field(int f, const void* src) = 0;
+	virtual int set_fields(int nfields, const int32_t* ids, const void* const* srcs) = 0;
 	virtual int get_field(int f, void* dst) = 0;
 	virtual int set_links(int which, const int32_t* off, const int32_t* idx) = 0;
 	virtual int get_links(int which, int32_t* off, int32_t* idx, int64_t* total) = 0;
@@ -113,6 +217,60 @@ struct pcm_batch
 
 namespace {
 
+// Block maps of a batch's flattened launches, one per grid shape the step uses: per agent
+// (BLOCK threads), 4, 8 or 16 lanes per agent, per spring slot (erase), per CLAIM_THREADS
+// agents (claim), per occupied voxel (neighbor search, at most 16384 blocks a member) and
+// the container update's move blocks plus one block per BLOCK agents.
+enum map_kind
+{
+	MAP_AGENT,
+	MAP_LANES4,
+	MAP_LANES8,
+	MAP_LANES16,
+	MAP_ERASE,
+	MAP_CLAIM,
+	MAP_NEIGHBORS,
+	MAP_COMPACT,
+	MAP_KINDS,
+	MAP_NONE = -1
+};
+
+inline int map_of_lanes(int lanes) { return lanes == 4 ? MAP_LANES4 : (lanes == 16 ? MAP_LANES16 : MAP_LANES8); }
+
+// blocks of one member in a launch of `kind`, for n agents and L spring slots
+inline int member_blocks(int kind, long long n, long long L)
+{
+	auto ceil_div = [](long long a, long long b) { return (a + b - 1) / b; };
+	long long b = 1;
+	switch (kind)
+	{
+		case MAP_AGENT:
+			b = ceil_div(n, BLOCK);
+			break;
+		case MAP_LANES4:
+			b = ceil_div(n * 4, BLOCK);
+			break;
+		case MAP_LANES8:
+			b = ceil_div(n * 8, BLOCK);
+			break;
+		case MAP_LANES16:
+			b = ceil_div(n * 16, BLOCK);
+			break;
+		case MAP_ERASE:
+			b = ceil_div(n * L, BLOCK);
+			break;
+		case MAP_CLAIM:
+			b = ceil_div(n, CLAIM_THREADS);
+			break;
+		case MAP_NEIGHBORS:
+			b = std::min<long long>(n, 16384);
+			break;
+		case MAP_COMPACT:
+			return MOVE_BLOCKS + (int)std::max<long long>(1, ceil_div(n, BLOCK));
+	}
+	return (int)std::max<long long>(1, b);
+}
+
 enum phase_id
 {
 	PH_MESH,
@@ -147,8 +305,8 @@ public:
 		}
 		for (hipEvent_t e : event_pool_)
 			(void)hipEventDestroy(e);
-		for (void* p : allocations_)
-			(void)hipFree(p);
+		for (const auto& a : allocations_)
+			pool().give(a.ptr, a.bytes, a.device);
 	}
 
 	int init() override
@@ -184,6 +342,17 @@ public:
 		h.springs = 1;
 
 		PCM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+		in_init_ = true;
+		const int rc = init_storage();
+		in_init_ = false;
+		PCM_TRY(rc);
+		PCM_TRY(push_ctx()); // waits for the zeroing and the copies of init_storage
+		return PCM_OK;
+	}
+
+	int init_storage()
+	{
+		auto& h = h_;
 		const int cap = std::max(1, h.cap);
 		const int dims = h.dims;
 
@@ -256,10 +425,10 @@ public:
 		std::vector<int> ident(cap);
 		for (int i = 0; i < cap; i++)
 			ident[i] = i;
-		PCM_HIP(hipMemcpy(h.remap, ident.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+		// ordered after the zeroing on the same stream; `ident` lives until init waits
+		remap_ident_ = std::move(ident);
+		PCM_HIP(hipMemcpyAsync(h.remap, remap_ident_.data(), sizeof(int) * cap, hipMemcpyHostToDevice, stream_));
 		PCM_TRY(alloc_t(&d_, 1));
-		PCM_TRY(push_ctx());
-		PCM_HIP(hipStreamSynchronize(stream_));
 		return PCM_OK;
 	}
 
@@ -268,33 +437,46 @@ public:
 	{
 		if (n < 0 || n > h_.cap)
 			return fail(PCM_ERR_CAPACITY, "agent count exceeds the handle's capacity");
-		PCM_HIP(hipStreamSynchronize(stream_));
-		Scalars s;
-		PCM_HIP(hipMemcpy(&s, h_.sc, sizeof(s), hipMemcpyDeviceToHost));
-		s.n = n;
-		s.status = 0;
-		s.cand_count = 0;
-		s.conflict = 0;
-		s.bad_count = 0;
-		s.moves = 0;
-		s.claim = 0;
-		s.log_len = 0;
-		PCM_HIP(hipMemcpy(h_.sc, &s, sizeof(s), hipMemcpyHostToDevice));
+		PCM_TRY(read_scalars());
+		// the voxel counts of the last binning are cleared here, over the agents it binned:
+		// later clears cover the new launch bound only, which may be smaller
+		if (sc_.n_binned > 0)
+		{
+			PCM_TRY(launch(PCM_K_BIN_CLEAR, k_bin_clear<R>, (sc_.n_binned + BLOCK - 1) / BLOCK, BLOCK));
+			sc_.n_binned = 0;
+			sc_.n_occ = 0;
+		}
+		binned_launch_ = 0;
+		sc_.n = n;
+		sc_.status = 0;
+		sc_.cand_count = 0;
+		sc_.conflict = 0;
+		sc_.bad_count = 0;
+		sc_.moves = 0;
+		sc_.claim = 0;
+		sc_.log_len = 0;
+		// stream-ordered copies and fills, one wait at the end (the sources are members)
+		PCM_HIP(hipMemcpyAsync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice, stream_));
 		n_launch_ = n;
 		mesh_fresh_ = false;
 		records_fresh_ = false;
-		std::vector<int> ident(std::max(n, 1));
-		for (int i = 0; i < n; i++)
-			ident[i] = i;
+		if ((int)origin_ident_.size() < n)
+		{
+			const int from = (int)origin_ident_.size();
+			origin_ident_.resize(n);
+			for (int i = from; i < n; i++)
+				origin_ident_[i] = i;
+		}
 		if (n > 0)
 		{
-			PCM_HIP(hipMemcpy(h_.origin, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-			PCM_HIP(hipMemset(h_.spr_cnt, 0, sizeof(int) * n));
-			PCM_HIP(hipMemset(h_.atc_cnt, 0, sizeof(int) * n));
-			PCM_HIP(hipMemset(h_.nbr_cnt, 0, sizeof(int) * n));
-			PCM_HIP(hipMemset(h_.contact, 0, n));
-			PCM_HIP(hipMemset(h_.bias_fn, 0, n));
+			PCM_HIP(hipMemcpyAsync(h_.origin, origin_ident_.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream_));
+			PCM_HIP(hipMemsetAsync(h_.spr_cnt, 0, sizeof(int) * n, stream_));
+			PCM_HIP(hipMemsetAsync(h_.atc_cnt, 0, sizeof(int) * n, stream_));
+			PCM_HIP(hipMemsetAsync(h_.nbr_cnt, 0, sizeof(int) * n, stream_));
+			PCM_HIP(hipMemsetAsync(h_.contact, 0, n, stream_));
+			PCM_HIP(hipMemsetAsync(h_.bias_fn, 0, n, stream_));
 		}
+		PCM_HIP(hipStreamSynchronize(stream_));
 		if (h_.contacts || h_.chemotaxis)
 		{
 			h_.contacts = 0;
@@ -317,7 +499,36 @@ public:
 			return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer");
 		// the current count: removals since the last upload leave it below n_launch_
 		PCM_TRY(read_scalars());
+		PCM_TRY(check_field(f, src, sc_.n));
+		PCM_HIP(hipMemcpyAsync(h_.fptr[f], src, (size_t)sc_.n * h_.fbytes[f], hipMemcpyHostToDevice, stream_));
+		PCM_HIP(hipStreamSynchronize(stream_));
+		return PCM_OK;
+	}
+
+	// Several fields in one call: one read of the count, every check first (nothing is
+	// copied when one fails), then the copies queued on the stream and one wait.
+	int set_fields(int nfields, const int32_t* ids, const void* const* srcs) override
+	{
+		if (nfields < 0 || (nfields > 0 && (!ids || !srcs)))
+			return fail(PCM_ERR_ARGUMENT, "null field table");
+		for (int k = 0; k < nfields; k++)
+			if (ids[k] < 0 || ids[k] >= PCM_FIELD_COUNT || !srcs[k])
+				return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer at entry " + std::to_string(k));
+		PCM_TRY(read_scalars());
 		const int n = sc_.n;
+		for (int k = 0; k < nfields; k++)
+			PCM_TRY(check_field(ids[k], srcs[k], n));
+		for (int k = 0; k < nfields; k++)
+			PCM_HIP(hipMemcpyAsync(h_.fptr[ids[k]], srcs[k], (size_t)n * h_.fbytes[ids[k]], hipMemcpyHostToDevice, stream_));
+		PCM_HIP(hipStreamSynchronize(stream_));
+		return PCM_OK;
+	}
+
+	// The checks and host-side consequences of uploading field f (n agents from src):
+	// value ranges the kernels index with, spring-row growth, the contact and chemotaxis
+	// switches of the context, and which derived device data goes stale.
+	int check_field(int f, const void* src, int n)
+	{
 		if (f == PCM_F_MAXIMUM_NUMBER_OF_ATTACHMENTS && n > 0)
 		{
 			const int* m = (const int*)src;
@@ -375,7 +586,6 @@ public:
 				PCM_TRY(push_ctx());
 			}
 		}
-		PCM_HIP(hipMemcpy(h_.fptr[f], src, (size_t)n * h_.fbytes[f], hipMemcpyHostToDevice));
 		if (f == PCM_F_POSITIONS || f == PCM_F_RADIUS || f == PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE)
 			mesh_fresh_ = false;
 		records_fresh_ = false;
@@ -629,7 +839,7 @@ public:
 		std::copy(comp.begin(), comp.end(), cols.begin() + 256);
 		PCM_HIP(hipMemcpy(pack_cols_, cols.data(), sizeof(int) * 512, hipMemcpyHostToDevice));
 		const int blocks = (int)((total + BLOCK - 1) / BLOCK);
-		hipLaunchKernelGGL(k_pack<R>, dim3(blocks), dim3(BLOCK), 0, stream_, (const SimCtx<R>*)d_,
+		hipLaunchKernelGGL(k_pack<R>, dim3(blocks), dim3(BLOCK), 0, stream_, (const SimCtx<R>*)d_, (const int4*)nullptr,
 						   (const int*)pack_cols_, (const int*)(pack_cols_ + 256), row, pack_out_);
 		PCM_HIP(hipGetLastError());
 		PCM_HIP(hipStreamSynchronize(stream_));
@@ -859,11 +1069,21 @@ private:
 	SimCtx<R>* d_ = nullptr;
 	Scalars sc_ {};
 	hipStream_t stream_ = nullptr;
-	std::vector<void*> allocations_;
+	struct allocation
+	{
+		void* ptr;
+		size_t bytes; // the pool block's size
+		int device;
+	};
+	std::vector<allocation> allocations_;
+	bool in_init_ = false; // allocations zero their memory asynchronously; init waits once at its end
 	int n_launch_ = 0; // agents at the last upload: an upper bound of the device count
 	// launch target: a batch points these at its context array for one segmented launch
 	// over all its simulations (grid.y = simulation); grid bounds then cover every member
 	const SimCtx<R>* launch_ctx_ = nullptr; // nullptr: this simulation's own context
+	const int4* flat_maps_ = nullptr;       // a batch's block maps of a flattened launch, or nullptr
+	int flat_off_[MAP_KINDS] = {};          // first entry of each map kind in flat_maps_
+	int flat_blocks_[MAP_KINDS] = {};       // grid size of each map kind (padded)
 	int launch_y_ = 1;
 	int grid_n_ = 0; // agent bound of grid sizes (0: n_launch_)
 	int grid_L_ = 0; // spring-slot bound of grid sizes (0: h_.L)
@@ -874,6 +1094,8 @@ private:
 	double* pack_out_ = nullptr;   // its output matrix
 	size_t pack_cap_ = 0;
 	int div_log_len_ = 0;         // divisions of the last phenotype update
+	std::vector<int> remap_ident_; // host source of init's identity copy
+	std::vector<int> origin_ident_; // host source of set_count's ORIGIN identity
 	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
@@ -900,9 +1122,14 @@ private:
 	int alloc(P* p, size_t bytes)
 	{
 		void* v = nullptr;
-		PCM_HIP(hipMalloc(&v, std::max<size_t>(bytes, 16)));
-		PCM_HIP(hipMemset(v, 0, std::max<size_t>(bytes, 16)));
-		allocations_.push_back(v);
+		size_t got = 0;
+		int dev = 0;
+		PCM_TRY(pool().take(bytes, &v, &got, &dev));
+		allocations_.push_back({ v, got, dev });
+		PCM_HIP(hipMemsetAsync(v, 0, std::max<size_t>(bytes, 16), stream_));
+		// outside init the caller may follow with copies on the null stream
+		if (!in_init_)
+			PCM_HIP(hipStreamSynchronize(stream_));
 		*p = (P)v;
 		return PCM_OK;
 	}
@@ -916,12 +1143,13 @@ private:
 		return PCM_OK;
 	}
 
+	// the caller has synchronized the stream: the block is idle
 	void release(void* p)
 	{
-		auto it = std::find(allocations_.begin(), allocations_.end(), p);
+		auto it = std::find_if(allocations_.begin(), allocations_.end(), [p](const allocation& a) { return a.ptr == p; });
 		if (it != allocations_.end())
 		{
-			(void)hipFree(p);
+			pool().give(it->ptr, it->bytes, it->device);
 			allocations_.erase(it);
 		}
 	}
@@ -1072,8 +1300,8 @@ private:
 
 	int read_scalars()
 	{
+		PCM_HIP(hipMemcpyAsync(&sc_, h_.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream_));
 		PCM_HIP(hipStreamSynchronize(stream_));
-		PCM_HIP(hipMemcpy(&sc_, h_.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
 		return PCM_OK;
 	}
 
@@ -1097,8 +1325,10 @@ private:
 	int gl() const { return grid_L_ ? grid_L_ : h_.L; }
 	int grid_agents() const { return std::max(1, (gn() + BLOCK - 1) / BLOCK); }
 
+	// `kind`: the launch's block map in a batch's flattened launch (MAP_NONE: one block row
+	// per simulation, as for single-block kernels)
 	template <class Kern>
-	int launch(int tag, Kern kernel, int blocks, int threads)
+	int launch(int tag, Kern kernel, int blocks, int threads, int kind = MAP_NONE)
 	{
 		const bool timed = ((prof_mask_ >> tag) & 1ull) && prof_seen_[tag]++ % prof_every_ == 0;
 		hipEvent_t ev[2] = { nullptr, nullptr };
@@ -1107,8 +1337,16 @@ private:
 			PCM_TRY(take_events(ev));
 			PCM_HIP(hipEventRecord(ev[0], stream_));
 		}
-		hipLaunchKernelGGL(kernel, dim3(blocks, launch_y_), dim3(threads), 0, stream_,
-						   launch_ctx_ ? launch_ctx_ : (const SimCtx<R>*)d_);
+		const int4* map = nullptr;
+		int gy = launch_y_;
+		if (flat_maps_ && kind != MAP_NONE)
+		{
+			map = flat_maps_ + flat_off_[kind];
+			blocks = flat_blocks_[kind];
+			gy = 1;
+		}
+		hipLaunchKernelGGL(kernel, dim3(blocks, gy), dim3(threads), 0, stream_,
+						   launch_ctx_ ? launch_ctx_ : (const SimCtx<R>*)d_, map);
 		PCM_HIP(hipGetLastError());
 		if (timed)
 		{
@@ -1121,14 +1359,14 @@ private:
 	template <class Kern>
 	int per_agent(int tag, Kern kernel)
 	{
-		return launch(tag, kernel, grid_agents(), BLOCK);
+		return launch(tag, kernel, grid_agents(), BLOCK, MAP_AGENT);
 	}
 
 	// `lanes` threads per agent
 	template <class Kern>
 	int per_lanes(int tag, Kern kernel, int lanes)
 	{
-		return launch(tag, kernel, std::max(1, (int)(((long long)gn() * lanes + BLOCK - 1) / BLOCK)), BLOCK);
+		return launch(tag, kernel, std::max(1, (int)(((long long)gn() * lanes + BLOCK - 1) / BLOCK)), BLOCK, map_of_lanes(lanes));
 	}
 
 	struct prof_record
@@ -1219,11 +1457,11 @@ private:
 			switch (tune_.neighbor_waves)
 			{
 				case 1:
-					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 1>, neighbor_blocks(), 64);
+					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 1>, neighbor_blocks(), 64, MAP_NEIGHBORS);
 				case 4:
-					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 4>, neighbor_blocks(), 256);
+					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 4>, neighbor_blocks(), 256, MAP_NEIGHBORS);
 				default:
-					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 2>, neighbor_blocks(), 128);
+					return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 2>, neighbor_blocks(), 128, MAP_NEIGHBORS);
 			}
 		}
 		// the voxel buckets predate position changes: search each agent's stencil around
@@ -1256,7 +1494,7 @@ private:
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 		PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D, FUSED>));
 		PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
-					   CLAIM_THREADS));
+					   CLAIM_THREADS, MAP_CLAIM));
 		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R>));
 		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 	}
@@ -1297,22 +1535,22 @@ private:
 	{
 		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R, IN_STEP>));
 		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R, IN_STEP>,
-					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK));
+					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK, MAP_ERASE));
 		// the first resolve round runs one block per collect block (segments of BLOCK / G)
 		const int segments = std::max(1, (int)(((long long)gn() * tune_.event_lanes + BLOCK - 1) / BLOCK));
 		switch (tune_.event_lanes)
 		{
 			case 4:
 				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 4>, 4));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 4>, segments, 64));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 4>, segments, 64, MAP_LANES4));
 				break;
 			case 16:
 				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 16>, 16));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 16>, segments, 64));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 16>, segments, 64, MAP_LANES16));
 				break;
 			default:
 				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 8>, 8));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 8>, segments, 64));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 8>, segments, 64, MAP_LANES8));
 		}
 		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
 		return IN_STEP ? PCM_OK : per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
@@ -1328,7 +1566,7 @@ private:
 
 	int compact_apply_kernel()
 	{
-		return launch(PCM_K_COMPACT_APPLY, k_compact_apply<R>, MOVE_BLOCKS + grid_agents(), BLOCK);
+		return launch(PCM_K_COMPACT_APPLY, k_compact_apply<R>, MOVE_BLOCKS + grid_agents(), BLOCK, MAP_COMPACT);
 	}
 
 	template <int D>
@@ -1527,13 +1765,21 @@ public:
 
 	// `steps` fused steps of every simulation in `ctxs` (this one is the batch's first
 	// member), as one launch per kernel with grid.y = m
+	// maps: the batch's block maps (device, offsets and padded totals per map kind) for
+	// flattened launches, or nullptr for one block row per member
 	int run_segmented(const SimCtx<R>* ctxs, int m, int grid_n, int grid_L, bool records_fresh, bool fold_clear,
-					  int steps)
+					  int steps, const int4* maps = nullptr, const int* offs = nullptr, const int* totals = nullptr)
 	{
 		launch_ctx_ = ctxs;
 		launch_y_ = m;
 		grid_n_ = grid_n;
 		grid_L_ = grid_L;
+		flat_maps_ = maps;
+		for (int k = 0; k < MAP_KINDS; k++)
+		{
+			flat_off_[k] = maps ? offs[k] : 0;
+			flat_blocks_[k] = maps ? totals[k] : 0;
+		}
 		int rc = PCM_OK;
 		for (int k = 0; k < steps && rc == PCM_OK; k++)
 		{
@@ -1545,6 +1791,7 @@ public:
 		launch_y_ = 1;
 		grid_n_ = 0;
 		grid_L_ = 0;
+		flat_maps_ = nullptr;
 		return rc;
 	}
 
@@ -1616,6 +1863,8 @@ public:
 				(void)hipFree(g.d_ctx);
 			if (g.d_sc)
 				(void)hipFree(g.d_sc);
+			if (g.d_map)
+				(void)hipFree(g.d_map);
 		}
 	}
 
@@ -1693,6 +1942,7 @@ public:
 	}
 
 	void use_graphs(bool on) { graphs_ = on; }
+	void use_flattened_grids(bool on) { flatten_ = on; }
 
 	// tune each group's launch shape on its real steps (graphs are off while tuning)
 	int autotune(const pcm_step_params* p, int steps) override
@@ -1731,6 +1981,7 @@ private:
 	struct capture_key
 	{
 		int slots = 0, grid_n = 0, grid_L = 0, springs = -1;
+		unsigned long long maps = 0; // map version of a flattened capture (0: none)
 		int velocity_lanes = 0, event_lanes = 0, neighbor_waves = 0;
 		bool operator==(const capture_key&) const = default;
 	};
@@ -1747,10 +1998,74 @@ private:
 		capture_key key;
 		std::vector<SimCtx<R>> h_ctx;
 		std::vector<Scalars> h_sc;
+		// block maps of flattened launches (map_kind order, concatenated), the layout they
+		// describe (per slot: launch bound and spring slots) and a version for graph keys
+		int4* d_map = nullptr;
+		size_t map_cap = 0;
+		std::vector<int4> h_map;
+		std::vector<long long> map_sig;
+		int map_off[MAP_KINDS] = {};
+		int map_tot[MAP_KINDS] = {};
+		unsigned long long map_version = 0;
 		bool active = false; // launched by the current step call
 	};
 	group groups_[3];
 	bool graphs_ = false;
+	bool flatten_ = true;
+
+	// The block maps of the group's members (slots 0..m-1), rebuilt and uploaded only when
+	// a slot's simulation, launch bound or spring-slot count changed.  Each map's grid is
+	// padded to four significant bits, so small changes keep the graph's grid sizes.
+	int build_maps(group& g, int m)
+	{
+		std::vector<long long> sig;
+		sig.reserve(2 * (size_t)m);
+		for (int k = 0; k < m; k++)
+		{
+			auto* sim = g.slots[k].sim;
+			sig.push_back(sim ? sim->launch_bound() : -1);
+			sig.push_back(sim ? sim->spring_slots() : 0);
+		}
+		if (g.d_map && sig == g.map_sig)
+			return PCM_OK;
+		g.h_map.clear();
+		for (int kind = 0; kind < MAP_KINDS; kind++)
+		{
+			g.map_off[kind] = (int)g.h_map.size();
+			long long total = 0;
+			for (int k = 0; k < m; k++)
+			{
+				auto* sim = g.slots[k].sim;
+				if (!sim)
+					continue;
+				const int b = member_blocks(kind, sim->launch_bound(), sim->spring_slots());
+				for (int x = 0; x < b; x++)
+					g.h_map.push_back(make_int4(k, x, b, 0));
+				total += b;
+			}
+			if (total > 0x7fffffffLL / 2)
+				return fail(PCM_ERR_CAPACITY, "flattened batch grid too large");
+			const int padded = round_up_grid((int)std::max<long long>(total, 1));
+			for (long long x = total; x < padded; x++)
+				g.h_map.push_back(make_int4(-1, 0, 0, 0));
+			g.map_tot[kind] = padded;
+		}
+		// the previous maps may still be read by queued launches
+		PCM_HIP(hipStreamSynchronize(g.stream));
+		if (g.h_map.size() > g.map_cap)
+		{
+			if (g.d_map)
+				PCM_HIP(hipFree(g.d_map));
+			g.d_map = nullptr;
+			const size_t cap = std::max(g.h_map.size(), 2 * g.map_cap);
+			PCM_HIP(hipMalloc((void**)&g.d_map, sizeof(int4) * cap));
+			g.map_cap = cap;
+		}
+		PCM_HIP(hipMemcpyAsync(g.d_map, g.h_map.data(), sizeof(int4) * g.h_map.size(), hipMemcpyHostToDevice, g.stream));
+		g.map_sig = std::move(sig);
+		g.map_version++;
+		return PCM_OK;
+	}
 
 	int ensure_group(group& g, int dims)
 	{
@@ -1865,7 +2180,7 @@ private:
 		bool fold_clear = true;
 		for (auto& slot : g.slots)
 			if (slot.sim)
-				fold_clear = fold_clear && slot.sim->binned_within(grid_n);
+				fold_clear = fold_clear && slot.sim->binned_within(flatten_ ? slot.sim->launch_bound() : grid_n);
 		g.h_ctx.resize(m);
 		for (int k = 0; k < m; k++)
 			g.h_ctx[k] = g.slots[k].sim ? g.slots[k].sim->host_ctx() : g.placeholder->host_ctx();
@@ -1879,18 +2194,27 @@ private:
 				g.slots[k].start = g.h_sc[k].step;
 			}
 		auto* leader = g.placeholder.get();
+		const int4* maps = nullptr;
+		if (flatten_)
+		{
+			PCM_TRY(build_maps(g, m));
+			maps = g.d_map;
+		}
+		auto run = [&](bool fresh, bool clear, int k) {
+			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, fresh, clear, k, maps, g.map_off, g.map_tot);
+		};
 		int done = 0;
 		if (!records_fresh || !fold_clear)
 		{
 			// the first step refreshes records or clears counts for some member: no graph
-			PCM_TRY(leader->run_segmented(g.d_ctx, m, grid_n, grid_L, records_fresh, fold_clear, 1));
+			PCM_TRY(run(records_fresh, fold_clear, 1));
 			done = 1;
 		}
 		if (!graphs_)
-			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, true, true, steps - done);
+			return run(true, true, steps - done);
 		const pcm_tuning t = leader->tuning();
-		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs, t.velocity_lanes, t.event_lanes,
-								t.neighbor_waves };
+		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs, maps ? g.map_version : 0,
+								t.velocity_lanes, t.event_lanes, t.neighbor_waves };
 		if (!g.exec || !(key == g.key))
 		{
 			if (g.exec)
@@ -1898,12 +2222,26 @@ private:
 				PCM_HIP(hipGraphExecDestroy(g.exec));
 				g.exec = nullptr;
 			}
+			// while other threads are inside the library, this call's steps run as plain
+			// launches instead of waiting for them; a later call captures
+			std::unique_lock<std::shared_mutex> no_other_calls(g_capture_mu, std::try_to_lock);
+			if (!no_other_calls.owns_lock())
+				return run(true, true, steps - done);
 			hipGraph_t graph = nullptr;
 			PCM_HIP(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-			const int rc = leader->run_segmented(g.d_ctx, m, grid_n, grid_L, true, true, 1);
+			const int rc = run(true, true, 1);
 			const hipError_t end = hipStreamEndCapture(g.stream, &graph);
+			no_other_calls.unlock();
+			if (end != hipSuccess)
+			{
+				// a capture the runtime invalidated (its launches then fail too) ran nothing:
+				// this call's steps run as plain launches, the next call captures again
+				(void)hipGetLastError();
+				if (graph)
+					(void)hipGraphDestroy(graph);
+				return run(true, true, steps - done);
+			}
 			PCM_TRY(rc);
-			PCM_HIP(end);
 			const hipError_t inst = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
 			PCM_HIP(hipGraphDestroy(graph));
 			PCM_HIP(inst);
@@ -2036,12 +2374,12 @@ int noalloc_build(const SimCtx<R>& proto, unsigned char* area, int* needed, hipS
 	hipLaunchKernelGGL(k_noalloc_setup<R>, dim3(clear_blocks), dim3(BLOCK), 0, stream, h, d_ctx, needed);
 	PCM_HIP(hipGetLastError());
 	const SimCtx<R>* ctx = d_ctx;
-	hipLaunchKernelGGL((k_bin_count<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx);
+	hipLaunchKernelGGL((k_bin_count<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL((k_bin_claim<R, false>), dim3(std::max(1, (agents + CLAIM_THREADS - 1) / CLAIM_THREADS)),
-					   dim3(CLAIM_THREADS), 0, stream, ctx);
-	hipLaunchKernelGGL(k_bin_scatter<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx);
-	hipLaunchKernelGGL((k_bin_order<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx);
-	hipLaunchKernelGGL((k_neighbors<R, D, 2>), dim3(std::max(1, std::min(agents, 16384))), dim3(128), 0, stream, ctx);
+					   dim3(CLAIM_THREADS), 0, stream, ctx, (const int4*)nullptr);
+	hipLaunchKernelGGL(k_bin_scatter<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
+	hipLaunchKernelGGL((k_bin_order<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
+	hipLaunchKernelGGL((k_neighbors<R, D, 2>), dim3(std::max(1, std::min(agents, 16384))), dim3(128), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL(k_noalloc_finish<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx, needed);
 	PCM_HIP(hipGetLastError());
 	return PCM_OK;
@@ -2137,6 +2475,7 @@ int pcm_create(const pcm_config* config, pcm_sim** out)
 	int devices = 0;
 	if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
 		return fail(PCM_ERR_DEVICE, "no HIP device is available");
+	std::shared_lock<std::shared_mutex> capture_guard(g_capture_mu);
 	std::unique_ptr<pcm_sim> s;
 	if (c.real_bytes == 4)
 		s.reset(new sim_impl<float>(c));
@@ -2155,6 +2494,7 @@ int pcm_destroy(pcm_sim* sim)
 {
 	if (!sim)
 		return PCM_OK;
+	std::shared_lock<std::shared_mutex> capture_guard(g_capture_mu);
 	if (auto* s4 = dynamic_cast<sim_impl<float>*>(sim); s4 && s4->batch())
 		PCM_TRY(((pcm_batch*)s4->batch())->retire(sim));
 	if (auto* s8 = dynamic_cast<sim_impl<double>*>(sim); s8 && s8->batch())
@@ -2246,6 +2586,17 @@ int pcm_autotune(pcm_sim* sim, const pcm_step_params* params, int32_t steps_per_
 	return sim->autotune(params, steps_per_candidate, chosen);
 }
 
+int pcm_batch_flatten(pcm_batch* batch, int32_t enable)
+{
+	if (!batch)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	if (auto* b4 = dynamic_cast<batch_impl<float>*>(batch))
+		b4->use_flattened_grids(enable != 0);
+	else if (auto* b8 = dynamic_cast<batch_impl<double>*>(batch))
+		b8->use_flattened_grids(enable != 0);
+	return PCM_OK;
+}
+
 int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t steps_per_candidate)
 {
 	if (!batch)
@@ -2266,7 +2617,8 @@ int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable)
 
 #define PCM_NEED(sim)                                                                                                  \
 	if (!(sim))                                                                                                        \
-	return fail(PCM_ERR_ARGUMENT, "null handle")
+		return fail(PCM_ERR_ARGUMENT, "null handle");                                                                  \
+	std::shared_lock<std::shared_mutex> capture_guard_(g_capture_mu)
 
 int pcm_set_count(pcm_sim* sim, int32_t count)
 {
@@ -2286,6 +2638,12 @@ int pcm_set_field(pcm_sim* sim, int32_t field, const void* host)
 {
 	PCM_NEED(sim);
 	return sim->set_field(field, host);
+}
+
+int pcm_set_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, const void* const* hosts)
+{
+	PCM_NEED(sim);
+	return sim->set_fields(nfields, fields, hosts);
 }
 
 int pcm_get_field(pcm_sim* sim, int32_t field, void* host)

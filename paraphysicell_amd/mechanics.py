@@ -133,11 +133,22 @@ class _LibSim:
         if state.count > self.capacity:
             raise PcmError(abi.PCM_ERR_CAPACITY, "state larger than capacity")
         self._call("set_count", self._h, state.count)
-        for name, _, _ in abi.FIELDS:
-            if fields is not None and name not in fields:
-                continue
-            a = np.ascontiguousarray(state.arrays[name])
-            self._call("set_field", self._h, abi.FIELD_ID[name], a.ctypes.data)
+        arrays = [(abi.FIELD_ID[name], np.ascontiguousarray(state.arrays[name])) for name, _, _ in abi.FIELDS
+                  if fields is None or name in fields]
+        bulk = getattr(self._lib, self.prefix + "set_fields", None)
+        if bulk is not None and arrays:
+            # every field in one call: one count read, one wait (pcm_set_fields)
+            ids = (ctypes.c_int32 * len(arrays))(*[fid for fid, _ in arrays])
+            ptrs = (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for _, a in arrays])
+            bulk.argtypes = [_PTR, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+            bulk.restype = ctypes.c_int
+            rc = bulk(self._h, len(arrays), ids, ptrs)
+            if rc != abi.PCM_OK:
+                msg = self._fn("last_error")()
+                raise PcmError(rc, msg.decode() if msg else "")
+        else:
+            for fid, a in arrays:
+                self._call("set_field", self._h, fid, a.ctypes.data)
         for which in (abi.LINKS_SPRINGS, abi.LINKS_ATTACHED):
             off, idx = state.links[which]
             off = np.ascontiguousarray(off, dtype=np.int32)
@@ -338,6 +349,7 @@ def load_library(path: str | None = None):
                        ("batch_size", [_PTR, ctypes.POINTER(ctypes.c_int32)]),
                        ("batch_step", [_PTR, ctypes.POINTER(abi.StepParams), ctypes.c_int32]),
                        ("batch_synchronize", [_PTR]), ("batch_use_graphs", [_PTR, ctypes.c_int32]),
+                       ("batch_flatten", [_PTR, ctypes.c_int32]),
                        ("batch_autotune", [_PTR, ctypes.POINTER(abi.StepParams), ctypes.c_int32]),
                        ("set_tuning", [_PTR, ctypes.POINTER(abi.Tuning)]),
                        ("get_tuning", [_PTR, ctypes.POINTER(abi.Tuning)]),
@@ -490,6 +502,10 @@ class Batch:
 
     def use_graphs(self, on: bool) -> None:
         self._check(self._lib.pcm_batch_use_graphs(self._h, 1 if on else 0))
+
+    def use_flattened_grids(self, on: bool) -> None:
+        """One launch over exactly the members' blocks (default) or one block row per member."""
+        self._check(self._lib.pcm_batch_flatten(self._h, 1 if on else 0))
 
     def autotune(self, params=None, steps_per_candidate: int = 5) -> None:
         """Pick each dimensionality group's launch shape on the batch's own steps."""

@@ -74,41 +74,83 @@ def _result(sim: DeviceSim) -> dict:
     return out
 
 
-class Server:
-    """Continuous batching over one pcm batch (hipGraph-captured steps by default)."""
+def _finish(sim: DeviceSim):
+    """Read a finished request's result back and free its simulation (memory returns to the
+    library's pool); returns the result and the time it was ready."""
+    result = _result(sim)
+    sim.close()
+    return result, time.perf_counter()
 
-    def __init__(self, budget_bytes: float = 200e9, chunk: int = 10, graphs: bool = True):
+
+class Server:
+    """Continuous batching over one pcm batch (hipGraph-captured steps by default).
+
+    The stepping thread only advances the batch and moves requests in and out of it; set-up
+    (handle, upload, first mesh and neighbor build) and tear-down (result read-back, free)
+    run on worker threads, each on the request's own stream, so they overlap the batch's
+    steps on the device.  A request counts against the HBM budget from the start of its
+    set-up to the end of its tear-down."""
+
+    def __init__(self, budget_bytes: float = 200e9, chunk: int = 10, graphs: bool = True, workers: int = 8):
+        from concurrent.futures import ThreadPoolExecutor
+
         self.budget = budget_bytes
         self.chunk = chunk
         self.batch = Batch(4, graphs=graphs)
         self.used = 0
         self.agent_steps = 0
+        self.workers = ThreadPoolExecutor(max_workers=max(1, workers), thread_name_prefix="pcm-serve")
 
     def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
         """Serve `requests` (sorted by arrival).  With realtime, a request only becomes
         visible at its arrival time; without, all are present at once (saturation)."""
+        from concurrent.futures import FIRST_COMPLETED, wait
+
         pending = sorted(requests, key=lambda q: q.arrival)
-        live: list[Request] = []
+        starting: list[tuple[Request, object]] = []   # set-up in flight
+        live: list[Request] = []                       # in the batch
+        finishing: list[tuple[Request, object]] = []  # tear-down in flight
         done: list[Request] = []
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
-        while pending or live:
-            # admit every arrived request that fits the budget, in arrival order
+        while pending or starting or live or finishing:
+            # arrived requests start their set-up while the budget allows, in arrival order
             while pending and (not realtime or pending[0].arrival <= now()):
                 q = pending[0]
                 need = footprint_bytes(q.mesh, q.state.real_bytes, q.state.defs, q.state.subs, q.agents)
-                if live and self.used + need > self.budget:
+                if (live or starting or finishing) and self.used + need > self.budget:
                     break
                 pending.pop(0)
-                q.sim = _start(q)
-                q.remaining = q.steps
-                q.admitted = now()
                 q.need = need
+                q.admitted = now()
                 self.used += need
+                starting.append((q, self.workers.submit(_start, q)))
+            # finished set-ups join the batch between two step chunks
+            for item in [x for x in starting if x[1].done()]:
+                q, fut = item
+                starting.remove(item)
+                q.sim = fut.result()
+                q.remaining = q.steps
                 self.batch.admit(q.sim)
                 live.append(q)
+            for item in [x for x in finishing if x[1].done()]:
+                q, fut = item
+                finishing.remove(item)
+                q.result, ready = fut.result()
+                q.finished = ready - t0
+                self.used -= q.need
+                done.append(q)
+            # at least as many requests setting up as stepping: they join within a few
+            # milliseconds, and stepping the few live ones alone meanwhile would leave the
+            # GPU mostly idle now and the late joiners stepping alone at the end
+            if starting and len(starting) >= len(live):
+                wait([f for _, f in starting], timeout=0.05, return_when=FIRST_COMPLETED)
+                continue
             if not live:
-                if pending:
+                futures = [f for _, f in finishing]
+                if futures:
+                    wait(futures, timeout=0.05, return_when=FIRST_COMPLETED)
+                elif pending:
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
                 continue
             k = min(self.chunk, min(q.remaining for q in live))
@@ -120,17 +162,14 @@ class Server:
                 q.remaining -= k
                 if q.remaining == 0:
                     self.batch.retire(q.sim)
-                    q.result = _result(q.sim)
-                    q.sim.close()
-                    q.sim = None
-                    q.finished = now()
-                    self.used -= q.need
                     live.remove(q)
-                    done.append(q)
+                    finishing.append((q, self.workers.submit(_finish, q.sim)))
+                    q.sim = None
         self.elapsed = now()
         return done
 
     def close(self):
+        self.workers.shutdown(wait=True)
         self.batch.close()
 
 

@@ -47,10 +47,11 @@ def _member(spec, real_bytes=4):
 SPECS = [(2, 40, 1), (3, 300, 2), (2, 500, 3), (3, 64, 4), (3, 900, 5)]
 
 
+@pytest.mark.parametrize("flatten", [True, False])
 @pytest.mark.parametrize("graphs", [False, True])
 @pytest.mark.parametrize("real_bytes", [4, 8])
-def test_batch_equals_alone(real_bytes, graphs):
-    print(f"batch vs alone, real_bytes {real_bytes}, graphs {graphs}", flush=True)
+def test_batch_equals_alone(real_bytes, graphs, flatten):
+    print(f"batch vs alone, real_bytes {real_bytes}, graphs {graphs}, flattened grids {flatten}", flush=True)
     alone = []
     for spec in SPECS:
         sim, params = _member(spec, real_bytes)
@@ -58,6 +59,7 @@ def test_batch_equals_alone(real_bytes, graphs):
         alone.append(_state_of(sim))
         sim.close()
     batch = Batch(real_bytes, graphs=graphs)
+    batch.use_flattened_grids(flatten)
     sims = []
     for spec in SPECS:
         sim, params = _member(spec, real_bytes)
