@@ -298,19 +298,14 @@ __device__ __forceinline__ bool candidate_within(const R* pi, const R* q)
 	}
 }
 
-// One agent's list straight from HBM.  DIRECT: the buckets may predate the current
-// positions, so the stencil is centred on the agent's current voxel and candidates are
-// tested with their current positions and radii (what the reference does with stale
-// buckets); otherwise the voxel-sorted copy made by the mesh update is used.
+// One agent's list from its 3^d stencil, in stencil order and bucket order: position pi and
+// reach mi = rel_adh * radius of agent i, voxel coordinates co.  DIRECT: the buckets may
+// predate the current positions, so candidates are tested with their current positions and
+// radii (what the reference does with stale buckets); otherwise the voxel-sorted copy made
+// by the mesh update is used.
 template <class R, int D, bool DIRECT>
-__device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
+__device__ __forceinline__ void neighbors_from(const SimCtx<R>& c, int i, const R* pi, R mi, const int* co)
 {
-	R pi[3] = { 0, 0, 0 };
-	for (int k = 0; k < D; k++)
-		pi[k] = c.pos[(size_t)i * D + k];
-	const R mi = c.rel_adh[i] * c.radius[i];
-	int co[3];
-	voxel_coords<R>(c, DIRECT ? voxel_index<R>(c, c.pos + (size_t)i * D) : c.vox_of[i], co);
 	int* out = c.nbr_idx + (size_t)i * c.K;
 	int cnt = 0;
 	for (int z = -1; z <= 1; z++)
@@ -362,6 +357,18 @@ __device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
 		cnt = c.K;
 	}
 	c.nbr_cnt[i] = cnt;
+}
+
+template <class R, int D, bool DIRECT>
+__device__ __forceinline__ void neighbors_one_agent(const SimCtx<R>& c, int i)
+{
+	R pi[3] = { 0, 0, 0 };
+	for (int k = 0; k < D; k++)
+		pi[k] = c.pos[(size_t)i * D + k];
+	const R mi = c.rel_adh[i] * c.radius[i];
+	int co[3];
+	voxel_coords<R>(c, DIRECT ? voxel_index<R>(c, c.pos + (size_t)i * D) : c.vox_of[i], co);
+	neighbors_from<R, D, DIRECT>(c, i, pi, mi, co);
 }
 
 // the path for stencils larger than the LDS tile, out of line so that its registers do
@@ -498,6 +505,68 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 		}
 		__syncthreads();
 	}
+}
+
+// Stencils of a few dozen candidates (2D and 1D populations at tissue densities) leave a
+// wavefront per agent mostly idle and spend a block per voxel on staging: one lane per agent
+// instead, agents taken in bucket order, so that the lanes of a wavefront share stencil
+// voxels and their candidate loads hit the same lines; the agent's own position and reach
+// come from its voxel-sorted record.  Candidates, order and test are those of k_neighbors.
+template <class R, int D>
+__global__ void k_neighbors_lanes(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+{
+	PCM_PROLOGUE_NEIGHBORS(c)
+	static_assert(D < 3, "3D stencils use k_neighbors");
+	constexpr int E = D == 1 ? 3 : 9; // stencil voxels, x fastest (the z row of a 2D grid is 0)
+	const int k0 = GID;
+	if (k0 >= c.sc->n_binned)
+		return;
+	const int i = c.bucket[k0];
+	R pi[4];
+	for (int d = 0; d < 4; d++)
+		pi[d] = c.sorted[(size_t)k0 * 4 + d];
+	int co[3];
+	// the record holds the binned position exactly, so this is the agent's bucket voxel
+	voxel_coords<R>(c, voxel_index<R>(c, pi), co);
+	// every stencil voxel's count and start requested at once
+	int cnt_e[E], start_e[E];
+#pragma unroll
+	for (int e = 0; e < E; e++)
+	{
+		const int x = co[0] + e % 3 - 1, y = D == 2 ? co[1] + e / 3 - 1 : 0;
+		const bool in = x >= 0 && x < c.grid[0] && y >= 0 && y < c.grid[1];
+		const int nv = in ? x + c.grid[0] * y : 0;
+		cnt_e[e] = in ? c.vcnt[nv] : 0;
+		start_e[e] = in ? c.vstart[nv] : 0;
+	}
+	int* out = c.nbr_idx + (size_t)i * c.K;
+	int cnt = 0;
+	const R mi = pi[3];
+#pragma unroll
+	for (int e = 0; e < E; e++)
+	{
+		const int s = start_e[e], m = cnt_e[e];
+		for (int k = s; k < s + m; k++)
+		{
+			const int j = c.bucket[k];
+			R q[4];
+			for (int d = 0; d < 4; d++)
+				q[d] = c.sorted[(size_t)k * 4 + d];
+			if (j != i && pair_within<D, R>(pi, q, mi + q[3]))
+			{
+				if (cnt < c.K)
+					out[cnt] = j;
+				cnt++;
+			}
+		}
+	}
+	if (cnt > c.K)
+	{
+		atomicOr(&c.sc->status, ST_NEIGHBOR_OVERFLOW);
+		atomicMax(&c.sc->max_neighbors, cnt);
+		cnt = c.K;
+	}
+	c.nbr_cnt[i] = cnt;
 }
 
 template <class R, int D>

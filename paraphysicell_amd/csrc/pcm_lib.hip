@@ -1447,11 +1447,18 @@ public:
 
 private:
 
+	static constexpr int LANES_NEIGHBOR_MIN_AGENTS = 4096;
+
 	int neighbor_blocks() const { return std::max(1, std::min(gn(), 16384)); }
 
 	template <int D>
 	int neighbors_d()
 	{
+		// 1D and 2D stencils hold a few dozen candidates: above a few thousand agents one
+		// lane per agent beats a block per occupied voxel (far fewer blocks); below, the
+		// block per voxel has more parallelism to hide its loads
+		if (mesh_fresh_ && D < 3 && gn() >= LANES_NEIGHBOR_MIN_AGENTS)
+			return per_agent(PCM_K_NEIGHBORS, k_neighbors_lanes<R, D < 3 ? D : 2>);
 		if (mesh_fresh_)
 		{
 			switch (tune_.neighbor_waves)

@@ -34,12 +34,17 @@ def _check_mesh_invariants(dev, st_mesh):
             assert np.all(inside | edge)
 
 
-@pytest.mark.parametrize("case", ["disc_900_2d", "ball_14k_3d_f32", "ball_14k_3d_f64", "ball_88k_3d"])
+@pytest.mark.parametrize("case", ["disc_900_2d", "disc_12k_2d", "ball_14k_3d_f32", "ball_14k_3d_f64", "ball_88k_3d"])
 def test_workload_full_size(case):
     print(f"full-size workload {case}", flush=True)
     if case == "disc_900_2d":
         mesh, st, params = synth.config0(4)
         steps = 10
+    elif case == "disc_12k_2d":
+        # the 900-agent disc's density at 12,000 agents: above the size where 2D neighbor
+        # lists are built one lane per agent
+        mesh, st, params = synth.ball_population(2, 12000, 400.0 * (12000 / 900) ** 0.5, 2000, 9, 4)
+        steps = 5
     elif case == "ball_14k_3d_f32":
         mesh, st, params = synth.config1(4)
         steps = 3
