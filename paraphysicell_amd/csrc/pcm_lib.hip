@@ -597,7 +597,7 @@ public:
 		if (f < 0 || f >= PCM_FIELD_COUNT || !dst)
 			return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer");
 		PCM_TRY(read_scalars());
-		PCM_HIP(hipMemcpy(dst, h_.fptr[f], (size_t)sc_.n * h_.fbytes[f], hipMemcpyDeviceToHost));
+		PCM_TRY(copy_sync(dst, h_.fptr[f], (size_t)sc_.n * h_.fbytes[f], hipMemcpyDeviceToHost));
 		return PCM_OK;
 	}
 
@@ -641,8 +641,8 @@ public:
 		int* di = which == PCM_LINKS_SPRINGS ? h_.spr_idx : h_.atc_idx;
 		if (n > 0)
 		{
-			PCM_HIP(hipMemcpy(dc, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-			PCM_HIP(hipMemcpy(di, rows.data(), sizeof(int) * (size_t)n * W, hipMemcpyHostToDevice));
+			PCM_TRY(copy_sync(dc, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+			PCM_TRY(copy_sync(di, rows.data(), sizeof(int) * (size_t)n * W, hipMemcpyHostToDevice));
 		}
 		return PCM_OK;
 	}
@@ -657,9 +657,9 @@ public:
 		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * W);
 		if (n > 0)
 		{
-			PCM_HIP(hipMemcpy(cnt.data(), which == PCM_LINKS_SPRINGS ? h_.spr_cnt : h_.atc_cnt, sizeof(int) * n,
+			PCM_TRY(copy_sync(cnt.data(), which == PCM_LINKS_SPRINGS ? h_.spr_cnt : h_.atc_cnt, sizeof(int) * n,
 							  hipMemcpyDeviceToHost));
-			PCM_HIP(hipMemcpy(rows.data(), which == PCM_LINKS_SPRINGS ? h_.spr_idx : h_.atc_idx,
+			PCM_TRY(copy_sync(rows.data(), which == PCM_LINKS_SPRINGS ? h_.spr_idx : h_.atc_idx,
 							  sizeof(int) * (size_t)n * W, hipMemcpyDeviceToHost));
 		}
 		int64_t t = 0;
@@ -686,8 +686,8 @@ public:
 		std::vector<uint8_t> mov(std::max(n, 1));
 		if (n > 0)
 		{
-			PCM_HIP(hipMemcpy(cnt.data(), h_.nbr_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
-			PCM_HIP(hipMemcpy(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(cnt.data(), h_.nbr_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
 		}
 		int64_t t = 0;
 		for (int i = 0; i < n; i++)
@@ -697,7 +697,7 @@ public:
 			return PCM_OK; // the size query needs no rows
 		std::unique_ptr<int[]> rows(new int[(size_t)std::max(n, 1) * K]);
 		if (n > 0)
-			PCM_HIP(hipMemcpy(rows.get(), h_.nbr_idx, sizeof(int) * (size_t)n * K, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(rows.get(), h_.nbr_idx, sizeof(int) * (size_t)n * K, hipMemcpyDeviceToHost));
 		t = 0;
 		for (int i = 0; i < n; i++)
 		{
@@ -725,7 +725,7 @@ public:
 		const int n = sc_.n;
 		std::vector<uint8_t> mov(std::max(n, 1));
 		if (n > 0)
-			PCM_HIP(hipMemcpy(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
 		// two passes over the flat lists: count (a movable agent's own entries, a
 		// non-movable agent's listers), then fill the fixed-stride rows
 		std::vector<int> cnt(std::max(n, 1), 0);
@@ -766,8 +766,8 @@ public:
 			std::fill(rows.get() + (size_t)i * K + cnt[i], rows.get() + (size_t)(i + 1) * K, -1);
 		if (n > 0)
 		{
-			PCM_HIP(hipMemcpy(h_.nbr_cnt, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-			PCM_HIP(hipMemcpy(h_.nbr_idx, rows.get(), sizeof(int) * (size_t)n * K, hipMemcpyHostToDevice));
+			PCM_TRY(copy_sync(h_.nbr_cnt, cnt.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+			PCM_TRY(copy_sync(h_.nbr_idx, rows.get(), sizeof(int) * (size_t)n * K, hipMemcpyHostToDevice));
 		}
 		return PCM_OK;
 	}
@@ -779,12 +779,12 @@ public:
 		PCM_TRY(read_scalars());
 		const size_t V = (size_t)h_.vcount;
 		std::vector<int> vc(V), vs(V);
-		PCM_HIP(hipMemcpy(vc.data(), h_.vcnt, sizeof(int) * V, hipMemcpyDeviceToHost));
-		PCM_HIP(hipMemcpy(vs.data(), h_.vstart, sizeof(int) * V, hipMemcpyDeviceToHost));
+		PCM_TRY(copy_sync(vc.data(), h_.vcnt, sizeof(int) * V, hipMemcpyDeviceToHost));
+		PCM_TRY(copy_sync(vs.data(), h_.vstart, sizeof(int) * V, hipMemcpyDeviceToHost));
 		const int nb = std::max(sc_.n_binned, 1);
 		std::vector<int> b(nb);
 		if (sc_.n_binned > 0)
-			PCM_HIP(hipMemcpy(b.data(), h_.bucket, sizeof(int) * sc_.n_binned, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(b.data(), h_.bucket, sizeof(int) * sc_.n_binned, hipMemcpyDeviceToHost));
 		int64_t t = 0;
 		for (size_t v = 0; v < V; v++)
 		{
@@ -837,13 +837,13 @@ public:
 		std::vector<int> cols(512, 0);
 		std::copy(fld.begin(), fld.end(), cols.begin());
 		std::copy(comp.begin(), comp.end(), cols.begin() + 256);
-		PCM_HIP(hipMemcpy(pack_cols_, cols.data(), sizeof(int) * 512, hipMemcpyHostToDevice));
+		PCM_TRY(copy_sync(pack_cols_, cols.data(), sizeof(int) * 512, hipMemcpyHostToDevice));
 		const int blocks = (int)((total + BLOCK - 1) / BLOCK);
 		hipLaunchKernelGGL(k_pack<R>, dim3(blocks), dim3(BLOCK), 0, stream_, (const SimCtx<R>*)d_, (const int4*)nullptr,
 						   (const int*)pack_cols_, (const int*)(pack_cols_ + 256), row, pack_out_);
 		PCM_HIP(hipGetLastError());
 		PCM_HIP(hipStreamSynchronize(stream_));
-		PCM_HIP(hipMemcpy(out, pack_out_, sizeof(double) * (size_t)total, hipMemcpyDeviceToHost));
+		PCM_TRY(copy_sync(out, pack_out_, sizeof(double) * (size_t)total, hipMemcpyDeviceToHost));
 		return PCM_OK;
 	}
 
@@ -862,7 +862,7 @@ public:
 		if (capacity < div_log_len_)
 			return fail(PCM_ERR_CAPACITY, "division log longer than the buffer");
 		PCM_HIP(hipStreamSynchronize(stream_));
-		PCM_HIP(hipMemcpy(parents, h_.div_list, sizeof(int32_t) * div_log_len_, hipMemcpyDeviceToHost));
+		PCM_TRY(copy_sync(parents, h_.div_list, sizeof(int32_t) * div_log_len_, hipMemcpyDeviceToHost));
 		return PCM_OK;
 	}
 
@@ -920,7 +920,7 @@ public:
 	{
 		PCM_TRY(read_scalars());
 		sc_.step = s;
-		PCM_HIP(hipMemcpy(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
+		PCM_TRY(copy_sync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
 		return PCM_OK;
 	}
 
@@ -954,7 +954,7 @@ public:
 			grad_count_ = count;
 		}
 		if (count > 0)
-			PCM_HIP(hipMemcpy((void*)h_.grad, g, sizeof(R) * count, hipMemcpyHostToDevice));
+			PCM_TRY(copy_sync((void*)h_.grad, g, sizeof(R) * count, hipMemcpyHostToDevice));
 		for (int k = 0; k < 3; k++)
 		{
 			h_.dvox[k] = voxel[k];
@@ -1027,9 +1027,9 @@ public:
 		if (capacity < len)
 			return fail(PCM_ERR_ARGUMENT, "output buffer smaller than the removal log");
 		if (len > 0)
-			PCM_HIP(hipMemcpy(positions, h_.rm_log, sizeof(int) * len, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(positions, h_.rm_log, sizeof(int) * len, hipMemcpyDeviceToHost));
 		sc_.log_len = 0;
-		PCM_HIP(hipMemcpy(&h_.sc->log_len, &sc_.log_len, sizeof(int), hipMemcpyHostToDevice));
+		PCM_TRY(copy_sync(&h_.sc->log_len, &sc_.log_len, sizeof(int), hipMemcpyHostToDevice));
 		return PCM_OK;
 	}
 
@@ -1044,8 +1044,8 @@ public:
 		if (segments > 0)
 		{
 			std::vector<int> cnt(segments), ev(segments);
-			PCM_HIP(hipMemcpy(cnt.data(), h_.blk_cnt, sizeof(int) * segments, hipMemcpyDeviceToHost));
-			PCM_HIP(hipMemcpy(ev.data(), h_.blk_ev, sizeof(int) * segments, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(cnt.data(), h_.blk_cnt, sizeof(int) * segments, hipMemcpyDeviceToHost));
+			PCM_TRY(copy_sync(ev.data(), h_.blk_ev, sizeof(int) * segments, hipMemcpyDeviceToHost));
 			for (int b = 0; b < segments; b++)
 			{
 				out[0] += ev[b];
@@ -1250,13 +1250,13 @@ private:
 	{
 		const size_t cap = std::max(1, h_.cap);
 		std::vector<int> rows(cap * oldW), grown(cap * newW, -1);
-		PCM_HIP(hipMemcpy(rows.data(), (void*)*idx, sizeof(int) * cap * oldW, hipMemcpyDeviceToHost));
+		PCM_TRY(copy_sync(rows.data(), (void*)*idx, sizeof(int) * cap * oldW, hipMemcpyDeviceToHost));
 		for (size_t i = 0; i < cap; i++)
 			for (int s = 0; s < oldW; s++)
 				grown[i * newW + s] = rows[i * oldW + s];
 		release((void*)*idx);
 		PCM_TRY(alloc(idx, sizeof(int) * cap * newW));
-		PCM_HIP(hipMemcpy((void*)*idx, grown.data(), sizeof(int) * cap * newW, hipMemcpyHostToDevice));
+		PCM_TRY(copy_sync((void*)*idx, grown.data(), sizeof(int) * cap * newW, hipMemcpyHostToDevice));
 		return PCM_OK;
 	}
 
@@ -1298,6 +1298,18 @@ private:
 		return push_ctx();
 	}
 
+	// a copy ordered on this simulation's stream, waited for before returning: copies of
+	// different simulations (a serving loop's worker threads) do not queue behind each
+	// other on the null stream
+	int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+	{
+		if (bytes == 0)
+			return PCM_OK;
+		PCM_HIP(hipMemcpyAsync(dst, src, bytes, kind, stream_));
+		PCM_HIP(hipStreamSynchronize(stream_));
+		return PCM_OK;
+	}
+
 	int read_scalars()
 	{
 		PCM_HIP(hipMemcpyAsync(&sc_, h_.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream_));
@@ -1313,7 +1325,7 @@ private:
 		PCM_TRY(grow_neighbors(sc_.max_neighbors));
 		sc_.status = 0;
 		sc_.max_neighbors = 0;
-		PCM_HIP(hipMemcpy(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
+		PCM_TRY(copy_sync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
 		PCM_TRY(launch_neighbors());
 		PCM_TRY(read_scalars());
 		if (sc_.status)

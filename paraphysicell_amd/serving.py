@@ -104,7 +104,7 @@ class Server:
     def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
         """Serve `requests` (sorted by arrival).  With realtime, a request only becomes
         visible at its arrival time; without, all are present at once (saturation)."""
-        from concurrent.futures import FIRST_COMPLETED, wait
+        from concurrent.futures import ALL_COMPLETED, FIRST_COMPLETED, wait
 
         pending = sorted(requests, key=lambda q: q.arrival)
         starting: list[tuple[Request, object]] = []   # set-up in flight
@@ -140,11 +140,13 @@ class Server:
                 q.finished = ready - t0
                 self.used -= q.need
                 done.append(q)
-            # at least as many requests setting up as stepping: they join within a few
-            # milliseconds, and stepping the few live ones alone meanwhile would leave the
-            # GPU mostly idle now and the late joiners stepping alone at the end
-            if starting and len(starting) >= len(live):
-                wait([f for _, f in starting], timeout=0.05, return_when=FIRST_COMPLETED)
+            # An empty batch waits for every set-up in flight, and a batch waits while at
+            # least as many requests are setting up as stepping: they join within a few
+            # milliseconds, while stepping the few live ones alone meanwhile would leave the
+            # GPU mostly idle now and the late joiners stepping alone at the end.
+            if starting and (not live or len(starting) >= len(live)):
+                wait([f for _, f in starting], timeout=0.05,
+                     return_when=ALL_COMPLETED if not live else FIRST_COMPLETED)
                 continue
             if not live:
                 futures = [f for _, f in finishing]
