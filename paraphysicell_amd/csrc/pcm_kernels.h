@@ -386,10 +386,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 {
 	PCM_PROLOGUE_NEIGHBORS(c)
 	constexpr int TILE = nbr_tile<R>();
-	__shared__ R s_rec[TILE * 4];
+	__shared__ R s_rec[(TILE + 64) * 4]; // candidates, then 64 far-away records
 	__shared__ int s_idx[TILE];
 	__shared__ int s_pre[28];
 	__shared__ int s_vs[27];
+	__shared__ R s_rmax[W]; // largest reach (rel_adh * radius) among the staged candidates, per wavefront
 	const int tid = threadIdx.x;
 	const int lane = tid & 63;
 	const int wave = tid >> 6;
@@ -433,6 +434,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 			// number of row starts at or below it; lane k of every wavefront holds the start
 			// of entry k, read back with readlane), then the entry within the row
 			const int lane_pre = s_pre[lane < 28 ? lane : 27];
+			R rmax = 0;
 			for (int t = tid; t < C; t += 64 * W)
 			{
 				int row = 0;
@@ -449,10 +451,19 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 				s_idx[t] = j;
 				for (int k = 0; k < 4; k++)
 					s_rec[t * 4 + k] = rec[k];
+				rmax = rec[3] > rmax ? rec[3] : rmax;
 			}
-			// the rest of the last 64-candidate tile: far-away records, so that the test
-			// below runs on every lane with no bounds branch (their lanes are masked too)
-			for (int t = C + tid; t < ((C + 63) & ~63); t += 64 * W)
+			for (int o = 32; o > 0; o >>= 1)
+			{
+				const R other = __shfl_xor(rmax, o, 64);
+				rmax = other > rmax ? other : rmax;
+			}
+			if (lane == 0)
+				s_rmax[wave] = rmax;
+			// 64 far-away records after the candidates, so that the test below runs on every
+			// lane of a 64-candidate chunk starting anywhere with no bounds branch (their
+			// lanes are masked too)
+			for (int t = C + tid; t < C + 64; t += 64 * W)
 			{
 				for (int k = 0; k < 3; k++)
 					s_rec[t * 4 + k] = far_coordinate<R>();
@@ -460,6 +471,14 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 			}
 			__syncthreads();
 			const int own0 = __builtin_amdgcn_readlane(lane_pre, 13);
+			R block_rmax = 0;
+			for (int k = 0; k < W; k++)
+				block_rmax = s_rmax[k] > block_rmax ? s_rmax[k] : block_rmax;
+			// the voxel's z faces (exact in real_t: integers of at most 24 bits)
+			const R z_lo = (R)c.bmin[2] + (R)co[2] * (R)c.vox[2];
+			const R z_hi = z_lo + (R)c.vox[2];
+			const int plane0 = __builtin_amdgcn_readlane(lane_pre, 9);  // first entry of the dz = 0 plane
+			const int plane1 = __builtin_amdgcn_readlane(lane_pre, 18); // first entry of the dz = +1 plane
 			for (int a = wave; a < m; a += W)
 			{
 				const int self_t = own0 + a;
@@ -469,10 +488,26 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 					pi[k] = s_rec[self_t * 4 + k];
 				int* out = c.nbr_idx + (size_t)i * c.K;
 				int base = 0;
-				for (int t0 = 0; t0 < C; t0 += 64)
+				// Stencil entries run z-major, so the dz = -1 and dz = +1 planes are the
+				// first and last runs of candidates.  A plane farther from the agent than its
+				// reach plus the largest candidate reach (plus a margin far above the
+				// rounding of the voxel assignment) holds no neighbor and is skipped; the
+				// candidates tested keep their order, so lists are unchanged.
+				int t_begin = 0, t_end = C;
+				if (D == 3)
+				{
+					const R bound = pi[3] + block_rmax + (R)0.01;
+					if (pi[2] - z_lo > bound)
+						t_begin = plane0;
+					if (z_hi - pi[2] > bound)
+						t_end = plane1;
+					t_begin = __builtin_amdgcn_readfirstlane(t_begin);
+					t_end = __builtin_amdgcn_readfirstlane(t_end);
+				}
+				for (int t0 = t_begin; t0 < t_end; t0 += 64)
 				{
 					const int t = t0 + lane;
-					const bool pass = candidate_within<D, R>(pi, s_rec + t * 4) && t != self_t && t < C;
+					const bool pass = candidate_within<D, R>(pi, s_rec + t * 4) && t != self_t && t < t_end;
 					const unsigned long long mask = __ballot(pass);
 					if (pass)
 					{

@@ -1461,7 +1461,15 @@ private:
 
 	static constexpr int LANES_NEIGHBOR_MIN_AGENTS = 4096;
 
-	int neighbor_blocks() const { return std::max(1, std::min(gn(), 16384)); }
+	// one block per occupied voxel (the kernel strides over more): sized from the occupied
+	// voxels the host saw last, with headroom, so that few blocks start with no voxel
+	int neighbor_blocks() const
+	{
+		int b = std::min(gn(), 16384);
+		if (!grid_n_ && sc_.n_occ > 0)
+			b = std::min(b, sc_.n_occ + sc_.n_occ / 4 + 64);
+		return std::max(1, b);
+	}
 
 	template <int D>
 	int neighbors_d()
