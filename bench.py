@@ -58,8 +58,8 @@ def measured_traffic(tag: str):
         with open(path) as f:
             for line in f:
                 cols = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cols) == 5 and cols[1] == tag:
-                    return float(cols[4])
+                if len(cols) == 6 and cols[1] == tag:
+                    return float(cols[5])
     except OSError:
         return None
     return None
