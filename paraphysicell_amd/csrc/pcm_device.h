@@ -48,7 +48,7 @@ struct Scalars
 	int log_len;           // entries in the removal log (may exceed log_cap: log lost)
 	int compact_limit;     // > 0: the removal loop visits agents below it only
 	int res_open;          // actors the first resolve round left with undecided events
-	int pad1;
+	int ticket;            // blocks of the current launch that have finished (last-block hand-off)
 	// diagnostics of the last step (pcm_get_stats): attachment events, resolver rounds,
 	// resolver chunks, interaction candidates, speculation conflict, removals, occupied voxels
 	int stat[8];

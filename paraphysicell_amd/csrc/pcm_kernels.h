@@ -119,20 +119,12 @@ __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs, const int4* __re
 		c.sc->n_occ = 0;
 }
 
-// RESTORE: also return the container update's remap entries to the identity
-template <class R, int D, bool RESTORE>
+template <class R, int D>
 __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	const int i = GID;
 	const int n = c.sc->n;
-	if (RESTORE)
-	{
-		if (i < c.sc->bad_count)
-			c.remap[c.bad[i]] = c.bad[i];
-		if (i < c.sc->moves)
-			c.remap[c.move_src[i]] = c.move_src[i];
-	}
 	if (i == 0)
 		c.sc->n_binned = n;
 	if (i >= n)
@@ -146,17 +138,20 @@ __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs, const int4* __re
 // per block of CLAIM_THREADS so that a block issues one 64-bit atomic on a packed counter
 // (agents in the high word, occupied voxels in the low word): device-scope atomics on one
 // cache line serialise across the XCDs, so there are as few of them as possible.
-// RESET: also zero the container update's counters (their readers have finished)
-template <class R, bool RESET>
+// RESTORE: also return the container update's remap entries to the identity (the link
+// edits that read them have finished; k_bin_scatter then zeroes the update's counters)
+template <class R, bool RESTORE>
 __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	const int i = GID;
 	const int n = c.sc->n;
-	if (RESET && i == 0)
+	if (RESTORE)
 	{
-		c.sc->bad_count = 0;
-		c.sc->moves = 0;
+		if (i < c.sc->bad_count)
+			c.remap[c.bad[i]] = c.bad[i];
+		if (i < c.sc->moves)
+			c.remap[c.move_src[i]] = c.move_src[i];
 	}
 	// blocks past the agents skip the block-wide scans (block 0 always claims, for n = 0)
 	if (bid_ > 0 && (int)bid_ * CLAIM_THREADS >= n)
@@ -183,7 +178,8 @@ __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __
 	}
 }
 
-template <class R>
+// RESET: also zero the container update's counters (their readers have finished)
+template <class R, bool RESET>
 __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
@@ -195,6 +191,11 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __
 		// the claims have finished
 		c.sc->n_occ = (int)(c.sc->claim & 0xffffffffull);
 		c.sc->claim = 0;
+		if (RESET)
+		{
+			c.sc->bad_count = 0;
+			c.sc->moves = 0;
+		}
 	}
 	c.btmp[c.vstart[c.vox_of[i]] + c.vrank[i]] = i;
 }
@@ -1764,18 +1765,15 @@ __device__ __forceinline__ bool outside_box(const SimCtx<R>& c, const R* p)
 	return out;
 }
 
-// FLAG: also decide, from the agent's final state, whether the container update removes
-// it, and append it to the removal list (the speculation is exact here, so an agent with
-// a removal event targeting it is removed)
+// One agent's events (i = its global thread).  FLAG: also decide, from the agent's final
+// state, whether the container update removes it, and append it to the removal list (the
+// speculation is exact here, so an agent with a removal event targeting it is removed).
+// Every lane of a wavefront calls it; returns whether the lane stored a removal entry.
 template <class R, int D, bool FLAG>
-__global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+__device__ __forceinline__ bool interact_apply_agent(const SimCtx<R>& c, int i)
 {
-	PCM_PROLOGUE(c)
-	if (c.sc->conflict)
-		return;
-	const int i = GID;
 	if ((i & ~63) >= c.sc->n)
-		return; // the whole wavefront is past the agents: nothing to reserve
+		return false; // the whole wavefront is past the agents: nothing to reserve
 	const bool live_thread = i < c.sc->n;
 	bool removed = false;
 	if (live_thread)
@@ -1845,28 +1843,46 @@ __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs, const int4*
 		if (removed)
 			c.bad[k] = i;
 	}
+	return removed;
+}
+
+template <class R, int D, bool FLAG>
+__global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+{
+	PCM_PROLOGUE(c)
+	if (c.sc->conflict)
+		return;
+	interact_apply_agent<R, D, FLAG>(c, GID);
 }
 
 // Exact replay of the reference loop over the candidates, in (actor, slot) order, when the
 // speculative pass found a dependency.  Candidates are ordered CAND_CHUNK at a time.
 template <class R>
-__device__ void compact_plan_block(const SimCtx<R>& c, int* sb);
+__device__ void compact_plan_block(const SimCtx<R>& c, int* sb, int sb_cap, int* s_red, int b, int n0, int limit, int log);
+
+// LDS of the serial pass: candidates ordered per pass, removed agents ordered at once
+template <int CAND, int BADC, bool PLAN>
+struct serial_lds
+{
+	unsigned long long key[CAND];
+	int val[CAND];
+	int count, red;
+	int sb[PLAN ? BADC : 1];
+};
 
 // PLAN: then build the removal list when the replay ran (the apply pass built it otherwise)
-// and plan the container update in the same block.
-template <class R, int D, bool PLAN>
-__global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+// and plan the container update in the same block.  Runs on one block of any size.
+template <class R, int D, bool PLAN, int CAND, int BADC>
+__device__ void interact_serial_block(const SimCtx<R>& c, serial_lds<CAND, BADC, PLAN>& sh)
 {
-	(void)bmap; // one block per simulation: never flattened
-	const SimCtx<R>& c = ctxs[blockIdx.y];
-	__shared__ unsigned long long key[CAND_CHUNK];
-	__shared__ int val[CAND_CHUNK];
-	__shared__ int s_count, s_red;
-	__shared__ int sb[PLAN ? BAD_CHUNK : 1];
-	if (c.sc->status)
+	// every scalar the pass reads, requested together
+	const int status = c.sc->status, cand_count = c.sc->cand_count, conflict = c.sc->conflict;
+	const int bad0 = *(volatile int*)&c.sc->bad_count, n0 = c.sc->n;
+	const int limit = c.sc->compact_limit, log0 = c.sc->log_len;
+	if (status)
 		return;
-	const int C = min(c.sc->cand_count, c.cand_cap);
-	const bool replay = c.sc->conflict != 0;
+	const int C = min(cand_count, c.cand_cap);
+	const bool replay = conflict != 0;
 	if (replay && C > 0)
 	{
 		const int n = c.sc->n;
@@ -1878,32 +1894,40 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCt
 		int lo = 0;
 		while (lo < n)
 		{
-			const int hi = block_chunk_end(c.cand, 2, C, lo, n, CAND_CHUNK, &s_red);
+			const int hi = block_chunk_end(c.cand, 2, C, lo, n, CAND, &sh.red);
 			if (threadIdx.x == 0)
-				s_count = 0;
+				sh.count = 0;
 			__syncthreads();
 			for (int e = threadIdx.x; e < C; e += blockDim.x)
 			{
 				const int ac = c.cand[2 * e];
 				if (ac >= lo && ac < hi)
 				{
-					const int k = atomicAdd(&s_count, 1);
-					if (k < CAND_CHUNK)
+					const int k = atomicAdd(&sh.count, 1);
+					if (k < CAND)
 					{
-						key[k] = ((unsigned long long)ac << 32) | (unsigned)c.cand[2 * e + 1];
-						val[k] = e;
+						sh.key[k] = ((unsigned long long)ac << 32) | (unsigned)c.cand[2 * e + 1];
+						sh.val[k] = e;
 					}
 				}
 			}
 			__syncthreads();
-			const int m = min(s_count, CAND_CHUNK);
-			block_sort_pairs(key, val, m);
+			// one actor's candidates (at most one per list slot) always fit a pass unless the
+			// rows are longer than the pass
+			if (sh.count > CAND)
+			{
+				if (threadIdx.x == 0)
+					atomicOr(&c.sc->status, ST_CAPACITY);
+				return;
+			}
+			const int m = sh.count;
+			block_sort_pairs(sh.key, sh.val, m);
 			if (threadIdx.x == 0)
 			{
 				for (int q = 0; q < m; q++)
 				{
-					const int i = (int)(key[q] >> 32);
-					const int s = (int)(key[q] & 0xffffffffu);
+					const int i = (int)(sh.key[q] >> 32);
+					const int s = (int)(sh.key[q] & 0xffffffffu);
 					if (i != cur)
 					{
 						cur = i;
@@ -1989,7 +2013,7 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCt
 	if (threadIdx.x == 0)
 	{
 		c.sc->stat[3] = C;
-		c.sc->stat[4] = c.sc->conflict;
+		c.sc->stat[4] = conflict;
 		c.sc->conflict = 0;
 		c.sc->cand_count = 0;
 	}
@@ -2016,8 +2040,63 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCt
 		}
 		__threadfence();
 		__syncthreads();
-		compact_plan_block<R>(c, sb);
+		// the replay appends to the removal list: its count is read again
+		const int b = replay ? *(volatile int*)&c.sc->bad_count : bad0;
+		compact_plan_block<R>(c, sh.sb, BADC, &sh.red, b, n0, limit, log0);
 	}
+}
+
+template <class R, int D, bool PLAN>
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_interact_serial(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+{
+	(void)bmap; // one block per simulation: never flattened
+	__shared__ serial_lds<CAND_CHUNK, BAD_CHUNK, PLAN> sh;
+	interact_serial_block<R, D, PLAN, CAND_CHUNK, BAD_CHUNK>(ctxs[blockIdx.y], sh);
+}
+
+// LDS passes of the serial pass when it runs in the last block of the apply launch (a
+// smaller footprint than the stand-alone kernel's, which every apply block would carry)
+constexpr int TAIL_CAND_CHUNK = 1024;
+constexpr int TAIL_BAD_CHUNK = 1024;
+
+// The step's apply pass (FLAG mode), then the serial pass with the container-update plan in
+// the block that finishes last, so that the pair costs one launch.  Every block takes a
+// ticket from the scalars' counter whatever the status word says (the counter returns to
+// zero for the next launch); the block that takes the last ticket resets it and runs the
+// serial pass after all other blocks' work.  Removal-list entries are the only data the
+// serial pass reads that this launch wrote: the wavefronts that wrote one, and the block's
+// ticket taker, publish them with a device-scope fence first.
+template <class R, int D>
+__global__ void __launch_bounds__(BLOCK) k_interact_apply_plan(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+{
+	PCM_GEOMETRY
+	const SimCtx<R>& c = ctxs[slot_];
+	__shared__ serial_lds<TAIL_CAND_CHUNK, TAIL_BAD_CHUNK, true> sh;
+	__shared__ int s_state;
+	if (threadIdx.x == 0)
+		s_state = c.sc->status | c.sc->conflict;
+	__syncthreads();
+	bool wrote = false;
+	if (s_state == 0)
+	{
+		wrote = interact_apply_agent<R, D, true>(c, GID);
+		if (__any(wrote))
+			__threadfence();
+	}
+	const int written = __syncthreads_or(wrote);
+	if (threadIdx.x == 0)
+	{
+		if (written)
+			__threadfence();
+		s_state = atomicAdd(&c.sc->ticket, 1) == nblk_ - 1;
+	}
+	__syncthreads();
+	if (!s_state)
+		return;
+	__threadfence();
+	if (threadIdx.x == 0)
+		c.sc->ticket = 0;
+	interact_serial_block<R, D, true, TAIL_CAND_CHUNK, TAIL_BAD_CHUNK>(c, sh);
 }
 
 // ------------------------------------------------------------------ container compaction
@@ -2081,10 +2160,8 @@ __device__ inline void block_sort_ints(int* a, int count)
 // agent, it is moved into a visited hole; otherwise it stays.  `list` holds the bad agents
 // ascending.  Single thread.
 template <class R>
-__device__ void compact_plan_limited(const SimCtx<R>& c, const int* list, int b, int limit)
+__device__ void compact_plan_limited(const SimCtx<R>& c, const int* list, int b, int limit, int n0, int log)
 {
-	const int n0 = c.sc->n;
-	int log = c.sc->log_len;
 	int top = n0, tb = b - 1, q = 0, moves = 0, removed = 0;
 	long long deaths = 0;
 	for (int h = 0; h < b; h++)
@@ -2126,20 +2203,22 @@ __device__ void compact_plan_limited(const SimCtx<R>& c, const int* list, int b,
 	c.sc->stat[5] = removed;
 	c.sc->stat[6] = c.sc->n_occ;
 	c.sc->n = top;
-	c.sc->deaths += deaths;
+	atomicAdd((unsigned long long*)&c.sc->deaths, (unsigned long long)deaths);
 }
 
+// The container update's plan on one block.  b, n0, limit and log are the removal count,
+// agent count, compact_limit and log_len as the caller read them (read together, so the
+// plan's chain of dependent loads is short); s_red is an LDS int.
 template <class R>
-__device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
+__device__ void compact_plan_block(const SimCtx<R>& c, int* sb, int sb_cap, int* s_red, int b, int n0, int limit, int log)
 {
-	// the removal list may have been filled earlier in the same kernel: read past the L1
-	const int b = *(volatile int*)&c.sc->bad_count;
-	const volatile int* bad_list = c.bad;
 	if (b == 0)
 		return;
 	// ascending list of removed agents: in LDS when it fits, else sorted in place in HBM
+	// (the list may have been filled earlier in the same kernel: read past the L1)
+	const volatile int* bad_list = c.bad;
 	int* list = sb;
-	if (b <= BAD_CHUNK)
+	if (b <= sb_cap)
 	{
 		for (int t = threadIdx.x; t < b; t += blockDim.x)
 			sb[t] = bad_list[t];
@@ -2153,26 +2232,32 @@ __device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
 		list = c.bad;
 	}
 	__syncthreads();
-	if (threadIdx.x != 0)
-		return;
-	const int n0 = c.sc->n;
-	const int limit = c.sc->compact_limit;
 	if (limit != 0)
 	{
-		compact_plan_limited<R>(c, list, b, limit < 0 ? 0 : limit);
+		if (threadIdx.x == 0)
+			compact_plan_limited<R>(c, list, b, limit < 0 ? 0 : limit, n0, log);
 		return;
 	}
+	// removed agents flagged for removal are deaths: their flags are read in parallel
+	if (threadIdx.x == 0)
+		*s_red = 0;
+	__syncthreads();
+	int d = 0;
+	for (int t = threadIdx.x; t < b; t += blockDim.x)
+		d += c.flags[list[t]] == PCM_FLAG_TO_REMOVE;
+	if (d)
+		atomicAdd(s_red, d);
+	__syncthreads();
+	if (threadIdx.x != 0)
+		return;
+	const long long deaths = *s_red;
 	const int n1 = n0 - b;
-	long long deaths = 0;
-	for (int t = 0; t < b; t++)
-		deaths += c.flags[list[t]] == PCM_FLAG_TO_REMOVE;
 	int holes = 0;
 	while (holes < b && list[holes] < n1)
 		holes++;
 	// walk the tail [n1, n0) downwards: good agents fill holes, bad ones get their
 	// position in the removal sequence
 	int q = 0;
-	int log = c.sc->log_len;
 	auto log_removal = [&](int position) {
 		if (log < c.log_cap)
 			c.rm_log[log] = position;
@@ -2225,7 +2310,7 @@ __device__ void compact_plan_block(const SimCtx<R>& c, int* sb)
 	c.sc->stat[5] = b;
 	c.sc->stat[6] = c.sc->n_occ;
 	c.sc->n = n1;
-	c.sc->deaths += deaths;
+	atomicAdd((unsigned long long*)&c.sc->deaths, (unsigned long long)deaths);
 }
 
 template <class R>
@@ -2234,9 +2319,12 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_compact_plan(const SimCtx<R
 	(void)bmap; // one block per simulation: never flattened
 	const SimCtx<R>& c = ctxs[blockIdx.y];
 	__shared__ int sb[BAD_CHUNK];
-	if (c.sc->status)
+	__shared__ int s_red;
+	const int status = c.sc->status, b = *(volatile int*)&c.sc->bad_count, n0 = c.sc->n;
+	const int limit = c.sc->compact_limit, log = c.sc->log_len;
+	if (status)
 		return;
-	compact_plan_block<R>(c, sb);
+	compact_plan_block<R>(c, sb, BAD_CHUNK, &s_red, b, n0, limit, log);
 }
 
 // remove_attached (swap with back) for removed partners in removal order, then
@@ -2313,6 +2401,64 @@ __device__ inline void edit_links(int* list, int* cnt_ptr, const int* remap)
 // the other blocks edit the link lists of the survivors that stay in place.
 constexpr int MOVE_BLOCKS = 16;
 
+// one surviving tail agent's move into its hole, by one wavefront
+template <class R>
+__device__ __forceinline__ void compact_move(const SimCtx<R>& c, int src, int dst, int lane)
+{
+	// the copy plan's words (at most MOVE_UNITS per lane): every lane reads its plan
+	// entries and source words first, then stores, so the words cost one round trip
+	// instead of one per 64 (the stores could alias the plan for the compiler)
+	constexpr int MOVE_UNITS = (int)(sizeof(c.mv_field) / sizeof(c.mv_field[0]) + 63) / 64;
+	uint8_t* dst_at[MOVE_UNITS];
+	uint32_t word[MOVE_UNITS];
+	int bytes[MOVE_UNITS];
+#pragma unroll
+	for (int r = 0; r < MOVE_UNITS; r++)
+	{
+		const int u = lane + 64 * r;
+		bytes[r] = 0;
+		if (u < c.mv_n)
+		{
+			const int f = c.mv_field[u];
+			const size_t nb = (size_t)c.fbytes[f];
+			bytes[r] = c.mv_bytes[u];
+			const size_t off = (size_t)c.mv_unit[u] * bytes[r];
+			const uint8_t* base = (const uint8_t*)c.fptr[f];
+			dst_at[r] = (uint8_t*)base + (size_t)dst * nb + off;
+			word[r] = bytes[r] == 4 ? *(const uint32_t*)(base + (size_t)src * nb + off)
+									: (uint32_t)base[(size_t)src * nb + off];
+		}
+	}
+#pragma unroll
+	for (int r = 0; r < MOVE_UNITS; r++)
+	{
+		if (bytes[r] == 4)
+			*(uint32_t*)dst_at[r] = word[r];
+		else if (bytes[r] == 1)
+			*dst_at[r] = (uint8_t)word[r];
+	}
+	// list rows: lanes over the entries (a serial copy costs one round trip per entry)
+	for (int t = lane; t < c.K; t += 64)
+		c.nbr_idx[(size_t)dst * c.K + t] = c.nbr_idx[(size_t)src * c.K + t];
+	for (int t = lane; t < c.L; t += 64)
+		c.spr_idx[(size_t)dst * c.L + t] = c.spr_idx[(size_t)src * c.L + t];
+	for (int t = lane; t < c.LA; t += 64)
+		c.atc_idx[(size_t)dst * c.LA + t] = c.atc_idx[(size_t)src * c.LA + t];
+	if (lane == 0)
+	{
+		c.nbr_cnt[dst] = c.nbr_cnt[src];
+		c.spr_cnt[dst] = c.spr_cnt[src];
+		c.atc_cnt[dst] = c.atc_cnt[src];
+	}
+	// the lane-0 edits below read rows the other lanes wrote
+	__threadfence_block();
+	if (lane == 0)
+	{
+		edit_links(c.spr_idx + (size_t)dst * c.L, c.spr_cnt + dst, c.remap);
+		edit_links(c.atc_idx + (size_t)dst * c.LA, c.atc_cnt + dst, c.remap);
+	}
+}
+
 template <class R>
 __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
@@ -2321,65 +2467,10 @@ __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* 
 		return;
 	if (bid_ < MOVE_BLOCKS)
 	{
-		const int lane = lane_id();
 		const int waves = MOVE_BLOCKS * (blockDim.x >> 6);
 		const int moves = c.sc->moves;
 		for (int k = bid_ * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
-		{
-			const int src = c.move_src[k], dst = c.move_dst[k];
-			// the copy plan's words (at most MOVE_UNITS per lane): every lane reads its plan
-			// entries and source words first, then stores, so the words cost one round trip
-			// instead of one per 64 (the stores could alias the plan for the compiler)
-			constexpr int MOVE_UNITS = (int)(sizeof(c.mv_field) / sizeof(c.mv_field[0]) + 63) / 64;
-			uint8_t* dst_at[MOVE_UNITS];
-			uint32_t word[MOVE_UNITS];
-			int bytes[MOVE_UNITS];
-#pragma unroll
-			for (int r = 0; r < MOVE_UNITS; r++)
-			{
-				const int u = lane + 64 * r;
-				bytes[r] = 0;
-				if (u < c.mv_n)
-				{
-					const int f = c.mv_field[u];
-					const size_t nb = (size_t)c.fbytes[f];
-					bytes[r] = c.mv_bytes[u];
-					const size_t off = (size_t)c.mv_unit[u] * bytes[r];
-					const uint8_t* base = (const uint8_t*)c.fptr[f];
-					dst_at[r] = (uint8_t*)base + (size_t)dst * nb + off;
-					word[r] = bytes[r] == 4 ? *(const uint32_t*)(base + (size_t)src * nb + off)
-											: (uint32_t)base[(size_t)src * nb + off];
-				}
-			}
-#pragma unroll
-			for (int r = 0; r < MOVE_UNITS; r++)
-			{
-				if (bytes[r] == 4)
-					*(uint32_t*)dst_at[r] = word[r];
-				else if (bytes[r] == 1)
-					*dst_at[r] = (uint8_t)word[r];
-			}
-			// list rows: lanes over the entries (a serial copy costs one round trip per entry)
-			for (int t = lane; t < c.K; t += 64)
-				c.nbr_idx[(size_t)dst * c.K + t] = c.nbr_idx[(size_t)src * c.K + t];
-			for (int t = lane; t < c.L; t += 64)
-				c.spr_idx[(size_t)dst * c.L + t] = c.spr_idx[(size_t)src * c.L + t];
-			for (int t = lane; t < c.LA; t += 64)
-				c.atc_idx[(size_t)dst * c.LA + t] = c.atc_idx[(size_t)src * c.LA + t];
-			if (lane == 0)
-			{
-				c.nbr_cnt[dst] = c.nbr_cnt[src];
-				c.spr_cnt[dst] = c.spr_cnt[src];
-				c.atc_cnt[dst] = c.atc_cnt[src];
-			}
-			// the lane-0 edits below read rows the other lanes wrote
-			__threadfence_block();
-			if (lane == 0)
-			{
-				edit_links(c.spr_idx + (size_t)dst * c.L, c.spr_cnt + dst, c.remap);
-				edit_links(c.atc_idx + (size_t)dst * c.LA, c.atc_cnt + dst, c.remap);
-			}
-		}
+			compact_move<R>(c, c.move_src[k], c.move_dst[k], lane_id());
 		return;
 	}
 	const int i = (bid_ - MOVE_BLOCKS) * blockDim.x + threadIdx.x;
@@ -2387,6 +2478,54 @@ __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* 
 		return;
 	edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
 	edit_links(c.atc_idx + (size_t)i * c.LA, c.atc_cnt + i, c.remap);
+}
+
+// The container update's data movement and the next mesh update's voxel histogram in one
+// launch.  Each agent of the compacted population is counted into the voxel of its final
+// position: an agent that stays by its own thread, a moved one by the wavefront that moves
+// it (with the position it moves), so the count never reads a slot a mover writes.  The
+// remap entries the link edits read are returned to the identity by the claim kernel.
+template <class R, int D>
+__global__ void k_compact_count(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+{
+	PCM_PROLOGUE(c)
+	const bool compacting = c.sc->bad_count != 0;
+	const int n = c.sc->n;
+	if (bid_ < MOVE_BLOCKS)
+	{
+		if (!compacting)
+			return;
+		const int waves = MOVE_BLOCKS * (blockDim.x >> 6);
+		const int moves = c.sc->moves;
+		const int lane = lane_id();
+		for (int k = bid_ * (blockDim.x >> 6) + (threadIdx.x >> 6); k < moves; k += waves)
+		{
+			const int src = c.move_src[k], dst = c.move_dst[k];
+			compact_move<R>(c, src, dst, lane);
+			if (lane == 0)
+			{
+				const int v = voxel_index<R>(c, c.pos + (size_t)src * D);
+				c.vox_of[dst] = v;
+				c.vrank[dst] = atomicAdd(&c.vcnt[v], 1);
+			}
+		}
+		return;
+	}
+	const int i = (bid_ - MOVE_BLOCKS) * blockDim.x + threadIdx.x;
+	if (i == 0)
+		c.sc->n_binned = n;
+	if (i >= n)
+		return;
+	if (compacting)
+	{
+		if (c.remap[i] < 0)
+			return; // a hole: its mover counts it
+		edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
+		edit_links(c.atc_idx + (size_t)i * c.LA, c.atc_cnt + i, c.remap);
+	}
+	const int v = voxel_index<R>(c, c.pos + (size_t)i * D);
+	c.vox_of[i] = v;
+	c.vrank[i] = atomicAdd(&c.vcnt[v], 1);
 }
 
 template <class R>

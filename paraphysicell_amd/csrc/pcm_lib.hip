@@ -1509,8 +1509,8 @@ private:
 		}
 	}
 
-	// FUSED: the clear ran in this step's velocity kernel and the container update's
-	// remap restore and counter reset ride on the count and claim kernels
+	// FUSED: the histogram ran in the step's compaction kernel (after the clear), and the
+	// container update's remap restore and counter reset ride on the claim and scatter kernels
 	template <int D, bool FUSED = false>
 	int mesh_kernels()
 	{
@@ -1518,11 +1518,13 @@ private:
 		records_fresh_ = true;
 		binned_launch_ = n_launch_;
 		if (!FUSED)
+		{
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
-		PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D, FUSED>));
+			PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D>));
+		}
 		PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
 					   CLAIM_THREADS, MAP_CLAIM));
-		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R>));
+		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R, FUSED>));
 		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 	}
 
@@ -1673,11 +1675,11 @@ private:
 		else
 			PCM_TRY((launch_velocity<D, false>()));
 		PCM_TRY((launch_tentative<D, true>()));
-		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply<R, D, true>));
-		PCM_TRY(launch(PCM_K_INTERACT_SERIAL, k_interact_serial<R, D, true>, 1, RESOLVE_THREADS));
-		PCM_TRY(compact_apply_kernel());
+		PCM_TRY(per_agent(PCM_K_INTERACT_APPLY, k_interact_apply_plan<R, D>));
 		if (!fold_clear)
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
+		// the container update's moves and the mesh update's histogram
+		PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D>, MOVE_BLOCKS + grid_agents(), BLOCK, MAP_COMPACT));
 		PCM_TRY((mesh_kernels<D, true>()));
 		return neighbors_d<D>();
 	}
@@ -2401,10 +2403,10 @@ int noalloc_build(const SimCtx<R>& proto, unsigned char* area, int* needed, hipS
 	hipLaunchKernelGGL(k_noalloc_setup<R>, dim3(clear_blocks), dim3(BLOCK), 0, stream, h, d_ctx, needed);
 	PCM_HIP(hipGetLastError());
 	const SimCtx<R>* ctx = d_ctx;
-	hipLaunchKernelGGL((k_bin_count<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
+	hipLaunchKernelGGL((k_bin_count<R, D>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL((k_bin_claim<R, false>), dim3(std::max(1, (agents + CLAIM_THREADS - 1) / CLAIM_THREADS)),
 					   dim3(CLAIM_THREADS), 0, stream, ctx, (const int4*)nullptr);
-	hipLaunchKernelGGL(k_bin_scatter<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
+	hipLaunchKernelGGL((k_bin_scatter<R, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL((k_bin_order<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL((k_neighbors<R, D, 2>), dim3(std::max(1, std::min(agents, 16384))), dim3(128), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL(k_noalloc_finish<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx, needed);
