@@ -252,7 +252,10 @@ __global__ void k_force_records(const SimCtx<R>* __restrict__ ctxs, const int4* 
 template <class R>
 constexpr int nbr_tile()
 {
-	return sizeof(R) == 4 ? 448 : 320; // candidates staged per wavefront (a multiple of 64)
+	// candidates staged per block (a multiple of 64): at tissue densities a 3D stencil holds
+	// about 240 candidates and up to ~370 (a 30 um voxel, radii up to 11 um); larger ones take
+	// the per-lane path from HBM
+	return 448;
 }
 
 // a coordinate whose square overflows: no staged padding record is within reach
@@ -407,7 +410,12 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 			int cnt = 0, vs = 0;
 			if (lane < 27)
 			{
-				const int x = co[0] + lane % 3 - 1, y = co[1] + (lane / 3) % 3 - 1, z = co[2] + lane / 9 - 1;
+				// the lane's stencil offsets are recomputed per voxel (the empty asm hides that
+				// they are loop-invariant): kept live across the voxel loop they cost a register
+				// spill per wavefront
+				int l = lane;
+				asm volatile("" : "+v"(l));
+				const int x = co[0] + l % 3 - 1, y = co[1] + (l / 3) % 3 - 1, z = co[2] + l / 9 - 1;
 				if (x >= 0 && x < c.grid[0] && y >= 0 && y < c.grid[1] && z >= 0 && z < c.grid[2])
 				{
 					const int nv = x + c.grid[0] * (y + c.grid[1] * z);
