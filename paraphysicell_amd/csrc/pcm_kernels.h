@@ -108,6 +108,26 @@ __global__ void k_gather_scalars(const SimCtx<R>* __restrict__ ctxs, Scalars* __
 // agent order; the device reproduces that order exactly: histogram -> bucket allocation ->
 // unordered scatter -> rank by "number of bucket members with a larger index".
 
+// One agent's detachment draws for the spring update keyed by `step` (one per list slot)
+// and its cleared erase mask.  A batch launches the spring kernels when any member has
+// springs on; a member with them off fires nothing and draws nothing (its erase pass then
+// only stores the step counter).
+template <class R>
+__device__ __forceinline__ void spring_fire_agent(const SimCtx<R>& c, int i, unsigned long long step)
+{
+	const int cnt = c.springs ? c.spr_cnt[i] : 0;
+	const R rate = c.det_rate[i] * c.dt;
+	uint32_t mask = 0;
+	for (int s = 0; s < cnt; s++)
+	{
+		const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_DETACH, (uint32_t)i, (uint32_t)s, 0u);
+		if (u <= rate)
+			mask |= 1u << s;
+	}
+	c.fire[i] = mask;
+	c.erase[i] = 0;
+}
+
 template <class R>
 __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
@@ -178,12 +198,19 @@ __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __
 	}
 }
 
-// RESET: also zero the container update's counters (their readers have finished)
-template <class R, bool RESET>
+// IN_STEP: the mesh update of a fused step.  It also zeroes the container update's
+// counters (their readers have finished) and does the next step's k_spring_fire work:
+// spring lists, counts, detachment rates and agent indices are final for this step once
+// the compaction has run, and the next step's draws are keyed by the stored step counter
+// plus one, as k_spring_fire<true> keys them at the next step's start.  The host skips
+// that step's fire launch unless something changed the agents in between.
+template <class R, bool IN_STEP>
 __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	const int i = GID;
+	if (IN_STEP && i == 0)
+		c.sc->res_open = 0;
 	if (i >= c.sc->n)
 		return;
 	if (i == 0)
@@ -191,13 +218,15 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __
 		// the claims have finished
 		c.sc->n_occ = (int)(c.sc->claim & 0xffffffffull);
 		c.sc->claim = 0;
-		if (RESET)
+		if (IN_STEP)
 		{
 			c.sc->bad_count = 0;
 			c.sc->moves = 0;
 		}
 	}
 	c.btmp[c.vstart[c.vox_of[i]] + c.vrank[i]] = i;
+	if (IN_STEP)
+		spring_fire_agent<R>(c, i, c.sc->step + 1);
 }
 
 // RECORDS: also write the agents' force records for the step's gather kernels (a mesh
@@ -1130,20 +1159,7 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs, const int4* __
 		c.sc->res_open = 0; // the resolver's open-actor list starts empty every update
 	if (i >= c.sc->n)
 		return;
-	// a batch launches the spring kernels when any member has springs on; a member with them
-	// off fires nothing and draws nothing (its erase pass then only stores the step counter)
-	const int cnt = c.springs ? c.spr_cnt[i] : 0;
-	const R rate = c.det_rate[i] * c.dt;
-	const unsigned long long step = c.sc->step + (STARTS_STEP ? 1 : 0);
-	uint32_t mask = 0;
-	for (int s = 0; s < cnt; s++)
-	{
-		const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_DETACH, (uint32_t)i, (uint32_t)s, 0u);
-		if (u <= rate)
-			mask |= 1u << s;
-	}
-	c.fire[i] = mask;
-	c.erase[i] = 0;
+	spring_fire_agent<R>(c, i, c.sc->step + (STARTS_STEP ? 1 : 0));
 }
 
 template <class R, bool STARTS_STEP>

@@ -460,6 +460,7 @@ public:
 		n_launch_ = n;
 		mesh_fresh_ = false;
 		records_fresh_ = false;
+		fire_ready_ = false;
 		if ((int)origin_ident_.size() < n)
 		{
 			const int from = (int)origin_ident_.size();
@@ -589,6 +590,7 @@ public:
 		if (f == PCM_F_POSITIONS || f == PCM_F_RADIUS || f == PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE)
 			mesh_fresh_ = false;
 		records_fresh_ = false;
+		fire_ready_ = false;
 		return PCM_OK;
 	}
 
@@ -603,6 +605,7 @@ public:
 
 	int set_links(int which, const int32_t* off, const int32_t* idx) override
 	{
+		fire_ready_ = false;
 		if (which != PCM_LINKS_SPRINGS && which != PCM_LINKS_ATTACHED)
 			return fail(PCM_ERR_ARGUMENT, "bad link list id");
 		if (!off)
@@ -719,6 +722,7 @@ public:
 	// that list it, in ascending order.
 	int set_neighbors(const int32_t* off, const int32_t* idx) override
 	{
+		fire_ready_ = false;
 		if (!off || !idx)
 			return fail(PCM_ERR_ARGUMENT, "null neighbor lists");
 		PCM_TRY(read_scalars());
@@ -889,6 +893,7 @@ public:
 		n_launch_ = (int)total;
 		mesh_fresh_ = false;
 		records_fresh_ = false;
+		fire_ready_ = false;
 		const int blocks = std::max(1, (sc_.div_count + BLOCK - 1) / BLOCK);
 		switch (h_.dims)
 		{
@@ -918,6 +923,7 @@ public:
 
 	int set_step(uint64_t s) override
 	{
+		fire_ready_ = false;
 		PCM_TRY(read_scalars());
 		sc_.step = s;
 		PCM_TRY(copy_sync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
@@ -933,6 +939,7 @@ public:
 
 	int set_gradients(const int32_t* voxel, const void* g) override
 	{
+		fire_ready_ = false;
 		int grid[3];
 		size_t voxels = 1;
 		for (int k = 0; k < 3; k++)
@@ -975,6 +982,7 @@ public:
 	int phase(int which, const pcm_step_params* p) override
 	{
 		PCM_TRY(ready());
+		fire_ready_ = false;
 		if (p)
 			PCM_TRY(set_params(*p));
 		switch (h_.dims)
@@ -1100,6 +1108,10 @@ private:
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
 	bool records_fresh_ = false; // force records describe the current agent fields
+	// the last work enqueued was a fused step, whose mesh update drew the next step's
+	// detachment masks, and nothing has changed the agents, their lists or the step
+	// parameters since
+	bool fire_ready_ = false;
 	int binned_launch_ = 1 << 30; // launch bound of the last mesh update (>= agents it binned)
 
 	int width(int kind) const
@@ -1292,6 +1304,7 @@ private:
 		const int springs = p.automated_spring_adhesion != 0;
 		if (dt == h_.dt && wall == h_.wall && springs == h_.springs)
 			return PCM_OK;
+		fire_ready_ = false; // the masks were drawn against the old rate * dt
 		h_.dt = dt;
 		h_.wall = wall;
 		h_.springs = springs;
@@ -1320,6 +1333,7 @@ private:
 	// neighbor overflow: grow K, rebuild the lists of the aborted step, clear the status
 	int recover()
 	{
+		fire_ready_ = false;
 		if (sc_.status & ~ST_NEIGHBOR_OVERFLOW)
 			return fail(PCM_ERR_STATE, "device status word is set: " + std::to_string(sc_.status));
 		PCM_TRY(grow_neighbors(sc_.max_neighbors));
@@ -1559,10 +1573,12 @@ private:
 	// IN_STEP: the kernels open a fused mechanics step, so they also advance the step
 	// counter (the detachment draws use the new value; the erase kernel stores it), and the
 	// step's velocity kernel appends the accepted attachments instead of k_spring_append
+	// fire_ready: the masks were drawn by the previous fused step's mesh update
 	template <bool IN_STEP>
-	int spring_update_kernels()
+	int spring_update_kernels(bool fire_ready = false)
 	{
-		PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R, IN_STEP>));
+		if (!fire_ready)
+			PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R, IN_STEP>));
 		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R, IN_STEP>,
 					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK, MAP_ERASE));
 		// the first resolve round runs one block per collect block (segments of BLOCK / G)
@@ -1659,11 +1675,12 @@ private:
 	// one mechanics step (simulator.cpp:173-192): the spring-list update runs first since
 	// forces, motility and the membrane do not read springs; velocity contributions are
 	// then summed in the reference's order by the fused velocity kernel.
+	// fire_ready: the previous fused step's mesh update drew this step's detachment masks
 	template <int D>
-	int step_d(bool records_fresh, bool fold_clear)
+	int step_d(bool records_fresh, bool fold_clear, bool fire_ready)
 	{
 		if (h_.springs)
-			PCM_TRY(spring_update_kernels<true>());
+			PCM_TRY(spring_update_kernels<true>(fire_ready));
 		else
 			PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 		if (!records_fresh)
@@ -1684,18 +1701,25 @@ private:
 		return neighbors_d<D>();
 	}
 
-	int dispatch_step() { return dispatch_step_flags(records_fresh_, binned_launch_ <= n_launch_); }
+	int dispatch_step()
+	{
+		const bool ready = fire_ready_;
+		fire_ready_ = false;
+		PCM_TRY(dispatch_step_flags(records_fresh_, binned_launch_ <= n_launch_, ready));
+		fire_ready_ = true;
+		return PCM_OK;
+	}
 
-	int dispatch_step_flags(bool records_fresh, bool fold_clear)
+	int dispatch_step_flags(bool records_fresh, bool fold_clear, bool fire_ready)
 	{
 		switch (h_.dims)
 		{
 			case 1:
-				return step_d<1>(records_fresh, fold_clear);
+				return step_d<1>(records_fresh, fold_clear, fire_ready);
 			case 2:
-				return step_d<2>(records_fresh, fold_clear);
+				return step_d<2>(records_fresh, fold_clear, fire_ready);
 			default:
-				return step_d<3>(records_fresh, fold_clear);
+				return step_d<3>(records_fresh, fold_clear, fire_ready);
 		}
 	}
 
@@ -1709,6 +1733,7 @@ public:
 	int launch_bound() const { return n_launch_; }
 	int spring_slots() const { return h_.L; }
 	bool records_fresh() const { return records_fresh_; }
+	bool fire_ready() const { return fire_ready_; }
 	bool binned_within(int grid_n) const { return binned_launch_ <= grid_n; }
 	void*& batch() { return batch_; }
 	int apply_params(const pcm_step_params& p) { return set_params(p); }
@@ -1797,7 +1822,8 @@ public:
 	// maps: the batch's block maps (device, offsets and padded totals per map kind) for
 	// flattened launches, or nullptr for one block row per member
 	int run_segmented(const SimCtx<R>* ctxs, int m, int grid_n, int grid_L, bool records_fresh, bool fold_clear,
-					  int steps, const int4* maps = nullptr, const int* offs = nullptr, const int* totals = nullptr)
+					  bool fire_ready, int steps, const int4* maps = nullptr, const int* offs = nullptr,
+					  const int* totals = nullptr)
 	{
 		launch_ctx_ = ctxs;
 		launch_y_ = m;
@@ -1812,9 +1838,10 @@ public:
 		int rc = PCM_OK;
 		for (int k = 0; k < steps && rc == PCM_OK; k++)
 		{
-			rc = dispatch_step_flags(records_fresh, fold_clear);
+			rc = dispatch_step_flags(records_fresh, fold_clear, fire_ready);
 			records_fresh = true;
 			fold_clear = true;
+			fire_ready = true;
 		}
 		launch_ctx_ = nullptr;
 		launch_y_ = 1;
@@ -1829,6 +1856,7 @@ public:
 	{
 		mesh_fresh_ = true;
 		records_fresh_ = true;
+		fire_ready_ = true;
 		binned_launch_ = n_launch_;
 	}
 
@@ -2183,7 +2211,7 @@ private:
 		const int m = (int)g.slots.size();
 		PCM_TRY(reserve(g, m));
 		int grid_n = 1, grid_L = 1;
-		bool records_fresh = true;
+		bool records_fresh = true, fire_ready = true;
 		// the leader launches the spring kernels when any member has springs on (members with
 		// them off skip the work inside those kernels)
 		int any_springs = 0;
@@ -2204,6 +2232,7 @@ private:
 				grid_n = std::max(grid_n, slot.sim->launch_bound());
 				grid_L = std::max(grid_L, slot.sim->spring_slots());
 				records_fresh = records_fresh && slot.sim->records_fresh();
+				fire_ready = fire_ready && slot.sim->fire_ready();
 			}
 		grid_n = round_up_grid(grid_n);
 		bool fold_clear = true;
@@ -2229,18 +2258,20 @@ private:
 			PCM_TRY(build_maps(g, m));
 			maps = g.d_map;
 		}
-		auto run = [&](bool fresh, bool clear, int k) {
-			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, fresh, clear, k, maps, g.map_off, g.map_tot);
+		auto run = [&](bool fresh, bool clear, bool fire, int k) {
+			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, fresh, clear, fire, k, maps, g.map_off,
+										 g.map_tot);
 		};
 		int done = 0;
-		if (!records_fresh || !fold_clear)
+		if (!records_fresh || !fold_clear || !fire_ready)
 		{
-			// the first step refreshes records or clears counts for some member: no graph
-			PCM_TRY(run(records_fresh, fold_clear, 1));
+			// the first step refreshes records, clears counts or draws the detachment masks for
+			// some member: no graph
+			PCM_TRY(run(records_fresh, fold_clear, fire_ready, 1));
 			done = 1;
 		}
 		if (!graphs_)
-			return run(true, true, steps - done);
+			return run(true, true, true, steps - done);
 		const pcm_tuning t = leader->tuning();
 		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs, maps ? g.map_version : 0,
 								t.velocity_lanes, t.event_lanes, t.neighbor_waves };
@@ -2255,10 +2286,10 @@ private:
 			// launches instead of waiting for them; a later call captures
 			std::unique_lock<std::shared_mutex> no_other_calls(g_capture_mu, std::try_to_lock);
 			if (!no_other_calls.owns_lock())
-				return run(true, true, steps - done);
+				return run(true, true, true, steps - done);
 			hipGraph_t graph = nullptr;
 			PCM_HIP(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-			const int rc = run(true, true, 1);
+			const int rc = run(true, true, true, 1);
 			const hipError_t end = hipStreamEndCapture(g.stream, &graph);
 			no_other_calls.unlock();
 			if (end != hipSuccess)
@@ -2268,7 +2299,7 @@ private:
 				(void)hipGetLastError();
 				if (graph)
 					(void)hipGraphDestroy(graph);
-				return run(true, true, steps - done);
+				return run(true, true, true, steps - done);
 			}
 			PCM_TRY(rc);
 			const hipError_t inst = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);

@@ -3,6 +3,8 @@
 Bar: voxel buckets and neighbor lists bit-exact; integer/byte fields and link lists
 bit-exact; real fields within 1e-5 (FP32) / 1e-12 (FP64) relative to max(|oracle|, 1).
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -171,6 +173,26 @@ def test_fused_step(dims, real_bytes):
         # the busy rates leave attachment events undecided after the first resolver round,
         # so the later rounds (the single-block resolver) are covered too
         assert max(rounds) >= 2, rounds
+
+
+@pytest.mark.parametrize("real_bytes", [4, 8])
+def test_consecutive_fused_steps(real_bytes):
+    """Fused steps back to back, with no upload between them, against the oracle: from the
+    second step on, the detachment masks come from the previous step's mesh update instead
+    of their own launch; a parameter change in between brings that launch back."""
+    mesh, st, p = parity.rich_population(3, 1500, 31, real_bytes, rates="busy")
+    dev, ora = parity.make_pair(mesh, st, seed=4)
+    for s in (dev, ora):
+        s.initialize_mesh()
+    dev.step(p, 3)
+    ora.step(p, 3)
+    parity.assert_same_state(dev, ora, what="three fused steps")
+    parity.assert_same_neighbors(dev, ora, "three fused steps")
+    q = dataclasses.replace(p, dt=0.5 * p.dt)
+    dev.step(q, 2)
+    ora.step(q, 2)
+    parity.assert_same_state(dev, ora, what="two more steps at half the time step")
+    parity.assert_same_neighbors(dev, ora, "two more steps at half the time step")
 
 
 @pytest.mark.parametrize("dims", [2, 3])
