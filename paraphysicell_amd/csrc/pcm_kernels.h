@@ -128,6 +128,85 @@ __device__ __forceinline__ void spring_fire_agent(const SimCtx<R>& c, int i, uns
 	c.erase[i] = 0;
 }
 
+// Whether agent i's non-fired slot s (partner p, fired masks fi and fp, p's list lp of
+// length cp) is erased by the partner's firing.
+template <class R>
+__device__ __forceinline__ bool erased_by_partner(const int* li, int cnt, int s, int i, int p, uint32_t fi, const int* lp,
+												  int cp, uint32_t fp)
+{
+	if (i < p)
+	{
+		int f_i = 0, nf_rank = 0;
+		for (int u = 0; u < cnt; u++)
+			if (li[u] == p)
+			{
+				const bool fu = (fi >> u) & 1u;
+				f_i += fu;
+				if (!fu && u < s)
+					nf_rank++;
+			}
+		int g = 0, occ = 0;
+		for (int u = 0; u < cp; u++)
+			if (lp[u] == i)
+			{
+				occ++;
+				if (occ > f_i && ((fp >> u) & 1u))
+					g++;
+			}
+		return nf_rank < g;
+	}
+	int rank = 0;
+	for (int u = 0; u < s; u++)
+		rank += li[u] == p;
+	int f_p = 0;
+	for (int u = 0; u < cp; u++)
+		f_p += lp[u] == i && ((fp >> u) & 1u);
+	return rank < f_p;
+}
+
+// k_spring_erase's decisions for agent i's slots lane, lane + ERASE_LANES, ..., as the
+// fused step's mesh update makes them for the next step (after the fired slots are drawn):
+// the lane's partners, then their fired masks, are requested together; the list rules run
+// only for the rare slots whose partner fired
+constexpr int ERASE_LANES = 4;
+
+template <class R>
+__device__ __forceinline__ void spring_erase_lanes(const SimCtx<R>& c, int i, int lane, int n)
+{
+	constexpr int U = 32 / ERASE_LANES; // slots per lane at the largest list width
+	const int cnt = c.springs ? c.spr_cnt[i] : 0;
+	if (lane >= cnt)
+		return; // the erase mask was cleared with the fired masks
+	const int* li = c.spr_idx + (size_t)i * c.L;
+	const uint32_t fi = c.fire[i];
+	int ps[U];
+	uint32_t fps[U];
+#pragma unroll
+	for (int u = 0; u < U; u++)
+	{
+		const int s = lane + u * ERASE_LANES;
+		ps[u] = s < cnt ? li[s] : -1;
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++)
+		fps[u] = ps[u] >= 0 && ps[u] < n ? c.fire[ps[u]] : 0u;
+	uint32_t erase = 0;
+#pragma unroll
+	for (int u = 0; u < U; u++)
+	{
+		const int s = lane + u * ERASE_LANES, p = ps[u];
+		if (s >= cnt || p < 0 || p >= n)
+			continue;
+		if ((fi >> s) & 1u)
+			erase |= 1u << s;
+		else if (fps[u] != 0
+				 && erased_by_partner<R>(li, cnt, s, i, p, fi, c.spr_idx + (size_t)p * c.L, c.spr_cnt[p], fps[u]))
+			erase |= 1u << s;
+	}
+	if (erase)
+		atomicOr(&c.erase[i], erase);
+}
+
 template <class R>
 __global__ void k_bin_clear(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
@@ -230,13 +309,30 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __
 }
 
 // RECORDS: also write the agents' force records for the step's gather kernels (a mesh
-// update for the neighbor lists alone has no force-record buffer)
-template <class R, int D, bool RECORDS = true>
+// update for the neighbor lists alone has no force-record buffer).  DETACH: the mesh update
+// of a fused step also decides the next step's erased spring slots (k_spring_erase's work;
+// k_bin_scatter drew the fired slots)
+// With DETACH the launch has the member's ceil(n / BLOCK) ordering blocks followed by its
+// erase blocks (ERASE_LANES lanes per agent).
+template <class R, int D, bool RECORDS = true, bool DETACH = false>
 __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
+	const int n = c.sc->n;
+	if (DETACH)
+	{
+		const int order_blocks = (n + BLOCK - 1) / BLOCK;
+		if (bid_ >= order_blocks)
+		{
+			const long long t = (long long)(bid_ - order_blocks) * blockDim.x + threadIdx.x;
+			const int a = (int)(t / ERASE_LANES);
+			if (a < n)
+				spring_erase_lanes<R>(c, a, (int)(t % ERASE_LANES), n);
+			return;
+		}
+	}
 	const int i = GID;
-	if (i >= c.sc->n)
+	if (i >= n)
 		return;
 	// the agent's own data first: its loads overlap the rank chain below
 	if (RECORDS)
@@ -1149,7 +1245,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 //   a > p: erased iff s fired, or s is among the first f_p entries of p in a's list
 //          (f_p = fired entries of a in p's list).
 // STARTS_STEP: the kernel opens a mechanics step, whose counter is one above the stored
-// one until k_spring_erase stores it (k_begin_step's work, without its launch)
+// one until k_spring_resolve_first stores it (k_begin_step's work, without its launch)
 template <class R, bool STARTS_STEP>
 __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
@@ -1162,13 +1258,12 @@ __global__ void k_spring_fire(const SimCtx<R>* __restrict__ ctxs, const int4* __
 	spring_fire_agent<R>(c, i, c.sc->step + (STARTS_STEP ? 1 : 0));
 }
 
-template <class R, bool STARTS_STEP>
+// one thread per (agent, slot)
+template <class R>
 __global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	const long long t = (long long)bid_ * blockDim.x + threadIdx.x;
-	if (STARTS_STEP && t == 0)
-		c.sc->step++; // no thread of this kernel reads the counter
 	const int i = (int)(t / c.L), s = (int)(t % c.L);
 	const int n = c.sc->n;
 	if (i >= n)
@@ -1188,42 +1283,8 @@ __global__ void k_spring_erase(const SimCtx<R>* __restrict__ ctxs, const int4* _
 	}
 	const uint32_t fp = c.fire[p];
 	if (fp == 0)
-		return; // both rules below need a fired entry in p's list
-	const int* lp = c.spr_idx + (size_t)p * c.L;
-	const int cp = c.spr_cnt[p];
-	bool erased = false;
-	if (i < p)
-	{
-		int f_i = 0, nf_rank = 0;
-		for (int u = 0; u < cnt; u++)
-			if (li[u] == p)
-			{
-				const bool fu = (fi >> u) & 1u;
-				f_i += fu;
-				if (!fu && u < s)
-					nf_rank++;
-			}
-		int g = 0, occ = 0;
-		for (int u = 0; u < cp; u++)
-			if (lp[u] == i)
-			{
-				occ++;
-				if (occ > f_i && ((fp >> u) & 1u))
-					g++;
-			}
-		erased = nf_rank < g;
-	}
-	else
-	{
-		int rank = 0;
-		for (int u = 0; u < s; u++)
-			rank += li[u] == p;
-		int f_p = 0;
-		for (int u = 0; u < cp; u++)
-			f_p += lp[u] == i && ((fp >> u) & 1u);
-		erased = rank < f_p;
-	}
-	if (erased)
+		return; // both rules need a fired entry in p's list
+	if (erased_by_partner<R>(li, cnt, s, i, p, fi, c.spr_idx + (size_t)p * c.L, c.spr_cnt[p], fp))
 		atomicOr(&c.erase[i], 1u << s);
 }
 
@@ -1248,7 +1309,8 @@ __device__ __forceinline__ void spring_compact_agent(const SimCtx<R>& c, int i)
 // keeps its events' partners in slot order (own_tgt); each target collects (actor, k) references in
 // an unordered inbox; actors with events are listed for the resolver.  G lanes share an
 // agent: lane l draws for slots l, l+G, ...; a ballot ranks the events in slot order.
-template <class R, int G>
+// STARTS_STEP: the step counter is one above the stored one (k_spring_resolve_first stores it)
+template <class R, int G, bool STARTS_STEP>
 __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE_BLOCK(c)
@@ -1273,7 +1335,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring_collect(const SimCtx<R>* __res
 		const int ti = c.type[i];
 		const R rate_i = c.att_rate[i] * c.dt;
 		const type_row<R> aff_i = load_type_row<R>(c.affinity + (size_t)i * c.T, c.T);
-		const unsigned long long step = c.sc->step;
+		const unsigned long long step = c.sc->step + (STARTS_STEP ? 1 : 0);
 		int j_next = lane < cnt ? nb[lane] : -1; // each slot is requested one iteration ahead
 		for (int base = 0; base < cnt; base += G)
 		{
@@ -1389,7 +1451,9 @@ __device__ __forceinline__ bool resolve_actor(const SimCtx<R>& c, volatile uint8
 //
 // First round: every actor at once, one block per collect block's actor segment; actors
 // left with undecided events go to att_c for the single-block rounds that follow.
-template <class R, int G>
+// STARTS_STEP: also store the step counter the step's earlier kernels used (no thread of
+// this kernel or of k_spring_resolve reads it)
+template <class R, int G, bool STARTS_STEP>
 __global__ void __launch_bounds__(64) k_spring_resolve_first(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
@@ -1397,7 +1461,11 @@ __global__ void __launch_bounds__(64) k_spring_resolve_first(const SimCtx<R>* __
 	const int b = bid_;
 	const int n = c.sc->n;
 	if (b == 0 && threadIdx.x == 0)
+	{
 		c.sc->stat[7] = (n + APB - 1) / APB; // collect segments, for the event and actor counts
+		if (STARTS_STEP)
+			c.sc->step++;
+	}
 	if (b * APB >= n)
 		return;
 	const int m = c.blk_cnt[b];
@@ -1792,17 +1860,19 @@ __device__ __forceinline__ bool outside_box(const SimCtx<R>& c, const R* p)
 // One agent's events (i = its global thread).  FLAG: also decide, from the agent's final
 // state, whether the container update removes it, and append it to the removal list (the
 // speculation is exact here, so an agent with a removal event targeting it is removed).
-// Every lane of a wavefront calls it; returns whether the lane stored a removal entry.
+// Every lane of a wavefront calls it; returns whether the lane stored what the container
+// plan reads: a removal entry, or the removal flag of an agent it ingested or fused.
 template <class R, int D, bool FLAG>
 __device__ __forceinline__ bool interact_apply_agent(const SimCtx<R>& c, int i)
 {
 	if ((i & ~63) >= c.sc->n)
 		return false; // the whole wavefront is past the agents: nothing to reserve
 	const bool live_thread = i < c.sc->n;
-	bool removed = false;
+	bool removed = false, flagged = false;
 	if (live_thread)
 	{
 	const int nr = c.rem_cnt[i];
+	flagged = nr > 0;
 	const unsigned long long w = c.ev_t[i];
 	// the removal test's inputs are requested with the counters, in one round trip; only
 	// the actor's own fusion moves it, so then its position is read again after its events
@@ -1867,7 +1937,7 @@ __device__ __forceinline__ bool interact_apply_agent(const SimCtx<R>& c, int i)
 		if (removed)
 			c.bad[k] = i;
 	}
-	return removed;
+	return removed || flagged;
 }
 
 template <class R, int D, bool FLAG>
@@ -2087,9 +2157,10 @@ constexpr int TAIL_BAD_CHUNK = 1024;
 // the block that finishes last, so that the pair costs one launch.  Every block takes a
 // ticket from the scalars' counter whatever the status word says (the counter returns to
 // zero for the next launch); the block that takes the last ticket resets it and runs the
-// serial pass after all other blocks' work.  Removal-list entries are the only data the
-// serial pass reads that this launch wrote: the wavefronts that wrote one, and the block's
-// ticket taker, publish them with a device-scope fence first.
+// serial pass after all other blocks' work.  The serial pass reads two things this launch
+// writes: removal-list entries, and the removal flags that ingestion and fusion set on
+// their targets (the plan counts flagged removals as deaths).  The wavefronts that wrote
+// either, and their block's ticket taker, publish them with a device-scope fence first.
 template <class R, int D>
 __global__ void __launch_bounds__(BLOCK) k_interact_apply_plan(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {

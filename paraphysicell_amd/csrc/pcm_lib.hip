@@ -219,8 +219,9 @@ namespace {
 
 // Block maps of a batch's flattened launches, one per grid shape the step uses: per agent
 // (BLOCK threads), 4, 8 or 16 lanes per agent, per spring slot (erase), per CLAIM_THREADS
-// agents (claim), per occupied voxel (neighbor search, at most 16384 blocks a member) and
-// the container update's move blocks plus one block per BLOCK agents.
+// agents (claim), per occupied voxel (neighbor search, at most 16384 blocks a member), the
+// container update's move blocks plus one block per BLOCK agents, and the fused mesh
+// update's ordering blocks plus its erase blocks (ERASE_LANES lanes per agent).
 enum map_kind
 {
 	MAP_AGENT,
@@ -231,6 +232,7 @@ enum map_kind
 	MAP_CLAIM,
 	MAP_NEIGHBORS,
 	MAP_COMPACT,
+	MAP_ORDER,
 	MAP_KINDS,
 	MAP_NONE = -1
 };
@@ -267,6 +269,9 @@ inline int member_blocks(int kind, long long n, long long L)
 			break;
 		case MAP_COMPACT:
 			return MOVE_BLOCKS + (int)std::max<long long>(1, ceil_div(n, BLOCK));
+		case MAP_ORDER:
+			b = ceil_div(n, BLOCK) + ceil_div(n * ERASE_LANES, BLOCK);
+			break;
 	}
 	return (int)std::max<long long>(1, b);
 }
@@ -460,7 +465,7 @@ public:
 		n_launch_ = n;
 		mesh_fresh_ = false;
 		records_fresh_ = false;
-		fire_ready_ = false;
+		detach_ready_ = false;
 		if ((int)origin_ident_.size() < n)
 		{
 			const int from = (int)origin_ident_.size();
@@ -590,7 +595,7 @@ public:
 		if (f == PCM_F_POSITIONS || f == PCM_F_RADIUS || f == PCM_F_RELATIVE_MAXIMUM_ADHESION_DISTANCE)
 			mesh_fresh_ = false;
 		records_fresh_ = false;
-		fire_ready_ = false;
+		detach_ready_ = false;
 		return PCM_OK;
 	}
 
@@ -605,7 +610,7 @@ public:
 
 	int set_links(int which, const int32_t* off, const int32_t* idx) override
 	{
-		fire_ready_ = false;
+		detach_ready_ = false;
 		if (which != PCM_LINKS_SPRINGS && which != PCM_LINKS_ATTACHED)
 			return fail(PCM_ERR_ARGUMENT, "bad link list id");
 		if (!off)
@@ -722,7 +727,7 @@ public:
 	// that list it, in ascending order.
 	int set_neighbors(const int32_t* off, const int32_t* idx) override
 	{
-		fire_ready_ = false;
+		detach_ready_ = false;
 		if (!off || !idx)
 			return fail(PCM_ERR_ARGUMENT, "null neighbor lists");
 		PCM_TRY(read_scalars());
@@ -893,7 +898,7 @@ public:
 		n_launch_ = (int)total;
 		mesh_fresh_ = false;
 		records_fresh_ = false;
-		fire_ready_ = false;
+		detach_ready_ = false;
 		const int blocks = std::max(1, (sc_.div_count + BLOCK - 1) / BLOCK);
 		switch (h_.dims)
 		{
@@ -923,7 +928,7 @@ public:
 
 	int set_step(uint64_t s) override
 	{
-		fire_ready_ = false;
+		detach_ready_ = false;
 		PCM_TRY(read_scalars());
 		sc_.step = s;
 		PCM_TRY(copy_sync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
@@ -939,7 +944,7 @@ public:
 
 	int set_gradients(const int32_t* voxel, const void* g) override
 	{
-		fire_ready_ = false;
+		detach_ready_ = false;
 		int grid[3];
 		size_t voxels = 1;
 		for (int k = 0; k < 3; k++)
@@ -982,7 +987,7 @@ public:
 	int phase(int which, const pcm_step_params* p) override
 	{
 		PCM_TRY(ready());
-		fire_ready_ = false;
+		detach_ready_ = false;
 		if (p)
 			PCM_TRY(set_params(*p));
 		switch (h_.dims)
@@ -1108,10 +1113,10 @@ private:
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
 	bool records_fresh_ = false; // force records describe the current agent fields
-	// the last work enqueued was a fused step, whose mesh update drew the next step's
-	// detachment masks, and nothing has changed the agents, their lists or the step
-	// parameters since
-	bool fire_ready_ = false;
+	// the last work enqueued was a fused step, whose mesh update decided the next step's
+	// detachments (fired and erased spring slots), and nothing has changed the agents, their
+	// lists, the step counter or the step parameters since
+	bool detach_ready_ = false;
 	int binned_launch_ = 1 << 30; // launch bound of the last mesh update (>= agents it binned)
 
 	int width(int kind) const
@@ -1304,7 +1309,7 @@ private:
 		const int springs = p.automated_spring_adhesion != 0;
 		if (dt == h_.dt && wall == h_.wall && springs == h_.springs)
 			return PCM_OK;
-		fire_ready_ = false; // the masks were drawn against the old rate * dt
+		detach_ready_ = false; // the fired slots were drawn against the old rate * dt
 		h_.dt = dt;
 		h_.wall = wall;
 		h_.springs = springs;
@@ -1333,7 +1338,7 @@ private:
 	// neighbor overflow: grow K, rebuild the lists of the aborted step, clear the status
 	int recover()
 	{
-		fire_ready_ = false;
+		detach_ready_ = false;
 		if (sc_.status & ~ST_NEIGHBOR_OVERFLOW)
 			return fail(PCM_ERR_STATE, "device status word is set: " + std::to_string(sc_.status));
 		PCM_TRY(grow_neighbors(sc_.max_neighbors));
@@ -1539,7 +1544,11 @@ private:
 		PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
 					   CLAIM_THREADS, MAP_CLAIM));
 		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R, FUSED>));
-		return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
+		if (!FUSED)
+			return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
+		// the fused step's ordering blocks, then the next step's erase decisions
+		const int erase_blocks = (int)(((long long)gn() * ERASE_LANES + BLOCK - 1) / BLOCK);
+		return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true>, grid_agents() + erase_blocks, BLOCK, MAP_ORDER);
 	}
 
 	template <int D, bool CLEAR>
@@ -1573,29 +1582,37 @@ private:
 	// IN_STEP: the kernels open a fused mechanics step, so they also advance the step
 	// counter (the detachment draws use the new value; the erase kernel stores it), and the
 	// step's velocity kernel appends the accepted attachments instead of k_spring_append
-	// fire_ready: the masks were drawn by the previous fused step's mesh update
+	// IN_STEP: the kernels open a fused mechanics step, so the step counter they key their
+	// draws with is one above the stored one until the first resolve round stores it, and
+	// the step's velocity kernel appends the accepted attachments instead of k_spring_append.
+	// detach_ready: the previous fused step's mesh update decided the fired and erased slots
 	template <bool IN_STEP>
-	int spring_update_kernels(bool fire_ready = false)
+	int spring_update_kernels(bool detach_ready = false)
 	{
-		if (!fire_ready)
+		if (!detach_ready)
+		{
 			PCM_TRY(per_agent(PCM_K_SPRING_FIRE, k_spring_fire<R, IN_STEP>));
-		PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R, IN_STEP>,
-					   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK, MAP_ERASE));
+			PCM_TRY(launch(PCM_K_SPRING_ERASE, k_spring_erase<R>,
+						   std::max(1, (int)(((long long)gn() * gl() + BLOCK - 1) / BLOCK)), BLOCK, MAP_ERASE));
+		}
 		// the first resolve round runs one block per collect block (segments of BLOCK / G)
 		const int segments = std::max(1, (int)(((long long)gn() * tune_.event_lanes + BLOCK - 1) / BLOCK));
 		switch (tune_.event_lanes)
 		{
 			case 4:
-				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 4>, 4));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 4>, segments, 64, MAP_LANES4));
+				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 4, IN_STEP>, 4));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 4, IN_STEP>, segments, 64,
+							   MAP_LANES4));
 				break;
 			case 16:
-				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 16>, 16));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 16>, segments, 64, MAP_LANES16));
+				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 16, IN_STEP>, 16));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 16, IN_STEP>, segments, 64,
+							   MAP_LANES16));
 				break;
 			default:
-				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 8>, 8));
-				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 8>, segments, 64, MAP_LANES8));
+				PCM_TRY(per_lanes(PCM_K_SPRING_COLLECT, k_spring_collect<R, 8, IN_STEP>, 8));
+				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 8, IN_STEP>, segments, 64,
+							   MAP_LANES8));
 		}
 		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
 		return IN_STEP ? PCM_OK : per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
@@ -1675,12 +1692,12 @@ private:
 	// one mechanics step (simulator.cpp:173-192): the spring-list update runs first since
 	// forces, motility and the membrane do not read springs; velocity contributions are
 	// then summed in the reference's order by the fused velocity kernel.
-	// fire_ready: the previous fused step's mesh update drew this step's detachment masks
+	// detach_ready: the previous fused step's mesh update decided this step's detachments
 	template <int D>
-	int step_d(bool records_fresh, bool fold_clear, bool fire_ready)
+	int step_d(bool records_fresh, bool fold_clear, bool detach_ready)
 	{
 		if (h_.springs)
-			PCM_TRY(spring_update_kernels<true>(fire_ready));
+			PCM_TRY(spring_update_kernels<true>(detach_ready));
 		else
 			PCM_TRY(launch(PCM_K_BEGIN_STEP, k_begin_step<R>, 1, 64));
 		if (!records_fresh)
@@ -1703,23 +1720,23 @@ private:
 
 	int dispatch_step()
 	{
-		const bool ready = fire_ready_;
-		fire_ready_ = false;
+		const bool ready = detach_ready_;
+		detach_ready_ = false;
 		PCM_TRY(dispatch_step_flags(records_fresh_, binned_launch_ <= n_launch_, ready));
-		fire_ready_ = true;
+		detach_ready_ = true;
 		return PCM_OK;
 	}
 
-	int dispatch_step_flags(bool records_fresh, bool fold_clear, bool fire_ready)
+	int dispatch_step_flags(bool records_fresh, bool fold_clear, bool detach_ready)
 	{
 		switch (h_.dims)
 		{
 			case 1:
-				return step_d<1>(records_fresh, fold_clear, fire_ready);
+				return step_d<1>(records_fresh, fold_clear, detach_ready);
 			case 2:
-				return step_d<2>(records_fresh, fold_clear, fire_ready);
+				return step_d<2>(records_fresh, fold_clear, detach_ready);
 			default:
-				return step_d<3>(records_fresh, fold_clear, fire_ready);
+				return step_d<3>(records_fresh, fold_clear, detach_ready);
 		}
 	}
 
@@ -1733,7 +1750,7 @@ public:
 	int launch_bound() const { return n_launch_; }
 	int spring_slots() const { return h_.L; }
 	bool records_fresh() const { return records_fresh_; }
-	bool fire_ready() const { return fire_ready_; }
+	bool detach_ready() const { return detach_ready_; }
 	bool binned_within(int grid_n) const { return binned_launch_ <= grid_n; }
 	void*& batch() { return batch_; }
 	int apply_params(const pcm_step_params& p) { return set_params(p); }
@@ -1822,7 +1839,7 @@ public:
 	// maps: the batch's block maps (device, offsets and padded totals per map kind) for
 	// flattened launches, or nullptr for one block row per member
 	int run_segmented(const SimCtx<R>* ctxs, int m, int grid_n, int grid_L, bool records_fresh, bool fold_clear,
-					  bool fire_ready, int steps, const int4* maps = nullptr, const int* offs = nullptr,
+					  bool detach_ready, int steps, const int4* maps = nullptr, const int* offs = nullptr,
 					  const int* totals = nullptr)
 	{
 		launch_ctx_ = ctxs;
@@ -1838,10 +1855,10 @@ public:
 		int rc = PCM_OK;
 		for (int k = 0; k < steps && rc == PCM_OK; k++)
 		{
-			rc = dispatch_step_flags(records_fresh, fold_clear, fire_ready);
+			rc = dispatch_step_flags(records_fresh, fold_clear, detach_ready);
 			records_fresh = true;
 			fold_clear = true;
-			fire_ready = true;
+			detach_ready = true;
 		}
 		launch_ctx_ = nullptr;
 		launch_y_ = 1;
@@ -1856,7 +1873,7 @@ public:
 	{
 		mesh_fresh_ = true;
 		records_fresh_ = true;
-		fire_ready_ = true;
+		detach_ready_ = true;
 		binned_launch_ = n_launch_;
 	}
 
@@ -2211,7 +2228,7 @@ private:
 		const int m = (int)g.slots.size();
 		PCM_TRY(reserve(g, m));
 		int grid_n = 1, grid_L = 1;
-		bool records_fresh = true, fire_ready = true;
+		bool records_fresh = true, detach_ready = true;
 		// the leader launches the spring kernels when any member has springs on (members with
 		// them off skip the work inside those kernels)
 		int any_springs = 0;
@@ -2232,7 +2249,7 @@ private:
 				grid_n = std::max(grid_n, slot.sim->launch_bound());
 				grid_L = std::max(grid_L, slot.sim->spring_slots());
 				records_fresh = records_fresh && slot.sim->records_fresh();
-				fire_ready = fire_ready && slot.sim->fire_ready();
+				detach_ready = detach_ready && slot.sim->detach_ready();
 			}
 		grid_n = round_up_grid(grid_n);
 		bool fold_clear = true;
@@ -2258,16 +2275,16 @@ private:
 			PCM_TRY(build_maps(g, m));
 			maps = g.d_map;
 		}
-		auto run = [&](bool fresh, bool clear, bool fire, int k) {
-			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, fresh, clear, fire, k, maps, g.map_off,
+		auto run = [&](bool fresh, bool clear, bool detach, int k) {
+			return leader->run_segmented(g.d_ctx, m, grid_n, grid_L, fresh, clear, detach, k, maps, g.map_off,
 										 g.map_tot);
 		};
 		int done = 0;
-		if (!records_fresh || !fold_clear || !fire_ready)
+		if (!records_fresh || !fold_clear || !detach_ready)
 		{
-			// the first step refreshes records, clears counts or draws the detachment masks for
+			// the first step refreshes records, clears counts or decides the detachments for
 			// some member: no graph
-			PCM_TRY(run(records_fresh, fold_clear, fire_ready, 1));
+			PCM_TRY(run(records_fresh, fold_clear, detach_ready, 1));
 			done = 1;
 		}
 		if (!graphs_)

@@ -160,8 +160,10 @@ def test_fused_step(dims, real_bytes):
         s.initialize_mesh()
     rounds = []
     for step in range(4):
+        d0, o0 = dev.deaths_count, ora.deaths_count
         dev.step(p, 1)
         ora.step(p, 1)
+        assert dev.deaths_count - d0 == ora.deaths_count - o0, f"fused step {step}: deaths"
         parity.assert_same_state(dev, ora, what=f"fused step {step}")
         parity.assert_same_buckets(dev, ora, f"fused step {step}")
         parity.assert_same_neighbors(dev, ora, f"fused step {step}")
@@ -186,11 +188,13 @@ def test_consecutive_fused_steps(real_bytes):
         s.initialize_mesh()
     dev.step(p, 3)
     ora.step(p, 3)
+    assert dev.deaths_count == ora.deaths_count, "three fused steps: deaths"
     parity.assert_same_state(dev, ora, what="three fused steps")
     parity.assert_same_neighbors(dev, ora, "three fused steps")
     q = dataclasses.replace(p, dt=0.5 * p.dt)
     dev.step(q, 2)
     ora.step(q, 2)
+    assert dev.deaths_count == ora.deaths_count, "two more steps: deaths"
     parity.assert_same_state(dev, ora, what="two more steps at half the time step")
     parity.assert_same_neighbors(dev, ora, "two more steps at half the time step")
 
