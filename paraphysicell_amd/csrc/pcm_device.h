@@ -27,6 +27,7 @@ enum : int
 	ST_NEIGHBOR_OVERFLOW = 1, // some agent has more than K neighbors: grow K, rebuild
 	ST_LIST_OVERFLOW = 2,     // a link list would exceed its fixed capacity
 	ST_CAPACITY = 4,          // a scratch buffer capacity was exceeded
+	ST_BUCKET_OVERFLOW = 8,   // a voxel holds more agents than its unordered bucket table: grow B, rebuild
 };
 
 struct Scalars
@@ -49,6 +50,8 @@ struct Scalars
 	int compact_limit;     // > 0: the removal loop visits agents below it only
 	int res_open;          // actors the first resolve round left with undecided events
 	int ticket;            // blocks of the current launch that have finished (last-block hand-off)
+	int max_bucket;        // largest voxel population seen by an overflowing fused mesh update
+	int pad2;
 	// diagnostics of the last step (pcm_get_stats): attachment events, resolver rounds,
 	// resolver chunks, interaction candidates, speculation conflict, removals, occupied voxels
 	int stat[8];
@@ -84,6 +87,7 @@ struct SimCtx
 	int bmin[3], bmax[3], vox[3], grid[3];
 	long long vcount;
 	int K;       // neighbor slots per agent
+	int B;       // slots per voxel of the unordered bucket table (3D fused mesh update)
 	int L;       // spring slots per agent
 	int LA;      // attached-cell slots per agent
 	int cand_cap; // interaction candidate buffer
@@ -135,6 +139,7 @@ struct SimCtx
 	gptr<int> nbr_cnt, nbr_idx;
 	// mechanics voxel grid
 	gptr<int> vcnt, vstart, vox_of, vrank, btmp, bucket, occ;
+	gptr<int> vmem; // 3D: B slots per voxel, the voxel's agents in arrival order (fused mesh update)
 	gptr<R> sorted; // 4 reals per bucket slot: x, y, z, rel*radius
 	gptr<R> frec;   // 8 reals per agent: x, y, z, radius, repulsion, adhesion, rel, type|movable<<30 bits
 

@@ -238,7 +238,8 @@ __global__ void k_bin_count(const SimCtx<R>* __restrict__ ctxs, const int4* __re
 // (agents in the high word, occupied voxels in the low word): device-scope atomics on one
 // cache line serialise across the XCDs, so there are as few of them as possible.
 // RESTORE: also return the container update's remap entries to the identity (the link
-// edits that read them have finished; k_bin_scatter then zeroes the update's counters)
+// edits that read them have finished; the next step's interaction pass zeroes the update's
+// counters)
 template <class R, bool RESTORE>
 __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
@@ -277,19 +278,11 @@ __global__ void __launch_bounds__(CLAIM_THREADS) k_bin_claim(const SimCtx<R>* __
 	}
 }
 
-// IN_STEP: the mesh update of a fused step.  It also zeroes the container update's
-// counters (their readers have finished) and does the next step's k_spring_fire work:
-// spring lists, counts, detachment rates and agent indices are final for this step once
-// the compaction has run, and the next step's draws are keyed by the stored step counter
-// plus one, as k_spring_fire<true> keys them at the next step's start.  The host skips
-// that step's fire launch unless something changed the agents in between.
-template <class R, bool IN_STEP>
+template <class R>
 __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	const int i = GID;
-	if (IN_STEP && i == 0)
-		c.sc->res_open = 0;
 	if (i >= c.sc->n)
 		return;
 	if (i == 0)
@@ -297,31 +290,28 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __
 		// the claims have finished
 		c.sc->n_occ = (int)(c.sc->claim & 0xffffffffull);
 		c.sc->claim = 0;
-		if (IN_STEP)
-		{
-			c.sc->bad_count = 0;
-			c.sc->moves = 0;
-		}
 	}
 	c.btmp[c.vstart[c.vox_of[i]] + c.vrank[i]] = i;
-	if (IN_STEP)
-		spring_fire_agent<R>(c, i, c.sc->step + 1);
 }
 
 // RECORDS: also write the agents' force records for the step's gather kernels (a mesh
 // update for the neighbor lists alone has no force-record buffer).  DETACH: the mesh update
 // of a fused step also decides the next step's erased spring slots (k_spring_erase's work;
 // k_bin_scatter drew the fired slots)
-// With DETACH the launch has the member's ceil(n / BLOCK) ordering blocks followed by its
-// erase blocks (ERASE_LANES lanes per agent).
-template <class R, int D, bool RECORDS = true, bool DETACH = false>
+// With DETACH the launch has the member's ordering blocks followed by its erase blocks
+// (ERASE_LANES lanes per agent).  FIXED (the 3D fused mesh update, after k_compact_count
+// with FIXED): the voxel's members come from its unordered table, the occupied-voxel count
+// is taken from the claim counter, and the compaction's remap entries are returned to the
+// identity (the ordering blocks then cover the removal list too).
+template <class R, int D, bool RECORDS = true, bool DETACH = false, bool FIXED = false>
 __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE(c)
 	const int n = c.sc->n;
+	const int restore = FIXED ? c.sc->bad_count : 0; // moves <= removals
 	if (DETACH)
 	{
-		const int order_blocks = (n + BLOCK - 1) / BLOCK;
+		const int order_blocks = (max(n, restore) + BLOCK - 1) / BLOCK;
 		if (bid_ >= order_blocks)
 		{
 			const long long t = (long long)(bid_ - order_blocks) * blockDim.x + threadIdx.x;
@@ -332,6 +322,24 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs, const int4* __re
 		}
 	}
 	const int i = GID;
+	if (FIXED)
+	{
+		if (i == 0)
+		{
+			c.sc->n_occ = (int)(c.sc->claim & 0xffffffffull);
+			c.sc->claim = 0;
+		}
+		if (i < restore)
+		{
+			const int x = c.bad[i];
+			c.remap[x] = x;
+		}
+		if (i < c.sc->moves)
+		{
+			const int x = c.move_src[i];
+			c.remap[x] = x;
+		}
+	}
 	if (i >= n)
 		return;
 	// the agent's own data first: its loads overlap the rank chain below
@@ -343,10 +351,11 @@ __global__ void k_bin_order(const SimCtx<R>* __restrict__ ctxs, const int4* __re
 	rec[3] = c.rel_adh[i] * c.radius[i];
 	const int v = c.vox_of[i];
 	const int s = c.vstart[v], m = c.vcnt[v];
+	const int* members = FIXED ? c.vmem + (size_t)v * c.B : c.btmp + s;
 	int rank = 0;
 #pragma unroll 8
-	for (int k = s; k < s + m; k++)
-		rank += c.btmp[k] > i;
+	for (int k = 0; k < m; k++)
+		rank += members[k] > i;
 	const int slot = s + rank;
 	c.bucket[slot] = i;
 	R* out = c.sorted + (size_t)slot * 4;
@@ -1700,6 +1709,13 @@ __global__ void __launch_bounds__(BLOCK) k_interact_tentative(const SimCtx<R>* _
 	const int lane = t & (G - 1);
 	const int group_shift = (int)(threadIdx.x & 63) & ~(G - 1);
 	const unsigned long long group_mask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+	if (AB2 && t == 0)
+	{
+		// the previous step's container counters: their last readers (the remap restore of
+		// the mesh update) have finished, and this step's apply pass counts from zero
+		c.sc->bad_count = 0;
+		c.sc->moves = 0;
+	}
 	if (g >= c.sc->n)
 		return;
 	const int i = g;
@@ -2579,13 +2595,43 @@ __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* 
 // launch.  Each agent of the compacted population is counted into the voxel of its final
 // position: an agent that stays by its own thread, a moved one by the wavefront that moves
 // it (with the position it moves), so the count never reads a slot a mover writes.  The
-// remap entries the link edits read are returned to the identity by the claim kernel.
-template <class R, int D>
+// remap entries the link edits read are returned to the identity by a later kernel (the
+// voxel claim, or with FIXED the ordering kernel).  The thread that finishes an agent's
+// spring list also draws the next step's detachments for it (k_spring_fire's work, keyed
+// by the stored step counter plus one; spring lists, counts, rates and indices are final
+// here).
+// FIXED (3D): the agent also takes its slot in the voxel's unordered table (B slots per
+// voxel, `vmem`), and the voxel's first agent gives the voxel the next range of B bucket
+// slots (one counter atomic per block) and lists it as occupied: no claim or scatter launch
+// follows.  A voxel with more than B agents raises ST_BUCKET_OVERFLOW (the host grows B
+// and rebuilds the mesh the dense way).
+template <class R, int D, bool FIXED>
+__device__ __forceinline__ int count_into_voxel(const SimCtx<R>& c, int a, int src)
+{
+	const int v = voxel_index<R>(c, c.pos + (size_t)src * D);
+	const int r = atomicAdd(&c.vcnt[v], 1);
+	c.vox_of[a] = v;
+	c.vrank[a] = r;
+	if (FIXED)
+	{
+		if (r < c.B)
+			c.vmem[(size_t)v * c.B + r] = a;
+		else
+		{
+			atomicOr(&c.sc->status, ST_BUCKET_OVERFLOW);
+			atomicMax(&c.sc->max_bucket, r + 1);
+		}
+	}
+	return r == 0 ? v : -1;
+}
+
+template <class R, int D, bool FIXED>
 __global__ void k_compact_count(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
-	PCM_PROLOGUE(c)
+	PCM_PROLOGUE_BLOCK(c)
 	const bool compacting = c.sc->bad_count != 0;
 	const int n = c.sc->n;
+	const unsigned long long next_step = c.sc->step + 1;
 	if (bid_ < MOVE_BLOCKS)
 	{
 		if (!compacting)
@@ -2599,28 +2645,52 @@ __global__ void k_compact_count(const SimCtx<R>* __restrict__ ctxs, const int4* 
 			compact_move<R>(c, src, dst, lane);
 			if (lane == 0)
 			{
-				const int v = voxel_index<R>(c, c.pos + (size_t)src * D);
-				c.vox_of[dst] = v;
-				c.vrank[dst] = atomicAdd(&c.vcnt[v], 1);
+				// the moved fields and the edited list were written by this wavefront
+				spring_fire_agent<R>(c, dst, next_step);
+				const int v = count_into_voxel<R, D, FIXED>(c, dst, src);
+				if (FIXED && v >= 0)
+				{
+					const int w = (int)atomicAdd(&c.sc->claim, 1ull);
+					c.occ[w] = v;
+					c.vstart[v] = w * c.B;
+				}
 			}
 		}
 		return;
 	}
 	const int i = (bid_ - MOVE_BLOCKS) * blockDim.x + threadIdx.x;
 	if (i == 0)
-		c.sc->n_binned = n;
-	if (i >= n)
-		return;
-	if (compacting)
 	{
-		if (c.remap[i] < 0)
-			return; // a hole: its mover counts it
-		edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
-		edit_links(c.atc_idx + (size_t)i * c.LA, c.atc_cnt + i, c.remap);
+		c.sc->n_binned = n;
+		c.sc->res_open = 0; // the next step's resolver starts with no open actors
 	}
-	const int v = voxel_index<R>(c, c.pos + (size_t)i * D);
-	c.vox_of[i] = v;
-	c.vrank[i] = atomicAdd(&c.vcnt[v], 1);
+	int first = -1; // the voxel this thread's agent opened (FIXED)
+	// a hole receives its agent, counted and drawn for by the mover
+	if (i < n && !(compacting && c.remap[i] < 0))
+	{
+		if (compacting)
+		{
+			edit_links(c.spr_idx + (size_t)i * c.L, c.spr_cnt + i, c.remap);
+			edit_links(c.atc_idx + (size_t)i * c.LA, c.atc_cnt + i, c.remap);
+		}
+		spring_fire_agent<R>(c, i, next_step);
+		first = count_into_voxel<R, D, FIXED>(c, i, i);
+	}
+	if (FIXED)
+	{
+		__shared__ int s_base;
+		int total;
+		const int pre = block_exclusive_scan(first >= 0 ? 1 : 0, total);
+		if (threadIdx.x == 0)
+			s_base = total ? (int)atomicAdd(&c.sc->claim, (unsigned long long)total) : 0;
+		__syncthreads();
+		if (first >= 0)
+		{
+			const int w = s_base + pre;
+			c.occ[w] = first;
+			c.vstart[first] = w * c.B;
+		}
+	}
 }
 
 template <class R>

@@ -340,6 +340,7 @@ public:
 		if (h.vcount > 0x7fffffffLL)
 			return fail(PCM_ERR_CAPACITY, "mechanics voxel grid larger than 2^31 voxels");
 		h.K = h.dims == 3 ? 32 : (h.dims == 2 ? 16 : 8);
+		h.B = h.dims == 3 ? 32 : 0;
 		h.L = 12;
 		h.LA = 1;
 		h.seed = cfg_.seed;
@@ -399,9 +400,8 @@ public:
 		PCM_TRY(alloc_t(&h.vox_of, cap));
 		PCM_TRY(alloc_t(&h.vrank, cap));
 		PCM_TRY(alloc_t(&h.btmp, cap));
-		PCM_TRY(alloc_t(&h.bucket, cap));
 		PCM_TRY(alloc_t(&h.occ, cap));
-		PCM_TRY(alloc_t(&h.sorted, (size_t)cap * 4));
+		PCM_TRY(alloc_bucket_storage());
 		PCM_TRY(alloc_t(&h.frec, (size_t)cap * 8));
 		PCM_TRY(alloc_t(&h.own_cnt, cap));
 		PCM_TRY(alloc_t(&h.in_cnt, cap));
@@ -790,10 +790,14 @@ public:
 		std::vector<int> vc(V), vs(V);
 		PCM_TRY(copy_sync(vc.data(), h_.vcnt, sizeof(int) * V, hipMemcpyDeviceToHost));
 		PCM_TRY(copy_sync(vs.data(), h_.vstart, sizeof(int) * V, hipMemcpyDeviceToHost));
-		const int nb = std::max(sc_.n_binned, 1);
-		std::vector<int> b(nb);
-		if (sc_.n_binned > 0)
-			PCM_TRY(copy_sync(b.data(), h_.bucket, sizeof(int) * sc_.n_binned, hipMemcpyDeviceToHost));
+		// bucket slots in use: one range per occupied voxel (dense, or B apart in 3D)
+		size_t span = 0;
+		for (size_t v = 0; v < V; v++)
+			if (vc[v] > 0)
+				span = std::max(span, (size_t)vs[v] + (size_t)vc[v]);
+		std::vector<int> b(std::max<size_t>(span, 1));
+		if (span > 0)
+			PCM_TRY(copy_sync(b.data(), h_.bucket, sizeof(int) * span, hipMemcpyDeviceToHost));
 		int64_t t = 0;
 		for (size_t v = 0; v < V; v++)
 		{
@@ -1249,6 +1253,32 @@ private:
 		return PCM_OK;
 	}
 
+	// bucket slots: with the 3D unordered tables, B per occupied voxel (at most one voxel
+	// per agent), else one per agent
+	int alloc_bucket_storage()
+	{
+		const size_t cap = std::max(1, h_.cap);
+		const size_t slots = cap * (size_t)std::max(1, h_.B);
+		PCM_TRY(alloc_t(&h_.bucket, slots));
+		PCM_TRY(alloc_t(&h_.sorted, slots * 4));
+		if (h_.B > 0)
+			PCM_TRY(alloc_t(&h_.vmem, (size_t)h_.vcount * h_.B));
+		return PCM_OK;
+	}
+
+	// a voxel held more agents than its unordered table: grow the tables (nothing in them
+	// survives a mesh update)
+	int grow_buckets(int needed)
+	{
+		PCM_HIP(hipStreamSynchronize(stream_));
+		release(h_.bucket);
+		release(h_.sorted);
+		release(h_.vmem);
+		h_.B = next_pow2(std::max(needed, 2 * h_.B));
+		PCM_TRY(alloc_bucket_storage());
+		return push_ctx();
+	}
+
 	int grow_neighbors(int needed)
 	{
 		PCM_HIP(hipStreamSynchronize(stream_));
@@ -1339,6 +1369,36 @@ private:
 	int recover()
 	{
 		detach_ready_ = false;
+		if (sc_.status & ST_BUCKET_OVERFLOW)
+		{
+			// the fused 3D mesh update stopped after the compaction kernel (its histogram and
+			// voxel assignment are complete): finish the step's container update and mesh
+			// update the dense way with larger tables, then its neighbor lists
+			if (sc_.status & ~(ST_BUCKET_OVERFLOW | ST_NEIGHBOR_OVERFLOW))
+				return fail(PCM_ERR_STATE, "device status word is set: " + std::to_string(sc_.status));
+			PCM_TRY(grow_buckets(sc_.max_bucket));
+			sc_.status = 0;
+			sc_.max_bucket = 0;
+			sc_.claim = 0; // the occupied voxels the fused update counted are claimed again
+			PCM_TRY(copy_sync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
+			PCM_TRY(per_agent(PCM_K_COMPACT_RESTORE, k_compact_restore<R>));
+			PCM_TRY(launch(PCM_K_COMPACT_DONE, k_compact_done<R>, 1, 64));
+			switch (h_.dims)
+			{
+				case 1:
+					PCM_TRY(mesh_kernels<1>());
+					break;
+				case 2:
+					PCM_TRY(mesh_kernels<2>());
+					break;
+				default:
+					PCM_TRY(mesh_kernels<3>());
+			}
+			PCM_TRY(launch_neighbors());
+			PCM_TRY(read_scalars());
+			if (!(sc_.status & ST_NEIGHBOR_OVERFLOW))
+				return sc_.status ? fail(PCM_ERR_STATE, "mesh rebuild failed after bucket growth") : PCM_OK;
+		}
 		if (sc_.status & ~ST_NEIGHBOR_OVERFLOW)
 			return fail(PCM_ERR_STATE, "device status word is set: " + std::to_string(sc_.status));
 		PCM_TRY(grow_neighbors(sc_.max_neighbors));
@@ -1528,8 +1588,11 @@ private:
 		}
 	}
 
-	// FUSED: the histogram ran in the step's compaction kernel (after the clear), and the
-	// container update's remap restore and counter reset ride on the claim and scatter kernels
+	// FUSED: the histogram ran in the step's compaction kernel (after the clear).  In 3D that
+	// kernel also filled the voxels' unordered tables and gave each occupied voxel its bucket
+	// range, so the ordering kernel follows at once; in 1D and 2D the claim and scatter
+	// kernels lay the buckets out densely (the lane-per-agent neighbor search walks them).
+	// The ordering kernel also decides the next step's erased spring slots.
 	template <int D, bool FUSED = false>
 	int mesh_kernels()
 	{
@@ -1541,14 +1604,19 @@ private:
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 			PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D>));
 		}
-		PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
-					   CLAIM_THREADS, MAP_CLAIM));
-		PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R, FUSED>));
+		constexpr bool FIXED = FUSED && D == 3;
+		if (!FIXED)
+		{
+			PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
+						   CLAIM_THREADS, MAP_CLAIM));
+			PCM_TRY(per_agent(PCM_K_BIN_SCATTER, k_bin_scatter<R>));
+		}
 		if (!FUSED)
 			return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 		// the fused step's ordering blocks, then the next step's erase decisions
 		const int erase_blocks = (int)(((long long)gn() * ERASE_LANES + BLOCK - 1) / BLOCK);
-		return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true>, grid_agents() + erase_blocks, BLOCK, MAP_ORDER);
+		return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true, FIXED>, grid_agents() + erase_blocks, BLOCK,
+					  MAP_ORDER);
 	}
 
 	template <int D, bool CLEAR>
@@ -1713,7 +1781,8 @@ private:
 		if (!fold_clear)
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 		// the container update's moves and the mesh update's histogram
-		PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D>, MOVE_BLOCKS + grid_agents(), BLOCK, MAP_COMPACT));
+		PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, D == 3>, MOVE_BLOCKS + grid_agents(), BLOCK,
+					   MAP_COMPACT));
 		PCM_TRY((mesh_kernels<D, true>()));
 		return neighbors_d<D>();
 	}
@@ -2454,7 +2523,7 @@ int noalloc_build(const SimCtx<R>& proto, unsigned char* area, int* needed, hipS
 	hipLaunchKernelGGL((k_bin_count<R, D>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL((k_bin_claim<R, false>), dim3(std::max(1, (agents + CLAIM_THREADS - 1) / CLAIM_THREADS)),
 					   dim3(CLAIM_THREADS), 0, stream, ctx, (const int4*)nullptr);
-	hipLaunchKernelGGL((k_bin_scatter<R, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
+	hipLaunchKernelGGL(k_bin_scatter<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL((k_bin_order<R, D, false>), dim3(blocks), dim3(BLOCK), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL((k_neighbors<R, D, 2>), dim3(std::max(1, std::min(agents, 16384))), dim3(128), 0, stream, ctx, (const int4*)nullptr);
 	hipLaunchKernelGGL(k_noalloc_finish<R>, dim3(blocks), dim3(BLOCK), 0, stream, ctx, needed);

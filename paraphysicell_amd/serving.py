@@ -28,9 +28,11 @@ def footprint_bytes(mesh: abi.MeshSpec, real_bytes: int, defs: int, subs: int, c
     fields = sum(abi.field_width(w, d, defs, subs) * (r if t == abi.T_REAL else (4 if t == abi.T_I32 else 1))
                  for _, t, w in abi.FIELDS)
     k = {1: 8, 2: 16, 3: 32}[d]
-    per_agent = fields + 4 * (k + 12 + 1 + 2 * k + k) + k + 12 * r + 160
+    b = 32 if d == 3 else 1  # bucket slots per agent (3D: a range of 32 per occupied voxel)
+    per_agent = fields + 4 * (k + 12 + 1 + 2 * k + k) + k + 12 * r + 160 + (4 + 4 * r) * (b - 1)
     g = mesh.grid_shape()
-    return int(capacity * per_agent + 8 * g[0] * g[1] * g[2] + (1 << 16))
+    per_voxel = 8 + (4 * b if d == 3 else 0)  # counts, starts and the 3D unordered tables
+    return int(capacity * per_agent + per_voxel * g[0] * g[1] * g[2] + (1 << 16))
 
 
 @dataclass

@@ -199,6 +199,28 @@ def test_consecutive_fused_steps(real_bytes):
     parity.assert_same_neighbors(dev, ora, "two more steps at half the time step")
 
 
+@pytest.mark.parametrize("real_bytes", [4, 8])
+def test_crowded_voxel_grows_the_bucket_tables(real_bytes):
+    """70 agents packed into one 30 um voxel: the fused 3D mesh update's unordered voxel
+    tables (32 slots) overflow, the step finishes the dense way with larger tables, and
+    the next steps use them; every step agrees with the oracle."""
+    mesh, st, p = parity.rich_population(3, 700, 71, real_bytes, rates="busy")
+    pos = st["positions"].reshape(-1, 3).copy()
+    rng = np.random.default_rng(5)
+    pos[100:170] = rng.uniform(4.0, 26.0, size=(70, 3))  # the voxel [0, 30)^3
+    st["positions"] = pos.reshape(-1)
+    dev, ora = parity.make_pair(mesh, st, seed=6)
+    for s in (dev, ora):
+        s.initialize_mesh()
+    for step in range(3):
+        dev.step(p, 1)
+        ora.step(p, 1)
+        assert dev.deaths_count == ora.deaths_count, f"step {step}: deaths"
+        parity.assert_same_state(dev, ora, what=f"crowded voxel, step {step}")
+        parity.assert_same_buckets(dev, ora, f"crowded voxel, step {step}")
+        parity.assert_same_neighbors(dev, ora, f"crowded voxel, step {step}")
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_neighbors_with_stale_mesh(dims):
     """Neighbors rebuilt after the agents moved but before a new mesh update: the stencil is
