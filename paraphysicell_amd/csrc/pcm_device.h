@@ -51,7 +51,7 @@ struct Scalars
 	int res_open;          // actors the first resolve round left with undecided events
 	int ticket;            // blocks of the current launch that have finished (last-block hand-off)
 	int max_bucket;        // largest voxel population seen by an overflowing fused mesh update
-	int pad2;
+	unsigned res_done;     // low word of the step whose later attach-resolve rounds have finished
 	// diagnostics of the last step (pcm_get_stats): attachment events, resolver rounds,
 	// resolver chunks, interaction candidates, speculation conflict, removals, occupied voxels
 	int stat[8];

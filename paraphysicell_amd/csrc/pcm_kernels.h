@@ -982,7 +982,8 @@ __device__ __forceinline__ R pair_force_rec(const force_rec<R>& a, const force_r
 // that reads it back in the contraction loop, and lane 0 stores the counters.  Returns the
 // list length.  in_cnt, na and cnt are the agent's in_cnt, own_cnt and spr_cnt.
 template <class R, int G>
-__device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, int lane, int in_cnt, int na, int cnt)
+__device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, int lane, int in_cnt, int na, int cnt,
+												  bool coherent)
 {
 	const int nin = min(in_cnt, c.K);
 	if (nin + na == 0)
@@ -1003,7 +1004,7 @@ __device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, in
 			}
 		}
 		last = best;
-		if (c.own_st[(size_t)best * c.K + best_k] == 1)
+		if (event_state<R>(c, (size_t)best * c.K + best_k, coherent) == 1)
 		{
 			if (cnt < c.L)
 			{
@@ -1016,7 +1017,7 @@ __device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, in
 		}
 	}
 	for (int k = 0; k < na; k++)
-		if (c.own_st[(size_t)a * c.K + k] == 1)
+		if (event_state<R>(c, (size_t)a * c.K + k, coherent) == 1)
 		{
 			const int j = c.own_tgt[(size_t)a * c.K + k];
 			if (cnt < c.L)
@@ -1049,10 +1050,35 @@ __device__ __forceinline__ int spring_append_group(const SimCtx<R>& c, int a, in
 // (lane 0 writes), spring contraction is spread over lanes like the neighbor loop.
 // CLEAR: also clear the voxel counts of the last mesh update (nothing reads them until
 // this step's mesh update)
+// The step's later attach-resolve rounds (over the actors the first round left open) run
+// first in the member's first block, which then marks them done; only the wavefronts whose
+// agents have attachment events wait for the mark, just before the append.  The first
+// block is dispatched before the member's other blocks, so waiting cannot starve it.
 template <class R, int D, bool CLEAR, int G>
 __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
-	PCM_PROLOGUE(c)
+	PCM_GEOMETRY
+	const SimCtx<R>& c = ctxs[slot_];
+	const bool later_rounds = c.springs && c.sc->res_open > 0;
+	const unsigned step_tag = (unsigned)c.sc->step;
+	if (bid_ == 0)
+	{
+		__shared__ int s_go, s_next;
+		if (threadIdx.x == 0)
+			s_go = c.sc->status == 0 && c.springs ? (c.sc->res_open > 0 ? 1 : 2) : 0;
+		__syncthreads();
+		if (s_go == 1)
+		{
+			resolve_rounds<R>(c, &s_next, false);
+			__threadfence();
+			if (threadIdx.x == 0)
+				__hip_atomic_store(&c.sc->res_done, step_tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		else if (s_go == 2 && threadIdx.x == 0)
+			c.sc->stat[1] = 1;
+	}
+	if (c.sc->status)
+		return;
 	constexpr int B = VELOCITY_ROUNDS;
 	const int t = GID;
 	if (CLEAR && t < c.sc->n_binned)
@@ -1183,7 +1209,12 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 		membrane_apply<D, R>(c, me.r, bm_rep, p, v);
 	// the step's attachments join the list first (no separate append launch in a step)
 	if (c.springs)
-		ns = spring_append_group<R, G>(c, i, lane, in_cnt, own_cnt, ns);
+	{
+		const bool wait = later_rounds && __any(in_cnt + own_cnt > 0);
+		if (wait)
+			wait_for_resolve_rounds<R>(c, step_tag);
+		ns = spring_append_group<R, G>(c, i, lane, in_cnt, own_cnt, ns, wait);
+	}
 	if (mov && c.springs)
 	{
 		// spring_contract_function (position_solver.cpp:337-370)
@@ -1494,20 +1525,17 @@ __global__ void __launch_bounds__(64) k_spring_resolve_first(const SimCtx<R>* __
 		c.blk_ev[b] = events;
 }
 
-// Later rounds, one block, over the actors the first round left open.
+// Later rounds, on one block of any size, over the actors the first round left open.
+// clear_open: reset the open count when done (the fused step's velocity kernel keeps it:
+// its wavefronts read it to know whether to wait for these rounds).
 template <class R>
-__global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+__device__ void resolve_rounds(const SimCtx<R>& c, int* s_next, bool clear_open)
 {
-	(void)bmap; // one block per simulation: never flattened
-	const SimCtx<R>& c = ctxs[blockIdx.y];
-	__shared__ int s_next;
-	if (c.sc->status)
-		return;
 	int count = c.sc->res_open;
 	int* cur = c.att_c;
 	int* nxt = c.att_c2;
 	if (threadIdx.x == 0)
-		s_next = 0;
+		*s_next = 0;
 	__syncthreads();
 	volatile uint8_t* st = c.own_st;
 	int rounds = 1;
@@ -1523,15 +1551,15 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 		{
 			const int a = cur[t];
 			if (resolve_actor<R>(c, st, a, c.own_cnt[a]))
-				nxt[atomicAdd(&s_next, 1)] = a;
+				nxt[atomicAdd(s_next, 1)] = a;
 		}
 		__threadfence();
 		__syncthreads();
-		count = s_next;
+		count = *s_next;
 		rounds++;
 		__syncthreads();
 		if (threadIdx.x == 0)
-			s_next = 0;
+			*s_next = 0;
 		int* t = cur;
 		cur = nxt;
 		nxt = t;
@@ -1540,8 +1568,54 @@ __global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx
 	if (threadIdx.x == 0)
 	{
 		c.sc->stat[1] = rounds;
-		c.sc->res_open = 0;
+		if (clear_open)
+			c.sc->res_open = 0;
 	}
+}
+
+// The later rounds as a launch of their own (the springs phase; a fused step runs them in
+// the velocity kernel's first block)
+template <class R>
+__global__ void __launch_bounds__(RESOLVE_THREADS) k_spring_resolve(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
+{
+	(void)bmap; // one block per simulation: never flattened
+	const SimCtx<R>& c = ctxs[blockIdx.y];
+	__shared__ int s_next;
+	if (c.sc->status)
+		return;
+	resolve_rounds<R>(c, &s_next, true);
+}
+
+// A wavefront whose agents have attachment events, in a step whose later resolve rounds
+// run in the velocity kernel's first block, waits for that block's done mark (the step's
+// counter), or for the step being abandoned (a raised status word); a bound on the wait
+// raises ST_CAPACITY instead of hanging.
+template <class R>
+__device__ __forceinline__ void wait_for_resolve_rounds(const SimCtx<R>& c, unsigned tag)
+{
+	for (int spin = 0;; spin++)
+	{
+		if (__hip_atomic_load(&c.sc->res_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == tag)
+			break;
+		if (__hip_atomic_load(&c.sc->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+			break;
+		if (spin > (1 << 22))
+		{
+			atomicOr(&c.sc->status, ST_CAPACITY);
+			break;
+		}
+		__builtin_amdgcn_s_sleep(8);
+	}
+}
+
+// an attachment event's decided state; `coherent` reads past this XCD's L2 (states the
+// velocity kernel's first block decided in this launch)
+template <class R>
+__device__ __forceinline__ int event_state(const SimCtx<R>& c, size_t k, bool coherent)
+{
+	if (coherent)
+		return __hip_atomic_load(&c.own_st[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return c.own_st[k];
 }
 
 // Successful springs are appended to each endpoint's list in the order the reference

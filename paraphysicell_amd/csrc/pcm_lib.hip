@@ -460,6 +460,7 @@ public:
 		sc_.moves = 0;
 		sc_.claim = 0;
 		sc_.log_len = 0;
+		sc_.res_done = ~0u;
 		// stream-ordered copies and fills, one wait at the end (the sources are members)
 		PCM_HIP(hipMemcpyAsync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice, stream_));
 		n_launch_ = n;
@@ -935,6 +936,7 @@ public:
 		detach_ready_ = false;
 		PCM_TRY(read_scalars());
 		sc_.step = s;
+		sc_.res_done = ~0u; // no step's later resolve rounds are marked done
 		PCM_TRY(copy_sync(h_.sc, &sc_, sizeof(Scalars), hipMemcpyHostToDevice));
 		return PCM_OK;
 	}
@@ -1682,8 +1684,12 @@ private:
 				PCM_TRY(launch(PCM_K_SPRING_RESOLVE_FIRST, k_spring_resolve_first<R, 8, IN_STEP>, segments, 64,
 							   MAP_LANES8));
 		}
+		// a fused step runs the later rounds in its velocity kernel's first block, whose
+		// append waits for them
+		if (IN_STEP)
+			return PCM_OK;
 		PCM_TRY(launch(PCM_K_SPRING_RESOLVE, k_spring_resolve<R>, 1, RESOLVE_THREADS));
-		return IN_STEP ? PCM_OK : per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
+		return per_agent(PCM_K_SPRING_APPEND, k_spring_append<R>);
 	}
 
 	template <int D>
