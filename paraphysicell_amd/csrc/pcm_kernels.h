@@ -6,6 +6,8 @@
 // simulation's status word is non-zero (an aborted step).
 #pragma once
 
+#include <type_traits>
+
 #include "pcm_device.h"
 
 namespace pcm {
@@ -975,6 +977,51 @@ __device__ __forceinline__ R pair_force_rec(const force_rec<R>& a, const force_r
 	return term;
 }
 
+// Lane SRC of each aligned group of G lanes, broadcast to the group (G <= 32): ds_swizzle
+// in bit mode with and-mask ~(G - 1) and or-mask SRC, so no address register is needed
+template <int G, int SRC>
+__device__ __forceinline__ float group_bcast(float x)
+{
+	static_assert(G <= 32 && (G & (G - 1)) == 0 && SRC < G, "groups of a power of two lanes within 32");
+	constexpr int pattern = (32 - G) | (SRC << 5);
+	return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), pattern));
+}
+
+template <int G, int SRC>
+__device__ __forceinline__ double group_bcast(double x)
+{
+	static_assert(G <= 32 && (G & (G - 1)) == 0 && SRC < G, "groups of a power of two lanes within 32");
+	constexpr int pattern = (32 - G) | (SRC << 5);
+	const long long b = __double_as_longlong(x);
+	const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffll), pattern);
+	const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), pattern);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// f(integral_constant<int, S>) for S = 0 .. G-1, unrolled at compile time
+template <int G, int S = 0, class F>
+__device__ __forceinline__ void for_each_lane(F&& f)
+{
+	if constexpr (S < G)
+	{
+		f(std::integral_constant<int, S> {});
+		for_each_lane<G, S + 1>(f);
+	}
+}
+
+// v[k] += the group's f[k] in lane order (the ordered sum of one slot chunk).  Lanes past
+// the list hold -0 in f, so their terms leave v exactly as it is; the loop over the group's
+// lanes is unrolled at compile time, so the broadcasts are requested ahead of the adds
+// instead of one shuffle round trip per lane.
+template <int G, int D, class R>
+__device__ __forceinline__ void add_group_terms(R* v, const R* f)
+{
+	for_each_lane<G>([&](auto s) {
+		for (int k = 0; k < D; k++)
+			v[k] += group_bcast<G, decltype(s)::value>(f[k]);
+	});
+}
+
 // Accepted attachments appended to agent a's spring list (the append step of
 // update_spring_attachments_internal, position_solver.cpp:430-449): incoming partners in
 // ascending actor order, then the agent's own accepted events in event order.  The G
@@ -1118,7 +1165,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 		{
 			R f[D];
 			for (int k = 0; k < D; k++)
-				f[k] = 0;
+				f[k] = -(R)0; // -0 + x is exactly x for every x
 			if (base + lane < na)
 			{
 				const force_rec<R> o = read_force_record<R>(c, row[base + lane]);
@@ -1126,10 +1173,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 				for (int k = 0; k < D; k++)
 					f[k] = a * (o.p[k] - p[k]);
 			}
-			const int m = min(G, na - base);
-			for (int src = 0; src < m; src++)
-				for (int k = 0; k < D; k++)
-					v[k] += __shfl(f[k], src, G);
+			add_group_terms<G, D, R>(v, f);
 		}
 	}
 	const type_row<R> aff_row = load_type_row<R>(c.affinity + (size_t)i * c.T, c.T);
@@ -1168,18 +1212,19 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 			}
 		}
 		// the slot terms in list order
+		// (lanes past the list hold -0 forces and +0 pressure terms, and the pressure sum is
+		// never -0, so adding them changes nothing)
 #pragma unroll
 		for (int u = 0; u < B; u++)
 		{
-			const int m = min(G, cnt - base - u * G);
-			for (int src = 0; src < m; src++)
-			{
-				if (mov)
-					for (int k = 0; k < D; k++)
-						v[k] += __shfl(f[u][k], src, G);
-				pressure += __shfl(mine[u], src, G);
-				pressure += __shfl(theirs[u], src, G);
-			}
+			if (base + u * G >= cnt)
+				continue;
+			if (mov)
+				add_group_terms<G, D, R>(v, f[u]);
+			for_each_lane<G>([&](auto s) {
+				pressure += group_bcast<G, decltype(s)::value>(mine[u]);
+				pressure += group_bcast<G, decltype(s)::value>(theirs[u]);
+			});
 		}
 #pragma unroll
 		for (int u = 0; u < B; u++)
@@ -1249,7 +1294,7 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 			for (int u = 0; u < B; u++)
 			{
 				for (int k = 0; k < D; k++)
-					f[u][k] = 0;
+					f[u][k] = -(R)0; // -0 + x is exactly x for every x
 				if (base + u * G + lane < ns)
 				{
 					const R kij = ki * kj[u] * aff_row.at(other[u].type) * aff_ba[u];
@@ -1260,12 +1305,8 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 			}
 #pragma unroll
 			for (int u = 0; u < B; u++)
-			{
-				const int m = min(G, ns - base - u * G);
-				for (int src = 0; src < m; src++)
-					for (int k = 0; k < D; k++)
-						v[k] += __shfl(f[u][k], src, G);
-			}
+				if (base + u * G < ns)
+					add_group_terms<G, D, R>(v, f[u]);
 		}
 	}
 	if (lane == 0)
