@@ -239,27 +239,30 @@ def serving_leg(count: int, rate: float, steps: int):
         return [Request(k, float(times[k]), mesh, st, params, steps, seed=1000 + k)
                 for k, (mesh, st, params) in enumerate(specs)]
 
-    server = Server(budget_bytes=200e9, chunk=10, graphs=True)
+    # the simulations resident on the device first: batched steps against one at a time,
+    # each with the launch shape the tuner picks for it (per group, or per simulation)
+    from paraphysicell_amd.serving import run_ensemble
+
+    members = [(mesh, st, 1000 + k) for k, (mesh, st, _) in enumerate(specs)]
+    group_shapes, member_shapes = {}, {}
+    t_rb, as_rb, sims = run_ensemble(members, specs[0][2], steps, batched=True, shapes=group_shapes)
+    for sim in sims:
+        sim.close()
+    t_rs, as_rs, sims = run_ensemble(members, specs[0][2], steps, batched=False, shapes=member_shapes)
+    for sim in sims:
+        sim.close()
+    # served: the groups' launch shapes tuned above on the same kind of workload
+    server = Server(budget_bytes=200e9, chunk=10, graphs=True, tuning=group_shapes)
     done = server.run(requests(arrivals), realtime=True)
     served_rate = server.agent_steps / server.elapsed
     lat = np.array([q.latency for q in done]) * 1000.0
     server.close()
-    sat = Server(budget_bytes=200e9, chunk=10, graphs=True)
+    sat = Server(budget_bytes=200e9, chunk=10, graphs=True, tuning=group_shapes)
     sat.run(requests(np.zeros(count)), realtime=False)
     sat_rate = sat.agent_steps / sat.elapsed
     sat.close()
-    t_alone, as_alone = run_alone(requests(np.zeros(count)))
+    t_alone, as_alone = run_alone(requests(np.zeros(count)), shapes=[member_shapes[k] for k in range(count)])
     alone_rate = as_alone / t_alone
-    # the same simulations resident on the device: batched steps against one at a time
-    from paraphysicell_amd.serving import run_ensemble
-
-    members = [(mesh, st, 1000 + k) for k, (mesh, st, _) in enumerate(specs)]
-    t_rb, as_rb, sims = run_ensemble(members, specs[0][2], steps, batched=True)
-    for sim in sims:
-        sim.close()
-    t_rs, as_rs, sims = run_ensemble(members, specs[0][2], steps, batched=False)
-    for sim in sims:
-        sim.close()
     agents = [st.count for _, st, _ in specs]
     return {
         "workload": f"{count} requests, Poisson {rate}/s, {min(agents)}-{max(agents)} agents (mean "
@@ -275,6 +278,7 @@ def serving_leg(count: int, rate: float, steps: int):
         "resident_batched_throughput": round(as_rb / t_rb, 1),
         "resident_one_at_a_time_throughput": round(as_rs / t_rs, 1),
         "resident_speedup": round((as_rb / t_rb) / (as_rs / t_rs), 2),
+        "served_launch_shapes": {f"{d}d": list(v) for d, v in sorted(group_shapes.items())},
         "latency": "arrival to result (upload, mesh build, steps, read-back included)",
         "throughput_note": "throughput follows the 20/s arrival rate; saturated_* runs every request at once "
                            "(set-up included), resident_* steps the same simulations already on the device",

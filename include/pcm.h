@@ -197,6 +197,10 @@ int pcm_get_field(pcm_sim* sim, int32_t field, void* host);
  * field is checked before any is copied, the copies are queued together and the call waits
  * once, so uploading a whole cell_data costs one round trip instead of one per field. */
 int pcm_set_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, const void* const* hosts);
+/* pcm_get_field for `nfields` fields in one call (fields[k] written to hosts[k]): one read
+ * of the count, the copies queued together through page-locked staging, one wait (the
+ * read-back of a finished request in a serving loop). */
+int pcm_get_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, void* const* hosts);
 /* springs / attached_cells as CSR (offsets has count+1 entries) */
 int pcm_set_links(pcm_sim* sim, int32_t which, const int32_t* offsets, const int32_t* indices);
 int pcm_get_links_size(pcm_sim* sim, int32_t which, int64_t* total);
@@ -358,6 +362,15 @@ int pcm_batch_flatten(pcm_batch* batch, int32_t enable);
 /* pcm_autotune for every dimensionality group of the batch, on the batch's own steps (one
  * shape per group, used by its segmented launches); members advance 1 + 6 * steps. */
 int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t steps_per_candidate);
+/* The launch shape of the batch's `dims` group (1, 2 or 3), as pcm_set_tuning sets it for
+ * one simulation: kept for members that join later, so a server can apply a shape tuned
+ * earlier on the same kind of workload instead of tuning on live requests. */
+int pcm_batch_set_tuning(pcm_batch* batch, int32_t dims, const pcm_tuning* tuning);
+int pcm_batch_get_tuning(pcm_batch* batch, int32_t dims, pcm_tuning* tuning);
+/* The agent count of each of `count` members as the batch's last pcm_batch_step left it
+ * (every step call gathers all members' scalars at its end): no device access and no wait,
+ * unlike pcm_get_count.  Fails with PCM_ERR_ARGUMENT for a simulation that is not a member. */
+int pcm_batch_counts(pcm_batch* batch, int32_t count, pcm_sim* const* sims, int32_t* counts);
 
 /* ---------------------------------------------------------------- kernel timing */
 /* Counterpart of the reference's per-phase timers (simulator.cpp:14-29,

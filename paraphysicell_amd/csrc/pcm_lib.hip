@@ -165,6 +165,115 @@ device_pool& pool()
 	return *p;
 }
 
+// ---------------------------------------------------------------------------- pinned staging and streams
+// Reads of device arrays into caller (pageable) memory go device -> page-locked block (DMA,
+// every array of a call queued together) -> one wait -> host copies out.  A staged pageable
+// copy instead waits once per array and shares the runtime's staging buffer with every
+// other thread's copies, which serialises the read-back of concurrent tear-downs.  Blocks
+// are powers of two from 64 KiB, kept for reuse up to PINNED_CACHE_LIMIT.
+constexpr size_t PINNED_CACHE_LIMIT = 1ull << 30;
+
+class pinned_pool
+{
+public:
+	int take(size_t bytes, void** out, size_t* got)
+	{
+		size_t want = 64u << 10;
+		while (want < bytes)
+			want <<= 1;
+		{
+			std::lock_guard<std::mutex> lock(mu_);
+			auto it = free_.find(want);
+			if (it != free_.end())
+			{
+				*out = it->second;
+				*got = want;
+				cached_ -= want;
+				free_.erase(it);
+				return PCM_OK;
+			}
+		}
+		void* p = nullptr;
+		PCM_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+		*out = p;
+		*got = want;
+		return PCM_OK;
+	}
+
+	void give(void* p, size_t bytes)
+	{
+		std::lock_guard<std::mutex> lock(mu_);
+		if (cached_ + bytes > PINNED_CACHE_LIMIT)
+		{
+			(void)hipHostFree(p);
+			return;
+		}
+		free_.emplace(bytes, p);
+		cached_ += bytes;
+	}
+
+private:
+	std::mutex mu_;
+	std::multimap<size_t, void*> free_;
+	size_t cached_ = 0;
+};
+
+pinned_pool& pinned()
+{
+	static pinned_pool* p = new pinned_pool(); // never destroyed, like the device pool
+	return *p;
+}
+
+// Streams of destroyed handles are kept for later handles of the same device: creating and
+// destroying a stream per request costs the serving loop's set-up and tear-down threads
+// a runtime round trip each.  A stream is given back only once it is idle.  All streams
+// have the default priority.
+constexpr size_t STREAM_CACHE_LIMIT = 256;
+
+class stream_pool
+{
+public:
+	int take(hipStream_t* out)
+	{
+		int dev = 0;
+		PCM_HIP(hipGetDevice(&dev));
+		{
+			std::lock_guard<std::mutex> lock(mu_);
+			for (auto it = free_.begin(); it != free_.end(); ++it)
+				if (it->first == dev)
+				{
+					*out = it->second;
+					free_.erase(it);
+					return PCM_OK;
+				}
+		}
+		PCM_HIP(hipStreamCreateWithFlags(out, hipStreamNonBlocking));
+		return PCM_OK;
+	}
+
+	// `dev`: the device the stream was created on
+	void give(hipStream_t s, int dev)
+	{
+		std::lock_guard<std::mutex> lock(mu_);
+		if (free_.size() >= STREAM_CACHE_LIMIT)
+		{
+			(void)hipStreamDestroy(s);
+			return;
+		}
+		free_.emplace_back(dev, s);
+	}
+
+private:
+	std::mutex mu_;
+	std::vector<std::pair<int, hipStream_t>> free_;
+};
+
+stream_pool& streams()
+{
+	static stream_pool* p = new stream_pool();
+	return *p;
+}
+
 } // namespace
 
 struct pcm_sim
@@ -176,6 +285,7 @@ struct pcm_sim
 	virtual int set_field(int f, const void* src) = 0;
 	virtual int set_fields(int nfields, const int32_t* ids, const void* const* srcs) = 0;
 	virtual int get_field(int f, void* dst) = 0;
+	virtual int get_fields(int nfields, const int32_t* ids, void* const* dsts) = 0;
 	virtual int set_links(int which, const int32_t* off, const int32_t* idx) = 0;
 	virtual int get_links(int which, int32_t* off, int32_t* idx, int64_t* total) = 0;
 	virtual int get_neighbors(int32_t* off, int32_t* idx, int64_t* total) = 0;
@@ -213,6 +323,9 @@ struct pcm_batch
 	virtual int step(const pcm_step_params* p, int steps) = 0;
 	virtual int synchronize() = 0;
 	virtual int autotune(const pcm_step_params* p, int steps) = 0;
+	virtual int counts(int n, pcm_sim* const* sims, int32_t* out) = 0;
+	virtual int set_tuning(int dims, const pcm_tuning* t) = 0;
+	virtual int get_tuning(int dims, pcm_tuning* t) = 0;
 };
 
 namespace {
@@ -239,8 +352,22 @@ enum map_kind
 
 inline int map_of_lanes(int lanes) { return lanes == 4 ? MAP_LANES4 : (lanes == 16 ? MAP_LANES16 : MAP_LANES8); }
 
-// blocks of one member in a launch of `kind`, for n agents and L spring slots
-inline int member_blocks(int kind, long long n, long long L)
+// x rounded up to four significant bits (at most 1/8 above it): eight values per octave, so
+// grid sizes and block counts derived from slowly drifting counts change rarely
+inline int round_up_sig4(int x)
+{
+	if (x <= 8)
+		return std::max(x, 1);
+	int s = 0;
+	while ((x >> s) >= 16)
+		s++;
+	const long long step = 1ll << s;
+	return (int)(((x + step - 1) / step) * step);
+}
+
+// blocks of one member in a launch of `kind`, for n agents, L spring slots and `nbr` blocks
+// of the neighbor search (from the member's occupied voxels)
+inline int member_blocks(int kind, long long n, long long L, int nbr)
 {
 	auto ceil_div = [](long long a, long long b) { return (a + b - 1) / b; };
 	long long b = 1;
@@ -265,7 +392,7 @@ inline int member_blocks(int kind, long long n, long long L)
 			b = ceil_div(n, CLAIM_THREADS);
 			break;
 		case MAP_NEIGHBORS:
-			b = std::min<long long>(n, 16384);
+			b = std::min<long long>(n, nbr);
 			break;
 		case MAP_COMPACT:
 			return MOVE_BLOCKS + (int)std::max<long long>(1, ceil_div(n, BLOCK));
@@ -300,8 +427,9 @@ public:
 	{
 		if (stream_)
 		{
+			// the handle's own stream goes back to the stream cache once idle
 			(void)hipStreamSynchronize(stream_);
-			(void)hipStreamDestroy(stream_);
+			streams().give(stream_, stream_device_);
 		}
 		for (auto& r : prof_records_)
 		{
@@ -347,7 +475,8 @@ public:
 		h.dt = (R)0.1;
 		h.springs = 1;
 
-		PCM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+		PCM_HIP(hipGetDevice(&stream_device_));
+		PCM_TRY(streams().take(&stream_));
 		in_init_ = true;
 		const int rc = init_storage();
 		in_init_ = false;
@@ -602,11 +731,23 @@ public:
 
 	int get_field(int f, void* dst) override
 	{
-		if (f < 0 || f >= PCM_FIELD_COUNT || !dst)
-			return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer");
+		return get_fields(1, &f, &dst);
+	}
+
+	// Several fields in one call: one read of the count, the copies queued together into a
+	// pinned block, one wait.
+	int get_fields(int nfields, const int32_t* ids, void* const* dsts) override
+	{
+		if (nfields < 0 || (nfields > 0 && (!ids || !dsts)))
+			return fail(PCM_ERR_ARGUMENT, "null field table");
+		for (int k = 0; k < nfields; k++)
+			if (ids[k] < 0 || ids[k] >= PCM_FIELD_COUNT || !dsts[k])
+				return fail(PCM_ERR_ARGUMENT, "bad field id or null buffer at entry " + std::to_string(k));
 		PCM_TRY(read_scalars());
-		PCM_TRY(copy_sync(dst, h_.fptr[f], (size_t)sc_.n * h_.fbytes[f], hipMemcpyDeviceToHost));
-		return PCM_OK;
+		std::vector<staged_read> reads(nfields);
+		for (int k = 0; k < nfields; k++)
+			reads[k] = { dsts[k], h_.fptr[ids[k]], (size_t)sc_.n * h_.fbytes[ids[k]] };
+		return read_staged(reads.data(), nfields);
 	}
 
 	int set_links(int which, const int32_t* off, const int32_t* idx) override
@@ -663,13 +804,15 @@ public:
 		PCM_TRY(read_scalars());
 		const int n = sc_.n;
 		const int W = which == PCM_LINKS_SPRINGS ? h_.L : h_.LA;
-		std::vector<int> cnt(std::max(n, 1)), rows((size_t)std::max(n, 1) * W);
+		// the rows only when entries are wanted (a size query needs the counts)
+		std::vector<int> cnt(std::max(n, 1)), rows(idx ? (size_t)std::max(n, 1) * W : 0);
 		if (n > 0)
 		{
-			PCM_TRY(copy_sync(cnt.data(), which == PCM_LINKS_SPRINGS ? h_.spr_cnt : h_.atc_cnt, sizeof(int) * n,
-							  hipMemcpyDeviceToHost));
-			PCM_TRY(copy_sync(rows.data(), which == PCM_LINKS_SPRINGS ? h_.spr_idx : h_.atc_idx,
-							  sizeof(int) * (size_t)n * W, hipMemcpyDeviceToHost));
+			const staged_read reads[2] = {
+				{ cnt.data(), which == PCM_LINKS_SPRINGS ? h_.spr_cnt : h_.atc_cnt, sizeof(int) * (size_t)n },
+				{ rows.data(), which == PCM_LINKS_SPRINGS ? h_.spr_idx : h_.atc_idx, idx ? sizeof(int) * (size_t)n * W : 0 },
+			};
+			PCM_TRY(read_staged(reads, 2));
 		}
 		int64_t t = 0;
 		for (int i = 0; i < n; i++)
@@ -1088,6 +1231,7 @@ private:
 	SimCtx<R>* d_ = nullptr;
 	Scalars sc_ {};
 	hipStream_t stream_ = nullptr;
+	int stream_device_ = 0; // the device stream_ was created on
 	struct allocation
 	{
 		void* ptr;
@@ -1351,6 +1495,50 @@ private:
 	// a copy ordered on this simulation's stream, waited for before returning: copies of
 	// different simulations (a serving loop's worker threads) do not queue behind each
 	// other on the null stream
+	struct staged_read
+	{
+		void* dst;       // host destination (any memory)
+		const void* src; // device source
+		size_t bytes;
+	};
+
+	// device -> pinned block (all reads queued together) -> one wait -> host copies out
+	int read_staged(const staged_read* reads, int count)
+	{
+		size_t total = 0;
+		for (int k = 0; k < count; k++)
+			total += (reads[k].bytes + 255) & ~(size_t)255;
+		if (total == 0)
+			return PCM_OK;
+		void* block = nullptr;
+		size_t got = 0;
+		PCM_TRY(pinned().take(total, &block, &got));
+		size_t off = 0;
+		hipError_t e = hipSuccess;
+		for (int k = 0; k < count && e == hipSuccess; k++)
+		{
+			if (reads[k].bytes)
+				e = hipMemcpyAsync((char*)block + off, reads[k].src, reads[k].bytes, hipMemcpyDeviceToHost, stream_);
+			off += (reads[k].bytes + 255) & ~(size_t)255;
+		}
+		// wait even after a failed enqueue: the block must be idle before it is reused
+		const hipError_t w = hipStreamSynchronize(stream_);
+		if (e == hipSuccess && w == hipSuccess)
+		{
+			off = 0;
+			for (int k = 0; k < count; k++)
+			{
+				if (reads[k].bytes)
+					std::memcpy(reads[k].dst, (const char*)block + off, reads[k].bytes);
+				off += (reads[k].bytes + 255) & ~(size_t)255;
+			}
+		}
+		pinned().give(block, got);
+		PCM_HIP(e);
+		PCM_HIP(w);
+		return PCM_OK;
+	}
+
 	int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
 	{
 		if (bytes == 0)
@@ -1824,6 +2012,17 @@ public:
 	int dims() const { return h_.dims; }
 	int launch_bound() const { return n_launch_; }
 	int spring_slots() const { return h_.L; }
+	// neighbor-search blocks of this member in a batch's flattened launch: its occupied
+	// voxels at the last step seen, with headroom, rounded to four significant bits so that
+	// the block maps (and the captured graph) change only when the count drifts by about
+	// 1/8; the kernel strides over voxels beyond its blocks, so any count is correct
+	int neighbor_map_blocks() const
+	{
+		int b = std::min(n_launch_, 16384);
+		if (sc_.n_occ > 0)
+			b = std::min(b, round_up_sig4(sc_.n_occ + sc_.n_occ / 4 + 64));
+		return std::max(1, b);
+	}
 	bool records_fresh() const { return records_fresh_; }
 	bool detach_ready() const { return detach_ready_; }
 	bool binned_within(int grid_n) const { return binned_launch_ <= grid_n; }
@@ -2093,6 +2292,39 @@ public:
 	void use_graphs(bool on) { graphs_ = on; }
 	void use_flattened_grids(bool on) { flatten_ = on; }
 
+	// agent counts of members as the batch's last step left them: the scalars each step call
+	// gathers for all members at its end (no device access)
+	int counts(int n, pcm_sim* const* sims, int32_t* out) override
+	{
+		for (int k = 0; k < n; k++)
+		{
+			auto* sim = dynamic_cast<sim_impl<R>*>(sims[k]);
+			if (!sim || sim->batch() != static_cast<pcm_batch*>(this))
+				return fail(PCM_ERR_ARGUMENT, "entry " + std::to_string(k) + " is not a member of this batch");
+			out[k] = sim->scalars().n;
+		}
+		return PCM_OK;
+	}
+
+	// the launch shape of a dimensionality group (kept for members that join later)
+	int set_tuning(int dims, const pcm_tuning* t) override
+	{
+		if (dims < 1 || dims > 3)
+			return fail(PCM_ERR_ARGUMENT, "dims must be 1, 2 or 3");
+		group& g = groups_[dims - 1];
+		PCM_TRY(ensure_group(g, dims));
+		return g.placeholder->set_tuning(t);
+	}
+
+	int get_tuning(int dims, pcm_tuning* t) override
+	{
+		if (dims < 1 || dims > 3)
+			return fail(PCM_ERR_ARGUMENT, "dims must be 1, 2 or 3");
+		group& g = groups_[dims - 1];
+		PCM_TRY(ensure_group(g, dims));
+		return g.placeholder->get_tuning(t);
+	}
+
 	// tune each group's launch shape on its real steps (graphs are off while tuning)
 	int autotune(const pcm_step_params* p, int steps) override
 	{
@@ -2174,6 +2406,7 @@ private:
 			auto* sim = g.slots[k].sim;
 			sig.push_back(sim ? sim->launch_bound() : -1);
 			sig.push_back(sim ? sim->spring_slots() : 0);
+			sig.push_back(sim ? sim->neighbor_map_blocks() : 0);
 		}
 		if (g.d_map && sig == g.map_sig)
 			return PCM_OK;
@@ -2187,7 +2420,7 @@ private:
 				auto* sim = g.slots[k].sim;
 				if (!sim)
 					continue;
-				const int b = member_blocks(kind, sim->launch_bound(), sim->spring_slots());
+				const int b = member_blocks(kind, sim->launch_bound(), sim->spring_slots(), sim->neighbor_map_blocks());
 				for (int x = 0; x < b; x++)
 					g.h_map.push_back(make_int4(k, x, b, 0));
 				total += b;
@@ -2280,16 +2513,7 @@ private:
 	// a group's grid bound: its largest member rounded up to four significant bits, at most
 	// 1/8 above it (a power of two wasted up to half of the blocks); eight bounds per octave
 	// keep graph recaptures rare as members join and leave
-	static int round_up_grid(int x)
-	{
-		if (x <= 8)
-			return std::max(x, 1);
-		int s = 0;
-		while ((x >> s) >= 16)
-			s++;
-		const long long step = 1ll << s;
-		return (int)(((x + step - 1) / step) * step);
-	}
+	static int round_up_grid(int x) { return round_up_sig4(x); }
 
 	int launch_group(group& g, const pcm_step_params* p, int steps)
 	{
@@ -2756,6 +2980,27 @@ int pcm_batch_autotune(pcm_batch* batch, const pcm_step_params* params, int32_t 
 	return batch->autotune(params, steps_per_candidate);
 }
 
+int pcm_batch_counts(pcm_batch* batch, int32_t count, pcm_sim* const* sims, int32_t* counts)
+{
+	if (!batch || count < 0 || (count > 0 && (!sims || !counts)))
+		return fail(PCM_ERR_ARGUMENT, "null argument");
+	return batch->counts(count, sims, counts);
+}
+
+int pcm_batch_set_tuning(pcm_batch* batch, int32_t dims, const pcm_tuning* tuning)
+{
+	if (!batch)
+		return fail(PCM_ERR_ARGUMENT, "null handle");
+	return batch->set_tuning(dims, tuning);
+}
+
+int pcm_batch_get_tuning(pcm_batch* batch, int32_t dims, pcm_tuning* tuning)
+{
+	if (!batch || !tuning)
+		return fail(PCM_ERR_ARGUMENT, "null argument");
+	return batch->get_tuning(dims, tuning);
+}
+
 int pcm_batch_use_graphs(pcm_batch* batch, int32_t enable)
 {
 	if (!batch)
@@ -2790,6 +3035,12 @@ int pcm_set_field(pcm_sim* sim, int32_t field, const void* host)
 {
 	PCM_NEED(sim);
 	return sim->set_field(field, host);
+}
+
+int pcm_get_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, void* const* hosts)
+{
+	PCM_NEED(sim);
+	return sim->get_fields(nfields, fields, hosts);
 }
 
 int pcm_set_fields(pcm_sim* sim, int32_t nfields, const int32_t* fields, const void* const* hosts)

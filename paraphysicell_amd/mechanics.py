@@ -184,6 +184,29 @@ class _LibSim:
         self._call("get_field", self._h, abi.FIELD_ID[name], out.ctypes.data)
         return out[: n * w]
 
+    def get_fields(self, names) -> dict:
+        """Several fields in one call (pcm_get_fields: one count read, one wait); field by
+        field where the library has no bulk read."""
+        bulk = getattr(self._lib, self.prefix + "get_fields", None)
+        if bulk is None:
+            return {name: self.get_field(name) for name in names}
+        n = self.count
+        out, widths = {}, {}
+        for name in names:
+            ftype, wkind = abi.FIELDS[abi.FIELD_ID[name]][1:]
+            widths[name] = abi.field_width(wkind, self.mesh.dims, self.defs, self.subs)
+            out[name] = np.zeros(max(n * widths[name], 1), dtype=abi.field_dtype(ftype, self.real_bytes))
+        if out:
+            ids = (ctypes.c_int32 * len(out))(*[abi.FIELD_ID[name] for name in out])
+            ptrs = (ctypes.c_void_p * len(out))(*[a.ctypes.data for a in out.values()])
+            bulk.argtypes = [_PTR, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+            bulk.restype = ctypes.c_int
+            rc = bulk(self._h, len(out), ids, ptrs)
+            if rc != abi.PCM_OK:
+                msg = self._fn("last_error")()
+                raise PcmError(rc, msg.decode() if msg else "")
+        return {name: a[: n * widths[name]] for name, a in out.items()}
+
     def download(self) -> AgentState:
         n = self.count
         st = AgentState(self.mesh, self.real_bytes, self.defs, self.subs, n)
@@ -511,6 +534,28 @@ class Batch:
         """Pick each dimensionality group's launch shape on the batch's own steps."""
         p = None if params is None else ctypes.byref(params.to_c())
         self._check(self._lib.pcm_batch_autotune(self._h, p, int(steps_per_candidate)))
+
+    def tuning(self, dims: int) -> tuple:
+        """The launch shape of the `dims` group (velocity lanes, event lanes, neighbor waves)."""
+        t = abi.Tuning()
+        self._check(self._lib.pcm_batch_get_tuning(self._h, int(dims), ctypes.byref(t)))
+        return t.as_tuple()
+
+    def set_tuning(self, dims: int, value) -> None:
+        """Launch shape of the `dims` group, kept for members that join later."""
+        t = abi.Tuning(*value)
+        self._check(self._lib.pcm_batch_set_tuning(self._h, int(dims), ctypes.byref(t)))
+
+    def counts(self, sims) -> list[int]:
+        """Agent counts of member simulations as the last step call left them (the scalars
+        the batch gathered then; no device read, unlike DeviceSim.count)."""
+        sims = list(sims)
+        if not sims:
+            return []
+        handles = (_PTR * len(sims))(*[s._h for s in sims])
+        out = (ctypes.c_int32 * len(sims))()
+        self._check(self._lib.pcm_batch_counts(self._h, len(sims), handles, out))
+        return list(out)
 
     def close(self) -> None:
         if self._h:

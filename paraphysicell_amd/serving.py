@@ -71,7 +71,7 @@ def _start(req: Request) -> DeviceSim:
 
 
 def _result(sim: DeviceSim) -> dict:
-    out = {name: sim.get_field(name) for name in ("positions", "velocities", "simple_pressure", "damage")}
+    out = sim.get_fields(("positions", "velocities", "simple_pressure", "damage"))
     out.update(count=sim.count, deaths=sim.deaths_count, springs=sim.links(abi.LINKS_SPRINGS))
     return out
 
@@ -91,32 +91,45 @@ class Server:
     (handle, upload, first mesh and neighbor build) and tear-down (result read-back, free)
     run on worker threads, each on the request's own stream, so they overlap the batch's
     steps on the device.  A request counts against the HBM budget from the start of its
-    set-up to the end of its tear-down."""
+    set-up to the end of its tear-down.
 
-    def __init__(self, budget_bytes: float = 200e9, chunk: int = 10, graphs: bool = True, workers: int = 8):
+    `tuning` maps a dimensionality (2 or 3) to a launch shape (velocity lanes, event lanes,
+    neighbor waves) for that group's launches, e.g. one that pcm_batch_autotune picked on
+    the same kind of workload; requests are not tuned on while they are served."""
+
+    def __init__(self, budget_bytes: float = 200e9, chunk: int = 10, graphs: bool = True, workers: int = 8,
+                 tuning: dict | None = None):
         from concurrent.futures import ThreadPoolExecutor
 
         self.budget = budget_bytes
         self.chunk = chunk
         self.batch = Batch(4, graphs=graphs)
+        for dims, shape in (tuning or {}).items():
+            self.batch.set_tuning(dims, shape)
         self.used = 0
         self.agent_steps = 0
         self.workers = ThreadPoolExecutor(max_workers=max(1, workers), thread_name_prefix="pcm-serve")
 
     def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
-        """Serve `requests` (sorted by arrival).  With realtime, a request only becomes
-        visible at its arrival time; without, all are present at once (saturation)."""
-        from concurrent.futures import ALL_COMPLETED, FIRST_COMPLETED, wait
+        """Serve `requests`.  With realtime, a request only becomes visible at its arrival
+        time; without, all are present at once (saturation).
 
-        pending = sorted(requests, key=lambda q: q.arrival)
+        Requests start their set-up in arrival order, the largest first among those that
+        arrive together (each request takes the same number of steps, so a large one that
+        joined last would step on alone at the end).  The batch steps in chunks whenever it
+        has members; finished set-ups join, and finished requests leave, between chunks."""
+        from concurrent.futures import FIRST_COMPLETED, wait
+
+        pending = sorted(requests, key=lambda q: (q.arrival, -q.agents, q.key))
         starting: list[tuple[Request, object]] = []   # set-up in flight
         live: list[Request] = []                       # in the batch
         finishing: list[tuple[Request, object]] = []  # tear-down in flight
+        counts: dict[int, int] = {}                    # agents of each live request (by key)
         done: list[Request] = []
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         while pending or starting or live or finishing:
-            # arrived requests start their set-up while the budget allows, in arrival order
+            # arrived requests start their set-up while the budget allows
             while pending and (not realtime or pending[0].arrival <= now()):
                 q = pending[0]
                 need = footprint_bytes(q.mesh, q.state.real_bytes, q.state.defs, q.state.subs, q.agents)
@@ -134,6 +147,7 @@ class Server:
                 q.sim = fut.result()
                 q.remaining = q.steps
                 self.batch.admit(q.sim)
+                counts[q.key] = q.agents
                 live.append(q)
             for item in [x for x in finishing if x[1].done()]:
                 q, fut = item
@@ -142,31 +156,30 @@ class Server:
                 q.finished = ready - t0
                 self.used -= q.need
                 done.append(q)
-            # An empty batch waits for every set-up in flight, and a batch waits while at
-            # least as many requests are setting up as stepping: they join within a few
-            # milliseconds, while stepping the few live ones alone meanwhile would leave the
-            # GPU mostly idle now and the late joiners stepping alone at the end.
-            if starting and (not live or len(starting) >= len(live)):
-                wait([f for _, f in starting], timeout=0.05,
-                     return_when=ALL_COMPLETED if not live else FIRST_COMPLETED)
-                continue
             if not live:
-                futures = [f for _, f in finishing]
+                # nothing to step: wait for a set-up or tear-down to end, or the next arrival
+                futures = [f for _, f in starting] + [f for _, f in finishing]
+                limit = 0.05
+                if pending and realtime:
+                    limit = min(limit, max(0.0, pending[0].arrival - now()))
                 if futures:
-                    wait(futures, timeout=0.05, return_when=FIRST_COMPLETED)
+                    wait(futures, timeout=limit, return_when=FIRST_COMPLETED)
                 elif pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
+                    time.sleep(min(limit, 0.002))
                 continue
             k = min(self.chunk, min(q.remaining for q in live))
-            before = sum(q.sim.count for q in live)
+            before = sum(counts[q.key] for q in live)
             self.batch.step(None, k)  # every member under its own request's parameters
-            after = sum(q.sim.count for q in live)
-            self.agent_steps += 0.5 * (before + after) * k
+            after_counts = self.batch.counts([q.sim for q in live])
+            for q, n in zip(live, after_counts):
+                counts[q.key] = n
+            self.agent_steps += 0.5 * (before + sum(after_counts)) * k
             for q in list(live):
                 q.remaining -= k
                 if q.remaining == 0:
                     self.batch.retire(q.sim)
                     live.remove(q)
+                    del counts[q.key]
                     finishing.append((q, self.workers.submit(_finish, q.sim)))
                     q.sim = None
         self.elapsed = now()
@@ -177,12 +190,16 @@ class Server:
         self.batch.close()
 
 
-def run_alone(requests: list[Request]) -> tuple[float, int]:
-    """Each request on its own, one after another: (seconds, agent-steps)."""
+def run_alone(requests: list[Request], shapes: list | None = None) -> tuple[float, int]:
+    """Each request on its own, one after another: (seconds, agent-steps).  `shapes`: the
+    launch shape of each request (as the tuner picked it for that simulation alone), or None
+    for the library default."""
     t0 = time.perf_counter()
     agent_steps = 0
-    for q in requests:
+    for k, q in enumerate(requests):
         sim = _start(q)
+        if shapes is not None:
+            sim.tuning = shapes[k]
         before = sim.count
         sim.step(q.params, q.steps)
         agent_steps += 0.5 * (before + sim.count) * q.steps
@@ -198,10 +215,12 @@ def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
 
 
 def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, graphs: bool = True,
-                 autotune: bool = True):
+                 autotune: bool = True, shapes: dict | None = None):
     """Advance every (mesh, state, seed) member `steps` steps; returns (seconds, agent-steps,
     sims).  The timed region covers the steps only: states are resident and the launch
-    shape tuned (batch groups, or each simulation alone) before it starts."""
+    shape tuned (batch groups, or each simulation alone) before it starts.  The shapes used
+    go into `shapes`: dims -> shape of that group when batched, member index -> its shape
+    when one at a time."""
     sims = []
     for mesh, st, seed in members:
         sim = DeviceSim(mesh, st.real_bytes, st.defs, st.subs, capacity=st.count, seed=seed)
@@ -216,12 +235,17 @@ def run_ensemble(members, params: abi.Params, steps: int, batched: bool = True, 
             batch.admit(s)
         if autotune:
             batch.autotune(params, 2)  # launch shape per dims group, on real steps
+        if shapes is not None:
+            for d in sorted({s.mesh.dims for s in sims}):
+                shapes[d] = batch.tuning(d)
         batch.step(params, 1)  # warm-up: first-step refresh and graph capture
         batch.synchronize()
     else:
-        for s in sims:
+        for k, s in enumerate(sims):
             if autotune:
                 s.autotune(params, 2)
+            if shapes is not None:
+                shapes[k] = s.tuning
             s.step(params, 1)
             s.synchronize()
     t0 = time.perf_counter()
