@@ -394,6 +394,9 @@ constexpr int nbr_tile()
 	return 448;
 }
 
+// far-away records staged after the candidates (one 64-candidate chunk)
+constexpr int NBR_PAD = 64;
+
 // a coordinate whose square overflows: no staged padding record is within reach
 template <class R>
 constexpr R far_coordinate()
@@ -436,6 +439,55 @@ __device__ __forceinline__ bool candidate_within(const R* pi, const R* q)
 		}
 		return within_reach_lanes<R>(s, pi[3] + q[3]);
 	}
+}
+
+// The reach test of a staged candidate without a branch in the common case.  d is
+// |pi - q|^2 - reach^2 with the reference's rounded coordinate differences and reach, the
+// squares summed and the difference taken with fused multiply-adds.  Against the
+// reference's decision, sqrt(s) <= reach with s rounded in its own order: |s_fma - s| is at
+// most 6u*s (u the unit roundoff), and sqrt(s) <= reach holds exactly when s is below
+// (reach + ulp/2)^2, within reach^2 * (1 + 2u); the rounding of d adds u*|d|.  Near the
+// boundary s <= 2 reach^2, so the sign of d is the reference's decision whenever |d| > 16u
+// reach^2.  `band_limit` is 64u times an upper bound of reach^2 for every candidate of the
+// agent, plus a floor far above the subnormal range: lanes with |d| <= band_limit (rare)
+// take the exact test instead.  A padding record's coordinate overflows s to +inf, so d is
+// +inf: out, and not in the band.
+template <class R>
+__device__ __forceinline__ R fused_mul_add(R a, R b, R c)
+{
+	if constexpr (sizeof(R) == 4)
+		return __builtin_fmaf(a, b, c);
+	else
+		return __builtin_fma(a, b, c);
+}
+
+template <int D, class R>
+__device__ __forceinline__ R reach_excess(const R* pi, const R* q, R band_limit, bool& band)
+{
+	const R d0 = pi[0] - q[0], d1 = pi[1] - q[1];
+	R s;
+	if constexpr (D == 3)
+	{
+		const R d2 = pi[2] - q[2];
+		s = fused_mul_add<R>(d0, d0, fused_mul_add<R>(d1, d1, d2 * d2));
+	}
+	else
+		s = fused_mul_add<R>(d0, d0, d1 * d1);
+	const R reach = pi[3] + q[3];
+	const R d = fused_mul_add<R>(-reach, reach, s);
+	band = (d < (R)0 ? -d : d) <= band_limit;
+	return d;
+}
+
+// the band limit of an agent whose reach term is mi, among candidates whose largest reach
+// term is rmax (rmax >= mi, the agent being among them)
+template <class R>
+__device__ __forceinline__ R reach_band_limit(R mi, R rmax)
+{
+	const R u = sizeof(R) == 4 ? (R)5.9604644775390625e-8 : (R)1.1102230246251565e-16;
+	const R floor_ = sizeof(R) == 4 ? (R)1e-30 : (R)1e-290;
+	const R r = mi + rmax;
+	return (R)64 * u * (r * r) * (R)1.001 + floor_;
 }
 
 // One agent's list from its 3^d stencil, in stencil order and bucket order: position pi and
@@ -526,14 +578,14 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 {
 	PCM_PROLOGUE_NEIGHBORS(c)
 	constexpr int TILE = nbr_tile<R>();
-	__shared__ R s_rec[(TILE + 64) * 4]; // candidates, then 64 far-away records
+	__shared__ R s_rec[(TILE + NBR_PAD) * 4]; // candidates, then far-away records
 	__shared__ int s_idx[TILE];
 	__shared__ int s_pre[28];
 	__shared__ int s_vs[27];
 	__shared__ R s_rmax[W]; // largest reach (rel_adh * radius) among the staged candidates, per wavefront
 	const int tid = threadIdx.x;
 	const int lane = tid & 63;
-	const int wave = tid >> 6;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int n_occ = c.sc->n_occ;
 	const unsigned long long below = (1ull << lane) - 1ull;
 	for (int w = bid_; w < n_occ; w += nblk_)
@@ -605,10 +657,10 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 			}
 			if (lane == 0)
 				s_rmax[wave] = rmax;
-			// 64 far-away records after the candidates, so that the test below runs on every
-			// lane of a 64-candidate chunk starting anywhere with no bounds branch (their
+			// far-away records after the candidates, so that the test below runs on every lane
+			// of a 64-candidate chunk starting anywhere before C with no bounds branch (their
 			// lanes are masked too)
-			for (int t = C + tid; t < C + 64; t += 64 * W)
+			for (int t = C + tid; t < C + NBR_PAD; t += 64 * W)
 			{
 				for (int k = 0; k < 3; k++)
 					s_rec[t * 4 + k] = far_coordinate<R>();
@@ -649,17 +701,42 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(siz
 					t_begin = __builtin_amdgcn_readfirstlane(t_begin);
 					t_end = __builtin_amdgcn_readfirstlane(t_end);
 				}
+				const R band_limit = reach_band_limit<R>(pi[3], block_rmax);
+				// the chunk of the agent's own slot (t_begin <= self_t < t_end: the pruned
+				// planes never hold the agent's own voxel) and its bit there
+				const int self_chunk = __builtin_amdgcn_readfirstlane(t_begin + ((self_t - t_begin) & ~63));
+				const unsigned long long self_bit = 1ull << __builtin_amdgcn_readfirstlane(self_t - self_chunk);
 				for (int t0 = t_begin; t0 < t_end; t0 += 64)
 				{
 					const int t = t0 + lane;
-					const bool pass = candidate_within<D, R>(pi, s_rec + t * 4) && t != self_t && t < t_end;
-					const unsigned long long mask = __ballot(pass);
-					if (pass)
+					// decision value: the candidate is within reach when it is <= 0
+					R dec;
+					if constexpr (D == 1)
+						dec = candidate_within<D, R>(pi, s_rec + t * 4) ? (R)-1 : (R)1;
+					else
+					{
+						bool band;
+						dec = reach_excess<D, R>(pi, s_rec + t * 4, band_limit, band);
+						if (__ballot(band)) // rare (uniform branch): lanes in the rounding band decide exactly
+						{
+							const bool exact = candidate_within<D, R>(pi, s_rec + t * 4);
+							dec = band ? (exact ? (R)-1 : (R)1) : dec;
+						}
+					}
+					// The agent's own slot is cleared in the one chunk that holds it.  Lanes
+					// past t_end read the dz = +1 plane that was pruned (out of reach, for the
+					// reason it was pruned: its face is farther than any reach, with a margin
+					// above the voxel assignment's rounding) or the far-away padding, so the
+					// test rejects them itself.
+					unsigned long long mask = __ballot(dec <= (R)0);
+					if (t0 == self_chunk)
+						mask &= ~self_bit;
+					if (__builtin_amdgcn_inverse_ballot_w64(mask))
 					{
 						// rank among the passing lanes below, plus base
-						const int slot = (int)__builtin_amdgcn_mbcnt_hi(
+						const unsigned slot = __builtin_amdgcn_mbcnt_hi(
 							(unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, (unsigned)base));
-						if (slot < c.K)
+						if (slot < (unsigned)c.K)
 							out[slot] = s_idx[t];
 					}
 					base += __popcll(mask);

@@ -85,6 +85,14 @@ def test_batch_argument_errors(lib):
     assert lib.pcm_batch_step(b, None, -1) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_batch_step(b, None, 1) == abi.PCM_OK  # no members: nothing to do
     assert lib.pcm_batch_step(None, None, 1) == abi.PCM_ERR_ARGUMENT
+    out = (ctypes.c_int32 * 1)()
+    assert lib.pcm_batch_counts(None, 0, None, None) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_counts(b, 1, None, out) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_batch_counts(b, 0, None, None) == abi.PCM_OK  # nothing asked
+    assert lib.pcm_batch_get_tuning(b, 3, None) == abi.PCM_ERR_ARGUMENT
+    t = abi.Tuning(8, 8, 2)
+    assert lib.pcm_batch_set_tuning(None, 3, ctypes.byref(t)) == abi.PCM_ERR_ARGUMENT
+    assert lib.pcm_get_fields(None, 0, None, None) == abi.PCM_ERR_ARGUMENT
     assert lib.pcm_batch_destroy(b) == abi.PCM_OK
 
 

@@ -210,3 +210,41 @@ def test_autotune_keeps_results():
     for spec, s, r in zip(SPECS[:3], sims, refs):
         _assert_equal(_state_of(s), _state_of(r), f"batch autotuned member {spec}")
     batch.close()
+
+
+def test_batch_counts_shapes_and_bulk_reads():
+    """pcm_batch_counts gives each member's count as the last step left it (the scalars the
+    step gathered), pcm_batch_set_tuning's shape is the one the group runs and keeps for a
+    member that joins later, and pcm_get_fields reads what pcm_get_field reads."""
+    print("batch counts, group shapes, bulk field reads", flush=True)
+    sims = [_member(spec) for spec in SPECS]
+    params = sims[0][1]
+    batch = Batch(4, graphs=True)
+    batch.set_tuning(3, (4, 16, 1))
+    assert batch.tuning(3) == (4, 16, 1)
+    for sim, _ in sims[:3]:
+        batch.admit(sim)
+    batch.step(params, 6)
+    assert batch.counts([s for s, _ in sims[:3]]) == [s.count for s, _ in sims[:3]]
+    for sim, _ in sims[3:]:
+        batch.admit(sim)
+    batch.step(params, 6)
+    assert batch.tuning(3) == (4, 16, 1)
+    members = [s for s, _ in sims]
+    assert batch.counts(members) == [s.count for s in members]
+    with pytest.raises(Exception):
+        batch.counts([DeviceSim(*_member_args(SPECS[0]))])
+    names = ("positions", "velocities", "simple_pressure", "damage", "cell_definition_index", "dead")
+    for sim in members:
+        bulk = sim.get_fields(names)
+        for name in names:
+            assert np.array_equal(bulk[name], sim.get_field(name)), name
+    batch.close()
+    for sim in members:
+        sim.close()
+
+
+def _member_args(spec):
+    dims, n, seed = spec
+    mesh, st, _ = rich_population(dims, n, seed, 4)
+    return mesh, 4, st.defs, st.subs, n, seed
