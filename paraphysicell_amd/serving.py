@@ -110,6 +110,11 @@ class Server:
         self.agent_steps = 0
         self.workers = ThreadPoolExecutor(max_workers=max(1, workers), thread_name_prefix="pcm-serve")
 
+    @staticmethod
+    def order(q: Request):
+        """Set-up order: arrival, then the largest of simultaneous arrivals first."""
+        return (q.arrival, -q.agents, q.key)
+
     def run(self, requests: list[Request], realtime: bool = True) -> list[Request]:
         """Serve `requests`.  With realtime, a request only becomes visible at its arrival
         time; without, all are present at once (saturation).
@@ -120,7 +125,7 @@ class Server:
         has members; finished set-ups join, and finished requests leave, between chunks."""
         from concurrent.futures import FIRST_COMPLETED, wait
 
-        pending = sorted(requests, key=lambda q: (q.arrival, -q.agents, q.key))
+        pending = sorted(requests, key=self.order)
         starting: list[tuple[Request, object]] = []   # set-up in flight
         live: list[Request] = []                       # in the batch
         finishing: list[tuple[Request, object]] = []  # tear-down in flight
