@@ -836,20 +836,24 @@ public:
 		const int K = h_.K;
 		std::vector<int> cnt(std::max(n, 1));
 		std::vector<uint8_t> mov(std::max(n, 1));
+		// the rows only when entries are wanted (a size query needs the counts)
+		const bool rows_wanted = off || idx;
+		std::unique_ptr<int[]> rows(new int[rows_wanted ? (size_t)std::max(n, 1) * K : 1]);
 		if (n > 0)
 		{
-			PCM_TRY(copy_sync(cnt.data(), h_.nbr_cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
-			PCM_TRY(copy_sync(mov.data(), h_.movable, n, hipMemcpyDeviceToHost));
+			const staged_read reads[3] = {
+				{ cnt.data(), h_.nbr_cnt, sizeof(int) * (size_t)n },
+				{ mov.data(), h_.movable, (size_t)n },
+				{ rows.get(), h_.nbr_idx, rows_wanted ? sizeof(int) * (size_t)n * K : 0 },
+			};
+			PCM_TRY(read_staged(reads, 3));
 		}
 		int64_t t = 0;
 		for (int i = 0; i < n; i++)
 			t += mov[i] ? cnt[i] : 0;
 		*total = t;
-		if (!off && !idx)
-			return PCM_OK; // the size query needs no rows
-		std::unique_ptr<int[]> rows(new int[(size_t)std::max(n, 1) * K]);
-		if (n > 0)
-			PCM_TRY(copy_sync(rows.get(), h_.nbr_idx, sizeof(int) * (size_t)n * K, hipMemcpyDeviceToHost));
+		if (!rows_wanted)
+			return PCM_OK;
 		t = 0;
 		for (int i = 0; i < n; i++)
 		{
