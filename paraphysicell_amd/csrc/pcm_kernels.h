@@ -2339,8 +2339,9 @@ __device__ void interact_serial_block(const SimCtx<R>& c, serial_lds<CAND, BADC,
 				if (out)
 					c.bad[atomicAdd(&c.sc->bad_count, 1)] = i;
 			}
+			// the list entries other wavefronts appended are read by the plan below
+			__threadfence();
 		}
-		__threadfence();
 		__syncthreads();
 		// the replay appends to the removal list: its count is read again
 		const int b = replay ? *(volatile int*)&c.sc->bad_count : bad0;
@@ -2383,20 +2384,23 @@ __global__ void __launch_bounds__(BLOCK) k_interact_apply_plan(const SimCtx<R>* 
 	if (s_state == 0)
 	{
 		wrote = interact_apply_agent<R, D, true>(c, GID);
+		// publish (release: the XCD's L2 written back; no invalidation needed)
 		if (__any(wrote))
-			__threadfence();
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 	}
 	const int written = __syncthreads_or(wrote);
 	if (threadIdx.x == 0)
 	{
 		if (written)
-			__threadfence();
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 		s_state = atomicAdd(&c.sc->ticket, 1) == nblk_ - 1;
 	}
 	__syncthreads();
 	if (!s_state)
 		return;
-	__threadfence();
+	// the last block reads what the others published: acquire only (its L2 invalidated,
+	// nothing written back)
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 	if (threadIdx.x == 0)
 		c.sc->ticket = 0;
 	interact_serial_block<R, D, true, TAIL_CAND_CHUNK, TAIL_BAD_CHUNK>(c, sh);
