@@ -1194,7 +1194,9 @@ __global__ void __launch_bounds__(BLOCK) k_velocity_fused(const SimCtx<R>* __res
 		if (s_go == 1)
 		{
 			resolve_rounds<R>(c, &s_next, false);
-			__threadfence();
+			// publish the decisions before the done mark (release: the L2 written back; the
+			// waiting wavefronts read the states with agent-scope loads)
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 			if (threadIdx.x == 0)
 				__hip_atomic_store(&c.sc->res_done, step_tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		}
