@@ -397,6 +397,10 @@ constexpr int nbr_tile()
 // far-away records staged after the candidates (one 64-candidate chunk)
 constexpr int NBR_PAD = 64;
 
+// staged candidates per block for sparse populations (a multiple of 64): at a few agents per
+// occupied voxel a 3D stencil holds a few dozen candidates
+constexpr int NBR_TILE_SPARSE = 128;
+
 // a coordinate whose square overflows: no staged padding record is within reach
 template <class R>
 constexpr R far_coordinate()
@@ -572,12 +576,14 @@ __device__ __noinline__ void neighbors_one_agent_large_stencil(const SimCtx<R>& 
 	neighbors_one_agent<R, D, false>(c, i);
 }
 
-template <class R, int D, int W>
+// TILE: candidates staged per block (nbr_tile, or NBR_TILE_SPARSE for populations with a
+// few agents per occupied voxel, whose stencils are short: a smaller tile lets more blocks
+// share a CU's LDS)
+template <class R, int D, int W, int TILE = nbr_tile<R>()>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 7 : 6))) k_neighbors(
 	const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
 	PCM_PROLOGUE_NEIGHBORS(c)
-	constexpr int TILE = nbr_tile<R>();
 	__shared__ R s_rec[(TILE + NBR_PAD) * 4]; // candidates, then far-away records
 	__shared__ int s_idx[TILE];
 	__shared__ int s_pre[28];
@@ -2510,7 +2516,6 @@ __device__ void compact_plan_limited(const SimCtx<R>& c, const int* list, int b,
 	c.sc->log_len = log;
 	c.sc->moves = moves;
 	c.sc->stat[5] = removed;
-	c.sc->stat[6] = c.sc->n_occ;
 	c.sc->n = top;
 	atomicAdd((unsigned long long*)&c.sc->deaths, (unsigned long long)deaths);
 }
@@ -2522,7 +2527,11 @@ template <class R>
 __device__ void compact_plan_block(const SimCtx<R>& c, int* sb, int sb_cap, int* s_red, int b, int n0, int limit, int log)
 {
 	if (b == 0)
+	{
+		if (threadIdx.x == 0)
+			c.sc->stat[5] = 0;
 		return;
+	}
 	// ascending list of removed agents: in LDS when it fits, else sorted in place in HBM
 	// (the list may have been filled earlier in the same kernel: read past the L1)
 	const volatile int* bad_list = c.bad;
@@ -2617,7 +2626,6 @@ __device__ void compact_plan_block(const SimCtx<R>& c, int* sb, int sb_cap, int*
 	c.sc->log_len = log;
 	c.sc->moves = moves;
 	c.sc->stat[5] = b;
-	c.sc->stat[6] = c.sc->n_occ;
 	c.sc->n = n1;
 	atomicAdd((unsigned long long*)&c.sc->deaths, (unsigned long long)deaths);
 }

@@ -365,6 +365,12 @@ inline int round_up_sig4(int x)
 	return (int)(((x + step - 1) / step) * step);
 }
 
+// fewer than four agents per occupied voxel (3D stencils of a few dozen candidates)
+inline bool sparse_population(long long agents, long long occupied)
+{
+	return occupied > 0 && agents < 4 * occupied;
+}
+
 // blocks of one member in a launch of `kind`, for n agents, L spring slots and `nbr` blocks
 // of the neighbor search (from the member's occupied voxels)
 inline int member_blocks(int kind, long long n, long long L, int nbr)
@@ -1163,6 +1169,9 @@ public:
 		PCM_TRY(read_scalars());
 		if (sc_.status)
 			return fail(PCM_ERR_STATE, "device status word is set");
+		// alone, a small population's launch is bound by its latency, not by the blocks a CU
+		// can hold: below a few thousand agents the default shape stays
+		neighbors_sparse_ = sc_.n >= LANES_NEIGHBOR_MIN_AGENTS && sparse_population(sc_.n, sc_.n_occ);
 		const unsigned long long target = sc_.step + (unsigned long long)steps;
 		for (int guard = 0; guard < 64; guard++)
 		{
@@ -1204,6 +1213,7 @@ public:
 		PCM_TRY(read_scalars());
 		for (int k = 0; k < 8; k++)
 			out[k] = sc_.stat[k];
+		out[6] = sc_.n_occ; // occupied voxels, as the step's position update counted them
 		// attachment events and actors: summed over the collect segments of the last step
 		const int segments = sc_.stat[7];
 		out[0] = out[2] = out[7] = 0;
@@ -1266,6 +1276,10 @@ private:
 	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
+	// a 3D population with few agents per occupied voxel: its neighbor search stages short
+	// stencils, one wavefront per voxel with a small tile (set from the last scalars read, or
+	// by a batch for its group)
+	bool neighbors_sparse_ = false;
 	bool records_fresh_ = false; // force records describe the current agent fields
 	// the last work enqueued was a fused step, whose mesh update decided the next step's
 	// detachments (fired and erased spring slots), and nothing has changed the agents, their
@@ -1752,6 +1766,8 @@ private:
 		// block per voxel has more parallelism to hide its loads
 		if (mesh_fresh_ && D < 3 && gn() >= LANES_NEIGHBOR_MIN_AGENTS)
 			return per_agent(PCM_K_NEIGHBORS, k_neighbors_lanes<R, D < 3 ? D : 2>);
+		if (mesh_fresh_ && D == 3 && neighbors_sparse_)
+			return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 1, NBR_TILE_SPARSE>, neighbor_blocks(), 64, MAP_NEIGHBORS);
 		if (mesh_fresh_)
 		{
 			switch (tune_.neighbor_waves)
@@ -2015,6 +2031,7 @@ public:
 	uint64_t ctx_version() const { return ctx_version_; }
 	int dims() const { return h_.dims; }
 	int launch_bound() const { return n_launch_; }
+	void set_neighbors_sparse(bool on) { neighbors_sparse_ = on; }
 	int spring_slots() const { return h_.L; }
 	// neighbor-search blocks of this member in a batch's flattened launch: its occupied
 	// voxels at the last step seen, with headroom, rounded to four significant bits so that
@@ -2368,6 +2385,7 @@ private:
 		int slots = 0, grid_n = 0, grid_L = 0, springs = -1;
 		unsigned long long maps = 0; // map version of a flattened capture (0: none)
 		int velocity_lanes = 0, event_lanes = 0, neighbor_waves = 0;
+		bool sparse = false; // the neighbor search's sparse-population kernel
 		bool operator==(const capture_key&) const = default;
 	};
 	struct group
@@ -2555,6 +2573,16 @@ private:
 				detach_ready = detach_ready && slot.sim->detach_ready();
 			}
 		grid_n = round_up_grid(grid_n);
+		// the group's neighbor search kernel, from its members' agents per occupied voxel
+		long long agents = 0, occupied = 0;
+		for (auto& slot : g.slots)
+			if (slot.sim)
+			{
+				agents += slot.sim->scalars().n;
+				occupied += slot.sim->scalars().n_occ;
+			}
+		const bool sparse = sparse_population(agents, occupied);
+		g.placeholder->set_neighbors_sparse(sparse);
 		bool fold_clear = true;
 		for (auto& slot : g.slots)
 			if (slot.sim)
@@ -2594,7 +2622,7 @@ private:
 			return run(true, true, true, steps - done);
 		const pcm_tuning t = leader->tuning();
 		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs, maps ? g.map_version : 0,
-								t.velocity_lanes, t.event_lanes, t.neighbor_waves };
+								t.velocity_lanes, t.event_lanes, t.neighbor_waves, sparse };
 		if (!g.exec || !(key == g.key))
 		{
 			if (g.exec)

@@ -34,7 +34,8 @@ def _check_mesh_invariants(dev, st_mesh):
             assert np.all(inside | edge)
 
 
-@pytest.mark.parametrize("case", ["disc_900_2d", "disc_12k_2d", "ball_14k_3d_f32", "ball_14k_3d_f64", "ball_88k_3d"])
+@pytest.mark.parametrize("case", ["disc_900_2d", "disc_12k_2d", "ball_14k_3d_f32", "ball_14k_3d_f64", "ball_88k_3d",
+                                  "sparse_ball_9k_3d_f32", "sparse_ball_9k_3d_f64"])
 def test_workload_full_size(case):
     print(f"full-size workload {case}", flush=True)
     if case == "disc_900_2d":
@@ -51,6 +52,11 @@ def test_workload_full_size(case):
     elif case == "ball_14k_3d_f64":
         mesh, st, params = synth.config1(8)
         steps = 3
+    elif case.startswith("sparse_ball_9k_3d"):
+        # the ensemble's 3D density (about two agents per occupied voxel): the neighbor search
+        # stages short stencils with the sparse-population tile
+        mesh, st, params = synth.config3_member(1, 9000, 4 if case.endswith("f32") else 8)
+        steps = 4
     else:
         mesh, st, params = synth.config2(4)
         steps = 2
@@ -60,6 +66,9 @@ def test_workload_full_size(case):
     parity.assert_same_neighbors(dev, ora, f"{case} initial")
     for s in (dev, ora):
         s.step(params, steps)
+    if case.startswith("sparse"):
+        occupied = dev.stats()["occupied_voxels"]
+        assert dev.count < 4 * occupied, "not a sparse population"
     parity.assert_same_state(dev, ora, what=case)
     parity.assert_same_neighbors(dev, ora, case)
     parity.assert_same_buckets(dev, ora, case)
