@@ -390,19 +390,22 @@ def main():
     # them) when there are enough of them, then the kernel profile of the last few steps
     # picks the dominant kernel
     autotuned = args.warmup >= 13
-    tuned = dev.autotune(params, 2) if autotuned else dev.tuning
+    if autotuned:
+        dev.autotune(params, 2)
     dev.step(params, max(args.warmup - (13 if autotuned else 0) - 5, 0))
     names = kernel_names()
     dev.profile_begin(names)
     dev.step(params, min(5, args.warmup))
     warm_prof = dev.profile_end()
     dominant = max(warm_prof, key=lambda k: warm_prof[k][0])
+    tuned = dev.tuning  # the shape the warm-up steps ran with (tuned, or the library's default for this size)
 
     n_before = dev.count
     dev.synchronize()
     # the dominant kernel is timed with HIP events on the library's stream over the timed
-    # region, every second launch: each event pair stalls the stream for a few microseconds
-    dev.profile_every(2)
+    # region, every fourth launch: each event pair stalls the stream for a few microseconds,
+    # which the step time would carry (about 2 us per step when every second launch is timed)
+    dev.profile_every(4)
     dev.profile_begin([dominant])
     t0 = time.perf_counter()
     dev.step(params, args.steps)
@@ -455,7 +458,8 @@ def main():
             "last_step_counters": dev.stats(),
             "launch_shape": {"velocity_lanes": tuned[0], "event_lanes": tuned[1], "neighbor_waves": tuned[2],
                              "picked_by": "autotuner on warm-up steps" if autotuned
-                             else "library default (fewer than 13 warm-up steps: no tuning)"},
+                             else "library default for this population size (fewer than 13 warm-up steps: "
+                                  "no tuning)"},
         },
         "roofline": {
             "kernel": dominant,

@@ -311,7 +311,9 @@ int pcm_synchronize(pcm_sim* sim);
  * kernels of the step can run with several shapes, all giving bit-identical results:
  * lanes per agent of the force/velocity kernel and of the attachment and interaction
  * kernels (4, 8 or 16), and the neighbor search as 1, 2 or 4 wavefronts per occupied voxel
- * sharing an LDS copy of its stencil.  The default is {8, 8, 2}. */
+ * sharing an LDS copy of its stencil.  Until a shape is set or tuned, each pcm_step takes
+ * {4, 4, 2} from 65536 agents up and {8, 8, 2} below; pcm_get_tuning reports the shape in
+ * use.  A batch group's default is {8, 8, 2}. */
 typedef struct pcm_tuning
 {
 	int32_t velocity_lanes;

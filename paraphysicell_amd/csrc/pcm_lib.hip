@@ -1169,6 +1169,8 @@ public:
 		PCM_TRY(read_scalars());
 		if (sc_.status)
 			return fail(PCM_ERR_STATE, "device status word is set");
+		if (!tune_chosen_)
+			tune_ = default_tuning(sc_.n);
 		// alone, a small population's launch is bound by its latency, not by the blocks a CU
 		// can hold: below a few thousand agents the default shape stays
 		neighbors_sparse_ = sc_.n >= LANES_NEIGHBOR_MIN_AGENTS && sparse_population(sc_.n, sc_.n_occ);
@@ -1274,6 +1276,15 @@ private:
 	std::vector<int> remap_ident_; // host source of init's identity copy
 	std::vector<int> origin_ident_; // host source of set_count's ORIGIN identity
 	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
+	bool tune_chosen_ = false;      // set by pcm_set_tuning or the autotuner; else by size
+
+	// the launch shape when none was chosen: from 65536 agents up four lanes per agent (more
+	// agents per wavefront: 147 against 149 us per step at 88.7k agents), below that eight
+	// (more lanes spread fewer agents: 1-3% faster from 900 to 15k agents)
+	static pcm_tuning default_tuning(int n)
+	{
+		return n >= 65536 ? pcm_tuning { 4, 4, 2 } : pcm_tuning { 8, 8, 2 };
+	}
 	void* batch_ = nullptr;    // the batch this simulation belongs to, if any
 	bool mesh_fresh_ = false; // voxel buckets describe the current positions and radii
 	// a 3D population with few agents per occupied voxel: its neighbor search stages short
@@ -2065,6 +2076,7 @@ public:
 		if (!t || !valid_tuning(*t))
 			return fail(PCM_ERR_ARGUMENT, "tuning values must be 4, 8 or 16 lanes and 1, 2 or 4 wavefronts");
 		tune_ = *t;
+		tune_chosen_ = true;
 		return PCM_OK;
 	}
 
@@ -2072,7 +2084,7 @@ public:
 	{
 		if (!t)
 			return fail(PCM_ERR_ARGUMENT, "null tuning");
-		*t = tune_;
+		*t = tune_chosen_ ? tune_ : default_tuning(sc_.n);
 		return PCM_OK;
 	}
 
@@ -2089,7 +2101,8 @@ public:
 							  | (1ull << PCM_K_SPRING_RESOLVE) | (1ull << PCM_K_SPRING_RESOLVE_FIRST)
 							  | (1ull << PCM_K_INTERACT_TENTATIVE)
 							  | (1ull << PCM_K_NEIGHBORS);
-		PCM_TRY(run(1)); // warm-up with the current shape
+		tune_chosen_ = true; // the candidates' steps keep the shape they are given
+		PCM_TRY(run(1));     // warm-up with the current shape
 		double vel[3] = { 1e300, 1e300, 1e300 }, ev[3] = { 1e300, 1e300, 1e300 }, nb[3] = { 1e300, 1e300, 1e300 };
 		std::vector<double> ms(PCM_KERNEL_COUNT);
 		std::vector<int64_t> cnt(PCM_KERNEL_COUNT);
