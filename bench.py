@@ -429,7 +429,8 @@ def main():
     agents_mean = 0.5 * (n_before + n_after)
     bytes_per_launch = bytes_per_agent * agents_mean
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_agent else None
-    prefix = {"NEIGHBORS": "k_neighbors<float, 3, %d>" % tuned[2]}.get(dominant)
+    # the 3D kernel's name carries its wavefronts per voxel, then its candidate tile
+    prefix = {"NEIGHBORS": "k_neighbors<float, 3, %d, " % tuned[2]}.get(dominant)
     rp_avg = rocprof_average_us(prefix) if prefix else None
     step_bytes = ALG_BYTES["ball_88k_3d_f32"]
 
@@ -476,7 +477,7 @@ def main():
             "bytes_formula": "kernel_bytes() in bench.py at the end-of-run mean neighbors and springs per agent",
             "avg_launch_us": round(avg_ms * 1000.0, 2),
             "timed_launches": launches,
-            "timing": "HIP events on the library stream around every second launch of the timed region",
+            "timing": "HIP events on the library stream around every fourth launch of the timed region",
             "rocprof_avg_launch_us": rp_avg,
             "rocprof_source": "profiles/mechanics_step_88k_3d_f32_kernel_stats.md (same command under rocprofv3)",
             "frac_at_rocprof_avg": round(bytes_per_launch / (rp_avg * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
