@@ -1871,6 +1871,21 @@ __device__ inline int block_chunk_end(const int* rec, int stride, int count, int
 	return a;
 }
 
+// lane k's value of x (k wave-uniform)
+template <class R>
+__device__ __forceinline__ R lane_real(R x, int k)
+{
+	if constexpr (sizeof(R) == 4)
+		return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+	else
+	{
+		const long long b = __double_as_longlong(x);
+		const int lo = __builtin_amdgcn_readlane((int)b, k);
+		const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+		return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+	}
+}
+
 // ------------------------------------------------------------------ cell-cell interactions
 // update_cell_cell_interactions_internal (interactions_solver.cpp:143-253).
 //
@@ -2156,6 +2171,29 @@ __device__ __forceinline__ bool interact_apply_agent(const SimCtx<R>& c, int i)
 	return removed || flagged;
 }
 
+// Agent i's removal entry when the speculative pass found a conflict: the apply pass does
+// not run, so the entry reflects the state before the serial replay (removal flag set, or
+// outside the box); the replay then settles the agents its ingestions and fusions touched.
+// Every lane of a wavefront calls it.
+template <class R, int D>
+__device__ __forceinline__ bool interact_flag_agent(const SimCtx<R>& c, int i)
+{
+	if ((i & ~63) >= c.sc->n)
+		return false;
+	bool removed = false;
+	if (i < c.sc->n)
+	{
+		R p[D];
+		for (int k = 0; k < D; k++)
+			p[k] = c.pos[(size_t)i * D + k];
+		removed = (c.flags[i] == PCM_FLAG_TO_REMOVE) | outside_box<R, D>(c, p);
+	}
+	const int k = wave_reserve(&c.sc->bad_count, removed ? 1 : 0);
+	if (removed)
+		c.bad[k] = i;
+	return removed;
+}
+
 template <class R, int D, bool FLAG>
 __global__ void k_interact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* __restrict__ bmap)
 {
@@ -2177,6 +2215,7 @@ struct serial_lds
 	unsigned long long key[CAND];
 	int val[CAND];
 	int count, red;
+	int rescan; // the replay touched more agents than sb lists: the plan rescans every agent
 	int sb[PLAN ? BADC : 1];
 };
 
@@ -2193,6 +2232,9 @@ __device__ void interact_serial_block(const SimCtx<R>& c, serial_lds<CAND, BADC,
 		return;
 	const int C = min(cand_count, c.cand_cap);
 	const bool replay = conflict != 0;
+	// PLAN: the agents the replay's ingestions and fusions touch, listed in sh.sb (wavefront 0)
+	int touched = 0;
+	bool touch_overflow = false;
 	if (replay && C > 0)
 	{
 		const int n = c.sc->n;
@@ -2232,78 +2274,135 @@ __device__ void interact_serial_block(const SimCtx<R>& c, serial_lds<CAND, BADC,
 			}
 			const int m = sh.count;
 			block_sort_pairs(sh.key, sh.val, m);
-			if (threadIdx.x == 0)
+			// Wavefront 0 walks the ordered candidates 64 at a time.  Each lane first loads one
+			// candidate's inputs (target, draw, the four branch thresholds, the damage
+			// increment, both removal flags, the target's damage and attack time as they stand
+			// now), so that the walk runs on registers in uniform control flow instead of a few
+			// dependent global loads per candidate.  Of what the decisions read, only the
+			// removal flags change during the replay (ingestion and fusion flag their target);
+			// attacks add to their target's damage and attack time.  After each such event every
+			// lane that names the same agent takes the new value.  Thresholds are computed as
+			// the reference's expressions are (rate * dt, attack * immunogenicity * dt).
+			if (threadIdx.x < 64)
 			{
-				for (int q = 0; q < m; q++)
+				const int lane = threadIdx.x;
+				const volatile uint8_t* vflags = c.flags;
+				const volatile R* vdamage = c.damage;
+				const volatile R* vattack_time = c.attack_time;
+				for (int g0 = 0; g0 < m; g0 += 64)
 				{
-					const int i = (int)(sh.key[q] >> 32);
-					const int s = (int)(sh.key[q] & 0xffffffffu);
-					if (i != cur)
+					const int q = g0 + lane;
+					int ai = -1, aj = -1, dead_j = 0, fl_i = 0, fl_j = 0;
+					R u = 1, p_dead = 0, p_live = 0, p_attack = 0, p_fuse = 0, dr = 0, dmg = 0, atk = 0;
+					if (q < m)
 					{
-						cur = i;
-						broken = attacked_once = phagocytosed_once = fused_once = false;
+						ai = (int)(sh.key[q] >> 32);
+						const int slot = (int)(sh.key[q] & 0xffffffffu);
+						aj = c.nbr_idx[(size_t)ai * c.K + slot];
+						const int ti = c.type[ai], tj = c.type[aj];
+						u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)ai, (uint32_t)slot, 0u);
+						dead_j = c.dead[aj];
+						p_dead = c.dead_phago[ai] * dt;
+						p_live = c.live_phago[(size_t)ai * c.T + tj] * dt;
+						p_attack = c.attack[(size_t)ai * c.T + tj] * c.immuno[(size_t)aj * c.T + ti] * dt;
+						p_fuse = c.fusion[(size_t)ai * c.T + tj] * dt;
+						dr = c.damage_rate[ai] * dt;
+						fl_i = vflags[ai];
+						fl_j = vflags[aj];
+						dmg = vdamage[aj];
+						atk = vattack_time[aj];
 					}
-					if (broken)
-						continue;
-					const int j = c.nbr_idx[(size_t)i * c.K + s];
-					const int ti = c.type[i], tj = c.type[j];
-					const R u = pcm_uniform01<R>(c.seed, step, PCM_STREAM_INTERACT, (uint32_t)i, (uint32_t)s, 0u);
-					if (c.dead[j] == 1)
+					const int qn = min(64, m - g0);
+					for (int k = 0; k < qn; k++)
 					{
-						if (u < c.dead_phago[i] * dt)
+						const int i = __builtin_amdgcn_readlane(ai, k);
+						if (i != cur)
 						{
-							if (c.flags[i] == PCM_FLAG_TO_REMOVE)
+							cur = i;
+							broken = attacked_once = phagocytosed_once = fused_once = false;
+						}
+						if (broken)
+							continue;
+						const int j = __builtin_amdgcn_readlane(aj, k);
+						const R uk = lane_real<R>(u, k);
+						const bool removed_i = __builtin_amdgcn_readlane(fl_i, k) == PCM_FLAG_TO_REMOVE;
+						const bool removed_j = __builtin_amdgcn_readlane(fl_j, k) == PCM_FLAG_TO_REMOVE;
+						// 0: ingest, 1: attack, 2: fuse
+						int event = -1;
+						if (__builtin_amdgcn_readlane(dead_j, k) == 1)
+						{
+							if (!(uk < lane_real<R>(p_dead, k)))
+								continue;
+							event = 0;
+						}
+						else if (!phagocytosed_once && uk < lane_real<R>(p_live, k))
+						{
+							event = 0;
+							if (!removed_i && !removed_j)
+								phagocytosed_once = true;
+						}
+						else if (!attacked_once && uk < lane_real<R>(p_attack, k))
+						{
+							event = 1;
+							if (!removed_i && !removed_j)
+								attacked_once = true;
+						}
+						else if (!fused_once && uk < lane_real<R>(p_fuse, k))
+						{
+							event = 2;
+							if (!removed_i && !removed_j)
+								fused_once = true;
+						}
+						else
+							continue;
+						if (removed_i)
+						{
+							broken = true;
+							continue;
+						}
+						if (removed_j)
+							continue;
+						if (event == 1)
+						{
+							const R nd = lane_real<R>(dmg, k) + lane_real<R>(dr, k);
+							const R na = lane_real<R>(atk, k) + dt;
+							if (lane == 0)
 							{
-								broken = true;
-								continue;
+								c.damage[j] = nd;
+								c.attack_time[j] = na;
 							}
-							if (c.flags[j] == PCM_FLAG_TO_REMOVE)
-								continue;
-							ingest<R>(c, i, j);
+							dmg = aj == j ? nd : dmg;
+							atk = aj == j ? na : atk;
 						}
-						continue;
-					}
-					if (!phagocytosed_once && u < c.live_phago[(size_t)i * c.T + tj] * dt)
-					{
-						if (c.flags[i] == PCM_FLAG_TO_REMOVE)
+						else
 						{
-							broken = true;
-							continue;
+							if (lane == 0)
+							{
+								if (event == 0)
+									ingest<R>(c, i, j);
+								else
+									fuse<R>(c, i, j);
+							}
+							fl_i = ai == j ? (int)PCM_FLAG_TO_REMOVE : fl_i;
+							fl_j = aj == j ? (int)PCM_FLAG_TO_REMOVE : fl_j;
+							if (PLAN)
+							{
+								if (touched + 2 <= BADC)
+								{
+									if (lane == 0)
+									{
+										sh.sb[touched] = i;
+										sh.sb[touched + 1] = j;
+									}
+									touched += 2;
+								}
+								else
+									touch_overflow = true;
+							}
 						}
-						if (c.flags[j] == PCM_FLAG_TO_REMOVE)
-							continue;
-						ingest<R>(c, i, j);
-						phagocytosed_once = true;
-						continue;
 					}
-					const R p_attack = c.attack[(size_t)i * c.T + tj] * c.immuno[(size_t)j * c.T + ti] * dt;
-					if (!attacked_once && u < p_attack)
-					{
-						if (c.flags[i] == PCM_FLAG_TO_REMOVE)
-						{
-							broken = true;
-							continue;
-						}
-						if (c.flags[j] == PCM_FLAG_TO_REMOVE)
-							continue;
-						c.damage[j] += c.damage_rate[i] * dt;
-						c.attack_time[j] += dt;
-						attacked_once = true;
-						continue;
-					}
-					if (!fused_once && u < c.fusion[(size_t)i * c.T + tj] * dt)
-					{
-						if (c.flags[i] == PCM_FLAG_TO_REMOVE)
-						{
-							broken = true;
-							continue;
-						}
-						if (c.flags[j] == PCM_FLAG_TO_REMOVE)
-							continue;
-						fuse<R>(c, i, j);
-						fused_once = true;
-						continue;
-					}
+					// the next lanes' loads read what lane 0 stored
+					__threadfence_block();
 				}
 			}
 			__threadfence();
@@ -2319,7 +2418,10 @@ __device__ void interact_serial_block(const SimCtx<R>& c, serial_lds<CAND, BADC,
 			c.ev_t[i] = 0;
 		}
 	}
+	if (threadIdx.x == 0)
+		sh.rescan = touch_overflow ? 1 : 0; // thread 0 walked the candidates
 	__syncthreads();
+	const bool rescan = sh.rescan != 0;
 	if (threadIdx.x == 0)
 	{
 		c.sc->stat[3] = C;
@@ -2329,8 +2431,18 @@ __device__ void interact_serial_block(const SimCtx<R>& c, serial_lds<CAND, BADC,
 	}
 	if (PLAN)
 	{
-		if (replay)
+		// The apply launch's other blocks listed the removals as they stood before the replay
+		// (removal flag set, or outside the box).  Of the replay's effects only ingestion and
+		// fusion change that (their target is flagged; a fusion moves its actor), so the
+		// agents they touched are settled one by one: listed if now removed, taken off the
+		// list if a fusion brought them back into the box.  When more were touched than the
+		// LDS list holds, the whole population is scanned again instead.
+		if (replay && rescan)
 		{
+			__syncthreads();
+			if (threadIdx.x == 0)
+				c.sc->bad_count = 0;
+			__syncthreads();
 			// the replay's writes are read back by other wavefronts: bypass the L1
 			const int n = c.sc->n;
 			const volatile uint8_t* flags = c.flags;
@@ -2348,6 +2460,35 @@ __device__ void interact_serial_block(const SimCtx<R>& c, serial_lds<CAND, BADC,
 					c.bad[atomicAdd(&c.sc->bad_count, 1)] = i;
 			}
 			// the list entries other wavefronts appended are read by the plan below
+			__threadfence();
+		}
+		else if (replay && threadIdx.x == 0)
+		{
+			// the other blocks' list entries were acquired at this block's start; the flags and
+			// positions read here changed, if at all, by this thread's own replay
+			int nb = c.sc->bad_count;
+			for (int t = 0; t < touched; t++)
+			{
+				const int x = sh.sb[t];
+				bool seen = false;
+				for (int u = 0; u < t; u++)
+					seen |= sh.sb[u] == x;
+				if (seen)
+					continue;
+				R p[D];
+				for (int k = 0; k < D; k++)
+					p[k] = c.pos[(size_t)x * D + k];
+				const bool out = (c.flags[x] == PCM_FLAG_TO_REMOVE) | outside_box<R, D>(c, p);
+				int at = -1;
+				for (int u = 0; u < nb; u++)
+					at = c.bad[u] == x ? u : at;
+				if (out && at < 0)
+					c.bad[nb++] = x;
+				else if (!out && at >= 0)
+					c.bad[at] = c.bad[--nb];
+			}
+			c.sc->bad_count = nb;
+			// the list is read by the plan's other wavefronts below
 			__threadfence();
 		}
 		__syncthreads();
@@ -2386,12 +2527,12 @@ __global__ void __launch_bounds__(BLOCK) k_interact_apply_plan(const SimCtx<R>* 
 	__shared__ serial_lds<TAIL_CAND_CHUNK, TAIL_BAD_CHUNK, true> sh;
 	__shared__ int s_state;
 	if (threadIdx.x == 0)
-		s_state = c.sc->status | c.sc->conflict;
+		s_state = c.sc->status ? 2 : (c.sc->conflict ? 1 : 0); // 2: stop, 1: replay follows
 	__syncthreads();
 	bool wrote = false;
-	if (s_state == 0)
+	if (s_state < 2)
 	{
-		wrote = interact_apply_agent<R, D, true>(c, GID);
+		wrote = s_state == 0 ? interact_apply_agent<R, D, true>(c, GID) : interact_flag_agent<R, D>(c, GID);
 		// publish (release: the XCD's L2 written back; no invalidation needed)
 		if (__any(wrote))
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");

@@ -158,7 +158,7 @@ def test_fused_step(dims, real_bytes):
     dev, ora = parity.make_pair(mesh, st, seed=9)
     for s in (dev, ora):
         s.initialize_mesh()
-    rounds = []
+    rounds, replays = [], []
     for step in range(4):
         d0, o0 = dev.deaths_count, ora.deaths_count
         dev.step(p, 1)
@@ -167,7 +167,9 @@ def test_fused_step(dims, real_bytes):
         parity.assert_same_state(dev, ora, what=f"fused step {step}")
         parity.assert_same_buckets(dev, ora, f"fused step {step}")
         parity.assert_same_neighbors(dev, ora, f"fused step {step}")
-        rounds.append(dev.stats()["resolve_rounds"])
+        stats = dev.stats()
+        rounds.append(stats["resolve_rounds"])
+        replays.append(stats["interaction_replay"])
         dev.upload(ora.download())
         dev.step_index = ora.step_index
         dev.initialize_mesh()
@@ -175,6 +177,8 @@ def test_fused_step(dims, real_bytes):
         # the busy rates leave attachment events undecided after the first resolver round,
         # so the later rounds (the single-block resolver) are covered too
         assert max(rounds) >= 2, rounds
+    # and their interaction events collide, so the exact serial replay ran and was compared
+    assert max(replays) > 0, replays
 
 
 @pytest.mark.parametrize("real_bytes", [4, 8])
