@@ -182,6 +182,35 @@ def test_fused_step(dims, real_bytes):
 
 
 @pytest.mark.parametrize("real_bytes", [4, 8])
+def test_fused_step_with_mass_removal(real_bytes):
+    """A busy 20k-agent population loses thousands of agents per step to ingestion, fusion
+    and the wall: the interaction replay then touches more agents than its LDS list holds,
+    so the container plan rescans the whole population (smaller populations settle only the
+    touched agents, covered by test_fused_step)."""
+    mesh, st, p = parity.rich_population(3, 20000, 77, real_bytes, box=323, rates="busy")
+    dev, ora = parity.make_pair(mesh, st, seed=5)
+    for s in (dev, ora):
+        s.initialize_mesh()
+    for step in range(3):
+        print(f"mass removal step {step} real_bytes {real_bytes}", flush=True)
+        d0, o0 = dev.deaths_count, ora.deaths_count
+        dev.step(p, 1)
+        ora.step(p, 1)
+        deaths = ora.deaths_count - o0
+        assert dev.deaths_count - d0 == deaths, f"mass removal step {step}: deaths"
+        parity.assert_same_state(dev, ora, what=f"mass removal step {step}")
+        parity.assert_same_buckets(dev, ora, f"mass removal step {step}")
+        parity.assert_same_neighbors(dev, ora, f"mass removal step {step}")
+        assert dev.stats()["interaction_replay"] > 0
+        if step == 0:
+            # well past 512 ingestions and fusions (two touched agents each)
+            assert deaths > 1500, deaths
+        dev.upload(ora.download())
+        dev.step_index = ora.step_index
+        dev.initialize_mesh()
+
+
+@pytest.mark.parametrize("real_bytes", [4, 8])
 def test_consecutive_fused_steps(real_bytes):
     """Fused steps back to back, with no upload between them, against the oracle: from the
     second step on, the detachment masks come from the previous step's mesh update instead
