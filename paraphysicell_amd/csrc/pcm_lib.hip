@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -2410,6 +2411,7 @@ private:
 		SimCtx<R>* d_ctx = nullptr;
 		Scalars* d_sc = nullptr;
 		int cap = 0;
+		int xcds = 0; // XCDs of the device (flattened maps deal blocks over them)
 		hipGraphExec_t exec = nullptr;
 		capture_key key;
 		std::vector<SimCtx<R>> h_ctx;
@@ -2445,27 +2447,124 @@ private:
 		}
 		if (g.d_map && sig == g.map_sig)
 			return PCM_OK;
+		// Workgroups go to the device's XCDs in turn (grid block b to XCD b mod X), and each XCD
+		// has its own L2.  The members' blocks, in member order, are cut into X equal shares,
+		// one per XCD, so that a member's gathers stay within the one or few XCDs its share
+		// covers instead of all of them.  A member spanning several XCDs deals its blocks among
+		// them in proportion to their parts, so that blocks of unequal cost (a member's ordering
+		// blocks before its erase blocks, its voxel blocks before its spare ones) spread evenly.
+		// Each member's block 0, which its other blocks may wait for in the velocity kernel,
+		// comes first in its XCD's list, ahead of every block that could wait.  Grid block
+		// i * X + x holds entry i of XCD x's list.
+		if (g.xcds < 1)
+		{
+			int dev = 0, x = 1;
+			PCM_HIP(hipGetDevice(&dev));
+			if (hipDeviceGetAttribute(&x, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || x < 1)
+				x = 1;
+			g.xcds = std::min(x, 64);
+		}
+		const int X = g.xcds;
 		g.h_map.clear();
+		std::vector<std::vector<int4>> heads(X), rest(X);
+		std::vector<long long> part(X), dealt(X);
 		for (int kind = 0; kind < MAP_KINDS; kind++)
 		{
 			g.map_off[kind] = (int)g.h_map.size();
 			long long total = 0;
+			for (int k = 0; k < m; k++)
+				if (auto* sim = g.slots[k].sim)
+					total += member_blocks(kind, sim->launch_bound(), sim->spring_slots(), sim->neighbor_map_blocks());
+			if (total > 0x7fffffffLL / 4)
+				return fail(PCM_ERR_CAPACITY, "flattened batch grid too large");
+			if (kind == MAP_NEIGHBORS)
+			{
+				// the neighbor search keeps one member's blocks dealt over all XCDs in turn
+				// (its voxel blocks cost far more than its spare ones, and per-XCD lists of
+				// unequal cost at equal positions held back the in-order dispatch)
+				for (int k = 0; k < m; k++)
+				{
+					auto* sim = g.slots[k].sim;
+					if (!sim)
+						continue;
+					const int b = member_blocks(kind, sim->launch_bound(), sim->spring_slots(), sim->neighbor_map_blocks());
+					for (int j = 0; j < b; j++)
+						g.h_map.push_back(make_int4(k, j, b, 0));
+				}
+				const int padded = round_up_grid((int)std::max<long long>(total, 1));
+				for (long long x = total; x < padded; x++)
+					g.h_map.push_back(make_int4(-1, 0, 0, 0));
+				g.map_tot[kind] = padded;
+				continue;
+			}
+			const long long share = std::max<long long>(1, (total + X - 1) / X);
+			for (int x = 0; x < X; x++)
+			{
+				heads[x].clear();
+				rest[x].clear();
+				rest[x].reserve((size_t)share);
+			}
+			long long pos = 0;
 			for (int k = 0; k < m; k++)
 			{
 				auto* sim = g.slots[k].sim;
 				if (!sim)
 					continue;
 				const int b = member_blocks(kind, sim->launch_bound(), sim->spring_slots(), sim->neighbor_map_blocks());
-				for (int x = 0; x < b; x++)
-					g.h_map.push_back(make_int4(k, x, b, 0));
-				total += b;
+				const long long start = pos;
+				const int x0 = (int)std::min<long long>(start / share, X - 1);
+				const int x1 = (int)std::min<long long>((start + b - 1) / share, X - 1);
+				pos += b;
+				if (x0 == x1)
+				{
+					heads[x0].push_back(make_int4(k, 0, b, 0));
+					for (int j = 1; j < b; j++)
+						rest[x0].push_back(make_int4(k, j, b, 0));
+					continue;
+				}
+				for (int x = x0; x <= x1; x++)
+				{
+					const long long lo = std::max<long long>(start, (long long)x * share);
+					const long long hi = x == X - 1 ? start + b : std::min<long long>(start + b, (long long)(x + 1) * share);
+					part[x] = std::max<long long>(hi - lo, 0);
+					dealt[x] = 0;
+				}
+				for (int j = 0; j < b; j++)
+				{
+					// the XCD furthest behind its part (block j of b due part * (j + 1) / b)
+					int best = x0;
+					long long best_lag = LLONG_MIN;
+					for (int x = x0; x <= x1; x++)
+					{
+						if (dealt[x] >= part[x])
+							continue;
+						const long long lag = part[x] * (j + 1) - dealt[x] * (long long)b;
+						if (lag > best_lag)
+						{
+							best_lag = lag;
+							best = x;
+						}
+					}
+					dealt[best]++;
+					(j == 0 ? heads[best] : rest[best]).push_back(make_int4(k, j, b, 0));
+				}
 			}
-			if (total > 0x7fffffffLL / 2)
-				return fail(PCM_ERR_CAPACITY, "flattened batch grid too large");
-			const int padded = round_up_grid((int)std::max<long long>(total, 1));
-			for (long long x = total; x < padded; x++)
-				g.h_map.push_back(make_int4(-1, 0, 0, 0));
-			g.map_tot[kind] = padded;
+			size_t longest = 1;
+			for (int x = 0; x < X; x++)
+				longest = std::max(longest, heads[x].size() + rest[x].size());
+			const int per = round_up_grid((int)longest);
+			for (int i = 0; i < per; i++)
+				for (int x = 0; x < X; x++)
+				{
+					const size_t h = heads[x].size();
+					if ((size_t)i < h)
+						g.h_map.push_back(heads[x][i]);
+					else if ((size_t)i - h < rest[x].size())
+						g.h_map.push_back(rest[x][i - h]);
+					else
+						g.h_map.push_back(make_int4(-1, 0, 0, 0));
+				}
+			g.map_tot[kind] = per * X;
 		}
 		// the previous maps may still be read by queued launches
 		PCM_HIP(hipStreamSynchronize(g.stream));
