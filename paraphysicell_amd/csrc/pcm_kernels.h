@@ -301,7 +301,7 @@ __global__ void k_bin_scatter(const SimCtx<R>* __restrict__ ctxs, const int4* __
 // of a fused step also decides the next step's erased spring slots (k_spring_erase's work;
 // k_bin_scatter drew the fired slots)
 // With DETACH the launch has the member's ordering blocks followed by its erase blocks
-// (ERASE_LANES lanes per agent).  FIXED (the 3D fused mesh update, after k_compact_count
+// (ERASE_LANES lanes per agent).  FIXED (the fused mesh update with voxel tables, after k_compact_count
 // with FIXED): the voxel's members come from its unordered table, the occupied-voxel count
 // is taken from the claim counter, and the compaction's remap entries are returned to the
 // identity (the ordering blocks then cover the removal list too).
@@ -2947,7 +2947,7 @@ __global__ void k_compact_apply(const SimCtx<R>* __restrict__ ctxs, const int4* 
 // spring list also draws the next step's detachments for it (k_spring_fire's work, keyed
 // by the stored step counter plus one; spring lists, counts, rates and indices are final
 // here).
-// FIXED (3D): the agent also takes its slot in the voxel's unordered table (B slots per
+// FIXED (voxel tables): the agent also takes its slot in the voxel's unordered table (B slots per
 // voxel, `vmem`), and the voxel's first agent gives the voxel the next range of B bucket
 // slots (one counter atomic per block) and lists it as occupied: no claim or scatter launch
 // follows.  A voxel with more than B agents raises ST_BUCKET_OVERFLOW (the host grows B

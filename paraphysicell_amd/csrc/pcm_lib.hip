@@ -475,7 +475,10 @@ public:
 		if (h.vcount > 0x7fffffffLL)
 			return fail(PCM_ERR_CAPACITY, "mechanics voxel grid larger than 2^31 voxels");
 		h.K = h.dims == 3 ? 32 : (h.dims == 2 ? 16 : 8);
-		h.B = h.dims == 3 ? 32 : 0;
+		// voxel tables (B slots per voxel) for the fused mesh update: every 3D simulation, and 2D
+		// ones whose capacity keeps them below the lane-per-agent neighbor search (which walks
+		// the dense bucket layout the claim and scatter launches build)
+		h.B = h.dims == 3 || (h.dims == 2 && h.cap < LANES_NEIGHBOR_MIN_AGENTS) ? 32 : 0;
 		h.L = 12;
 		h.LA = 1;
 		h.seed = cfg_.seed;
@@ -945,7 +948,7 @@ public:
 		std::vector<int> vc(V), vs(V);
 		PCM_TRY(copy_sync(vc.data(), h_.vcnt, sizeof(int) * V, hipMemcpyDeviceToHost));
 		PCM_TRY(copy_sync(vs.data(), h_.vstart, sizeof(int) * V, hipMemcpyDeviceToHost));
-		// bucket slots in use: one range per occupied voxel (dense, or B apart in 3D)
+		// bucket slots in use: one range per occupied voxel (dense, or B apart with voxel tables)
 		size_t span = 0;
 		for (size_t v = 0; v < V; v++)
 			if (vc[v] > 0)
@@ -1172,6 +1175,7 @@ public:
 			return fail(PCM_ERR_STATE, "device status word is set");
 		if (!tune_chosen_)
 			tune_ = default_tuning(sc_.n);
+		tables_ = h_.B > 0;
 		// alone, a small population's launch is bound by its latency, not by the blocks a CU
 		// can hold: below a few thousand agents the default shape stays
 		neighbors_sparse_ = sc_.n >= LANES_NEIGHBOR_MIN_AGENTS && sparse_population(sc_.n, sc_.n_occ);
@@ -1278,6 +1282,9 @@ private:
 	std::vector<int> origin_ident_; // host source of set_count's ORIGIN identity
 	pcm_tuning tune_ = { 8, 8, 2 }; // launch shape of the lane-parallel kernels
 	bool tune_chosen_ = false;      // set by pcm_set_tuning or the autotuner; else by size
+	// the fused mesh update fills the voxel tables (no claim or scatter launch): set per step,
+	// or by a batch for its group (only when every member has tables)
+	bool tables_ = false;
 
 	// the launch shape when none was chosen: from 65536 agents up four lanes per agent (more
 	// agents per wavefront: 147 against 149 us per step at 88.7k agents), below that eight
@@ -1429,8 +1436,8 @@ private:
 		return PCM_OK;
 	}
 
-	// bucket slots: with the 3D unordered tables, B per occupied voxel (at most one voxel
-	// per agent), else one per agent
+	// bucket slots: with voxel tables (3D, and 2D below the lane search), B per occupied voxel
+	// (at most one voxel per agent), else one per agent
 	int alloc_bucket_storage()
 	{
 		const size_t cap = std::max(1, h_.cap);
@@ -1776,7 +1783,7 @@ private:
 		// 1D and 2D stencils hold a few dozen candidates: above a few thousand agents one
 		// lane per agent beats a block per occupied voxel (far fewer blocks); below, the
 		// block per voxel has more parallelism to hide its loads
-		if (mesh_fresh_ && D < 3 && gn() >= LANES_NEIGHBOR_MIN_AGENTS)
+		if (mesh_fresh_ && D < 3 && !tables_ && gn() >= LANES_NEIGHBOR_MIN_AGENTS)
 			return per_agent(PCM_K_NEIGHBORS, k_neighbors_lanes<R, D < 3 ? D : 2>);
 		if (mesh_fresh_ && D == 3 && neighbors_sparse_)
 			return launch(PCM_K_NEIGHBORS, k_neighbors<R, D, 1, NBR_TILE_SPARSE>, neighbor_blocks(), 64, MAP_NEIGHBORS);
@@ -1810,11 +1817,12 @@ private:
 		}
 	}
 
-	// FUSED: the histogram ran in the step's compaction kernel (after the clear).  In 3D that
-	// kernel also filled the voxels' unordered tables and gave each occupied voxel its bucket
-	// range, so the ordering kernel follows at once; in 1D and 2D the claim and scatter
-	// kernels lay the buckets out densely (the lane-per-agent neighbor search walks them).
-	// The ordering kernel also decides the next step's erased spring slots.
+	// FUSED: the histogram ran in the step's compaction kernel (after the clear).  With voxel
+	// tables (tables_: 3D, and 2D below the lane search) that kernel also filled the voxels'
+	// unordered tables and gave each occupied voxel its bucket range, so the ordering kernel
+	// follows at once; otherwise the claim and scatter kernels lay the buckets out densely
+	// (the lane-per-agent neighbor search walks them).  The ordering kernel also decides the
+	// next step's erased spring slots.
 	template <int D, bool FUSED = false>
 	int mesh_kernels()
 	{
@@ -1826,8 +1834,8 @@ private:
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 			PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D>));
 		}
-		constexpr bool FIXED = FUSED && D == 3;
-		if (!FIXED)
+		const bool fixed = FUSED && D >= 2 && tables_;
+		if (!fixed)
 		{
 			PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
 						   CLAIM_THREADS, MAP_CLAIM));
@@ -1837,7 +1845,11 @@ private:
 			return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 		// the fused step's ordering blocks, then the next step's erase decisions
 		const int erase_blocks = (int)(((long long)gn() * ERASE_LANES + BLOCK - 1) / BLOCK);
-		return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true, FIXED>, grid_agents() + erase_blocks, BLOCK,
+		if constexpr (D >= 2)
+			if (fixed)
+				return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true, true>, grid_agents() + erase_blocks, BLOCK,
+							  MAP_ORDER);
+		return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true, false>, grid_agents() + erase_blocks, BLOCK,
 					  MAP_ORDER);
 	}
 
@@ -2007,8 +2019,18 @@ private:
 		if (!fold_clear)
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 		// the container update's moves and the mesh update's histogram
-		PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, D == 3>, MOVE_BLOCKS + grid_agents(), BLOCK,
-					   MAP_COMPACT));
+		if constexpr (D >= 2)
+		{
+			if (tables_)
+				PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, true>, MOVE_BLOCKS + grid_agents(), BLOCK,
+							   MAP_COMPACT));
+			else
+				PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, false>, MOVE_BLOCKS + grid_agents(), BLOCK,
+							   MAP_COMPACT));
+		}
+		else
+			PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, false>, MOVE_BLOCKS + grid_agents(), BLOCK,
+						   MAP_COMPACT));
 		PCM_TRY((mesh_kernels<D, true>()));
 		return neighbors_d<D>();
 	}
@@ -2044,6 +2066,8 @@ public:
 	int dims() const { return h_.dims; }
 	int launch_bound() const { return n_launch_; }
 	void set_neighbors_sparse(bool on) { neighbors_sparse_ = on; }
+	void set_tables(bool on) { tables_ = on; }
+	bool has_tables() const { return h_.B > 0; }
 	int spring_slots() const { return h_.L; }
 	// neighbor-search blocks of this member in a batch's flattened launch: its occupied
 	// voxels at the last step seen, with headroom, rounded to four significant bits so that
@@ -2400,6 +2424,7 @@ private:
 		unsigned long long maps = 0; // map version of a flattened capture (0: none)
 		int velocity_lanes = 0, event_lanes = 0, neighbor_waves = 0;
 		bool sparse = false; // the neighbor search's sparse-population kernel
+		bool tables = false; // the mesh update fills voxel tables
 		bool operator==(const capture_key&) const = default;
 	};
 	struct group
@@ -2695,6 +2720,12 @@ private:
 			}
 		const bool sparse = sparse_population(agents, occupied);
 		g.placeholder->set_neighbors_sparse(sparse);
+		// the group's mesh update fills voxel tables only when every member has them
+		bool tables = true;
+		for (auto& slot : g.slots)
+			if (slot.sim)
+				tables = tables && slot.sim->has_tables();
+		g.placeholder->set_tables(tables);
 		bool fold_clear = true;
 		for (auto& slot : g.slots)
 			if (slot.sim)
@@ -2734,7 +2765,7 @@ private:
 			return run(true, true, true, steps - done);
 		const pcm_tuning t = leader->tuning();
 		const capture_key key { m, grid_n, grid_L, leader->host_ctx().springs, maps ? g.map_version : 0,
-								t.velocity_lanes, t.event_lanes, t.neighbor_waves, sparse };
+								t.velocity_lanes, t.event_lanes, t.neighbor_waves, sparse, tables };
 		if (!g.exec || !(key == g.key))
 		{
 			if (g.exec)

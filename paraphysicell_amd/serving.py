@@ -28,10 +28,13 @@ def footprint_bytes(mesh: abi.MeshSpec, real_bytes: int, defs: int, subs: int, c
     fields = sum(abi.field_width(w, d, defs, subs) * (r if t == abi.T_REAL else (4 if t == abi.T_I32 else 1))
                  for _, t, w in abi.FIELDS)
     k = {1: 8, 2: 16, 3: 32}[d]
-    b = 32 if d == 3 else 1  # bucket slots per agent (3D: a range of 32 per occupied voxel)
+    # voxel tables (a range of 32 bucket slots per occupied voxel, 32 table slots per voxel):
+    # 3D, and 2D below the lane-per-agent neighbor search (capacity under 4096)
+    tables = d == 3 or (d == 2 and capacity < 4096)
+    b = 32 if tables else 1  # bucket slots per agent
     per_agent = fields + 4 * (k + 12 + 1 + 2 * k + k) + k + 12 * r + 160 + (4 + 4 * r) * (b - 1)
     g = mesh.grid_shape()
-    per_voxel = 8 + (4 * b if d == 3 else 0)  # counts, starts and the 3D unordered tables
+    per_voxel = 8 + (4 * b if tables else 0)  # counts, starts and the unordered tables
     return int(capacity * per_agent + per_voxel * g[0] * g[1] * g[2] + (1 << 16))
 
 
