@@ -475,10 +475,10 @@ public:
 		if (h.vcount > 0x7fffffffLL)
 			return fail(PCM_ERR_CAPACITY, "mechanics voxel grid larger than 2^31 voxels");
 		h.K = h.dims == 3 ? 32 : (h.dims == 2 ? 16 : 8);
-		// voxel tables (B slots per voxel) for the fused mesh update: every 3D simulation, and 2D
-		// ones whose capacity keeps them below the lane-per-agent neighbor search (which walks
-		// the dense bucket layout the claim and scatter launches build)
-		h.B = h.dims == 3 || (h.dims == 2 && h.cap < LANES_NEIGHBOR_MIN_AGENTS) ? 32 : 0;
+		// voxel tables (B slots per voxel) for the fused mesh update: every 3D simulation, and 1D
+		// and 2D ones whose capacity keeps them below the lane-per-agent neighbor search (which
+		// walks the dense bucket layout the claim and scatter launches build)
+		h.B = h.dims == 3 || h.cap < LANES_NEIGHBOR_MIN_AGENTS ? 32 : 0;
 		h.L = 12;
 		h.LA = 1;
 		h.seed = cfg_.seed;
@@ -1436,7 +1436,7 @@ private:
 		return PCM_OK;
 	}
 
-	// bucket slots: with voxel tables (3D, and 2D below the lane search), B per occupied voxel
+	// bucket slots: with voxel tables (3D, and 1D and 2D below the lane search), B per occupied voxel
 	// (at most one voxel per agent), else one per agent
 	int alloc_bucket_storage()
 	{
@@ -1818,7 +1818,7 @@ private:
 	}
 
 	// FUSED: the histogram ran in the step's compaction kernel (after the clear).  With voxel
-	// tables (tables_: 3D, and 2D below the lane search) that kernel also filled the voxels'
+	// tables (tables_: 3D, and 1D and 2D below the lane search) that kernel also filled the voxels'
 	// unordered tables and gave each occupied voxel its bucket range, so the ordering kernel
 	// follows at once; otherwise the claim and scatter kernels lay the buckets out densely
 	// (the lane-per-agent neighbor search walks them).  The ordering kernel also decides the
@@ -1834,7 +1834,7 @@ private:
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 			PCM_TRY(per_agent(PCM_K_BIN_COUNT, k_bin_count<R, D>));
 		}
-		const bool fixed = FUSED && D >= 2 && tables_;
+		const bool fixed = FUSED && tables_;
 		if (!fixed)
 		{
 			PCM_TRY(launch(PCM_K_BIN_CLAIM, k_bin_claim<R, FUSED>, std::max(1, (gn() + CLAIM_THREADS - 1) / CLAIM_THREADS),
@@ -1845,10 +1845,9 @@ private:
 			return per_agent(PCM_K_BIN_ORDER, k_bin_order<R, D>);
 		// the fused step's ordering blocks, then the next step's erase decisions
 		const int erase_blocks = (int)(((long long)gn() * ERASE_LANES + BLOCK - 1) / BLOCK);
-		if constexpr (D >= 2)
-			if (fixed)
-				return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true, true>, grid_agents() + erase_blocks, BLOCK,
-							  MAP_ORDER);
+		if (fixed)
+			return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true, true>, grid_agents() + erase_blocks, BLOCK,
+						  MAP_ORDER);
 		return launch(PCM_K_BIN_ORDER, k_bin_order<R, D, true, true, false>, grid_agents() + erase_blocks, BLOCK,
 					  MAP_ORDER);
 	}
@@ -2019,15 +2018,9 @@ private:
 		if (!fold_clear)
 			PCM_TRY(per_agent(PCM_K_BIN_CLEAR, k_bin_clear<R>));
 		// the container update's moves and the mesh update's histogram
-		if constexpr (D >= 2)
-		{
-			if (tables_)
-				PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, true>, MOVE_BLOCKS + grid_agents(), BLOCK,
-							   MAP_COMPACT));
-			else
-				PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, false>, MOVE_BLOCKS + grid_agents(), BLOCK,
-							   MAP_COMPACT));
-		}
+		if (tables_)
+			PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, true>, MOVE_BLOCKS + grid_agents(), BLOCK,
+						   MAP_COMPACT));
 		else
 			PCM_TRY(launch(PCM_K_COMPACT_APPLY, k_compact_count<R, D, false>, MOVE_BLOCKS + grid_agents(), BLOCK,
 						   MAP_COMPACT));

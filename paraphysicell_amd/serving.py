@@ -29,8 +29,8 @@ def footprint_bytes(mesh: abi.MeshSpec, real_bytes: int, defs: int, subs: int, c
                  for _, t, w in abi.FIELDS)
     k = {1: 8, 2: 16, 3: 32}[d]
     # voxel tables (a range of 32 bucket slots per occupied voxel, 32 table slots per voxel):
-    # 3D, and 2D below the lane-per-agent neighbor search (capacity under 4096)
-    tables = d == 3 or (d == 2 and capacity < 4096)
+    # 3D, and 1D and 2D below the lane-per-agent neighbor search (capacity under 4096)
+    tables = d == 3 or capacity < 4096
     b = 32 if tables else 1  # bucket slots per agent
     per_agent = fields + 4 * (k + 12 + 1 + 2 * k + k) + k + 12 * r + 160 + (4 + 4 * r) * (b - 1)
     g = mesh.grid_shape()
